@@ -1,0 +1,268 @@
+/*
+ * sphexa_amd -- C ABI of the MI355X (gfx950) SPH hydro hot path.
+ *
+ * One extern "C" entry point per phase of the reference's hydro step. Each one
+ * replaces the GPU plug-in point of the reference (the `HaveGpu` branch of the
+ * public template, dispatching to the extern templates declared in
+ * sph/include/sph/sph_gpu.hpp:11-98); the reference function each entry point
+ * stands in for is cited above its declaration (paths relative to the reference
+ * repository root).
+ *
+ * Conventions
+ *  - All field pointers are DEVICE pointers owned by the caller (the Dataset);
+ *    kernels borrow them and write outputs in place. Scratch (tree, neighbour
+ *    lists, counters) is owned by the workspace.
+ *  - Fields are SoA arrays indexed by SFC-sorted particle position, laid out as in
+ *    sphexa::ParticlesData (sph/include/sph/particles_data.hpp:220-266):
+ *    x,y,z,u,temp,du are RealType (double); the HydroType fields are double
+ *    (plain FP64 form, SPHX_F64) or float (the reference's SphTypes layout with the
+ *    mixed-precision kernels, SPHX_F32), selected by sphx_params_t.hydro_type.
+ *  - Work is queued on the workspace's stream (the stream passed at creation);
+ *    entry points that return a scalar (minDtCourant) or a status that depends on
+ *    device results synchronise that stream before returning.
+ *  - Every entry point returns an sphx_status (0 = success) and never exits the
+ *    process. Workspaces are independent: calls on different workspaces may run
+ *    concurrently; calls on one workspace must be serialised by the caller.
+ */
+#ifndef SPHEXA_AMD_H
+#define SPHEXA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sphx_status
+{
+    SPHX_OK                = 0,
+    SPHX_ERR_HIP           = 1, /* a HIP runtime call or kernel launch failed */
+    SPHX_ERR_ARG           = 2, /* invalid argument: null pointer, empty/inverted range, ng0 > ngmax */
+    SPHX_ERR_NOT_CONVERGED = 3, /* coupled h / neighbour-count iteration did not converge in 10 rounds */
+    SPHX_ERR_STATE         = 4, /* called out of order, e.g. pair loop before find_neighbors */
+    SPHX_ERR_ALLOC         = 5  /* device allocation failed */
+} sphx_status;
+
+typedef enum sphx_dtype
+{
+    SPHX_F64 = 0,
+    SPHX_F32 = 1
+} sphx_dtype;
+
+/* cstone::Box<double> (domain/include/cstone/sfc/box.hpp:95-174); boundary: 0 open, 1 periodic */
+typedef struct sphx_box
+{
+    double  lo[3];
+    double  hi[3];
+    int32_t boundary[3];
+} sphx_box_t;
+
+/* Device pointers to the ParticlesData fields used on the hot path
+ * (sph/include/sph/particles_data.hpp:220-248). Unused fields may be NULL. */
+typedef struct sphx_fields
+{
+    /* RealType (double) fields */
+    double* x;
+    double* y;
+    double* z;
+    double* u;
+    double* temp;
+    double* du;
+    /* HydroType fields (double or float, see sphx_params_t.hydro_type); m is Tmass (same type) */
+    void* h;
+    void* m;
+    void* vx;
+    void* vy;
+    void* vz;
+    void* rho;
+    void* p;
+    void* c;
+    void* prho;
+    void* tdpdTrho;
+    void* c11;
+    void* c12;
+    void* c13;
+    void* c22;
+    void* c23;
+    void* c33;
+    void* xm;
+    void* kx;
+    void* gradh;
+    void* divv;
+    void* curlv;
+    void* alpha;
+    void* ax;
+    void* ay;
+    void* az;
+    void* dV11;
+    void* dV12;
+    void* dV13;
+    void* dV22;
+    void* dV23;
+    void* dV33;
+    /* integer fields */
+    uint64_t* keys;
+    uint32_t* nc;
+} sphx_fields_t;
+
+/* Dataset scalars (particles_data.hpp:91-146) */
+typedef struct sphx_params
+{
+    double   K;              /* kernel normalisation, see sphx_workspace_set_kernel */
+    double   Kcour;          /* Courant factor (passed to tsKCourant as float, like the reference) */
+    double   gamma;          /* adiabatic index */
+    double   muiConst;       /* mean molecular weight, for temperature-based EOS */
+    double   minDt;          /* current global time step (AV switches) */
+    double   alphamin, alphamax, decay_constant;
+    double   Atmin, Atmax, ramp;
+    uint32_t ng0, ngmax;
+    int32_t  hydro_type;     /* sphx_dtype of the HydroType fields */
+    int32_t  reserved;
+} sphx_params_t;
+
+/* OctreeNsView<double, uint64_t> (domain/include/cstone/tree/octree.hpp:280-300) plus the
+ * cornerstone leaf counts; all device pointers into workspace-owned memory. */
+typedef struct sphx_tree_view
+{
+    int32_t         numLeafNodes;
+    int32_t         numInternalNodes;
+    int32_t         numNodes;
+    int32_t         reserved;
+    const uint64_t* prefixes;       /* [numNodes] placeholder-bit node keys, level-major order */
+    const int32_t*  childOffsets;   /* [numNodes] first child, 0 = leaf */
+    const int32_t*  parents;        /* [(numNodes-1)/8] */
+    const int32_t*  internalToLeaf; /* [numNodes] leaf index for leaves, negative for internal nodes */
+    const int32_t*  levelRange;     /* [23] */
+    const uint64_t* leaves;         /* [numLeafNodes+1] cornerstone leaf keys */
+    const uint32_t* counts;         /* [numLeafNodes] particles per leaf */
+    const uint32_t* layout;         /* [numLeafNodes+1] first particle of each leaf */
+    const double*   centers;        /* [3*numNodes] geometric centres (x,y,z interleaved) */
+    const double*   sizes;          /* [3*numNodes] half sizes */
+} sphx_tree_view_t;
+
+typedef struct sphx_workspace sphx_workspace_t;
+
+/* ---- workspace ------------------------------------------------------------------------------- */
+
+/* Create a workspace on `device`; all work is queued on `stream` (a hipStream_t, NULL = default
+ * stream). Replaces the Dataset-owned scratch of the reference: devData.traversalStack grown by
+ * allocateNcStacks (domain/include/cstone/traversal/find_neighbors.cuh:493-505). */
+int sphx_workspace_create(sphx_workspace_t** ws, int device, void* stream);
+int sphx_workspace_destroy(sphx_workspace_t* ws);
+
+/* Tabulate W and dW/dv (20000 points on [0,2]) for sinc kernel index `sincIndex` and upload them
+ * in both precisions; returns K through *K. Replaces ParticlesData::createTables
+ * (sph/include/sph/particles_data.hpp:380-387) + DeviceParticlesData::uploadTables. */
+int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K);
+
+/* Human-readable text for the last error recorded on this workspace. */
+const char* sphx_last_error(const sphx_workspace_t* ws);
+
+/* ---- keys, sort, tree (one-rank Domain::sync) ------------------------------------------------ */
+
+/* 64-bit Hilbert keys of n particles; keys equal to 2^63 (removeKey) are left unchanged.
+ * Replaces computeSfcKeysGpu (domain/include/cstone/sfc/sfc_gpu.cu:25-43). */
+int sphx_compute_sfc_keys(sphx_workspace_t* ws, const sphx_box_t* box, const double* x, const double* y,
+                          const double* z, uint64_t* keys, size_t n);
+
+/* Stable sort of keys; order[i] receives the original index of the i-th sorted key.
+ * Replaces sortByKeyGpu / GpuSfcSorter::setMapFromCodes (domain/include/cstone/primitives/
+ * primitives_gpu.cu:323-346, primitives/gather.cuh:50-128). */
+int sphx_sort_keys(sphx_workspace_t* ws, uint64_t* keys, uint32_t* order, size_t n);
+
+/* out[i] = in[order[i]] for elements of `elemBytes` (4 or 8). Replaces gatherGpu
+ * (domain/include/cstone/primitives/primitives_gpu.h:34-40). */
+int sphx_gather(sphx_workspace_t* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
+
+/* Cornerstone leaves (bucket size `bucketSize`), linked octree, leaf layout and geometric node
+ * centres from sorted keys, kept in the workspace. Replaces computeOctree / buildOctreeGpu /
+ * computeNodeLayout / nodeFpCenters as combined by Domain::sync + octreeProperties on one rank
+ * (domain/include/cstone/tree/csarray.hpp:430-440, tree/octree_gpu.cu:137-157,
+ * focus/source_center.hpp:130-141, domain/include/cstone/domain/domain.hpp:416-428). */
+int sphx_build_tree(sphx_workspace_t* ws, const uint64_t* sortedKeys, size_t n, uint32_t bucketSize,
+                    const sphx_box_t* box);
+
+/* Device view of the tree built by the last sphx_build_tree / sphx_domain_sync. */
+int sphx_tree_view(const sphx_workspace_t* ws, sphx_tree_view_t* out);
+
+/* keys -> stable sort -> permute every non-NULL field of `f` (conserved and dependent) -> tree.
+ * One-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234). */
+int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t* f, const sphx_params_t* prm,
+                     size_t n, uint32_t bucketSize);
+
+/* ---- neighbour search ---------------------------------------------------------------------- */
+
+/* Neighbour lists of targets [first,last) with the coupled h / nc iteration (at most 10 updates,
+ * h <- updateH(ng0, nc, h) while nc < ng0/4 or nc-1 > ngmax); writes f->nc[i] (self included)
+ * and updates f->h in place; lists stay in the workspace for the pair loops. With iterateH = 0
+ * the search runs once with the given h. Returns SPHX_ERR_NOT_CONVERGED (after completing all
+ * lists) if any particle did not converge. Replaces sph::findNeighborsSfc / findNeighborsSph
+ * (sph/include/sph/find_neighbors.hpp:11-56) and the h-loop of xmassGpu
+ * (sph/include/sph/hydro_ve/xmass_gpu.cu:57-102). */
+int sphx_find_neighbors(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                        const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH);
+
+/* Copy the stored neighbour list of targets [first,last) into `out` (device or host memory,
+ * row-major [last-first][ngmax], entries beyond min(nc-1, ngmax) unspecified). Test/inspection
+ * helper matching the CPU Dataset's d.neighbors layout (particles_data.hpp:250-251). */
+int sphx_copy_neighbors(sphx_workspace_t* ws, uint32_t* out, uint32_t first, uint32_t last);
+
+/* Pair-test statistics of the last search: total stored pairs and maximum count. */
+int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t* maxCount);
+
+/* ---- std SPH (sph/include/sph/hydro_std) ---------------------------------------------------- */
+
+/* rho = m / xm with xm from xmassJLoop. Replaces computeDensity (hydro_std/density.hpp:56-60). */
+int sphx_density_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                     const sphx_params_t* prm, uint32_t first, uint32_t last);
+
+/* Ideal-gas EOS from u (or temp when u is NULL): p, c. Replaces computeEOS_HydroStd with
+ * eosChoice = idealGas (hydro_std/eos.hpp:55-153). */
+int sphx_eos_std(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                 uint32_t last);
+
+/* IAD tensor inverse c11..c33. Replaces computeIAD (hydro_std/iad.hpp:75-79). */
+int sphx_iad_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_params_t* prm,
+                 uint32_t first, uint32_t last);
+
+/* ax, ay, az, du and the Courant time step (min over targets) in *minDtCourant.
+ * Replaces computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:97-101). */
+int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                             const sphx_params_t* prm, uint32_t first, uint32_t last, double* minDtCourant);
+
+/* ---- VE SPH (sph/include/sph/hydro_ve) ------------------------------------------------------ */
+
+/* xm. Replaces computeXMass (hydro_ve/xmass.hpp:67-72). */
+int sphx_xmass(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_params_t* prm,
+               uint32_t first, uint32_t last);
+
+/* kx, gradh. Replaces computeVeDefGradh (hydro_ve/ve_def_gradh.hpp:79-84). */
+int sphx_ve_def_gradh(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                      const sphx_params_t* prm, uint32_t first, uint32_t last);
+
+/* VE ideal-gas EOS: prho, c (and rho, p when non-NULL) from temp (u when temp is NULL).
+ * Replaces computeEOS with eosChoice = idealGas (hydro_ve/eos.hpp:53-197). */
+int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                uint32_t last);
+
+/* c11..c33, divv, curlv (and dV11..dV33 when non-NULL). Replaces computeIadDivvCurlv
+ * (hydro_ve/iad_divv_curlv.hpp:92-97). */
+int sphx_iad_divv_curlv(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                        const sphx_params_t* prm, uint32_t first, uint32_t last);
+
+/* alpha (in place). Replaces computeAVswitches (hydro_ve/av_switches.hpp:80-85). */
+int sphx_av_switches(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                     const sphx_params_t* prm, uint32_t first, uint32_t last);
+
+/* ax, ay, az, du, *minDtCourant; avClean selects the AV-cleaning variant.
+ * Replaces computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:106-115). */
+int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                            const sphx_params_t* prm, uint32_t first, uint32_t last, int avClean,
+                            double* minDtCourant);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHEXA_AMD_H */

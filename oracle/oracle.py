@@ -1,0 +1,214 @@
+"""ctypes loader of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/liboracle.so (the restatement, prefix ``orc_``) or oracle/_ref/libsphx_ref.so
+(the reference's own headers compiled by oracle/Makefile, prefix ``ref_``). Used by tests/,
+__graft_entry__.smoke() and the CPU-baseline leg of bench.py only, as the checker; the
+product library never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from sphexa_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {"orc": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libsphx_ref.so")}
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_U32 = C.c_uint32
+_BOXP = C.POINTER(abi.Box)
+_FP = C.POINTER(abi.Fields)
+_PP = C.POINTER(abi.Params)
+
+_SIGS = {
+    "sfc_keys": (None, [_BOXP, _P, _P, _P, _P, _SZ]),
+    "hilbert_encode": (None, [_P, _P, _P, _P, _SZ]),
+    "hilbert_decode": (None, [_P, _P, _P, _P, _SZ]),
+    "sort_keys": (None, [_P, _P, _SZ]),
+    "tree_build": (_P, [_P, _SZ, _U32, _BOXP]),
+    "tree_sizes": (None, [_P, _P, _P, _P]),
+    "tree_copy": (None, [_P] * 11),
+    "tree_free": (None, [_P]),
+    "find_neighbors": (_SZ, [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, _P]),
+    "kernel_K": (C.c_double, [C.c_double]),
+    "sphynx_K": (C.c_double, [C.c_double]),
+    "tables_f64": (None, [C.c_double, _P, _P]),
+    "tables_f32": (None, [C.c_double, _P, _P]),
+    "lookup_f64": (C.c_double, [_P, C.c_double]),
+    "density_std": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "eos_std": (None, [_FP, _PP, _U32, _U32]),
+    "iad_std": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "momentum_energy_std": (C.c_double, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P]),
+    "xmass": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "ve_def_gradh": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P]),
+    "eos_ve": (None, [_FP, _PP, _U32, _U32]),
+    "iad_divv_curlv": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "av_switches": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "momentum_energy_ve": (C.c_double, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, C.c_int, _P]),
+}
+
+_cache = {}
+
+
+def available(kind="orc"):
+    return os.path.exists(LIBS[kind])
+
+
+def load(kind="orc"):
+    """Return a namespace of the oracle's C functions (kind 'orc' or 'ref')."""
+    if kind in _cache:
+        return _cache[kind]
+    path = LIBS[kind]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (run: make -C oracle{' ref' if kind == 'ref' else ''})")
+    lib = C.CDLL(path)
+    ns = {}
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, f"{kind}_{name}")
+        fn.restype = res
+        fn.argtypes = args
+        ns[name] = fn
+    ns["_lib"] = lib
+    _cache[kind] = type("OracleLib", (), ns)
+    return _cache[kind]
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Tree:
+    """Cornerstone leaves + linked octree + layout + node centres (numpy copies)."""
+
+    def __init__(self, lib, sorted_keys, box, bucket=64):
+        keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+        self.lib = lib
+        self.handle = lib.tree_build(ptr(keys), keys.size, bucket, C.byref(box))
+        nl, ni, nn = C.c_int32(), C.c_int32(), C.c_int32()
+        lib.tree_sizes(self.handle, C.byref(nl), C.byref(ni), C.byref(nn))
+        self.numLeafNodes, self.numInternalNodes, self.numNodes = nl.value, ni.value, nn.value
+        L, N = self.numLeafNodes, self.numNodes
+        self.leaves = np.zeros(L + 1, np.uint64)
+        self.counts = np.zeros(L, np.uint32)
+        self.layout = np.zeros(L + 1, np.uint32)
+        self.prefixes = np.zeros(N, np.uint64)
+        self.childOffsets = np.zeros(N, np.int32)
+        self.parents = np.zeros(max(1, (N - 1) // 8), np.int32)
+        self.internalToLeaf = np.zeros(N, np.int32)
+        self.levelRange = np.zeros(23, np.int32)
+        self.centers = np.zeros((N, 3), np.float64)
+        self.sizes = np.zeros((N, 3), np.float64)
+        lib.tree_copy(self.handle, ptr(self.leaves), ptr(self.counts), ptr(self.layout), ptr(self.prefixes),
+                      ptr(self.childOffsets), ptr(self.parents), ptr(self.internalToLeaf), ptr(self.levelRange),
+                      ptr(self.centers), ptr(self.sizes))
+
+    def __del__(self):
+        try:
+            self.lib.tree_free(self.handle)
+        except Exception:
+            pass
+
+
+def tables(lib, sinc_index=6.0, dtype=np.float64):
+    wh = np.zeros(20000, dtype)
+    whd = np.zeros(20000, dtype)
+    fn = lib.tables_f64 if dtype == np.float64 else lib.tables_f32
+    fn(sinc_index, ptr(wh), ptr(whd))
+    return wh, whd
+
+
+def sfc_keys(lib, box, x, y, z):
+    keys = np.zeros(x.size, np.uint64)
+    lib.sfc_keys(C.byref(box), ptr(x), ptr(y), ptr(z), ptr(keys), x.size)
+    return keys
+
+
+def sort_keys(lib, keys):
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    order = np.zeros(k.size, np.uint32)
+    lib.sort_keys(ptr(k), ptr(order), k.size)
+    return k, order
+
+
+def find_neighbors(lib, tree, box, arrays, params, first, last, iterate_h=True):
+    """Run findNeighborsSph on [first,last); updates arrays['h'] and arrays['nc'] in place.
+    Returns (neighbors[last-first, ngmax], number of non-converged particles)."""
+    nb = np.zeros((last - first) * params.ngmax, np.uint32)
+    f = abi.fields_from(arrays)
+    fails = lib.find_neighbors(tree.handle, C.byref(box), C.byref(f), C.byref(params), first, last,
+                               int(iterate_h), ptr(nb))
+    return nb.reshape(last - first, params.ngmax), fails
+
+
+def sorted_lists(nb, nc, ngmax):
+    """Per-target sorted neighbour sets (only the stored min(nc-1, ngmax) entries)."""
+    out = []
+    for k in range(nb.shape[0]):
+        n = min(int(nc[k]) - 1, ngmax)
+        out.append(np.sort(nb[k, :n]))
+    return out
+
+
+class CpuPass:
+    """One std or VE hydro pass on host arrays through the oracle (or the reference wrapper)."""
+
+    def __init__(self, lib, box, arrays, params, nbrs, first, last, wh, whd):
+        self.lib, self.box, self.a, self.p = lib, box, arrays, params
+        self.nb, self.first, self.last, self.wh, self.whd = nbrs, first, last, wh, whd
+
+    def _f(self):
+        return abi.fields_from(self.a)
+
+    def density_std(self):
+        self.lib.density_std(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                             ptr(self.nb), ptr(self.wh))
+
+    def eos_std(self):
+        self.lib.eos_std(C.byref(self._f()), C.byref(self.p), self.first, self.last)
+
+    def iad_std(self):
+        self.lib.iad_std(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                         ptr(self.nb), ptr(self.wh))
+
+    def momentum_energy_std(self, maxvs=None):
+        return self.lib.momentum_energy_std(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first,
+                                            self.last, ptr(self.nb), ptr(self.wh), ptr(maxvs))
+
+    def xmass(self):
+        self.lib.xmass(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last, ptr(self.nb),
+                       ptr(self.wh))
+
+    def ve_def_gradh(self):
+        self.lib.ve_def_gradh(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                              ptr(self.nb), ptr(self.wh), ptr(self.whd))
+
+    def eos_ve(self):
+        self.lib.eos_ve(C.byref(self._f()), C.byref(self.p), self.first, self.last)
+
+    def iad_divv_curlv(self):
+        self.lib.iad_divv_curlv(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                                ptr(self.nb), ptr(self.wh))
+
+    def av_switches(self):
+        self.lib.av_switches(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                             ptr(self.nb), ptr(self.wh))
+
+    def momentum_energy_ve(self, av_clean=False, maxvs=None):
+        return self.lib.momentum_energy_ve(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first,
+                                           self.last, ptr(self.nb), ptr(self.wh), int(av_clean), ptr(maxvs))
+
+    def run_std(self):
+        self.density_std()
+        self.eos_std()
+        self.iad_std()
+        return self.momentum_energy_std()
+
+    def run_ve(self, av_clean=False):
+        self.xmass()
+        self.ve_def_gradh()
+        self.eos_ve()
+        self.iad_divv_curlv()
+        self.av_switches()
+        return self.momentum_energy_ve(av_clean)

@@ -1,0 +1,446 @@
+/*
+ * C ABI of the CPU oracle (TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the
+ * CPU-baseline leg of bench.py through ctypes; never linked into the product library).
+ *
+ * Host pointers throughout. The particle view structs are the product's own
+ * (include/sphexa_amd.h) so that one Python view feeds both sides with identical data.
+ * Drivers follow the reference OpenMP drivers cited on each function; the neighbour list of
+ * target i (first <= i < last) is row (i - first) of `nbrs` with `ngmax` columns, exactly like
+ * the CPU Dataset's d.neighbors (sph/include/sph/particles_data.hpp:250-251).
+ */
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+#include "sphexa_amd.h"
+#include "sphx_oracle.hpp"
+
+using namespace orc;
+
+namespace
+{
+
+Box toBox(const sphx_box_t* b)
+{
+    double lim[6] = {b->lo[0], b->hi[0], b->lo[1], b->hi[1], b->lo[2], b->hi[2]};
+    int    bc[3]  = {b->boundary[0], b->boundary[1], b->boundary[2]};
+    return Box(lim, bc);
+}
+
+template<class T>
+T* F(void* p)
+{
+    return static_cast<T*>(p);
+}
+
+template<class T>
+unsigned capped(const sphx_fields_t* f, LocalIndex i, unsigned ngmax)
+{
+    return std::min(f->nc[i] - 1, ngmax);
+}
+
+//! computeDensity (hydro_std/density.hpp:42-53): xm in rho's slot, then rho = m / xm
+template<class T>
+void densityStd(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+                const LocalIndex* nb, const T* wh)
+{
+    T* rho = F<T>(f->rho);
+    T* h   = F<T>(f->h);
+    T* m   = F<T>(f->m);
+#pragma omp parallel for
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        rho[i]      = xmassLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, h, m, wh);
+    }
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+        rho[i] = m[i] / rho[i];
+}
+
+//! computeXMassImpl, hydro_ve/xmass.hpp:40-65
+template<class T>
+void xmass(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+           const LocalIndex* nb, const T* wh)
+{
+    T* xm = F<T>(f->xm);
+#pragma omp parallel for
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        xm[i] = xmassLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->h),
+                          F<T>(f->m), wh);
+    }
+}
+
+//! computeIdealGasEOS_HydroStd_Impl, hydro_std/eos.hpp:55-82
+template<class T>
+void eosStd(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
+{
+    T* rho = F<T>(f->rho);
+    T* p   = F<T>(f->p);
+    T* c   = F<T>(f->c);
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        double pi, ci;
+        if (f->u != nullptr) { idealGasFromU(f->u[i], rho[i], prm->gamma, pi, ci); }
+        else
+        {
+            T    mui = T(prm->muiConst);
+            auto cv  = idealGasCv(mui, prm->gamma);
+            idealGasFromU(cv * f->temp[i], rho[i], prm->gamma, pi, ci);
+        }
+        p[i] = pi;
+        c[i] = ci;
+    }
+}
+
+//! computeIADImpl, hydro_std/iad.hpp:42-72
+template<class T>
+void iadStd(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+            const LocalIndex* nb, const T* wh)
+{
+    const T* m   = F<T>(f->m);
+    const T* rho = F<T>(f->rho);
+    auto     vol = [m, rho](LocalIndex j) { return m[j] / rho[j]; };
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        iadLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->h), wh, vol,
+                F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33));
+    }
+}
+
+//! computeMomentumEnergyStdImpl, hydro_std/momentum_energy.hpp:42-95
+template<class T>
+double momentumEnergyStd(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first,
+                         LocalIndex last, const LocalIndex* nb, const T* wh, T* maxvsOut)
+{
+    T minDt = INFINITY;
+    const T* h = F<T>(f->h);
+    const T* c = F<T>(f->c);
+#pragma omp parallel for schedule(static) reduction(min : minDt)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        T        maxvs = 0;
+        unsigned nn    = capped<T>(f, i, p->ngmax);
+        momentumEnergyStdLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->vx),
+                              F<T>(f->vy), F<T>(f->vz), h, F<T>(f->m), F<T>(f->rho), F<T>(f->p), c,
+                              F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33),
+                              wh, F<T>(f->ax), F<T>(f->ay), F<T>(f->az), f->du, &maxvs);
+        if (maxvsOut) { maxvsOut[i] = maxvs; }
+        T dt_i = courantDt(maxvs, h[i], c[i], float(p->Kcour));
+        minDt  = std::min(minDt, dt_i);
+    }
+    return minDt;
+}
+
+//! computeVeDefGradhImpl, hydro_ve/ve_def_gradh.hpp:40-77
+template<class T>
+void veDefGradh(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+                const LocalIndex* nb, const T* wh, const T* whd)
+{
+    T* kx    = F<T>(f->kx);
+    T* gradh = F<T>(f->gradh);
+#pragma omp parallel for
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        T        k, g;
+        veDefGradhLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->h),
+                       F<T>(f->m), wh, whd, F<T>(f->xm), k, g);
+        kx[i]    = k;
+        gradh[i] = g;
+    }
+}
+
+//! computeIdealGasEOS_Impl, hydro_ve/eos.hpp:53-95 (temp-based when temp != NULL, else u-based)
+template<class T>
+void eosVe(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
+{
+    const T* m     = F<T>(f->m);
+    const T* kx    = F<T>(f->kx);
+    const T* xm    = F<T>(f->xm);
+    const T* gradh = F<T>(f->gradh);
+    T*       prho  = F<T>(f->prho);
+    T*       c     = F<T>(f->c);
+    T*       rhoO  = F<T>(f->rho);
+    T*       pO    = F<T>(f->p);
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        auto   rho = kx[i] * m[i] / xm[i];
+        double pi, ci;
+        if (f->temp != nullptr)
+        {
+            T    mui = T(prm->muiConst);
+            auto cv  = idealGasCv(mui, prm->gamma);
+            idealGasFromU(cv * f->temp[i], rho, prm->gamma, pi, ci);
+        }
+        else { idealGasFromU(f->u[i], rho, prm->gamma, pi, ci); }
+        prho[i] = pi / (kx[i] * m[i] * m[i] * gradh[i]);
+        c[i]    = ci;
+        if (rhoO) { rhoO[i] = rho; }
+        if (pO) { pO[i] = pi; }
+    }
+}
+
+//! computeIadDivvCurlvImpl, hydro_ve/iad_divv_curlv.hpp:42-90
+template<class T>
+void iadDivvCurlv(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+                  const LocalIndex* nb, const T* wh)
+{
+    const T* xm      = F<T>(f->xm);
+    const T* kx      = F<T>(f->kx);
+    auto     vol     = [xm, kx](LocalIndex j) { return xm[j] / kx[j]; };
+    bool     doGradV = f->dV11 != nullptr;
+#pragma omp parallel for
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned          nn = capped<T>(f, i, p->ngmax);
+        const LocalIndex* ni = nb + size_t(i - first) * p->ngmax;
+        iadLoop(i, p->K, box, ni, nn, f->x, f->y, f->z, F<T>(f->h), wh, vol, F<T>(f->c11), F<T>(f->c12),
+                F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33));
+        divvCurlvLoop(i, p->K, box, ni, nn, f->x, f->y, f->z, F<T>(f->vx), F<T>(f->vy), F<T>(f->vz), F<T>(f->h),
+                      F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33), wh, kx, xm,
+                      F<T>(f->divv), F<T>(f->curlv), F<T>(f->dV11), F<T>(f->dV12), F<T>(f->dV13), F<T>(f->dV22),
+                      F<T>(f->dV23), F<T>(f->dV33), doGradV);
+    }
+}
+
+//! computeAVswitchesImpl, hydro_ve/av_switches.hpp:41-78
+template<class T>
+void avSwitches(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+                const LocalIndex* nb, const T* wh)
+{
+    T* alpha = F<T>(f->alpha);
+#pragma omp parallel for
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        alpha[i] = avSwitchesLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->vx),
+                                  F<T>(f->vy), F<T>(f->vz), F<T>(f->h), F<T>(f->c), F<T>(f->c11), F<T>(f->c12),
+                                  F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33), wh, F<T>(f->kx),
+                                  F<T>(f->xm), F<T>(f->divv), p->minDt, T(p->alphamin), T(p->alphamax),
+                                  T(p->decay_constant), alpha[i]);
+    }
+}
+
+//! computeMomentumEnergyImpl<avClean>, hydro_ve/momentum_energy.hpp:41-104
+template<bool avClean, class T>
+double momentumEnergyVe(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first,
+                        LocalIndex last, const LocalIndex* nb, const T* wh, T* maxvsOut)
+{
+    T        minDt = INFINITY;
+    const T* h     = F<T>(f->h);
+    const T* c     = F<T>(f->c);
+#pragma omp parallel for schedule(static) reduction(min : minDt)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn    = capped<T>(f, i, p->ngmax);
+        T        maxvs = 0;
+        momentumEnergyVeLoop<avClean>(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z,
+                                      F<T>(f->vx), F<T>(f->vy), F<T>(f->vz), h, F<T>(f->m), F<T>(f->prho),
+                                      F<T>(f->tdpdTrho), c, F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22),
+                                      F<T>(f->c23), F<T>(f->c33), T(p->Atmin), T(p->Atmax), T(p->ramp), wh,
+                                      F<T>(f->kx), F<T>(f->xm), F<T>(f->alpha), F<T>(f->dV11), F<T>(f->dV12),
+                                      F<T>(f->dV13), F<T>(f->dV22), F<T>(f->dV23), F<T>(f->dV33), F<T>(f->ax),
+                                      F<T>(f->ay), F<T>(f->az), f->du, &maxvs);
+        if (maxvsOut) { maxvsOut[i] = maxvs; }
+        T dt_i = courantDt(maxvs, h[i], c[i], float(p->Kcour));
+        minDt  = std::min(minDt, dt_i);
+    }
+    return minDt;
+}
+
+} // namespace
+
+extern "C"
+{
+
+/* ---- keys / sort ------------------------------------------------------------------------------ */
+void orc_sfc_keys(const sphx_box_t* b, const double* x, const double* y, const double* z, uint64_t* keys, size_t n)
+{
+    computeSfcKeys(x, y, z, keys, n, toBox(b));
+}
+
+void orc_hilbert_encode(const uint32_t* ix, const uint32_t* iy, const uint32_t* iz, uint64_t* keys, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        keys[i] = hilbertEncode(ix[i], iy[i], iz[i]);
+}
+
+void orc_hilbert_decode(const uint64_t* keys, uint32_t* ix, uint32_t* iy, uint32_t* iz, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        hilbertDecode(keys[i], ix[i], iy[i], iz[i]);
+}
+
+//! keys sorted in place (stable); order receives the original index of each sorted key
+void orc_sort_keys(uint64_t* keys, uint32_t* order, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        order[i] = uint32_t(i);
+    stableSortByKey(keys, order, n);
+}
+
+/* ---- tree ------------------------------------------------------------------------------------ */
+void* orc_tree_build(const uint64_t* sortedKeys, size_t n, uint32_t bucket, const sphx_box_t* b)
+{
+    auto* t = new NsTree;
+    buildNsTree(sortedKeys, n, bucket, toBox(b), *t);
+    return t;
+}
+
+void orc_tree_sizes(const void* tp, int32_t* numLeaf, int32_t* numInternal, int32_t* numNodes)
+{
+    auto* t      = static_cast<const NsTree*>(tp);
+    *numLeaf     = t->oct.numLeafNodes;
+    *numInternal = t->oct.numInternalNodes;
+    *numNodes    = t->oct.numNodes;
+}
+
+void orc_tree_copy(const void* tp, uint64_t* leaves, uint32_t* counts, uint32_t* layout, uint64_t* prefixes,
+                   int32_t* childOffsets, int32_t* parents, int32_t* internalToLeaf, int32_t* levelRange,
+                   double* centers, double* sizes)
+{
+    auto* t = static_cast<const NsTree*>(tp);
+    auto  N = size_t(t->oct.numNodes);
+    if (leaves) std::memcpy(leaves, t->leaves.data(), t->leaves.size() * 8);
+    if (counts) std::memcpy(counts, t->counts.data(), t->counts.size() * 4);
+    if (layout) std::memcpy(layout, t->layout.data(), t->layout.size() * 4);
+    if (prefixes) std::memcpy(prefixes, t->oct.prefixes.data(), N * 8);
+    if (childOffsets) std::memcpy(childOffsets, t->oct.childOffsets.data(), N * 4);
+    if (parents) std::memcpy(parents, t->oct.parents.data(), t->oct.parents.size() * 4);
+    if (internalToLeaf) std::memcpy(internalToLeaf, t->oct.internalToLeaf.data(), N * 4);
+    if (levelRange) std::memcpy(levelRange, t->oct.levelRange.data(), (kMaxLevel + 2) * 4);
+    if (centers) std::memcpy(centers, t->centers.data(), 3 * N * 8);
+    if (sizes) std::memcpy(sizes, t->sizes.data(), 3 * N * 8);
+}
+
+void orc_tree_free(void* tp) { delete static_cast<NsTree*>(tp); }
+
+/* ---- neighbours ------------------------------------------------------------------------------ */
+//! findNeighborsSfc -> findNeighborsSph (sph/include/sph/find_neighbors.hpp:11-56); nc[i], h[i]
+//! for first <= i < last. Returns the number of particles whose h/nc iteration failed.
+size_t orc_find_neighbors(const void* tp, const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p,
+                          uint32_t first, uint32_t last, int iterateH, uint32_t* nbrs)
+{
+    auto* t   = static_cast<const NsTree*>(tp);
+    Box   box = toBox(b);
+    if (p->hydro_type == SPHX_F32)
+        return findNeighborsSph(f->x, f->y, f->z, F<float>(f->h), first, last, box, *t, p->ng0, p->ngmax, nbrs,
+                                f->nc + first, iterateH != 0);
+    return findNeighborsSph(f->x, f->y, f->z, F<double>(f->h), first, last, box, *t, p->ng0, p->ngmax, nbrs,
+                            f->nc + first, iterateH != 0);
+}
+
+/* ---- tables ---------------------------------------------------------------------------------- */
+double orc_kernel_K(double sincIndex) { return kernelNormalization(sincIndex); }
+double orc_sphynx_K(double n) { return sphynxK(n); }
+
+void orc_tables_f64(double sincIndex, double* wh, double* whd)
+{
+    tabulate<double>([sincIndex](double v) { return sincKernel(v, sincIndex); }, wh);
+    tabulate<double>([sincIndex](double v) { return sincKernelDerivative(v, sincIndex); }, whd);
+}
+
+void orc_tables_f32(double sincIndex, float* wh, float* whd)
+{
+    tabulate<float>([sincIndex](double v) { return sincKernel(v, sincIndex); }, wh);
+    tabulate<float>([sincIndex](double v) { return sincKernelDerivative(v, sincIndex); }, whd);
+}
+
+double orc_lookup_f64(const double* table, double v) { return lookup(table, v); }
+
+/* ---- SPH passes (wh/whd in the hydro type) --------------------------------------------------- */
+#define ORC_DISPATCH(call)                                                                                             \
+    if (p->hydro_type == SPHX_F32) { call(float); }                                                                    \
+    else { call(double); }
+
+void orc_density_std(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                     uint32_t last, const uint32_t* nb, const void* wh)
+{
+#define C(T) densityStd<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_eos_std(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
+{
+#define C(T) eosStd<T>(f, p, first, last)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_iad_std(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                 const uint32_t* nb, const void* wh)
+{
+#define C(T) iadStd<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+double orc_momentum_energy_std(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                               uint32_t last, const uint32_t* nb, const void* wh, void* maxvs)
+{
+    if (p->hydro_type == SPHX_F32)
+        return momentumEnergyStd<float>(toBox(b), f, p, first, last, nb, (const float*)wh, (float*)maxvs);
+    return momentumEnergyStd<double>(toBox(b), f, p, first, last, nb, (const double*)wh, (double*)maxvs);
+}
+
+void orc_xmass(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+               const uint32_t* nb, const void* wh)
+{
+#define C(T) xmass<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_ve_def_gradh(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                      uint32_t last, const uint32_t* nb, const void* wh, const void* whd)
+{
+#define C(T) veDefGradh<T>(toBox(b), f, p, first, last, nb, (const T*)wh, (const T*)whd)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_eos_ve(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
+{
+#define C(T) eosVe<T>(f, p, first, last)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_iad_divv_curlv(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                        uint32_t last, const uint32_t* nb, const void* wh)
+{
+#define C(T) iadDivvCurlv<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_av_switches(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                     uint32_t last, const uint32_t* nb, const void* wh)
+{
+#define C(T) avSwitches<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+double orc_momentum_energy_ve(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                              uint32_t last, const uint32_t* nb, const void* wh, int avClean, void* maxvs)
+{
+    Box box = toBox(b);
+    if (p->hydro_type == SPHX_F32)
+    {
+        return avClean ? momentumEnergyVe<true, float>(box, f, p, first, last, nb, (const float*)wh, (float*)maxvs)
+                       : momentumEnergyVe<false, float>(box, f, p, first, last, nb, (const float*)wh, (float*)maxvs);
+    }
+    return avClean ? momentumEnergyVe<true, double>(box, f, p, first, last, nb, (const double*)wh, (double*)maxvs)
+                   : momentumEnergyVe<false, double>(box, f, p, first, last, nb, (const double*)wh, (double*)maxvs);
+}
+
+} // extern "C"

@@ -1,0 +1,1150 @@
+/*
+ * CPU restatement of SPH-EXA's hydro hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This header is the oracle: the checker the HIP product path is compared
+ * against. Only tests, the smoke check and the CPU-baseline leg of the bench
+ * may load it; the product library never links or calls it.
+ *
+ * Every function restates one reference function (file:line cited on each,
+ * paths relative to the reference repository root) and keeps the reference's
+ * operation order, operand types and promotions, so that -- built with
+ * -ffp-contract=off like the reference wrapper in oracle/_ref -- its results are
+ * bitwise identical to the reference headers instantiated with the same types.
+ *
+ * Types: Tc = coordinate type (always double here), T = HydroType (double for the
+ * plain FP64 form, float for the reference's SphTypes layout), Tm = mass type.
+ */
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <limits>
+#include <numeric>
+#include <set>
+#include <vector>
+
+namespace orc
+{
+
+using LocalIndex    = uint32_t;
+using TreeNodeIndex = int;
+using Key           = uint64_t;
+
+constexpr unsigned kMaxLevel = 21; // 64-bit keys: 21 bits per dimension
+constexpr unsigned kUnused   = 1;  // leading unused bit in a 64-bit key
+constexpr int      kTableSize = 20000;
+
+/* ------------------------------------------------------------------------------------------------
+ * Box  (domain/include/cstone/sfc/box.hpp:95-174)
+ * ----------------------------------------------------------------------------------------------*/
+struct Box
+{
+    double lo[3], hi[3];
+    double len[3], ilen[3];
+    int    pbc[3]; // 1 = periodic
+
+    Box() = default;
+    Box(const double* limits6, const int* bc3)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            lo[d]   = limits6[2 * d];
+            hi[d]   = limits6[2 * d + 1];
+            len[d]  = hi[d] - lo[d];
+            ilen[d] = 1.0 / (hi[d] - lo[d]);
+            pbc[d]  = bc3[d] == 1;
+        }
+    }
+    bool anyPbc() const { return pbc[0] || pbc[1] || pbc[2]; }
+};
+
+/* ------------------------------------------------------------------------------------------------
+ * Bit helpers (domain/include/cstone/sfc/common.hpp:109-213, 252-255, 271-275)
+ * ----------------------------------------------------------------------------------------------*/
+inline int clz64(Key x) { return x == 0 ? 64 : __builtin_clzll(x); }
+
+inline Key nodeRange(unsigned level) { return Key(1) << (3u * (kMaxLevel - level)); }
+
+//! level of a node whose key range is a power of 8 (common.hpp:186-191)
+inline unsigned treeLevel(Key range) { return (clz64(range - 1) - kUnused) / 3; }
+
+//! common.hpp:170-173
+inline int commonPrefix(Key a, Key b) { return clz64(a ^ b) - int(kUnused); }
+
+//! placeholder-bit (Warren-Salmon) encoding, common.hpp:203-210
+inline Key encodePlaceholder(Key key, int prefixLength)
+{
+    int nShifts = 3 * int(kMaxLevel) - prefixLength;
+    return (Key(1) << prefixLength) | (key >> nShifts);
+}
+
+//! common.hpp:220-224
+inline unsigned decodePrefixLength(Key code) { return 8 * sizeof(Key) - 1 - clz64(code); }
+
+//! common.hpp:234-241
+inline Key decodePlaceholder(Key code)
+{
+    int prefixLength = decodePrefixLength(code);
+    Key stripped     = code ^ (Key(1) << prefixLength);
+    return stripped << (3 * kMaxLevel - prefixLength);
+}
+
+inline unsigned octalDigit(Key code, unsigned position) { return (code >> (3u * (kMaxLevel - position))) & 7u; }
+
+/* ------------------------------------------------------------------------------------------------
+ * Hilbert curve (domain/include/cstone/sfc/hilbert.hpp:44-93 encode, :130-172 decode)
+ *
+ * Per level, the Morton octant (x<<2|y<<1|z) is mapped to the Hilbert digit with the
+ * table {0,1,3,2,7,6,4,5}; the remaining lower bits of the coordinates are then
+ * reflected and permuted so that the next level is traversed in the child's frame.
+ * ----------------------------------------------------------------------------------------------*/
+inline Key hilbertEncode(unsigned px, unsigned py, unsigned pz)
+{
+    static const unsigned digitOf[8] = {0, 1, 3, 2, 7, 6, 4, 5};
+    Key key = 0;
+    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
+    {
+        unsigned bx = (px >> level) & 1u, by = (py >> level) & 1u, bz = (pz >> level) & 1u;
+        key = (key << 3) + digitOf[(bx << 2) | (by << 1) | bz];
+
+        // reflections, applied to all bits (bits above `level` are never read again)
+        unsigned rx = bx & ((!by) | bz);
+        unsigned ry = (bx & (by | bz)) | (by & (!bz));
+        unsigned rz = (bx & (!by) & (!bz)) | (by & (!bz));
+        px ^= 0u - rx;
+        py ^= 0u - ry;
+        pz ^= 0u - rz;
+
+        if (bz) // rotate (x,y,z) <- (y,z,x)
+        {
+            unsigned t = px;
+            px = py, py = pz, pz = t;
+        }
+        else if (!by) // exchange x and z
+        {
+            std::swap(px, pz);
+        }
+    }
+    return key;
+}
+
+inline void hilbertDecode(Key key, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (unsigned level = 0; level < kMaxLevel; ++level)
+    {
+        unsigned digit = (key >> (3 * level)) & 7u;
+        unsigned bx = digit >> 2u, by = (digit >> 1u) & 1u, bz = digit & 1u;
+
+        if (by ^ bz) // inverse rotation: (x,y,z) <- (z,x,y)
+        {
+            unsigned t = px;
+            px = pz, pz = py, py = t;
+        }
+        else if ((!bx & !by & !bz) || (bx & by & bz)) { std::swap(px, pz); }
+
+        unsigned mask = (1u << level) - 1;
+        px ^= mask & (0u - (bx & (by | bz)));
+        py ^= mask & (0u - ((bx & ((!by) | (!bz))) | ((!bx) & by & bz)));
+        pz ^= mask & (0u - ((bx & (!by) & (!bz)) | (by & bz)));
+
+        px |= (bx << level);
+        py |= ((bx ^ by) << level);
+        pz |= ((by ^ bz) << level);
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+//! sfc3D, domain/include/cstone/sfc/sfc.hpp:142-178 (cube length 2^21, clamp to 2^21-1)
+inline Key sfcKey(double x, double y, double z, const Box& box)
+{
+    constexpr int mcoord = (1 << kMaxLevel) - 1;
+    const unsigned cube  = 1u << kMaxLevel;
+    double mx = cube * box.ilen[0], my = cube * box.ilen[1], mz = cube * box.ilen[2];
+
+    int ix = std::floor(x * mx) - box.lo[0] * mx;
+    int iy = std::floor(y * my) - box.lo[1] * my;
+    int iz = std::floor(z * mz) - box.lo[2] * mz;
+    ix = std::min(ix, mcoord);
+    iy = std::min(iy, mcoord);
+    iz = std::min(iz, mcoord);
+    return hilbertEncode(ix, iy, iz);
+}
+
+constexpr Key kRemoveKey = Key(1) << 63; // definitions.h removeKey
+
+//! computeSfcKeys, sfc/sfc.hpp:268-275 (keys equal to removeKey are left untouched)
+inline void computeSfcKeys(const double* x, const double* y, const double* z, Key* keys, size_t n, const Box& box)
+{
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (keys[i] != kRemoveKey) { keys[i] = sfcKey(x[i], y[i], z[i], box); }
+    }
+}
+
+//! sort_by_key with std::stable_sort, primitives/gather.hpp:43-68
+inline void stableSortByKey(Key* keys, LocalIndex* values, size_t n)
+{
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(), [keys](size_t a, size_t b) { return keys[a] < keys[b]; });
+    std::vector<Key>        k2(n);
+    std::vector<LocalIndex> v2(n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        k2[i] = keys[order[i]];
+        v2[i] = values[order[i]];
+    }
+    std::copy(k2.begin(), k2.end(), keys);
+    std::copy(v2.begin(), v2.end(), values);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Cornerstone leaf array (domain/include/cstone/tree/csarray.hpp:182-440)
+ * ----------------------------------------------------------------------------------------------*/
+inline unsigned nodeCount(Key a, Key b, const Key* keys, size_t n)
+{
+    auto lo = std::lower_bound(keys, keys + n, a);
+    auto hi = std::lower_bound(keys, keys + n, b);
+    return unsigned(hi - lo);
+}
+
+//! calculateNodeOp, csarray.hpp:247-290: 0 merge, 1 keep, 8/64/512/4096 split
+inline int nodeOp(const std::vector<Key>& tree, TreeNodeIndex i, const std::vector<unsigned>& counts, unsigned bucket)
+{
+    Key      node  = tree[i];
+    unsigned level = treeLevel(tree[i + 1] - node);
+    int      sib   = -1;
+    if (level > 0)
+    {
+        sib = octalDigit(node, level);
+        bool complete = tree[i - sib + 8] == tree[i - sib] + nodeRange(level - 1);
+        if (!complete) { sib = -1; }
+    }
+    if (sib > 0)
+    {
+        size_t parentCount = 0;
+        for (int s = 0; s < 8; ++s)
+            parentCount += counts[i - sib + s];
+        if (parentCount <= size_t(bucket)) { return 0; }
+    }
+    size_t c = counts[i];
+    if (c > size_t(bucket) * 512 && level + 3 < kMaxLevel) { return 4096; }
+    if (c > size_t(bucket) * 64 && level + 2 < kMaxLevel) { return 512; }
+    if (c > size_t(bucket) * 8 && level + 1 < kMaxLevel) { return 64; }
+    if (c > size_t(bucket) && level < kMaxLevel) { return 8; }
+    return 1;
+}
+
+//! computeOctree (csarray.hpp:409-440): iterate count / rebalance until no node changes
+inline void computeOctree(const Key* keys, size_t n, unsigned bucket, std::vector<Key>& tree, std::vector<unsigned>& counts)
+{
+    tree   = {0, nodeRange(0)};
+    counts = {unsigned(n)};
+    while (true)
+    {
+        TreeNodeIndex         numNodes = TreeNodeIndex(tree.size()) - 1;
+        std::vector<int>      ops(numNodes + 1, 0);
+        bool                  converged = true;
+        for (TreeNodeIndex i = 0; i < numNodes; ++i)
+        {
+            ops[i] = nodeOp(tree, i, counts, bucket);
+            if (ops[i] != 1) { converged = false; }
+        }
+        // rebalanceTree / processNode, csarray.hpp:340-389
+        std::exclusive_scan(ops.begin(), ops.end(), ops.begin(), 0);
+        std::vector<Key> next(ops[numNodes] + 1);
+        for (TreeNodeIndex i = 0; i < numNodes; ++i)
+        {
+            Key      node  = tree[i];
+            unsigned level = treeLevel(tree[i + 1] - node);
+            int      op    = ops[i + 1] - ops[i];
+            if (op == 1) { next[ops[i]] = node; }
+            else if (op >= 8)
+            {
+                unsigned down = (op == 8) ? 1 : (op == 64) ? 2 : (op == 512) ? 3 : 4;
+                for (int s = 0; s < op; ++s)
+                    next[ops[i] + s] = node + Key(s) * nodeRange(level + down);
+            }
+        }
+        next.back() = tree.back();
+        tree.swap(next);
+        counts.resize(tree.size() - 1);
+        for (size_t i = 0; i + 1 < tree.size(); ++i)
+            counts[i] = nodeCount(tree[i], tree[i + 1], keys, n);
+        if (converged) { break; }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Linked octree (domain/include/cstone/tree/octree.hpp:79-197)
+ *
+ * Same output as buildOctreeCpu: all nodes (internal + leaves) in placeholder-bit
+ * prefix order (level-major, SFC-minor), childOffsets (0 = leaf), parents per sibling
+ * group, levelRange[kMaxLevel+2] and internalToLeaf (= cornerstone leaf index for
+ * leaves). Internal nodes are found here as the set of proper ancestors of all
+ * leaves instead of the reference's binary-radix-tree index mapping.
+ * ----------------------------------------------------------------------------------------------*/
+struct Octree
+{
+    TreeNodeIndex              numLeafNodes{0}, numInternalNodes{0}, numNodes{0};
+    std::vector<Key>           prefixes;
+    std::vector<TreeNodeIndex> childOffsets, parents, levelRange, internalToLeaf;
+};
+
+inline void buildOctree(const std::vector<Key>& leaves, Octree& o)
+{
+    TreeNodeIndex L = TreeNodeIndex(leaves.size()) - 1;
+    o.numLeafNodes     = L;
+    o.numInternalNodes = (L - 1) / 7;
+    o.numNodes         = L + o.numInternalNodes;
+
+    std::vector<std::pair<Key, TreeNodeIndex>> nodes; // (prefix, leaf index or -1)
+    std::set<Key>                              internal;
+    for (TreeNodeIndex t = 0; t < L; ++t)
+    {
+        Key      k     = leaves[t];
+        unsigned level = treeLevel(leaves[t + 1] - k);
+        nodes.push_back({encodePlaceholder(k, 3 * level), t});
+        for (int l = int(level) - 1; l >= 0; --l)
+        {
+            Key anc = encodePlaceholder(k & ~(nodeRange(l) - 1), 3 * l);
+            if (!internal.insert(anc).second) { break; }
+        }
+    }
+    for (Key p : internal)
+        nodes.push_back({p, -1});
+    std::sort(nodes.begin(), nodes.end());
+
+    TreeNodeIndex N = TreeNodeIndex(nodes.size());
+    o.prefixes.resize(N);
+    o.internalToLeaf.resize(N);
+    for (TreeNodeIndex i = 0; i < N; ++i)
+    {
+        o.prefixes[i]       = nodes[i].first;
+        o.internalToLeaf[i] = nodes[i].second; // internal nodes: negative
+    }
+    o.levelRange.resize(kMaxLevel + 2);
+    for (unsigned l = 0; l <= kMaxLevel; ++l)
+        o.levelRange[l] =
+            TreeNodeIndex(std::lower_bound(o.prefixes.begin(), o.prefixes.end(), encodePlaceholder(0, 3 * l)) -
+                          o.prefixes.begin());
+    o.levelRange[kMaxLevel + 1] = N;
+
+    o.childOffsets.assign(N + 1, 0);
+    o.parents.assign(std::max(1, (N - 1) / 8), 0);
+    for (TreeNodeIndex i = 0; i < N; ++i)
+    {
+        if (o.internalToLeaf[i] >= 0) { continue; }
+        Key      prefix = o.prefixes[i];
+        unsigned plen   = decodePrefixLength(prefix);
+        unsigned level  = plen / 3;
+        Key      child  = encodePlaceholder(decodePlaceholder(prefix), plen + 3);
+        auto     b      = o.prefixes.begin() + o.levelRange[level + 1];
+        auto     e      = o.prefixes.begin() + o.levelRange[level + 2];
+        auto     it     = std::lower_bound(b, e, child);
+        if (it != e && *it == child)
+        {
+            TreeNodeIndex c   = TreeNodeIndex(it - o.prefixes.begin());
+            o.childOffsets[i] = c;
+            o.parents[(c - 1) / 8] = i;
+        }
+    }
+}
+
+//! nodeFpCenters, focus/source_center.hpp:130-141 with hilbertIBox (hilbert.hpp:259-274) and
+//! centerAndSize (sfc/box.hpp:318-334)
+inline void nodeCenters(const std::vector<Key>& prefixes, const Box& box, std::vector<double>& centers,
+                        std::vector<double>& sizes)
+{
+    size_t n = prefixes.size();
+    centers.resize(3 * n);
+    sizes.resize(3 * n);
+    const double uL = 1.0 / double(1 << kMaxLevel);
+    for (size_t i = 0; i < n; ++i)
+    {
+        Key      start = decodePlaceholder(prefixes[i]);
+        unsigned level = decodePrefixLength(prefixes[i]) / 3;
+        unsigned cube  = (1u << kMaxLevel) >> level;
+        unsigned mask  = ~(cube - 1);
+        unsigned ic[3];
+        hilbertDecode(start, ic[0], ic[1], ic[2]);
+        for (int d = 0; d < 3; ++d)
+        {
+            int    imin = int(ic[d] & mask), imax = imin + int(cube);
+            double half = 0.5 * uL * box.len[d];
+            centers[3 * i + d] = box.lo[d] + (imax + imin) * half;
+            sizes[3 * i + d]   = (imax - imin) * half;
+        }
+    }
+}
+
+//! OctreeNsView, tree/octree.hpp:280-300 (+ layout from leaf counts, focus centres)
+struct NsTree
+{
+    std::vector<Key>        leaves;
+    std::vector<unsigned>   counts;
+    std::vector<LocalIndex> layout;
+    Octree                  oct;
+    std::vector<double>     centers, sizes;
+};
+
+inline void buildNsTree(const Key* sortedKeys, size_t n, unsigned bucket, const Box& box, NsTree& t)
+{
+    computeOctree(sortedKeys, n, bucket, t.leaves, t.counts);
+    buildOctree(t.leaves, t.oct);
+    t.layout.assign(t.counts.size() + 1, 0);
+    std::inclusive_scan(t.counts.begin(), t.counts.end(), t.layout.begin() + 1);
+    nodeCenters(t.oct.prefixes, box, t.centers, t.sizes);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Neighbour search (domain/include/cstone/findneighbors.hpp:33-170,
+ * traversal/traversal.hpp:52-93, traversal/boxoverlap.hpp:184-215)
+ * ----------------------------------------------------------------------------------------------*/
+inline double pbcFold(double d, const Box& box, int dim)
+{
+    return d - double(box.pbc[dim]) * box.len[dim] * std::rint(d * box.ilen[dim]);
+}
+
+//! distanceSq<Pbc>(x1.., x2.., box): (x1-x2) folded, then dx*dx + dy*dy + dz*dz
+template<bool Pbc>
+inline double distSq(double x1, double y1, double z1, double x2, double y2, double z2, const Box& box)
+{
+    double dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    if (Pbc)
+    {
+        dx = pbcFold(dx, box, 0);
+        dy = pbcFold(dy, box, 1);
+        dz = pbcFold(dz, box, 2);
+    }
+    return dx * dx + dy * dy + dz * dz;
+}
+
+//! norm2(minDistance(X, center, size[, box])): right-fold dot product (util/array.hpp:237-240)
+template<bool Pbc>
+inline double cellDistSq(const double* X, const double* c, const double* s, const Box& box)
+{
+    double d[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        double v = c[k] - X[k];
+        if (Pbc) { v = pbcFold(v, box, k); }
+        v = std::abs(v) - s[k];
+        v += std::abs(v);
+        d[k] = v * 0.5;
+    }
+    return d[0] * d[0] + (d[1] * d[1] + d[2] * d[2]);
+}
+
+template<bool Pbc, class Th>
+inline unsigned searchOne(LocalIndex i, const double* x, const double* y, const double* z, Th hi, const NsTree& t,
+                          const Box& box, unsigned ngmax, LocalIndex* out)
+{
+    const double X[3]         = {x[i], y[i], z[i]};
+    auto         radiusSq     = Th(4.0) * hi * hi;
+    auto         cellRadiusSq = radiusSq * 1.0f * 1.0f; // searchExtFactor = 1 (octree.hpp:298)
+    unsigned     count        = 0;
+
+    auto overlaps = [&](TreeNodeIndex idx)
+    { return cellDistSq<Pbc>(X, &t.centers[3 * idx], &t.sizes[3 * idx], box) < cellRadiusSq; };
+
+    auto scanLeaf = [&](TreeNodeIndex idx)
+    {
+        TreeNodeIndex leaf = t.oct.internalToLeaf[idx];
+        for (LocalIndex j = t.layout[leaf]; j < t.layout[leaf + 1]; ++j)
+        {
+            if (j == i) { continue; }
+            if (distSq<Pbc>(x[j], y[j], z[j], X[0], X[1], X[2], box) < radiusSq)
+            {
+                if (count < ngmax) { out[count] = j; }
+                count++;
+            }
+        }
+    };
+
+    // depth-first walk: children 0..7 in order, leaves handled immediately, internal
+    // children pushed on a LIFO stack (traversal.hpp:52-93)
+    const TreeNodeIndex* child = t.oct.childOffsets.data();
+    if (!overlaps(0)) { return 0; }
+    if (child[0] == 0)
+    {
+        scanLeaf(0);
+        return count;
+    }
+    TreeNodeIndex stack[128];
+    stack[0]          = 0;
+    TreeNodeIndex top = 1, node = 0;
+    do
+    {
+        for (int oct = 0; oct < 8; ++oct)
+        {
+            TreeNodeIndex c = child[node] + oct;
+            if (overlaps(c))
+            {
+                if (child[c] == 0) { scanLeaf(c); }
+                else { stack[top++] = c; }
+            }
+        }
+        node = stack[--top];
+    } while (node != 0);
+    return count;
+}
+
+//! cstone::findNeighbors(i, ...), findneighbors.hpp:78-147 (count excludes self)
+template<class Th>
+inline unsigned findNeighbors(LocalIndex i, const double* x, const double* y, const double* z, const Th* h,
+                              const NsTree& t, const Box& box, unsigned ngmax, LocalIndex* out)
+{
+    Th     hi   = h[i];
+    double ext  = double(2) * hi;
+    bool   in   = true;
+    double X[3] = {x[i], y[i], z[i]};
+    for (int d = 0; d < 3; ++d)
+        in = in && (X[d] - ext >= box.lo[d]) && (X[d] + ext <= box.hi[d]);
+    bool usePbc = box.anyPbc() && !in;
+    return usePbc ? searchOne<true>(i, x, y, z, hi, t, box, ngmax, out)
+                  : searchOne<false>(i, x, y, z, hi, t, box, ngmax, out);
+}
+
+//! updateH, sph/include/sph/kernels.hpp:27-32
+template<class T>
+inline T updateH(unsigned ng0, unsigned nc, T h)
+{
+    constexpr T c0 = 1023.0;
+    constexpr T ex = 1.0 / 10.0;
+    return h * T(0.5) * std::pow(T(1) + c0 * ng0 / T(nc), ex);
+}
+
+//! findNeighborsSph, sph/include/sph/find_neighbors.hpp:11-44. nc includes self.
+//! Returns the number of particles whose h/nc iteration did not converge.
+template<class Th>
+inline size_t findNeighborsSph(const double* x, const double* y, const double* z, Th* h, LocalIndex first,
+                               LocalIndex last, const Box& box, const NsTree& t, unsigned ng0, unsigned ngmax,
+                               LocalIndex* neighbors, unsigned* nc, bool iterateH)
+{
+    unsigned ngmin    = ng0 / 4;
+    size_t   numFails = 0;
+#pragma omp parallel for reduction(+ : numFails) schedule(dynamic, 256)
+    for (LocalIndex k = 0; k < last - first; ++k)
+    {
+        LocalIndex id = k + first;
+        unsigned   n  = 1 + findNeighbors(id, x, y, z, h, t, box, ngmax, neighbors + size_t(k) * ngmax);
+        int        it = 0;
+        while (iterateH && (ngmin > n || (n - 1) > ngmax) && it++ < 10)
+        {
+            h[id] = updateH(ng0, n, h[id]);
+            n     = 1 + findNeighbors(id, x, y, z, h, t, box, ngmax, neighbors + size_t(k) * ngmax);
+        }
+        numFails += (it >= 10);
+        nc[k] = n;
+    }
+    return numFails;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * SPH kernel tables (sph/include/sph/kernels.hpp:36-61, sph_kernel_tables.hpp:31-101,
+ * table_lookup.hpp:10-26, particles_data.hpp:380-387)
+ * ----------------------------------------------------------------------------------------------*/
+inline double sincHalfPi(double v)
+{
+    if (v == 0.0) { return 1.0; }
+    const double Pv = M_PI_2 * v;
+    return std::sin(Pv) / Pv;
+}
+
+inline double sincHalfPiDerivative(double v)
+{
+    if (v == 0.0) return 0.0;
+    constexpr double piHalf = M_PI_2;
+    const double     Pv     = piHalf * v;
+    const double     s      = std::sin(Pv) / (Pv);
+    return s * piHalf * ((std::cos(Pv) / std::sin(Pv)) - 1.0 / Pv);
+}
+
+inline double sincKernel(double v, double n) { return std::pow(sincHalfPi(v), n); }
+inline double sincKernelDerivative(double v, double n)
+{
+    return n * std::pow(sincHalfPi(v), n - 1) * sincHalfPiDerivative(v);
+}
+
+//! composite Simpson rule with sorted partial sums, sph_kernel_tables.hpp:31-60
+template<class F>
+inline double simpson(double a, double b, uint64_t n, F&& f)
+{
+    uint64_t            numOdd  = n / 2;
+    uint64_t            numEven = (numOdd >= 1) ? numOdd - 1 : 0;
+    double              h       = (b - a) / double(n);
+    std::vector<double> odd(numOdd), even(numEven);
+    for (uint64_t i = 0; i < numOdd; ++i)
+        odd[i] = f(a + double(2 * (i + 1) - 1) * h);
+    for (uint64_t i = 0; i < numEven; ++i)
+        even[i] = f(a + double(2 * (i + 1)) * h);
+    std::sort(odd.begin(), odd.end());
+    std::sort(even.begin(), even.end());
+    return h / 3.0 *
+           (f(a) + f(b) + 4.0 * std::accumulate(odd.begin(), odd.end(), 0.0) +
+            2.0 * std::accumulate(even.begin(), even.end(), 0.0));
+}
+
+//! K = 1 / int_0^2 4 pi v^2 W(v) dv with 2000 Simpson intervals (sph_kernel_tables.hpp:78-83)
+inline double kernelNormalization(double sincIndex)
+{
+    auto vol = [sincIndex](double v) { return 4.0 * M_PI * v * v * sincKernel(v, sincIndex); };
+    return 1.0 / simpson(0, 2.0, 2000, vol);
+}
+
+//! SPHYNX interpolation constant used by the reference's known-answer tests (sph_kernel_tables.hpp:65-76)
+inline double sphynxK(double n)
+{
+    double b0 = 2.7012593e-2, b1 = 2.0410827e-2, b2 = 3.7451957e-3, b3 = 4.7013839e-2;
+    return b0 + b1 * std::sqrt(n) + b2 * n + b3 * std::sqrt(n * n * n);
+}
+
+//! tabulateFunction<T, 20000>(f, 0, 2): abscissa computed in T (sph_kernel_tables.hpp:88-101)
+template<class T, class F>
+inline void tabulate(F&& f, T* table)
+{
+    const T dx = (2.0 - 0.0) / (kTableSize - 1);
+    for (size_t i = 0; i < size_t(kTableSize); ++i)
+    {
+        T v      = 0.0 + i * dx;
+        table[i] = f(v);
+    }
+}
+
+//! lt::lookup, table_lookup.hpp:10-26
+template<class T>
+inline T lookup(const T* table, T v)
+{
+    constexpr int numIntervals = kTableSize - 1;
+    constexpr T   support      = 2.0;
+    constexpr T   dx           = support / numIntervals;
+    constexpr T   invDx        = T(1) / dx;
+
+    int idx        = v * invDx;
+    T   derivative = (idx >= numIntervals) ? 0.0 : (table[idx + 1] - table[idx]) * invDx;
+    return (idx >= numIntervals) ? 0.0 : table[idx] + derivative * (v - T(idx) * dx);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Periodic images and kernels helpers (sfc/box.hpp:218-250, sph/include/sph/kernels.hpp:11-99)
+ * ----------------------------------------------------------------------------------------------*/
+//! legacy minimum image: shift by one box length if |d| > r (box.hpp:218-238)
+template<class T>
+inline void foldLegacy(const Box& box, T r, T& xx, T& yy, T& zz)
+{
+    if (box.pbc[0] && xx > r) xx -= box.len[0];
+    else if (box.pbc[0] && xx < -r) xx += box.len[0];
+    if (box.pbc[1] && yy > r) yy -= box.len[1];
+    else if (box.pbc[1] && yy < -r) yy += box.len[1];
+    if (box.pbc[2] && zz > r) zz -= box.len[2];
+    else if (box.pbc[2] && zz < -r) zz += box.len[2];
+}
+
+//! distancePBC, box.hpp:241-250: pair distance in T after a legacy fold with r = 2h
+template<class T>
+inline T pairDistance(const Box& box, T hi, double x1, double y1, double z1, double x2, double y2, double z2)
+{
+    T xx = x1 - x2, yy = y1 - y2, zz = z1 - z2;
+    foldLegacy(box, T(2) * hi, xx, yy, zz);
+    return std::sqrt(xx * xx + yy * yy + zz * zz);
+}
+
+//! tsKCourant, kernels.hpp:11-16
+template<class T1, class T2, class T3>
+inline auto courantDt(T1 maxvsignal, T2 h, T3 c, float Kcour)
+{
+    using T = std::common_type_t<T1, T2, T3>;
+    T v     = maxvsignal > T(0) ? maxvsignal : c;
+    return T(Kcour * h / v);
+}
+
+//! artificial_viscosity, kernels.hpp:71-84 (beta = 2)
+template<class T>
+inline T avisc(T alpha_i, T alpha_j, T c_i, T c_j, T w_ij)
+{
+    constexpr T beta = 2.0;
+    T           visc = 0.0;
+    if (w_ij < 0.0)
+    {
+        T vsig = (alpha_i + alpha_j) / 4.0 * (c_i + c_j) - beta * w_ij;
+        visc   = -vsig * w_ij;
+    }
+    return visc;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Per-target pair loops ("JLoops") with neighbour lists
+ * ----------------------------------------------------------------------------------------------*/
+
+//! xmassJLoop, sph/include/sph/hydro_ve/xmass_kern.hpp:51-79
+template<class T, class Tm>
+inline T xmassLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                   const double* y, const double* z, const T* h, const Tm* m, const T* wh)
+{
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto hi = h[i];
+    auto mi = m[i];
+    T    hInv  = 1.0 / hi;
+    T    h3Inv = hInv * hInv * hInv;
+    T    rho0i = mi;
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j    = nb[k];
+        T          dist = pairDistance(box, hi, xi, yi, zi, x[j], y[j], z[j]);
+        T          vloc = dist * hInv;
+        T          w    = lookup(wh, vloc);
+        rho0i += w * m[j];
+    }
+    auto rhoZero = rho0i * K * h3Inv; // double (K is the coordinate type)
+    T    xmassi  = mi / rhoZero;      // veDefinition
+    return xmassi;
+}
+
+//! IADJLoopSTD (hydro_std/iad_kern.hpp:11-76) and IADJLoop (hydro_ve/iad_kern.hpp:44-109):
+//! tau accumulation with volume vol(j), 2^-mean(ilogb) scaling, adjugate inverse.
+template<class T, class Vol>
+inline void iadLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                    const double* y, const double* z, const T* h, const T* wh, Vol&& vol, T* c11, T* c12, T* c13,
+                    T* c22, T* c23, T* c33)
+{
+    T    tau11 = 0.0, tau12 = 0.0, tau13 = 0.0, tau22 = 0.0, tau23 = 0.0, tau33 = 0.0;
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto hi    = h[i];
+    auto hiInv = T(1) / hi;
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j  = nb[k];
+        T          rx = (xi - x[j]), ry = (yi - y[j]), rz = (zi - z[j]);
+        foldLegacy(box, T(2) * hi, rx, ry, rz);
+        T dist  = std::sqrt(rx * rx + ry * ry + rz * rz);
+        T vloc  = dist * hiInv;
+        T w     = lookup(wh, vloc);
+        T vol_w = vol(j) * w;
+        tau11 += rx * rx * vol_w;
+        tau12 += rx * ry * vol_w;
+        tau13 += rx * rz * vol_w;
+        tau22 += ry * ry * vol_w;
+        tau23 += ry * rz * vol_w;
+        tau33 += rz * rz * vol_w;
+    }
+    auto getExp = [](T val) { return (val == T(0) ? 0 : std::ilogb(val)); };
+    int  expSum = getExp(tau11) + getExp(tau12) + getExp(tau13) + getExp(tau22) + getExp(tau23) + getExp(tau33);
+    T    norm   = std::ldexp(T(1), -expSum / 6);
+    tau11 *= norm;
+    tau12 *= norm;
+    tau13 *= norm;
+    tau22 *= norm;
+    tau23 *= norm;
+    tau33 *= norm;
+    T det = tau11 * tau22 * tau33 + T(2) * tau12 * tau23 * tau13 - tau11 * tau23 * tau23 - tau22 * tau13 * tau13 -
+            tau33 * tau12 * tau12;
+    T factor = norm * (hi * hi * hi) / (det * K);
+    c11[i]   = (tau22 * tau33 - tau23 * tau23) * factor;
+    c12[i]   = (tau13 * tau23 - tau33 * tau12) * factor;
+    c13[i]   = (tau12 * tau23 - tau22 * tau13) * factor;
+    c22[i]   = (tau11 * tau33 - tau13 * tau13) * factor;
+    c23[i]   = (tau13 * tau12 - tau11 * tau23) * factor;
+    c33[i]   = (tau11 * tau22 - tau12 * tau12) * factor;
+}
+
+//! momentumAndEnergyJLoop (std), hydro_std/momentum_energy_kern.hpp:11-134
+template<class T, class Tm>
+inline void momentumEnergyStdLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn,
+                                  const double* x, const double* y, const double* z, const T* vx, const T* vy,
+                                  const T* vz, const T* h, const Tm* m, const T* rho, const T* p, const T* c,
+                                  const T* c11, const T* c12, const T* c13, const T* c22, const T* c23, const T* c33,
+                                  const T* wh, T* ax, T* ay, T* az, double* du, T* maxvsignal)
+{
+    constexpr T gradh_i = 1.0, gradh_j = 1.0;
+    auto        xi = x[i], yi = y[i], zi = z[i];
+    auto        vxi = vx[i], vyi = vy[i], vzi = vz[i];
+    auto        hi = h[i], roi = rho[i], pri = p[i], ci = c[i];
+    auto        mi_roi = m[i] / rho[i];
+    T           hiInv  = T(1) / hi;
+    T           hiInv3 = hiInv * hiInv * hiInv;
+    T           maxvs  = 0.0;
+    T           momx = 0.0, momy = 0.0, momz = 0.0, energy = 0.0;
+    auto        c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
+
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j  = nb[k];
+        T          rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        foldLegacy(box, T(2) * hi, rx, ry, rz);
+        T r2    = rx * rx + ry * ry + rz * rz;
+        T dist  = std::sqrt(r2);
+        T vx_ij = vxi - vx[j], vy_ij = vyi - vy[j], vz_ij = vzi - vz[j];
+        T hj    = h[j];
+        T hjInv = T(1) / hj;
+        T v1    = dist * hiInv;
+        T v2    = dist * hjInv;
+        T rv    = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+        T hjInv3 = hjInv * hjInv * hjInv;
+        T Wi     = hiInv3 * lookup(wh, v1);
+        T Wj     = hjInv3 * lookup(wh, v2);
+
+        T termA1_i = c11i * rx + c12i * ry + c13i * rz;
+        T termA2_i = c12i * rx + c22i * ry + c23i * rz;
+        T termA3_i = c13i * rx + c23i * ry + c33i * rz;
+        auto c11j = c11[j], c12j = c12[j], c13j = c13[j], c22j = c22[j], c23j = c23[j], c33j = c33[j];
+        T termA1_j = c11j * rx + c12j * ry + c13j * rz;
+        T termA2_j = c12j * rx + c22j * ry + c23j * rz;
+        T termA3_j = c13j * rx + c23j * ry + c33j * rz;
+
+        auto        roj   = rho[j];
+        auto        cj    = c[j];
+        T           wij   = rv / dist;
+        constexpr T alpha = T(1);
+        T           visc  = T(0.5) * avisc(alpha, alpha, ci, cj, wij);
+        T           vsig  = ci + cj - T(3) * wij;
+        maxvs             = (vsig > maxvs) ? vsig : maxvs;
+
+        auto mj        = m[j];
+        auto mj_roj_Wj = mj / roj * Wj;
+        T    mj_pro_i  = mj * pri / (gradh_i * roi * roi);
+        {
+            T a = Wi * (mj_pro_i + visc * mi_roi);
+            T b = mj_roj_Wj * (p[j] / (roj * gradh_j) + visc);
+            momx += a * termA1_i + b * termA1_j;
+            momy += a * termA2_i + b * termA2_j;
+            momz += a * termA3_i + b * termA3_j;
+        }
+        {
+            T a = Wi * (T(2) * mj_pro_i + visc * mi_roi);
+            T b = visc * mj_roj_Wj;
+            energy += vx_ij * (a * termA1_i + b * termA1_j) + vy_ij * (a * termA2_i + b * termA2_j) +
+                      vz_ij * (a * termA3_i + b * termA3_j);
+        }
+    }
+    du[i]       = -K * double(0.5) * energy;
+    ax[i]       = K * momx;
+    ay[i]       = K * momy;
+    az[i]       = K * momz;
+    *maxvsignal = maxvs;
+}
+
+//! veDefGradhJLoop, hydro_ve/ve_def_gradh_kern.hpp:44-90 -> (kx, gradh)
+template<class T, class Tm>
+inline void veDefGradhLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                           const double* y, const double* z, const T* h, const Tm* m, const T* wh, const T* whd,
+                           const T* xm, T& kxOut, T& gradhOut)
+{
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto hi = h[i];
+    auto mi = m[i];
+    auto xmassi = xm[i];
+    auto hInv   = T(1) / hi;
+    auto h3Inv  = hInv * hInv * hInv;
+    auto kxi      = xmassi;
+    auto whomegai = -T(3) * xmassi;
+    auto wrho0i   = -T(3) * mi;
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j     = nb[k];
+        T          dist  = pairDistance(box, hi, xi, yi, zi, x[j], y[j], z[j]);
+        T          vloc  = dist * hInv;
+        T          w     = lookup(wh, vloc);
+        T          dw    = lookup(whd, vloc);
+        T          dterh = -(T(3) * w + vloc * dw);
+        T          xmj   = xm[j];
+        kxi += w * xmj;
+        whomegai += dterh * xmj;
+        wrho0i += dterh * m[j];
+    }
+    kxi *= K * h3Inv;
+    whomegai *= K * h3Inv * hInv;
+    wrho0i *= K * h3Inv * hInv;
+    whomegai = whomegai * mi / xmassi + (kxi - K * xmassi * h3Inv) * wrho0i;
+    T rhoi   = kxi * mi / xmassi;
+    T dhdrho = -hi / (rhoi * T(3));
+    gradhOut = T(1) - dhdrho * whomegai;
+    kxOut    = kxi;
+}
+
+//! divV_curlVJLoop, hydro_ve/divv_curlv_kern.hpp:44-123
+template<class T>
+inline void divvCurlvLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                          const double* y, const double* z, const T* vx, const T* vy, const T* vz, const T* h,
+                          const T* c11, const T* c12, const T* c13, const T* c22, const T* c23, const T* c33,
+                          const T* wh, const T* kx, const T* xm, T* divv, T* curlv, T* dV11, T* dV12, T* dV13, T* dV22,
+                          T* dV23, T* dV33, bool doGradV)
+{
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto vxi = vx[i], vyi = vy[i], vzi = vz[i];
+    auto hi = h[i], kxi = kx[i];
+    auto hiInv  = T(1) / hi;
+    auto hiInv3 = hiInv * hiInv * hiInv;
+    T    gX[3] = {0., 0., 0.}, gY[3] = {0., 0., 0.}, gZ[3] = {0., 0., 0.};
+    auto c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
+
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j  = nb[k];
+        T          rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        foldLegacy(box, T(2) * hi, rx, ry, rz);
+        T r2    = rx * rx + ry * ry + rz * rz;
+        T dist  = std::sqrt(r2);
+        T vx_ji = vx[j] - vxi, vy_ji = vy[j] - vyi, vz_ji = vz[j] - vzi;
+        T v1    = dist * hiInv;
+        T Wi    = lookup(wh, v1);
+        T tA[3];
+        tA[0]   = -(c11i * rx + c12i * ry + c13i * rz) * Wi;
+        tA[1]   = -(c12i * rx + c22i * ry + c23i * rz) * Wi;
+        tA[2]   = -(c13i * rx + c23i * ry + c33i * rz) * Wi;
+        T xmj   = xm[j];
+        T sx = vx_ji * xmj, sy = vy_ji * xmj, sz = vz_ji * xmj;
+        for (int d = 0; d < 3; ++d)
+        {
+            gX[d] += sx * tA[d];
+            gY[d] += sy * tA[d];
+            gZ[d] += sz * tA[d];
+        }
+    }
+    T norm_kxi = K * hiInv3 / kxi;
+    divv[i]    = norm_kxi * (gX[0] + gY[1] + gZ[2]);
+    if (curlv != nullptr)
+    {
+        T cv[3] = {gZ[1] - gY[2], gX[2] - gZ[0], gY[0] - gX[1]};
+        curlv[i] = norm_kxi * std::sqrt(cv[0] * cv[0] + (cv[1] * cv[1] + cv[2] * cv[2]));
+    }
+    if (doGradV)
+    {
+        dV11[i] = norm_kxi * gX[0];
+        dV12[i] = norm_kxi * (gX[1] + gY[0]);
+        dV13[i] = norm_kxi * (gX[2] + gZ[0]);
+        dV22[i] = norm_kxi * gY[1];
+        dV23[i] = norm_kxi * (gY[2] + gZ[1]);
+        dV33[i] = norm_kxi * gZ[2];
+    }
+}
+
+//! AVswitchesJLoop, hydro_ve/av_switches_kern.hpp:44-137 -> new alpha_i
+template<class T>
+inline T avSwitchesLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                        const double* y, const double* z, const T* vx, const T* vy, const T* vz, const T* h,
+                        const T* c, const T* c11, const T* c12, const T* c13, const T* c22, const T* c23,
+                        const T* c33, const T* wh, const T* kx, const T* xm, const T* divv, const double dt,
+                        const T alphamin, const T alphamax, const T decay_constant, T alpha_i)
+{
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto vxi = vx[i], vyi = vy[i], vzi = vz[i];
+    auto hi = h[i], ci = c[i];
+    auto c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
+    T    vsig_i = T(1.e-40) * ci;
+    auto hiInv  = T(1) / hi;
+    auto hiInv3 = hiInv * hiInv * hiInv;
+    auto divv_i = divv[i];
+    T    gdx = 0.0, gdy = 0.0, gdz = 0.0;
+
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j  = nb[k];
+        T          rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        foldLegacy(box, T(2) * hi, rx, ry, rz);
+        T r2    = rx * rx + ry * ry + rz * rz;
+        T dist  = std::sqrt(r2);
+        T vx_ij = vxi - vx[j], vy_ij = vyi - vy[j], vz_ij = vzi - vz[j];
+        T rv    = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+        T vs_ij = 0.0;
+        if (rv < T(0)) { vs_ij = ci + c[j] - T(3) * rv / dist; }
+        vsig_i = std::max(vsig_i, vs_ij);
+        T v1   = dist * hiInv;
+        T Wi   = K * hiInv3 * lookup(wh, v1);
+        T tA1  = -(c11i * rx + c12i * ry + c13i * rz) * Wi;
+        T tA2  = -(c12i * rx + c22i * ry + c23i * rz) * Wi;
+        T tA3  = -(c13i * rx + c23i * ry + c33i * rz) * Wi;
+        T volj = xm[j] / kx[j];
+        T fac  = volj * (divv_i - divv[j]);
+        gdx += fac * tA1;
+        gdy += fac * tA2;
+        gdz += fac * tA3;
+    }
+    T graddivv = std::sqrt(gdx * gdx + gdy * gdy + gdz * gdz);
+    T alphaloc = 0.0;
+    if (divv_i < T(0))
+    {
+        T a_const = hi * hi * graddivv;
+        alphaloc  = alphamax * a_const / (a_const + hi * std::abs(divv_i) + T(0.05) * ci);
+    }
+    if (alphaloc >= alpha_i) { alpha_i = alphaloc; }
+    else
+    {
+        T decay    = hi / (decay_constant * vsig_i);
+        T alphadot = 0.0;
+        if (alphaloc >= alphamin) { alphadot = (alphaloc - alpha_i) / decay; }
+        else { alphadot = (alphamin - alpha_i) / decay; }
+        alpha_i += alphadot * dt;
+    }
+    return alpha_i;
+}
+
+//! symv with the reference's upper-triangular row layout (kernels.hpp:87-96) and the AV cleaning
+//! correction avRvCorrection (hydro_ve/momentum_energy_kern.hpp:44-63)
+template<class T>
+inline T avRvCorrection(const T R[3], T eta_ab, T eta_crit, const T gi[6], const T gj[6])
+{
+    auto symvDot = [&R](const T* g)
+    {
+        T s0 = g[0] * R[0] + g[1] * R[1] + g[2] * R[2];
+        T s1 = g[3] * R[1] + g[4] * R[2];
+        T s2 = g[5] * R[2];
+        return R[0] * s0 + (R[1] * s1 + R[2] * s2);
+    };
+    T dmy1 = symvDot(gi);
+    T dmy2 = symvDot(gj);
+    T dmy3 = T(1);
+    if (eta_ab < eta_crit)
+    {
+        T etaDiff = T(5) * (eta_ab - eta_crit);
+        dmy3      = std::exp(-etaDiff * etaDiff);
+    }
+    T A_ab   = (dmy2 != T(0)) ? dmy1 / dmy2 : T(0);
+    T A_abp1 = T(1) + A_ab;
+    T phi_ab = T(0.5) * dmy3 * std::max(T(0), std::min(T(1), T(4) * A_ab / (A_abp1 * A_abp1)));
+    return -phi_ab * (dmy1 + dmy2);
+}
+
+//! momentumAndEnergyJLoop<avClean> (VE), hydro_ve/momentum_energy_kern.hpp:66-222
+template<bool avClean, class T, class Tm>
+inline void momentumEnergyVeLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn,
+                                 const double* x, const double* y, const double* z, const T* vx, const T* vy,
+                                 const T* vz, const T* h, const Tm* m, const T* prho, const T* tdpdTrho, const T* c,
+                                 const T* c11, const T* c12, const T* c13, const T* c22, const T* c23, const T* c33,
+                                 const T Atmin, const T Atmax, const T ramp, const T* wh, const T* kx, const T* xm,
+                                 const T* alpha, const T* dV11, const T* dV12, const T* dV13, const T* dV22,
+                                 const T* dV23, const T* dV33, T* ax, T* ay, T* az, double* du, T* maxvsignal)
+{
+    auto xi = x[i], yi = y[i], zi = z[i];
+    auto vxi = vx[i], vyi = vy[i], vzi = vz[i];
+    auto hi = h[i];
+    auto mi = m[i];
+    auto ci = c[i];
+    auto kxi = kx[i];
+    auto alpha_i = alpha[i];
+    auto xmassi  = xm[i];
+    auto rhoi    = kxi * mi / xmassi;
+    auto prhoi   = prho[i];
+    T    hiInv   = T(1) / hi;
+    T    hiInv3  = hiInv * hiInv * hiInv;
+    T    maxvs   = 0.0;
+    T    momx = 0.0, momy = 0.0, momz = 0.0, energy = 0.0;
+    T    avEnergy = 0.0;
+    auto c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
+    T    gradV_i[6] = {0, 0, 0, 0, 0, 0};
+    if (avClean) { gradV_i[0] = dV11[i], gradV_i[1] = dV12[i], gradV_i[2] = dV13[i], gradV_i[3] = dV22[i],
+                   gradV_i[4] = dV23[i], gradV_i[5] = dV33[i]; }
+    T eta_crit = std::cbrt(T(32) * M_PI / T(3) / T(nn + 1));
+
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j  = nb[k];
+        T          rx = xi - x[j], ry = yi - y[j], rz = zi - z[j];
+        auto       vxj = vx[j], vyj = vy[j], vzj = vz[j];
+        foldLegacy(box, T(2) * hi, rx, ry, rz);
+        T r2     = rx * rx + ry * ry + rz * rz;
+        T dist   = std::sqrt(r2);
+        T vx_ij  = vxi - vxj, vy_ij = vyi - vyj, vz_ij = vzi - vzj;
+        T hj     = h[j];
+        T hjInv  = T(1) / hj;
+        T v1     = dist * hiInv;
+        T v2     = dist * hjInv;
+        T hjInv3 = hjInv * hjInv * hjInv;
+        T Wi     = hiInv3 * lookup(wh, v1);
+        T Wj     = hjInv3 * lookup(wh, v2);
+
+        T tA1_i = -(c11i * rx + c12i * ry + c13i * rz) * Wi;
+        T tA2_i = -(c12i * rx + c22i * ry + c23i * rz) * Wi;
+        T tA3_i = -(c13i * rx + c23i * ry + c33i * rz) * Wi;
+        auto c11j = c11[j], c12j = c12[j], c13j = c13[j], c22j = c22[j], c23j = c23[j], c33j = c33[j];
+        T tA1_j = -(c11j * rx + c12j * ry + c13j * rz) * Wj;
+        T tA2_j = -(c12j * rx + c22j * ry + c23j * rz) * Wj;
+        T tA3_j = -(c13j * rx + c23j * ry + c33j * rz) * Wj;
+
+        auto mj  = m[j];
+        auto cj  = c[j];
+        auto kxj = kx[j];
+        auto xmassj = xm[j];
+        auto rhoj   = kxj * mj / xmassj;
+
+        T rv = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+        if (avClean)
+        {
+            T R[3]      = {rx, ry, rz};
+            T gradV_j[6] = {dV11[j], dV12[j], dV13[j], dV22[j], dV23[j], dV33[j]};
+            rv += avRvCorrection(R, std::min(v1, v2), eta_crit, gradV_i, gradV_j);
+        }
+        T wij  = rv / dist;
+        T visc = avisc(alpha_i, alpha[j], ci, cj, wij);
+        T vsig = T(0.5) * (ci + cj) - T(2) * wij;
+        maxvs  = (vsig > maxvs) ? vsig : maxvs;
+
+        T a_mom, b_mom;
+        T Atwood = (std::abs(rhoi - rhoj)) / (rhoi + rhoj);
+        if (Atwood < Atmin)
+        {
+            a_mom = xmassi * xmassi;
+            b_mom = xmassj * xmassj;
+        }
+        else if (Atwood > Atmax)
+        {
+            a_mom = xmassi * xmassj;
+            b_mom = a_mom;
+        }
+        else
+        {
+            T sigma_ij = ramp * (Atwood - Atmin);
+            // unqualified pow in the reference resolves to the C library's double pow
+            a_mom = ::pow(xmassi, T(2) - sigma_ij) * ::pow(xmassj, sigma_ij);
+            b_mom = ::pow(xmassj, T(2) - sigma_ij) * ::pow(xmassi, sigma_ij);
+        }
+
+        auto a_visc   = mj / rhoi * visc;
+        auto b_visc   = mj / rhoj * visc;
+        T    a_visc_x = T(0.5) * (a_visc * tA1_i + b_visc * tA1_j);
+        T    a_visc_y = T(0.5) * (a_visc * tA2_i + b_visc * tA2_j);
+        T    a_visc_z = T(0.5) * (a_visc * tA3_i + b_visc * tA3_j);
+        avEnergy += a_visc_x * vx_ij + a_visc_y * vy_ij + a_visc_z * vz_ij;
+
+        energy += mj * a_mom * (vx_ij * tA1_i + vy_ij * tA2_i + vz_ij * tA3_i);
+
+        auto mom_i = mj * prhoi * a_mom;
+        auto mom_j = mj * prho[j] * b_mom;
+        momx += mom_i * tA1_i + mom_j * tA1_j + a_visc_x;
+        momy += mom_i * tA2_i + mom_j * tA2_j + a_visc_y;
+        momz += mom_i * tA3_i + mom_j * tA3_j + a_visc_z;
+    }
+    avEnergy = std::max(T(0), avEnergy);
+    T eCoeff = (tdpdTrho == nullptr) ? prhoi : tdpdTrho[i];
+    du[i]    = K * (eCoeff * energy + T(0.5) * avEnergy);
+    ax[i]    = -K * momx;
+    ay[i]    = -K * momy;
+    az[i]    = -K * momz;
+    *maxvsignal = maxvs;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Equations of state (sph/include/sph/eos.hpp:19-86)
+ * ----------------------------------------------------------------------------------------------*/
+template<class T1, class T2>
+inline T1 idealGasCv(T1 mui, T2 gamma)
+{
+    constexpr T1 R = 8.317e7;
+    return R / mui / (gamma - T1(1));
+}
+
+template<class T1, class T2, class T3>
+inline void idealGasFromU(T1 u, T2 rho, T3 gamma, std::common_type_t<T1, T2, T3>& p,
+                          std::common_type_t<T1, T2, T3>& c)
+{
+    using Tc = std::common_type_t<T1, T2, T3>;
+    Tc tmp   = u * (gamma - Tc(1));
+    p        = rho * tmp;
+    c        = std::sqrt(gamma * tmp);
+}
+
+} // namespace orc

@@ -1,0 +1,98 @@
+"""ctypes mirrors of the C ABI structs declared in include/sphexa_amd.h.
+
+Pure data-layout definitions (no library is loaded here), shared by the product
+wrapper (sphexa_amd.lib) and the CPU oracle loader so that one particle view can be
+handed to both sides.
+"""
+import ctypes as C
+
+SPHX_OK = 0
+SPHX_ERR_HIP = 1
+SPHX_ERR_ARG = 2
+SPHX_ERR_NOT_CONVERGED = 3
+SPHX_ERR_STATE = 4
+SPHX_ERR_ALLOC = 5
+
+SPHX_F64 = 0
+SPHX_F32 = 1
+
+OPEN = 0
+PERIODIC = 1
+
+# HydroType fields of sphexa::ParticlesData (sph/include/sph/particles_data.hpp:220-248) that the
+# hot path reads or writes, in the order of sphx_fields_t.
+REAL_FIELDS = ("x", "y", "z", "u", "temp", "du")
+HYDRO_FIELDS = ("h", "m", "vx", "vy", "vz", "rho", "p", "c", "prho", "tdpdTrho",
+                "c11", "c12", "c13", "c22", "c23", "c33", "xm", "kx", "gradh", "divv", "curlv",
+                "alpha", "ax", "ay", "az", "dV11", "dV12", "dV13", "dV22", "dV23", "dV33")
+INT_FIELDS = ("keys", "nc")
+ALL_FIELDS = REAL_FIELDS + HYDRO_FIELDS + INT_FIELDS
+
+
+class Box(C.Structure):
+    """cstone::Box<double> (domain/include/cstone/sfc/box.hpp:95-174)."""
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("boundary", C.c_int32 * 3)]
+
+    @classmethod
+    def make(cls, lo, hi, boundary=OPEN):
+        lo = [lo] * 3 if isinstance(lo, (int, float)) else list(lo)
+        hi = [hi] * 3 if isinstance(hi, (int, float)) else list(hi)
+        bc = [boundary] * 3 if isinstance(boundary, int) else list(boundary)
+        b = cls()
+        for d in range(3):
+            b.lo[d], b.hi[d], b.boundary[d] = float(lo[d]), float(hi[d]), int(bc[d])
+        return b
+
+    @property
+    def lengths(self):
+        return [self.hi[d] - self.lo[d] for d in range(3)]
+
+    @property
+    def periodic(self):
+        return [self.boundary[d] == PERIODIC for d in range(3)]
+
+
+class Fields(C.Structure):
+    _fields_ = ([(n, C.c_void_p) for n in REAL_FIELDS] + [(n, C.c_void_p) for n in HYDRO_FIELDS]
+                + [("keys", C.c_void_p), ("nc", C.c_void_p)])
+
+
+class Params(C.Structure):
+    """Dataset scalars (sph/include/sph/particles_data.hpp:91-146)."""
+    _fields_ = [("K", C.c_double), ("Kcour", C.c_double), ("gamma", C.c_double), ("muiConst", C.c_double),
+                ("minDt", C.c_double), ("alphamin", C.c_double), ("alphamax", C.c_double),
+                ("decay_constant", C.c_double), ("Atmin", C.c_double), ("Atmax", C.c_double),
+                ("ramp", C.c_double), ("ng0", C.c_uint32), ("ngmax", C.c_uint32),
+                ("hydro_type", C.c_int32), ("reserved", C.c_int32)]
+
+    @classmethod
+    def defaults(cls, K, hydro_type=SPHX_F64, **kw):
+        """Reference defaults: ng0 100, ngmax 150, Kcour 0.2, gamma 5/3, muiConst 10,
+        alphamin 0.05, alphamax 1, decay 0.2, Atmin 0.1, Atmax 0.2, ramp 1/(Atmax-Atmin)."""
+        p = cls(K=K, Kcour=0.2, gamma=5.0 / 3.0, muiConst=10.0, minDt=1e-12, alphamin=0.05, alphamax=1.0,
+                decay_constant=0.2, Atmin=0.1, Atmax=0.2, ramp=1.0 / (0.2 - 0.1), ng0=100, ngmax=150,
+                hydro_type=hydro_type, reserved=0)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+class TreeView(C.Structure):
+    _fields_ = [("numLeafNodes", C.c_int32), ("numInternalNodes", C.c_int32), ("numNodes", C.c_int32),
+                ("reserved", C.c_int32), ("prefixes", C.c_void_p), ("childOffsets", C.c_void_p),
+                ("parents", C.c_void_p), ("internalToLeaf", C.c_void_p), ("levelRange", C.c_void_p),
+                ("leaves", C.c_void_p), ("counts", C.c_void_p), ("layout", C.c_void_p),
+                ("centers", C.c_void_p), ("sizes", C.c_void_p)]
+
+
+def fields_from(arrays):
+    """Build a Fields struct from a mapping name -> array exposing an address
+    (numpy: .ctypes.data, torch: .data_ptr()); missing names stay NULL."""
+    f = Fields()
+    for name in ALL_FIELDS:
+        a = arrays.get(name)
+        if a is None:
+            continue
+        addr = a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        setattr(f, name, addr)
+    return f
