@@ -33,6 +33,8 @@ _SIGS = {
     "tree_free": (None, [_P]),
     "find_neighbors": (_SZ, [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, _P]),
     "kernel_K": (C.c_double, [C.c_double]),
+    "distance_sq_pbc": (C.c_double, [_BOXP] + [C.c_double] * 6),
+    "simpson_sin": (C.c_double, []),
     "sphynx_K": (C.c_double, [C.c_double]),
     "tables_f64": (None, [C.c_double, _P, _P]),
     "tables_f32": (None, [C.c_double, _P, _P]),
@@ -46,6 +48,7 @@ _SIGS = {
     "eos_ve": (None, [_FP, _PP, _U32, _U32]),
     "iad_divv_curlv": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "av_switches": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
+    "divv_curlv": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "momentum_energy_ve": (C.c_double, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, C.c_int, _P]),
 }
 
@@ -66,6 +69,8 @@ def load(kind="orc"):
     lib = C.CDLL(path)
     ns = {}
     for name, (res, args) in _SIGS.items():
+        if not hasattr(lib, f"{kind}_{name}"):
+            continue
         fn = getattr(lib, f"{kind}_{name}")
         fn.restype = res
         fn.argtypes = args

@@ -337,6 +337,15 @@ size_t orc_find_neighbors(const void* tp, const sphx_box_t* b, const sphx_fields
                             f->nc + first, iterateH != 0);
 }
 
+//! distanceSq<true> (domain/include/cstone/findneighbors.hpp:33-48)
+double orc_distance_sq_pbc(const sphx_box_t* b, double x1, double y1, double z1, double x2, double y2, double z2)
+{
+    return distSq<true>(x1, y1, z1, x2, y2, z2, toBox(b));
+}
+
+//! util::simpson of sin over [0, pi/2] with 2000 intervals (sph_kernel_tables.hpp:31-60)
+double orc_simpson_sin() { return simpson(0, M_PI / 2, 2000, [](double x) { return std::sin(x); }); }
+
 /* ---- tables ---------------------------------------------------------------------------------- */
 double orc_kernel_K(double sincIndex) { return kernelNormalization(sincIndex); }
 double orc_sphynx_K(double n) { return sphynxK(n); }
@@ -420,6 +429,24 @@ void orc_iad_divv_curlv(const sphx_box_t* b, const sphx_fields_t* f, const sphx_
 #define C(T) iadDivvCurlv<T>(toBox(b), f, p, first, last, nb, (const T*)wh)
     ORC_DISPATCH(C)
 #undef C
+}
+
+//! divV_curlVJLoop alone on the stored c_ij (the JLoop the reference's ve.cpp:122-139 exercises)
+void orc_divv_curlv(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                    uint32_t last, const uint32_t* nb, const void* whv)
+{
+    const double* wh = static_cast<const double*>(whv);
+    Box           box = toBox(b);
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<double>(f, i, p->ngmax);
+        divvCurlvLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<double>(f->vx),
+                      F<double>(f->vy), F<double>(f->vz), F<double>(f->h), F<double>(f->c11), F<double>(f->c12),
+                      F<double>(f->c13), F<double>(f->c22), F<double>(f->c23), F<double>(f->c33), wh,
+                      F<double>(f->kx), F<double>(f->xm), F<double>(f->divv), F<double>(f->curlv), F<double>(f->dV11),
+                      F<double>(f->dV12), F<double>(f->dV13), F<double>(f->dV22), F<double>(f->dV23),
+                      F<double>(f->dV33), f->dV11 != nullptr);
+    }
 }
 
 void orc_av_switches(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,

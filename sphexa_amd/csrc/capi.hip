@@ -1,0 +1,358 @@
+// extern "C" entry points of include/sphexa_amd.h: argument checks, error translation,
+// kernel-table generation and the one-rank domain sync.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sphx
+{
+
+namespace
+{
+
+template<class F>
+int guarded(sphx_workspace* ws, F&& body)
+{
+    if (ws == nullptr) { return SPHX_ERR_ARG; }
+    try
+    {
+        SPHX_HIP(hipSetDevice(ws->device));
+        int r = body();
+        if (r == SPHX_OK) { ws->lastError.clear(); }
+        return r;
+    }
+    catch (const Error& e)
+    {
+        ws->lastError = e.msg;
+        return e.code;
+    }
+    catch (const std::exception& e)
+    {
+        ws->lastError = e.what();
+        return SPHX_ERR_ARG;
+    }
+}
+
+//! sinc(pi/2 v)^n and its v-derivative (sph/include/sph/kernels.hpp:36-61, sph_kernel_tables.hpp:105-109)
+double sincPow(double v, double n)
+{
+    double s = v == 0.0 ? 1.0 : std::sin(M_PI_2 * v) / (M_PI_2 * v);
+    return std::pow(s, n);
+}
+
+double sincPowDerivative(double v, double n)
+{
+    if (v == 0.0) { return 0.0; }
+    const double pv = M_PI_2 * v;
+    const double s  = std::sin(pv) / pv;
+    const double ds = s * M_PI_2 * ((std::cos(pv) / std::sin(pv)) - 1.0 / pv);
+    return n * std::pow(s, n - 1) * ds;
+}
+
+//! 1 / int_0^2 4 pi v^2 W(v) dv, composite Simpson with 2000 intervals and ascending partial sums
+//! (sph_kernel_tables.hpp:31-60,78-83)
+double normalization(double n)
+{
+    const uint64_t      intervals = 2000;
+    const double        a = 0.0, b = 2.0, h = (b - a) / double(intervals);
+    auto                f = [n](double v) { return 4.0 * M_PI * v * v * sincPow(v, n); };
+    std::vector<double> odd(intervals / 2), even(intervals / 2 - 1);
+    for (uint64_t i = 0; i < odd.size(); ++i)
+        odd[i] = f(a + double(2 * i + 1) * h);
+    for (uint64_t i = 0; i < even.size(); ++i)
+        even[i] = f(a + double(2 * i + 2) * h);
+    std::sort(odd.begin(), odd.end());
+    std::sort(even.begin(), even.end());
+    double so = std::accumulate(odd.begin(), odd.end(), 0.0), se = std::accumulate(even.begin(), even.end(), 0.0);
+    return 1.0 / (h / 3.0 * (f(a) + f(b) + 4.0 * so + 2.0 * se));
+}
+
+//! tabulateFunction<T, 20000>(f, 0, 2): abscissae i * dx formed in T (sph_kernel_tables.hpp:88-101)
+template<class T, class F>
+std::vector<T> tabulate(F&& f)
+{
+    std::vector<T> t(kTableSize);
+    const T        dx = T(2.0 / (kTableSize - 1));
+    for (int i = 0; i < kTableSize; ++i)
+    {
+        T v  = T(0.0 + double(T(i * dx)));
+        t[i] = T(f(double(v)));
+    }
+    return t;
+}
+
+void requireTables(const sphx_workspace* ws)
+{
+    SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
+}
+
+size_t hydroBytes(const sphx_params_t* p) { return p->hydro_type == SPHX_F32 ? 4 : 8; }
+
+} // namespace
+} // namespace sphx
+
+using namespace sphx;
+
+extern "C"
+{
+
+int sphx_workspace_create(sphx_workspace_t** out, int device, void* stream)
+{
+    if (out == nullptr) { return SPHX_ERR_ARG; }
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return SPHX_ERR_HIP;
+    }
+    auto* ws   = new sphx_workspace;
+    ws->device = device;
+    ws->stream = static_cast<hipStream_t>(stream);
+    *out       = ws;
+    return SPHX_OK;
+}
+
+int sphx_workspace_destroy(sphx_workspace_t* ws)
+{
+    if (ws == nullptr) { return SPHX_OK; }
+    (void)hipSetDevice(ws->device);
+    (void)hipStreamSynchronize(ws->stream);
+    ws->wh64.release(), ws->whd64.release(), ws->wh32.release(), ws->whd32.release();
+    ws->tree.releaseAll();
+    ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release(), ws->idxTmp.release();
+    ws->fieldTmp.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
+    delete ws;
+    return SPHX_OK;
+}
+
+const char* sphx_last_error(const sphx_workspace_t* ws) { return ws ? ws->lastError.c_str() : "null workspace"; }
+
+int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(sincIndex > 0, SPHX_ERR_ARG, "sinc index must be positive");
+        auto w64  = tabulate<double>([sincIndex](double v) { return sincPow(v, sincIndex); });
+        auto wd64 = tabulate<double>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
+        auto w32  = tabulate<float>([sincIndex](double v) { return sincPow(v, sincIndex); });
+        auto wd32 = tabulate<float>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
+        ws->wh64.reserve(kTableSize), ws->whd64.reserve(kTableSize);
+        ws->wh32.reserve(kTableSize), ws->whd32.reserve(kTableSize);
+        SPHX_HIP(hipMemcpyAsync(ws->wh64.get(), w64.data(), kTableSize * 8, hipMemcpyHostToDevice, ws->stream));
+        SPHX_HIP(hipMemcpyAsync(ws->whd64.get(), wd64.data(), kTableSize * 8, hipMemcpyHostToDevice, ws->stream));
+        SPHX_HIP(hipMemcpyAsync(ws->wh32.get(), w32.data(), kTableSize * 4, hipMemcpyHostToDevice, ws->stream));
+        SPHX_HIP(hipMemcpyAsync(ws->whd32.get(), wd32.data(), kTableSize * 4, hipMemcpyHostToDevice, ws->stream));
+        SPHX_HIP(hipStreamSynchronize(ws->stream));
+        ws->K           = normalization(sincIndex);
+        ws->sincIndex   = sincIndex;
+        ws->tablesValid = true;
+        if (K) { *K = ws->K; }
+        return SPHX_OK;
+    });
+}
+
+int sphx_compute_sfc_keys(sphx_workspace_t* ws, const sphx_box_t* box, const double* x, const double* y,
+                          const double* z, uint64_t* keys, size_t n)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && (n == 0 || (x && y && z && keys)), SPHX_ERR_ARG, "null argument");
+        computeKeys(ws, makeDevBox(box), x, y, z, keys, n);
+        return SPHX_OK;
+    });
+}
+
+int sphx_sort_keys(sphx_workspace_t* ws, uint64_t* keys, uint32_t* order, size_t n)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(n == 0 || (keys && order), SPHX_ERR_ARG, "null argument");
+        SPHX_REQUIRE(n < (size_t(1) << 32), SPHX_ERR_ARG, "more than 2^32-1 particles");
+        sortKeys(ws, keys, order, n);
+        return SPHX_OK;
+    });
+}
+
+int sphx_gather(sphx_workspace_t* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(n == 0 || (order && in && out && in != out), SPHX_ERR_ARG, "null or aliased argument");
+        gather(ws, order, in, out, n, elemBytes);
+        return SPHX_OK;
+    });
+}
+
+int sphx_build_tree(sphx_workspace_t* ws, const uint64_t* sortedKeys, size_t n, uint32_t bucketSize,
+                    const sphx_box_t* box)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && (n == 0 || sortedKeys), SPHX_ERR_ARG, "null argument");
+        ws->nbValid = false;
+        buildTree(ws, sortedKeys, n, bucketSize, makeDevBox(box));
+        return SPHX_OK;
+    });
+}
+
+int sphx_tree_view(const sphx_workspace_t* wsc, sphx_tree_view_t* out)
+{
+    if (wsc == nullptr || out == nullptr) { return SPHX_ERR_ARG; }
+    const Tree& t = wsc->tree;
+    if (!t.valid) { return SPHX_ERR_STATE; }
+    out->numLeafNodes     = t.numLeafNodes;
+    out->numInternalNodes = t.numInternalNodes;
+    out->numNodes         = t.numNodes;
+    out->reserved         = 0;
+    out->prefixes         = t.prefixes.get();
+    out->childOffsets     = t.childOffsets.get();
+    out->parents          = t.parents.get();
+    out->internalToLeaf   = t.internalToLeaf.get();
+    out->levelRange       = t.levelRange.get();
+    out->leaves           = t.leaves.get();
+    out->counts           = t.counts.get();
+    out->layout           = t.layout.get();
+    out->centers          = t.centers.get();
+    out->sizes            = t.sizes.get();
+    return SPHX_OK;
+}
+
+int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t* f, const sphx_params_t* prm,
+                     size_t n, uint32_t bucketSize)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
+        SPHX_REQUIRE(n == 0 || (f->x && f->y && f->z), SPHX_ERR_ARG, "x, y, z are required");
+        SPHX_REQUIRE(n < (size_t(1) << 32), SPHX_ERR_ARG, "more than 2^32-1 particles");
+        ws->nbValid  = false;
+        DevBox b     = makeDevBox(box);
+        ws->keyTmp.reserve(n);
+        uint64_t* keys = f->keys;
+        DevBuf<uint64_t> own;
+        if (keys == nullptr)
+        {
+            own.reserve(n);
+            keys = own.get();
+        }
+        fill64(ws, keys, 0, n);
+        computeKeys(ws, b, f->x, f->y, f->z, keys, n);
+        DevBuf<uint32_t> order;
+        order.reserve(n);
+        sortKeys(ws, keys, order.get(), n);
+
+        // permute every allocated field through one staging buffer
+        ws->fieldTmp.reserve(n * 8);
+        void*                              tmp = ws->fieldTmp.get();
+        std::vector<std::pair<void*, int>> fields;
+        for (double* r : {f->x, f->y, f->z, f->u, f->temp, f->du})
+            if (r) fields.push_back({r, 8});
+        int   hb      = int(hydroBytes(prm));
+        void* hyd[]   = {f->h,   f->m,    f->vx,   f->vy,    f->vz,    f->rho,   f->p,    f->c,   f->prho, f->tdpdTrho,
+                         f->c11, f->c12,  f->c13,  f->c22,   f->c23,   f->c33,   f->xm,   f->kx,  f->gradh, f->divv,
+                         f->curlv, f->alpha, f->ax, f->ay,   f->az,    f->dV11,  f->dV12, f->dV13, f->dV22, f->dV23,
+                         f->dV33};
+        for (void* h : hyd)
+            if (h) fields.push_back({h, hb});
+        if (f->nc) fields.push_back({f->nc, 4});
+        for (auto& [ptr, eb] : fields)
+        {
+            gather(ws, order.get(), ptr, tmp, n, eb);
+            SPHX_HIP(hipMemcpyAsync(ptr, tmp, n * size_t(eb), hipMemcpyDeviceToDevice, ws->stream));
+        }
+        buildTree(ws, keys, n, bucketSize, b);
+        SPHX_HIP(hipStreamSynchronize(ws->stream));
+        order.release();
+        own.release();
+        return SPHX_OK;
+    });
+}
+
+int sphx_find_neighbors(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                        const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
+        return findNeighbors(ws, makeDevBox(box), f, prm, first, last, iterateH != 0);
+    });
+}
+
+int sphx_copy_neighbors(sphx_workspace_t* ws, uint32_t* out, uint32_t first, uint32_t last)
+{
+    return guarded(ws, [&] {
+        copyNeighbors(ws, out, first, last);
+        return SPHX_OK;
+    });
+}
+
+int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t* maxCount)
+{
+    if (ws == nullptr) { return SPHX_ERR_ARG; }
+    if (!ws->nbValid) { return SPHX_ERR_STATE; }
+    if (sumPairs) { *sumPairs = ws->statSumPairs; }
+    if (maxCount) { *maxCount = ws->statMaxCount; }
+    return SPHX_OK;
+}
+
+#define SPHX_PAIR_ENTRY(name, fn)                                                                                      \
+    int name(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_params_t* prm,            \
+             uint32_t first, uint32_t last)                                                                            \
+    {                                                                                                                  \
+        return guarded(ws, [&] {                                                                                       \
+            SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");                                              \
+            requireTables(ws);                                                                                         \
+            fn(ws, makeDevBox(box), f, prm, first, last);                                                              \
+            return SPHX_OK;                                                                                            \
+        });                                                                                                            \
+    }
+
+SPHX_PAIR_ENTRY(sphx_density_std, densityStd)
+SPHX_PAIR_ENTRY(sphx_iad_std, iadStd)
+SPHX_PAIR_ENTRY(sphx_xmass, xmassVe)
+SPHX_PAIR_ENTRY(sphx_ve_def_gradh, veDefGradh)
+SPHX_PAIR_ENTRY(sphx_iad_divv_curlv, iadDivvCurlv)
+SPHX_PAIR_ENTRY(sphx_av_switches, avSwitches)
+
+int sphx_eos_std(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                 uint32_t last)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
+        eosStd(ws, f, prm, first, last);
+        return SPHX_OK;
+    });
+}
+
+int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                uint32_t last)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
+        eosVe(ws, f, prm, first, last);
+        return SPHX_OK;
+    });
+}
+
+int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                             const sphx_params_t* prm, uint32_t first, uint32_t last, double* minDtCourant)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
+        requireTables(ws);
+        double dt = momentumEnergyStd(ws, makeDevBox(box), f, prm, first, last);
+        if (minDtCourant) { *minDtCourant = dt; }
+        return SPHX_OK;
+    });
+}
+
+int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                            const sphx_params_t* prm, uint32_t first, uint32_t last, int avClean,
+                            double* minDtCourant)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
+        requireTables(ws);
+        double dt = momentumEnergyVe(ws, makeDevBox(box), f, prm, first, last, avClean != 0);
+        if (minDtCourant) { *minDtCourant = dt; }
+        return SPHX_OK;
+    });
+}
+
+} // extern "C"

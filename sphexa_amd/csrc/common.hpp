@@ -1,0 +1,311 @@
+// Shared device helpers, workspace state and launch utilities of the gfx950 hydro path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "sphexa_amd.h"
+
+namespace sphx
+{
+
+using Key           = uint64_t;
+using LocalIndex    = uint32_t;
+using TreeNodeIndex = int32_t;
+
+constexpr unsigned kMaxLevel  = 21; // 64-bit Hilbert keys, 21 bits per dimension
+constexpr int      kTableSize = 20000;
+constexpr int      kWave      = 64; // gfx950 wavefront
+constexpr int      kGroup     = 64; // targets per neighbour-search group (one wavefront)
+
+/* ---------------------------------------------------------------------------------------------
+ * error handling: every entry point returns an sphx_status and records a message
+ * -------------------------------------------------------------------------------------------*/
+struct Error
+{
+    int         code;
+    std::string msg;
+};
+
+#define SPHX_HIP(call)                                                                                                 \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess)                                                                                          \
+            throw ::sphx::Error{SPHX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};                      \
+    } while (0)
+
+#define SPHX_CHECK_LAUNCH() SPHX_HIP(hipGetLastError())
+
+#define SPHX_REQUIRE(cond, code, text)                                                                                 \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(cond)) throw ::sphx::Error{(code), (text)};                                                              \
+    } while (0)
+
+//! growable device buffer owned by the workspace
+template<class T>
+struct DevBuf
+{
+    T*     ptr = nullptr;
+    size_t cap = 0;
+
+    T* get() const { return ptr; }
+    void reserve(size_t n)
+    {
+        if (n <= cap) { return; }
+        release();
+        size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
+        if (hipMalloc(&ptr, bytes) != hipSuccess)
+        {
+            ptr = nullptr;
+            cap = 0;
+            (void)hipGetLastError();
+            throw Error{SPHX_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        }
+        cap = n;
+    }
+    void release()
+    {
+        if (ptr) { (void)hipFree(ptr); }
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+//! per-build temporaries of the top-down tree construction, kept for reuse
+struct TreeBuildScratch
+{
+    DevBuf<Key>      nodeKey, leafKey, leafKeySorted;
+    DevBuf<uint32_t> lo, hi, flag, rank, leafFlag, leafRank, leafNode, leafNodeSorted;
+    DevBuf<uint8_t>  level;
+    DevBuf<int32_t>  levelStart;
+
+    void releaseAll()
+    {
+        nodeKey.release(), leafKey.release(), leafKeySorted.release(), lo.release(), hi.release();
+        flag.release(), rank.release(), leafFlag.release(), leafRank.release(), leafNode.release();
+        leafNodeSorted.release(), level.release(), levelStart.release();
+    }
+};
+
+struct Tree
+{
+    int32_t              numLeafNodes = 0, numInternalNodes = 0, numNodes = 0;
+    size_t               numParticles = 0;
+    DevBuf<Key>          prefixes, leaves;
+    DevBuf<TreeNodeIndex> childOffsets, parents, internalToLeaf, levelRange;
+    DevBuf<uint32_t>     counts, layout;
+    DevBuf<double>       centers, sizes;
+    bool                 valid = false;
+    TreeBuildScratch     build;
+
+    void releaseAll()
+    {
+        prefixes.release(), leaves.release(), childOffsets.release(), parents.release();
+        internalToLeaf.release(), levelRange.release(), counts.release(), layout.release();
+        centers.release(), sizes.release(), build.releaseAll();
+        valid = false;
+    }
+};
+
+} // namespace sphx
+
+struct sphx_workspace
+{
+    int         device = 0;
+    hipStream_t stream = nullptr;
+    std::string lastError;
+
+    // kernel tables: W and dW/dv on [0,2], 20000 points, in both precisions
+    double                 K = 0, sincIndex = 0;
+    sphx::DevBuf<double>   wh64, whd64;
+    sphx::DevBuf<float>    wh32, whd32;
+    bool                   tablesValid = false;
+
+    sphx::Tree tree;
+
+    // neighbour lists of the last search: group-interleaved [group][k][lane]
+    sphx::DevBuf<uint32_t> nbrs;
+    uint32_t               nbFirst = 0, nbLast = 0, nbNgmax = 0;
+    const uint32_t*        nbNc = nullptr; // nc array the lists were built with
+    bool                   nbValid = false;
+    uint64_t               statSumPairs = 0;
+    uint32_t               statMaxCount = 0;
+
+    // scratch
+    sphx::DevBuf<char>     scratch;      // sort / scan temporaries
+    sphx::DevBuf<uint64_t> keyTmp;
+    sphx::DevBuf<uint32_t> idxTmp;
+    sphx::DevBuf<char>     fieldTmp;     // permutation staging
+    sphx::DevBuf<uint32_t> counters;     // small device counters / flags
+    sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
+    sphx::DevBuf<double>   reduce64;     // per-block partial reductions
+};
+
+namespace sphx
+{
+
+//! scratch of at least `bytes`, 256-byte aligned
+inline void* scratchBytes(sphx_workspace* ws, size_t bytes)
+{
+    ws->scratch.reserve(bytes);
+    return ws->scratch.get();
+}
+
+inline unsigned ceilDiv(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
+
+/* ---------------------------------------------------------------------------------------------
+ * bit helpers (placeholder-bit keys, cf. domain/include/cstone/sfc/common.hpp)
+ * -------------------------------------------------------------------------------------------*/
+__host__ __device__ inline Key nodeRange(unsigned level) { return Key(1) << (3u * (kMaxLevel - level)); }
+
+__host__ __device__ inline Key encodePlaceholder(Key key, int prefixLength)
+{
+    return (Key(1) << prefixLength) | (key >> (3 * int(kMaxLevel) - prefixLength));
+}
+
+//! Hilbert key -> integer coordinates (same curve as domain/include/cstone/sfc/hilbert.hpp:130-172)
+__host__ __device__ inline void hilbertDecode(Key key, unsigned& ox, unsigned& oy, unsigned& oz)
+{
+    unsigned px = 0, py = 0, pz = 0;
+    for (unsigned level = 0; level < kMaxLevel; ++level)
+    {
+        unsigned d  = unsigned(key >> (3 * level)) & 7u;
+        unsigned bx = d >> 2, by = (d >> 1) & 1u, bz = d & 1u;
+        if (by ^ bz)
+        {
+            unsigned t = px;
+            px = pz, pz = py, py = t;
+        }
+        else if (bx == by && by == bz)
+        {
+            unsigned t = px;
+            px = pz, pz = t;
+        }
+        unsigned mask = (1u << level) - 1;
+        px ^= mask & (0u - (bx & (by | bz)));
+        py ^= mask & (0u - ((bx & ((by ^ 1u) | (bz ^ 1u))) | ((bx ^ 1u) & by & bz)));
+        pz ^= mask & (0u - ((bx & (by ^ 1u) & (bz ^ 1u)) | (by & bz)));
+        px |= bx << level;
+        py |= (bx ^ by) << level;
+        pz |= (by ^ bz) << level;
+    }
+    ox = px, oy = py, oz = pz;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * wave-level primitives (wave64)
+ * -------------------------------------------------------------------------------------------*/
+__device__ inline unsigned laneId() { return __lane_id(); }
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+//! number of set bits of `mask` below this lane
+__device__ inline unsigned laneRank(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(unsigned(mask >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(mask), 0u));
+}
+
+template<class T>
+__device__ inline T waveMin(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        T w = __shfl_xor(v, o, 64);
+        v   = w < v ? w : v;
+    }
+    return v;
+}
+
+template<class T>
+__device__ inline T waveMax(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        T w = __shfl_xor(v, o, 64);
+        v   = w > v ? w : v;
+    }
+    return v;
+}
+
+template<class T>
+__device__ inline T waveSum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+//! read a value held by lane `src` (uniform result)
+template<class T>
+__device__ inline T readLane(T v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * box in device-friendly form
+ * -------------------------------------------------------------------------------------------*/
+struct DevBox
+{
+    double lo[3], hi[3], len[3], ilen[3];
+    int    pbc[3];
+    int    anyPbc;
+};
+
+inline DevBox makeDevBox(const sphx_box_t* b)
+{
+    DevBox d;
+    d.anyPbc = 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        d.lo[k]   = b->lo[k];
+        d.hi[k]   = b->hi[k];
+        d.len[k]  = b->hi[k] - b->lo[k];
+        d.ilen[k] = 1.0 / (b->hi[k] - b->lo[k]);
+        d.pbc[k]  = b->boundary[k] == 1;
+        d.anyPbc |= d.pbc[k];
+    }
+    return d;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * host launchers implemented in the .hip translation units
+ * -------------------------------------------------------------------------------------------*/
+void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n);
+void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
+                 size_t n);
+void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
+void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
+void buildTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);
+
+int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                  uint32_t first, uint32_t last, bool iterateH);
+void copyNeighbors(sphx_workspace* ws, uint32_t* out, uint32_t first, uint32_t last);
+
+void densityStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                uint32_t first, uint32_t last);
+void eosStd(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last);
+void iadStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+            uint32_t last);
+double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                         uint32_t first, uint32_t last);
+
+void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+             uint32_t last);
+void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                uint32_t first, uint32_t last);
+void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last);
+void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                  uint32_t first, uint32_t last);
+void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                uint32_t first, uint32_t last);
+double momentumEnergyVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                        uint32_t first, uint32_t last, bool avClean);
+
+} // namespace sphx

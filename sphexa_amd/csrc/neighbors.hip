@@ -1,0 +1,410 @@
+// Neighbour search on the cornerstone octree: one wavefront per group of 64 SFC-consecutive
+// targets, with the coupled smoothing-length / neighbour-count iteration.
+//
+// Per group the wave walks the octree once with the group's bounding box grown by the largest
+// search radius 2*h_max (exact periodic box-box distance, so no cell holding a neighbour of any
+// target is skipped), collecting candidate leaves in LDS; internal nodes live on an LDS stack
+// popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Each candidate leaf's
+// particles are then streamed through the wave with uniform (scalar) loads while every lane
+// tests them against its own target with the reference predicate (domain/include/cstone/
+// findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with dx = x_j - x_i, periodic
+// images folded by one box length, unfused FP64 arithmetic, keep if d2 < Th(4)*h_i*h_i and
+// j != i. The kept set and count are therefore bit-identical to the CPU search; the first ngmax
+// hits are stored, group-interleaved as [group][k][lane] so that the pair loops read them with
+// one coalesced 256-byte load per neighbour slot.
+#include "common.hpp"
+
+namespace sphx
+{
+
+namespace
+{
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kStackCap      = 1024; // LDS node stack per wave (bound argued in traverse())
+constexpr int kLeafCap       = 256;  // LDS candidate-leaf buffer per wave
+
+struct SearchArgs
+{
+    const double*        x;
+    const double*        y;
+    const double*        z;
+    const TreeNodeIndex* childOffsets;
+    const TreeNodeIndex* internalToLeaf;
+    const uint32_t*      layout;
+    const double*        centers;
+    const double*        sizes;
+    DevBox               box;
+    double               slack; // absolute enlargement of the group radius (rounding margin)
+    uint32_t             first, last, ng0, ngmax;
+    int                  iterateH;
+    uint32_t*            nbrs;
+    uint32_t*            nc;
+    unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles
+};
+
+//! per-target state used by the leaf sweep
+struct Target
+{
+    double   x, y, z, r2;
+    uint32_t i;
+    bool     active;
+};
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniformD(double v)
+{
+    uint64_t b = __double_as_longlong(v);
+    uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(b)), hi = __builtin_amdgcn_readfirstlane(uint32_t(b >> 32));
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+//! Test all particles of one candidate leaf against the wave's targets; hits are appended to
+//! each lane's list. `shift` is the periodic image offset (multiple of the box length, exact)
+//! valid for every pair that can be a neighbour; `foldPerPair` selects the reference's rint fold
+//! per pair when the box is too small for a single image per leaf.
+__device__ __forceinline__ void sweepLeaf(const SearchArgs& a, const Target& t, uint32_t lo, uint32_t hi,
+                                          double sx, double sy, double sz, bool shifted, bool foldPerPair,
+                                          unsigned& count, uint32_t* myList)
+{
+#pragma clang fp contract(off)
+    const unsigned ngmax = a.ngmax;
+    if (!shifted && !foldPerPair)
+    {
+        for (uint32_t j = lo; j < hi; ++j)
+        {
+            double dx = a.x[j] - t.x, dy = a.y[j] - t.y, dz = a.z[j] - t.z;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (t.active && d2 < t.r2 && j != t.i)
+            {
+                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
+                count++;
+            }
+        }
+    }
+    else if (!foldPerPair)
+    {
+        for (uint32_t j = lo; j < hi; ++j)
+        {
+            double dx = (a.x[j] - t.x) - sx, dy = (a.y[j] - t.y) - sy, dz = (a.z[j] - t.z) - sz;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (t.active && d2 < t.r2 && j != t.i)
+            {
+                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
+                count++;
+            }
+        }
+    }
+    else
+    {
+        const DevBox& b = a.box;
+        for (uint32_t j = lo; j < hi; ++j)
+        {
+            double dx = a.x[j] - t.x, dy = a.y[j] - t.y, dz = a.z[j] - t.z;
+            dx        = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
+            dy        = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
+            dz        = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (t.active && d2 < t.r2 && j != t.i)
+            {
+                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
+                count++;
+            }
+        }
+    }
+}
+
+//! Sweep the buffered candidate leaves (wave-uniform loop).
+__device__ void flushLeaves(const SearchArgs& a, const Target& t, const int32_t* leafBuf, int nLeaf,
+                            const double gc[3], const double ge[3], double reach, unsigned& count, uint32_t* myList)
+{
+    const DevBox& b = a.box;
+    for (int q = 0; q < nLeaf; ++q)
+    {
+        int32_t  node = int32_t(uniform(uint32_t(leafBuf[q])));
+        uint32_t leaf = uniform(uint32_t(a.internalToLeaf[node]));
+        uint32_t lo = uniform(a.layout[leaf]), hi = uniform(a.layout[leaf + 1]);
+        if (lo == hi) { continue; }
+        double s[3], c[3], sh[3];
+        bool   shifted = false, fold = false;
+        for (int d = 0; d < 3; ++d)
+        {
+            c[d]  = uniformD(a.centers[3 * size_t(node) + d]);
+            s[d]  = uniformD(a.sizes[3 * size_t(node) + d]);
+            sh[d] = 0.0;
+            if (b.pbc[d])
+            {
+                double k = rint((c[d] - gc[d]) * b.ilen[d]);
+                sh[d]    = k * b.len[d];
+                shifted  = shifted || (k != 0.0);
+                // a single image serves all pairs only if group + leaf + radius span < L/4
+                fold = fold || (ge[d] + s[d] + reach > 0.25 * b.len[d]);
+            }
+        }
+        sweepLeaf(a, t, lo, hi, sh[0], sh[1], sh[2], shifted, fold, count, myList);
+    }
+}
+
+/* Collect all leaves whose (periodic) distance to the active targets' bounding box is below the
+ * largest active search radius and sweep them. Returns each lane's neighbour count.
+ *
+ * Stack bound: nodes are popped 64 at a time only while top + 7*64 <= kStackCap - 160, else one
+ * at a time. A one-at-a-time pop replaces a node by at most 8 children and the next pop takes
+ * the last child, i.e. it proceeds depth first, which adds at most 7 entries per tree level
+ * (<= 7*21 = 147 < 160) on top of the stack it started from. */
+__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, int32_t* stack, int32_t* leafBuf,
+                                uint32_t* myList)
+{
+#pragma clang fp contract(off)
+    const unsigned lane = laneId();
+    const DevBox&  b    = a.box;
+
+    double gmin[3], gmax[3], gc[3], ge[3];
+    double p[3] = {t.x, t.y, t.z};
+    for (int d = 0; d < 3; ++d)
+    {
+        gmin[d] = waveMin(t.active ? p[d] : INFINITY);
+        gmax[d] = waveMax(t.active ? p[d] : -INFINITY);
+        gc[d]   = 0.5 * (gmin[d] + gmax[d]);
+        ge[d]   = 0.5 * (gmax[d] - gmin[d]);
+    }
+    double reach  = waveMax(t.active ? radius : 0.0) + a.slack;
+    double reach2 = reach * reach;
+
+    unsigned count = 0;
+    int      top   = 1;
+    int      nLeaf = 0;
+    if (lane == 0) { stack[0] = 0; }
+    __builtin_amdgcn_wave_barrier();
+
+    while (top > 0)
+    {
+        int k = (top + 7 * kWave <= kStackCap - 160) ? min(kWave, top) : 1;
+        int node = -1;
+        if (int(lane) < k) { node = stack[top - k + int(lane)]; }
+        top -= k;
+        __builtin_amdgcn_wave_barrier();
+
+        bool ov = false, isLeaf = false;
+        int  child = 0;
+        if (node >= 0)
+        {
+            double d2 = 0.0;
+            for (int d = 0; d < 3; ++d)
+            {
+                double v = a.centers[3 * size_t(node) + d] - gc[d];
+                if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
+                v = fabs(v) - a.sizes[3 * size_t(node) + d] - ge[d];
+                v = v > 0.0 ? v : 0.0;
+                d2 += v * v;
+            }
+            ov     = d2 < reach2;
+            child  = a.childOffsets[node];
+            isLeaf = child == 0;
+        }
+        uint64_t mLeaf = ballot(ov && isLeaf);
+        uint64_t mInt  = ballot(ov && !isLeaf);
+        int      nl    = __popcll(mLeaf);
+        if (nLeaf + nl > kLeafCap)
+        {
+            flushLeaves(a, t, leafBuf, nLeaf, gc, ge, reach, count, myList);
+            nLeaf = 0;
+        }
+        if (ov && isLeaf) { leafBuf[nLeaf + laneRank(mLeaf)] = node; }
+        if (ov && !isLeaf)
+        {
+            int pos = top + 8 * int(laneRank(mInt));
+            for (int c = 0; c < 8; ++c)
+                stack[pos + c] = child + c;
+        }
+        nLeaf += nl;
+        top += 8 * __popcll(mInt);
+        __builtin_amdgcn_wave_barrier();
+    }
+    flushLeaves(a, t, leafBuf, nLeaf, gc, ge, reach, count, myList);
+    return count;
+}
+
+//! updateH (sph/include/sph/kernels.hpp:27-32)
+template<class Th>
+__device__ __forceinline__ Th updateH(unsigned ng0, unsigned nc, Th h)
+{
+#pragma clang fp contract(off)
+    constexpr Th c0 = 1023.0;
+    constexpr Th ex = 1.0 / 10.0;
+    return h * Th(0.5) * pow(Th(1) + c0 * ng0 / Th(nc), ex);
+}
+
+template<class Th>
+__global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArgs a, Th* __restrict__ h)
+{
+    __shared__ int32_t stackMem[kWavesPerBlock][kStackCap];
+    __shared__ int32_t leafMem[kWavesPerBlock][kLeafCap];
+
+    const unsigned wid  = threadIdx.x / kWave;
+    const unsigned lane = laneId();
+    const uint32_t g    = blockIdx.x * kWavesPerBlock + wid;
+    const uint32_t base = a.first + g * kGroup;
+    if (base >= a.last) { return; }
+
+    Target t;
+    t.i      = base + lane;
+    t.active = t.i < a.last;
+    Th hi    = t.active ? h[t.i] : Th(0);
+    t.x      = t.active ? a.x[t.i] : 0.0;
+    t.y      = t.active ? a.y[t.i] : 0.0;
+    t.z      = t.active ? a.z[t.i] : 0.0;
+    {
+#pragma clang fp contract(off)
+        Th r2 = Th(4.0) * hi * hi; // radius in h's type, as the reference
+        t.r2  = double(r2);
+    }
+    uint32_t* myList = a.nbrs + size_t(g) * a.ngmax * kGroup + lane;
+
+    bool     valid = t.active;
+    unsigned count = searchGroup(a, t, double(2) * double(hi), stackMem[wid], leafMem[wid], myList);
+    unsigned ncSph = 1 + count;
+
+    int      updates = 0;
+    unsigned ngmin   = a.ng0 / 4;
+    if (a.iterateH)
+    {
+        while (true)
+        {
+            bool repeat = valid && (ngmin > ncSph || (ncSph - 1) > a.ngmax) && updates < 10;
+            if (!ballot(repeat)) { break; }
+            if (repeat)
+            {
+                updates++;
+                hi = updateH(a.ng0, ncSph, hi);
+                {
+#pragma clang fp contract(off)
+                    Th r2 = Th(4.0) * hi * hi;
+                    t.r2  = double(r2);
+                }
+            }
+            t.active       = repeat;
+            unsigned again = searchGroup(a, t, double(2) * double(hi), stackMem[wid], leafMem[wid], myList);
+            if (repeat) { ncSph = 1 + again; }
+        }
+    }
+    if (valid)
+    {
+        if (updates > 0) { h[t.i] = hi; }
+        a.nc[t.i] = ncSph;
+    }
+
+    unsigned stored = valid ? min(ncSph - 1, a.ngmax) : 0;
+    unsigned sum    = waveSum(stored);
+    unsigned mx     = waveMax(valid ? ncSph : 0u);
+    unsigned fails  = __popcll(ballot(valid && updates >= 10));
+    if (lane == 0)
+    {
+        atomicAdd(&a.stats[0], (unsigned long long)sum);
+        atomicMax(&a.stats[1], (unsigned long long)mx);
+        if (fails) { atomicAdd(&a.stats[2], (unsigned long long)fails); }
+    }
+}
+
+__global__ void copyListsKernel(const uint32_t* __restrict__ nbrs, const uint32_t* __restrict__ nc,
+                                uint32_t* __restrict__ out, uint32_t first, uint32_t last, uint32_t ngmax)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) { return; }
+    uint32_t r = i - first, g = r / kGroup, lane = r % kGroup;
+    uint32_t n = min(nc[i] - 1, ngmax);
+    for (uint32_t k = 0; k < n; ++k)
+        out[size_t(r) * ngmax + k] = nbrs[(size_t(g) * ngmax + k) * kGroup + lane];
+}
+
+} // namespace
+
+int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                  uint32_t first, uint32_t last, bool iterateH)
+{
+    SPHX_REQUIRE(ws->tree.valid, SPHX_ERR_STATE, "find_neighbors called before the tree was built");
+    SPHX_REQUIRE(f->x && f->y && f->z && f->h && f->nc, SPHX_ERR_ARG, "find_neighbors needs x, y, z, h and nc");
+    SPHX_REQUIRE(first <= last && last <= ws->tree.numParticles, SPHX_ERR_ARG, "invalid target range");
+    SPHX_REQUIRE(p->ng0 <= p->ngmax, SPHX_ERR_ARG, "ng0 should be smaller than ngmax");
+    SPHX_REQUIRE(p->ngmax > 0, SPHX_ERR_ARG, "ngmax must be positive");
+
+    const Tree& t       = ws->tree;
+    uint32_t    numTgt  = last - first;
+    uint32_t    groups  = ceilDiv(numTgt, kGroup);
+    ws->nbrs.reserve(std::max<size_t>(1, size_t(groups) * p->ngmax * kGroup));
+    ws->counters.reserve(8);
+    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 3);
+
+    double scale = 0;
+    for (int d = 0; d < 3; ++d)
+        scale = std::max(scale, std::max(std::abs(box.lo[d]), std::abs(box.hi[d])));
+
+    SearchArgs a;
+    a.x = f->x, a.y = f->y, a.z = f->z;
+    a.childOffsets   = t.childOffsets.get();
+    a.internalToLeaf = t.internalToLeaf.get();
+    a.layout         = t.layout.get();
+    a.centers        = t.centers.get();
+    a.sizes          = t.sizes.get();
+    a.box            = box;
+    a.slack          = 1e-12 * scale;
+    a.first = first, a.last = last, a.ng0 = p->ng0, a.ngmax = p->ngmax;
+    a.iterateH = iterateH ? 1 : 0;
+    a.nbrs     = ws->nbrs.get();
+    a.nc       = f->nc;
+    a.stats    = reinterpret_cast<unsigned long long*>(ws->counters.get());
+
+    if (groups > 0)
+    {
+        unsigned blocks = ceilDiv(groups, kWavesPerBlock);
+        if (p->hydro_type == SPHX_F32)
+            searchKernel<float><<<blocks, kWavesPerBlock * kWave, 0, ws->stream>>>(a, static_cast<float*>(f->h));
+        else
+            searchKernel<double><<<blocks, kWavesPerBlock * kWave, 0, ws->stream>>>(a, static_cast<double*>(f->h));
+        SPHX_CHECK_LAUNCH();
+    }
+    unsigned long long stats[3];
+    SPHX_HIP(hipMemcpyAsync(stats, ws->counters.get(), sizeof(stats), hipMemcpyDeviceToHost, ws->stream));
+    SPHX_HIP(hipStreamSynchronize(ws->stream));
+
+    ws->nbFirst      = first;
+    ws->nbLast       = last;
+    ws->nbNgmax      = p->ngmax;
+    ws->nbNc         = f->nc;
+    ws->nbValid      = true;
+    ws->statSumPairs = stats[0];
+    ws->statMaxCount = uint32_t(stats[1]);
+    return stats[2] > 0 ? SPHX_ERR_NOT_CONVERGED : SPHX_OK;
+}
+
+void copyNeighbors(sphx_workspace* ws, uint32_t* out, uint32_t first, uint32_t last)
+{
+    SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "no neighbour lists: call find_neighbors first");
+    SPHX_REQUIRE(first == ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
+                 "copy range must start at the first searched target and stay inside the searched range");
+    SPHX_REQUIRE(out != nullptr, SPHX_ERR_ARG, "null output");
+    if (last == first) { return; }
+    size_t             len = size_t(last - first) * ws->nbNgmax;
+    hipPointerAttribute_t attr{};
+    bool               onDevice = hipPointerGetAttributes(&attr, out) == hipSuccess && attr.type == hipMemoryTypeDevice;
+    (void)hipGetLastError();
+    uint32_t* dst = out;
+    DevBuf<uint32_t> staging;
+    if (!onDevice)
+    {
+        staging.reserve(len);
+        dst = staging.get();
+    }
+    copyListsKernel<<<ceilDiv(last - first, 256), 256, 0, ws->stream>>>(ws->nbrs.get(), ws->nbNc, dst, first, last,
+                                                                          ws->nbNgmax);
+    SPHX_CHECK_LAUNCH();
+    if (!onDevice)
+    {
+        SPHX_HIP(hipMemcpyAsync(out, dst, len * sizeof(uint32_t), hipMemcpyDeviceToHost, ws->stream));
+        SPHX_HIP(hipStreamSynchronize(ws->stream));
+        staging.release();
+    }
+}
+
+} // namespace sphx

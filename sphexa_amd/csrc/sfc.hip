@@ -1,0 +1,125 @@
+// Hilbert keys, stable key sort and field permutation (the one-rank Domain::sync front end).
+#include <cstring> // rocprim uses memset without including it
+#include <rocprim/rocprim.hpp>
+
+#include "common.hpp"
+
+namespace sphx
+{
+
+/* Hilbert key of one particle. Integer coordinates follow sfc3D (domain/include/cstone/sfc/
+ * sfc.hpp:142-178): i = int(floor(x*m) - xmin*m), clamped to 2^21-1, with m = 2^21/L, evaluated
+ * without contraction so the keys equal the CPU's. The curve is walked from the top level: the
+ * Morton octant o = 4x+2y+z of each level maps to the Hilbert digit o ^ (o >> 1) ^ (o >> 2)
+ * (prefix parity: 0,1,3,2,7,6,4,5), after which the lower coordinate bits are reflected and
+ * permuted into the child's frame. */
+__device__ __forceinline__ Key hilbertKeyOf(double x, double y, double z, const DevBox& b)
+{
+#pragma clang fp contract(off)
+    const double cube = double(1u << kMaxLevel);
+    double       mx = cube * b.ilen[0], my = cube * b.ilen[1], mz = cube * b.ilen[2];
+    int          ix = int(floor(x * mx) - b.lo[0] * mx);
+    int          iy = int(floor(y * my) - b.lo[1] * my);
+    int          iz = int(floor(z * mz) - b.lo[2] * mz);
+    const int    mc = (1 << kMaxLevel) - 1;
+    unsigned     px = unsigned(min(ix, mc)), py = unsigned(min(iy, mc)), pz = unsigned(min(iz, mc));
+
+    Key key = 0;
+#pragma unroll
+    for (int level = int(kMaxLevel) - 1; level >= 0; --level)
+    {
+        unsigned bx = (px >> level) & 1u, by = (py >> level) & 1u, bz = (pz >> level) & 1u;
+        unsigned o  = (bx << 2) | (by << 1) | bz;
+        key         = (key << 3) | Key(o ^ (o >> 1) ^ (o >> 2));
+
+        unsigned nby = by ^ 1u, nbz = bz ^ 1u;
+        px ^= 0u - (bx & (nby | bz));
+        py ^= 0u - ((bx & (by | bz)) | (by & nbz));
+        pz ^= 0u - ((bx & nby & nbz) | (by & nbz));
+        unsigned tx = px, ty = py, tz = pz;
+        // bz: rotate (x,y,z) <- (y,z,x); else !by: exchange x and z
+        px = bz ? ty : (nby ? tz : tx);
+        py = bz ? tz : ty;
+        pz = bz ? tx : (nby ? tx : tz);
+    }
+    return key;
+}
+
+__global__ void __launch_bounds__(256) keysKernel(DevBox box, const double* __restrict__ x,
+                                                  const double* __restrict__ y, const double* __restrict__ z,
+                                                  Key* __restrict__ keys, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    constexpr Key removeKey = Key(1) << 63;
+    if (keys[i] != removeKey) { keys[i] = hilbertKeyOf(x[i], y[i], z[i], box); }
+}
+
+void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
+                 size_t n)
+{
+    if (n == 0) { return; }
+    keysKernel<<<ceilDiv(n, 256), 256, 0, ws->stream>>>(box, x, y, z, keys, n);
+    SPHX_CHECK_LAUNCH();
+}
+
+__global__ void fill64Kernel(uint64_t* p, uint64_t v, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) { p[i] = v; }
+}
+
+//! device fill (a kernel rather than hipMemsetAsync, so callers may capture it in a graph)
+void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n)
+{
+    if (n == 0) { return; }
+    fill64Kernel<<<ceilDiv(n, 256), 256, 0, ws->stream>>>(p, v, n);
+    SPHX_CHECK_LAUNCH();
+}
+
+__global__ void iotaKernel(uint32_t* v, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) { v[i] = uint32_t(i); }
+}
+
+//! Stable LSD radix sort of (key, original index) over all 64 key bits (removeKey = 2^63 sorts
+//! last), i.e. the order of std::stable_sort on the keys.
+void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n)
+{
+    if (n == 0) { return; }
+    ws->keyTmp.reserve(n);
+    ws->idxTmp.reserve(n);
+    iotaKernel<<<ceilDiv(n, 256), 256, 0, ws->stream>>>(ws->idxTmp.get(), n);
+    SPHX_CHECK_LAUNCH();
+
+    size_t tmpBytes = 0;
+    SPHX_HIP(rocprim::radix_sort_pairs(nullptr, tmpBytes, keys, ws->keyTmp.get(), ws->idxTmp.get(), order, n, 0, 64,
+                                       ws->stream));
+    void* tmp = scratchBytes(ws, tmpBytes);
+    SPHX_HIP(rocprim::radix_sort_pairs(tmp, tmpBytes, keys, ws->keyTmp.get(), ws->idxTmp.get(), order, n, 0, 64,
+                                       ws->stream));
+    SPHX_HIP(hipMemcpyAsync(keys, ws->keyTmp.get(), n * sizeof(Key), hipMemcpyDeviceToDevice, ws->stream));
+}
+
+template<class E>
+__global__ void __launch_bounds__(256) gatherKernel(const uint32_t* __restrict__ order, const E* __restrict__ in,
+                                                    E* __restrict__ out, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) { out[i] = in[order[i]]; }
+}
+
+void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes)
+{
+    if (n == 0) { return; }
+    if (elemBytes == 8)
+        gatherKernel<uint64_t><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(order, (const uint64_t*)in, (uint64_t*)out, n);
+    else if (elemBytes == 4)
+        gatherKernel<uint32_t><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(order, (const uint32_t*)in, (uint32_t*)out, n);
+    else
+        throw Error{SPHX_ERR_ARG, "gather: element size must be 4 or 8 bytes"};
+    SPHX_CHECK_LAUNCH();
+}
+
+} // namespace sphx

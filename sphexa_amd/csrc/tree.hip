@@ -1,0 +1,277 @@
+// Cornerstone leaf array + linked octree + layout + geometric node centres, built top-down.
+//
+// The reference converges a leaf array by repeated count/split/merge sweeps (computeOctree,
+// domain/include/cstone/tree/csarray.hpp:409-440) and then derives the internal tree by sorting
+// placeholder-bit prefixes (buildOctreeGpu, tree/octree_gpu.cu:137-157). Its fixed point is
+// unique: a node is internal iff it holds more than `bucket` particles and lies above the
+// deepest level. Descending level by level from the root and splitting exactly those nodes
+// produces that same tree, and because children are appended in key order behind their
+// level-ordered parents, the node array comes out directly in the reference's sorted
+// prefix order (level-major, SFC-minor). One host read-back per level sizes the next one.
+#include <cstring> // rocprim uses memset without including it
+#include <rocprim/rocprim.hpp>
+
+#include "common.hpp"
+
+namespace sphx
+{
+
+namespace
+{
+
+__device__ __forceinline__ uint32_t lowerBound(const Key* keys, uint32_t lo, uint32_t hi, Key v)
+{
+    while (lo < hi)
+    {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void initRoot(Key* nodeKey, uint32_t* lo, uint32_t* hi, uint8_t* level, uint32_t n)
+{
+    nodeKey[0] = 0;
+    lo[0]      = 0;
+    hi[0]      = n;
+    level[0]   = 0;
+}
+
+__global__ void classifyKernel(const uint32_t* lo, const uint32_t* hi, uint32_t* flag, uint32_t begin, uint32_t end,
+                               unsigned bucket, unsigned level)
+{
+    uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) { return; }
+    flag[i] = (hi[i] - lo[i] > bucket) && (level < kMaxLevel);
+}
+
+//! each internal node of the level writes its 8 children behind the level
+__global__ void expandKernel(const Key* __restrict__ keys, Key* nodeKey, uint32_t* lo, uint32_t* hi, uint8_t* nodeLevel,
+                             const uint32_t* flag, const uint32_t* inclRank, uint32_t internalBefore,
+                             uint32_t begin, uint32_t end, unsigned level)
+{
+    uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end || !flag[i]) { return; }
+    uint32_t r     = inclRank[i] - 1 - internalBefore; // rank among this level's internal nodes
+    uint32_t c0    = end + 8 * r;
+    Key      k     = nodeKey[i];
+    Key      range = nodeRange(level + 1);
+    uint32_t a = lo[i], b = hi[i];
+    uint32_t prev = a;
+    for (int c = 0; c < 8; ++c)
+    {
+        uint32_t next     = (c == 7) ? b : lowerBound(keys, prev, b, k + Key(c + 1) * range);
+        nodeKey[c0 + c]   = k + Key(c) * range;
+        lo[c0 + c]        = prev;
+        hi[c0 + c]        = next;
+        nodeLevel[c0 + c] = uint8_t(level + 1);
+        prev              = next;
+    }
+}
+
+//! prefixes, child links, parents, centres; leaves are collected for the key sort
+__global__ void linkKernel(const Key* nodeKey, const uint8_t* nodeLevel, const uint32_t* flag, const uint32_t* inclRank,
+                           const int32_t* levelStart, uint32_t numNodes, DevBox box, Key* prefixes,
+                           TreeNodeIndex* childOffsets, TreeNodeIndex* parents, TreeNodeIndex* internalToLeaf,
+                           double* centers, double* sizes, Key* leafKey, uint32_t* leafNode, const uint32_t* leafRank)
+{
+#pragma clang fp contract(off)
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= numNodes) { return; }
+    Key      k = nodeKey[i];
+    unsigned l = nodeLevel[i];
+    prefixes[i] = encodePlaceholder(k, 3 * l);
+
+    if (flag[i])
+    {
+        // ranks of internal nodes are global (inclusive scan over all nodes); the level's
+        // first internal rank is the count of internal nodes on shallower levels
+        uint32_t rankInLevel = inclRank[i] - 1 - (levelStart[l] == 0 ? 0 : inclRank[levelStart[l] - 1]);
+        TreeNodeIndex child  = levelStart[l + 1] + 8 * TreeNodeIndex(rankInLevel);
+        childOffsets[i]      = child;
+        parents[(child - 1) / 8] = TreeNodeIndex(i);
+        internalToLeaf[i]    = -1;
+    }
+    else
+    {
+        childOffsets[i]   = 0;
+        uint32_t s        = leafRank[i];
+        leafKey[s]        = k;
+        leafNode[s]       = i;
+    }
+
+    // geometric centre / half size: hilbertIBox + centerAndSize (sfc/hilbert.hpp:259-274,
+    // sfc/box.hpp:318-334)
+    unsigned ix, iy, iz;
+    hilbertDecode(k, ix, iy, iz);
+    unsigned     cube = (1u << kMaxLevel) >> l;
+    unsigned     mask = ~(cube - 1);
+    unsigned     ic[3] = {ix & mask, iy & mask, iz & mask};
+    const double uL    = 1.0 / double(1u << kMaxLevel);
+    for (int d = 0; d < 3; ++d)
+    {
+        int    imin = int(ic[d]), imax = int(ic[d] + cube);
+        double half = 0.5 * uL * box.len[d];
+        centers[3 * size_t(i) + d] = box.lo[d] + double(imax + imin) * half;
+        sizes[3 * size_t(i) + d]   = double(imax - imin) * half;
+    }
+}
+
+__global__ void leafFlagKernel(const uint32_t* flag, uint32_t* leafFlag, uint32_t numNodes)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numNodes) { leafFlag[i] = 1u - flag[i]; }
+}
+
+__global__ void exclusiveFromInclusive(const uint32_t* incl, const uint32_t* v, uint32_t* excl, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { excl[i] = incl[i] - v[i]; }
+}
+
+__global__ void leafFinalize(const Key* sortedLeafKey, const uint32_t* sortedLeafNode, const uint32_t* lo,
+                             const uint32_t* hi, Key* leaves, uint32_t* counts, uint32_t* layout,
+                             TreeNodeIndex* internalToLeaf, uint32_t L, uint32_t n)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > L) { return; }
+    if (s == L)
+    {
+        leaves[L] = nodeRange(0);
+        layout[L] = n;
+        return;
+    }
+    uint32_t node        = sortedLeafNode[s];
+    leaves[s]            = sortedLeafKey[s];
+    counts[s]            = hi[node] - lo[node];
+    layout[s]            = lo[node];
+    internalToLeaf[node] = TreeNodeIndex(s);
+}
+
+template<class T>
+void growCopy(sphx_workspace* ws, DevBuf<T>& buf, size_t used, size_t need)
+{
+    if (need <= buf.cap) { return; }
+    DevBuf<T> nb;
+    nb.reserve(std::max(need, 2 * buf.cap));
+    if (used) SPHX_HIP(hipMemcpyAsync(nb.ptr, buf.ptr, used * sizeof(T), hipMemcpyDeviceToDevice, ws->stream));
+    SPHX_HIP(hipStreamSynchronize(ws->stream));
+    buf.release();
+    buf = nb;
+}
+
+} // namespace
+
+void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned bucket, const DevBox& box)
+{
+    SPHX_REQUIRE(nParticles < (size_t(1) << 32), SPHX_ERR_ARG, "particle count must fit 32-bit indices");
+    SPHX_REQUIRE(bucket >= 1, SPHX_ERR_ARG, "bucket size must be positive");
+    hipStream_t  st = ws->stream;
+    uint32_t     n  = uint32_t(nParticles);
+    auto&        s   = ws->tree.build;
+    size_t       cap = std::max<size_t>(4096, 16 * (nParticles / (bucket + 1) + 1));
+    s.nodeKey.reserve(cap), s.lo.reserve(cap), s.hi.reserve(cap), s.flag.reserve(cap), s.rank.reserve(cap);
+    s.level.reserve(cap);
+
+    initRoot<<<1, 1, 0, st>>>(s.nodeKey.get(), s.lo.get(), s.hi.get(), s.level.get(), n);
+    SPHX_CHECK_LAUNCH();
+
+    std::vector<int32_t> levelStart(kMaxLevel + 3, 0);
+    uint32_t             begin = 0, end = 1;
+    unsigned             level = 0;
+    uint32_t             internalBefore = 0;
+    size_t               scanBytes      = 0;
+    while (true)
+    {
+        levelStart[level] = int32_t(begin);
+        uint32_t cnt      = end - begin;
+        classifyKernel<<<ceilDiv(cnt, 256), 256, 0, st>>>(s.lo.get(), s.hi.get(), s.flag.get(), begin, end, bucket,
+                                                           level);
+        SPHX_CHECK_LAUNCH();
+        // global inclusive scan of flags up to `end` (so ranks are tree-global)
+        SPHX_HIP(rocprim::inclusive_scan(nullptr, scanBytes, s.flag.get(), s.rank.get(), end, rocprim::plus<uint32_t>(),
+                                         st));
+        void* tmp = scratchBytes(ws, scanBytes);
+        SPHX_HIP(rocprim::inclusive_scan(tmp, scanBytes, s.flag.get(), s.rank.get(), end, rocprim::plus<uint32_t>(),
+                                         st));
+        uint32_t totalInternal = 0;
+        SPHX_HIP(hipMemcpyAsync(&totalInternal, s.rank.get() + end - 1, 4, hipMemcpyDeviceToHost, st));
+        SPHX_HIP(hipStreamSynchronize(st));
+        uint32_t numInternalHere = totalInternal - internalBefore;
+        if (numInternalHere == 0) { break; }
+
+        size_t need = size_t(end) + 8 * size_t(numInternalHere);
+        growCopy(ws, s.nodeKey, end, need), growCopy(ws, s.lo, end, need), growCopy(ws, s.hi, end, need);
+        growCopy(ws, s.flag, end, need), growCopy(ws, s.rank, end, need), growCopy(ws, s.level, end, need);
+
+        expandKernel<<<ceilDiv(cnt, 256), 256, 0, st>>>(keys, s.nodeKey.get(), s.lo.get(), s.hi.get(), s.level.get(),
+                                                        s.flag.get(), s.rank.get(), internalBefore, begin, end,
+                                                        level);
+        SPHX_CHECK_LAUNCH();
+        internalBefore = totalInternal;
+        begin          = end;
+        end            = uint32_t(need);
+        level++;
+    }
+    for (unsigned l = level + 1; l <= kMaxLevel + 2; ++l)
+        levelStart[l] = int32_t(end);
+
+    uint32_t numNodes    = end;
+    uint32_t numInternal = internalBefore;
+    uint32_t L           = numNodes - numInternal;
+
+    Tree& t = ws->tree;
+    t.prefixes.reserve(numNodes), t.childOffsets.reserve(numNodes + 1), t.internalToLeaf.reserve(numNodes);
+    t.parents.reserve(std::max<uint32_t>(1, (numNodes - 1) / 8)), t.levelRange.reserve(kMaxLevel + 2);
+    t.centers.reserve(3 * size_t(numNodes)), t.sizes.reserve(3 * size_t(numNodes));
+    t.leaves.reserve(L + 1), t.counts.reserve(L), t.layout.reserve(L + 1);
+
+    DevBuf<int32_t>& dLevelStart = s.levelStart;
+    dLevelStart.reserve(kMaxLevel + 3);
+    SPHX_HIP(hipMemcpyAsync(dLevelStart.get(), levelStart.data(), (kMaxLevel + 3) * 4, hipMemcpyHostToDevice, st));
+
+    // leaf ranks in level order, then sort leaves by key
+    s.leafFlag.reserve(numNodes), s.leafRank.reserve(numNodes);
+    s.leafKey.reserve(L), s.leafNode.reserve(L), s.leafKeySorted.reserve(L), s.leafNodeSorted.reserve(L);
+    leafFlagKernel<<<ceilDiv(numNodes, 256), 256, 0, st>>>(s.flag.get(), s.leafFlag.get(), numNodes);
+    SPHX_CHECK_LAUNCH();
+    SPHX_HIP(rocprim::inclusive_scan(nullptr, scanBytes, s.leafFlag.get(), s.leafRank.get(), numNodes,
+                                     rocprim::plus<uint32_t>(), st));
+    SPHX_HIP(rocprim::inclusive_scan(scratchBytes(ws, scanBytes), scanBytes, s.leafFlag.get(), s.leafRank.get(),
+                                     numNodes, rocprim::plus<uint32_t>(), st));
+    exclusiveFromInclusive<<<ceilDiv(numNodes, 256), 256, 0, st>>>(s.leafRank.get(), s.leafFlag.get(),
+                                                                    s.leafRank.get(), numNodes);
+    SPHX_CHECK_LAUNCH();
+
+    linkKernel<<<ceilDiv(numNodes, 256), 256, 0, st>>>(
+        s.nodeKey.get(), s.level.get(), s.flag.get(), s.rank.get(), dLevelStart.get(), numNodes, box, t.prefixes.get(),
+        t.childOffsets.get(), t.parents.get(), t.internalToLeaf.get(), t.centers.get(), t.sizes.get(), s.leafKey.get(),
+        s.leafNode.get(), s.leafRank.get());
+    SPHX_CHECK_LAUNCH();
+
+    size_t sortBytes = 0;
+    SPHX_HIP(rocprim::radix_sort_pairs(nullptr, sortBytes, s.leafKey.get(), s.leafKeySorted.get(), s.leafNode.get(),
+                                       s.leafNodeSorted.get(), L, 0, 64, st));
+    SPHX_HIP(rocprim::radix_sort_pairs(scratchBytes(ws, sortBytes), sortBytes, s.leafKey.get(), s.leafKeySorted.get(),
+                                       s.leafNode.get(), s.leafNodeSorted.get(), L, 0, 64, st));
+    leafFinalize<<<ceilDiv(L + 1, 256), 256, 0, st>>>(s.leafKeySorted.get(), s.leafNodeSorted.get(), s.lo.get(),
+                                                      s.hi.get(), t.leaves.get(), t.counts.get(), t.layout.get(),
+                                                      t.internalToLeaf.get(), L, n);
+    SPHX_CHECK_LAUNCH();
+
+    std::vector<int32_t> levelRange(kMaxLevel + 2);
+    for (unsigned l = 0; l <= kMaxLevel; ++l)
+        levelRange[l] = levelStart[l];
+    levelRange[kMaxLevel + 1] = int32_t(numNodes);
+    SPHX_HIP(hipMemcpyAsync(t.levelRange.get(), levelRange.data(), (kMaxLevel + 2) * 4, hipMemcpyHostToDevice, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+
+    t.numLeafNodes     = int32_t(L);
+    t.numInternalNodes = int32_t(numInternal);
+    t.numNodes         = int32_t(numNodes);
+    t.numParticles     = nParticles;
+    t.valid            = true;
+}
+
+} // namespace sphx

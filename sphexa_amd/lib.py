@@ -1,0 +1,164 @@
+"""ctypes binding of libsphexa_amd.so (the C ABI in include/sphexa_amd.h).
+
+torch provides device memory and the stream; every computation runs in the HIP library.
+Loading fails loudly if the library has not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsphexa_amd.so")
+
+_lib = None
+
+_P = C.c_void_p
+_BOXP = C.POINTER(abi.Box)
+_FP = C.POINTER(abi.Fields)
+_PP = C.POINTER(abi.Params)
+_U32 = C.c_uint32
+_SZ = C.c_size_t
+
+_SIGS = {
+    "sphx_workspace_create": [C.POINTER(_P), C.c_int, _P],
+    "sphx_workspace_destroy": [_P],
+    "sphx_workspace_set_kernel": [_P, C.c_double, C.POINTER(C.c_double)],
+    "sphx_compute_sfc_keys": [_P, _BOXP, _P, _P, _P, _P, _SZ],
+    "sphx_sort_keys": [_P, _P, _P, _SZ],
+    "sphx_gather": [_P, _P, _P, _P, _SZ, C.c_int],
+    "sphx_build_tree": [_P, _P, _SZ, _U32, _BOXP],
+    "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
+    "sphx_domain_sync": [_P, _BOXP, _FP, _PP, _SZ, _U32],
+    "sphx_find_neighbors": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int],
+    "sphx_copy_neighbors": [_P, _P, _U32, _U32],
+    "sphx_neighbor_stats": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "sphx_density_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_eos_std": [_P, _FP, _PP, _U32, _U32],
+    "sphx_iad_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_momentum_energy_std": [_P, _BOXP, _FP, _PP, _U32, _U32, C.POINTER(C.c_double)],
+    "sphx_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_ve_def_gradh": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_eos_ve": [_P, _FP, _PP, _U32, _U32],
+    "sphx_iad_divv_curlv": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_av_switches": [_P, _BOXP, _FP, _PP, _U32, _U32],
+    "sphx_momentum_energy_ve": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.POINTER(C.c_double)],
+}
+
+EXPORTED = tuple(_SIGS) + ("sphx_last_error",)
+
+
+class SphxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sphexa_amd error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load the HIP library (raises if it is missing: no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run `python -m sphexa_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.sphx_last_error.argtypes = [_P]
+    lib.sphx_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+class Workspace:
+    """Owns a sphx_workspace on one device, bound to a HIP stream (default: torch's current)."""
+
+    def __init__(self, device=0, stream=None, sinc_index=6.0):
+        import torch
+
+        self.lib = load()
+        self.device = device
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        self.stream = stream
+        h = _P()
+        self._check(self.lib.sphx_workspace_create(C.byref(h), device, _P(stream)), None)
+        self.handle = h
+        K = C.c_double()
+        self.call("sphx_workspace_set_kernel", sinc_index, C.byref(K))
+        self.K = K.value
+
+    def _check(self, code, handle):
+        if code != abi.SPHX_OK:
+            msg = self.lib.sphx_last_error(handle).decode() if handle else "workspace creation failed"
+            raise SphxError(code, msg)
+
+    def call(self, name, *args, allow=()):
+        code = getattr(self.lib, name)(self.handle, *args)
+        if code != abi.SPHX_OK and code not in allow:
+            self._check(code, self.handle)
+        return code
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.sphx_workspace_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- thin wrappers -----------------------------------------------------------------------
+    def sfc_keys(self, box, x, y, z, keys):
+        self.call("sphx_compute_sfc_keys", C.byref(box), x.data_ptr(), y.data_ptr(), z.data_ptr(), keys.data_ptr(),
+                  x.numel())
+
+    def sort_keys(self, keys, order):
+        self.call("sphx_sort_keys", keys.data_ptr(), order.data_ptr(), keys.numel())
+
+    def build_tree(self, sorted_keys, box, bucket=64):
+        self.call("sphx_build_tree", sorted_keys.data_ptr(), sorted_keys.numel(), bucket, C.byref(box))
+
+    def tree_view(self):
+        tv = abi.TreeView()
+        self.call("sphx_tree_view", C.byref(tv))
+        return tv
+
+    def domain_sync(self, box, fields, params, n, bucket=64):
+        self.call("sphx_domain_sync", C.byref(box), C.byref(fields), C.byref(params), n, bucket)
+
+    def find_neighbors(self, box, fields, params, first, last, iterate_h=True):
+        """Returns True when every particle's h/nc iteration converged."""
+        code = self.call("sphx_find_neighbors", C.byref(box), C.byref(fields), C.byref(params), first, last,
+                         int(iterate_h), allow=(abi.SPHX_ERR_NOT_CONVERGED,))
+        return code == abi.SPHX_OK
+
+    def copy_neighbors(self, out, first, last):
+        self.call("sphx_copy_neighbors", out.data_ptr(), first, last)
+
+    def neighbor_stats(self):
+        s, m = C.c_uint64(), C.c_uint32()
+        self.call("sphx_neighbor_stats", C.byref(s), C.byref(m))
+        return s.value, m.value
+
+    def pair(self, name, box, fields, params, first, last):
+        self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last)
+
+    def eos(self, name, fields, params, first, last):
+        self.call(name, C.byref(fields), C.byref(params), first, last)
+
+    def momentum_energy_std(self, box, fields, params, first, last):
+        dt = C.c_double()
+        self.call("sphx_momentum_energy_std", C.byref(box), C.byref(fields), C.byref(params), first, last,
+                  C.byref(dt))
+        return dt.value
+
+    def momentum_energy_ve(self, box, fields, params, first, last, av_clean=False):
+        dt = C.c_double()
+        self.call("sphx_momentum_energy_ve", C.byref(box), C.byref(fields), C.byref(params), first, last,
+                  int(av_clean), C.byref(dt))
+        return dt.value

@@ -1,0 +1,103 @@
+"""Device particle set named after sphexa::ParticlesData, and the hydro step drivers.
+
+ParticleSet holds one torch tensor per field on the GPU (SoA, SFC order after the domain
+sync): x, y, z, u, temp, du in float64 (RealType); the HydroType fields in float64 for the
+plain form or float32 for the mixed form (the reference's SphTypes layout,
+sph/include/sph/types.hpp:39-46); keys uint64; nc uint32.
+
+The drivers issue the phases in the order of the reference propagators:
+  std: HydroProp::computeForces   (main/src/propagator/std_hydro.hpp:124-168)
+  VE:  HydroVeProp::computeForces (main/src/propagator/ve_hydro.hpp:130-204)
+with the one-rank Domain::sync in front and no halo exchanges.
+"""
+import numpy as np
+
+from . import abi
+
+STD_FIELDS = ("x", "y", "z", "u", "du", "h", "m", "vx", "vy", "vz", "rho", "p", "c", "c11", "c12", "c13", "c22",
+              "c23", "c33", "ax", "ay", "az", "keys", "nc")
+VE_FIELDS = ("x", "y", "z", "temp", "du", "h", "m", "vx", "vy", "vz", "prho", "c", "c11", "c12", "c13", "c22",
+             "c23", "c33", "xm", "kx", "gradh", "divv", "curlv", "alpha", "ax", "ay", "az", "keys", "nc")
+GRADV_FIELDS = ("dV11", "dV12", "dV13", "dV22", "dV23", "dV33")
+
+IDEAL_GAS_R = 8.317e7
+
+
+def ideal_gas_cv(mui, gamma):
+    """idealGasCv (sph/include/sph/eos.hpp:19-23)."""
+    return IDEAL_GAS_R / mui / (gamma - 1.0)
+
+
+class ParticleSet:
+    def __init__(self, n, names, mixed=False, device="cuda"):
+        import torch
+
+        self.n = n
+        self.mixed = mixed
+        self.device = device
+        self.hydro_dtype = torch.float32 if mixed else torch.float64
+        self.t = {}
+        for name in names:
+            if name in abi.REAL_FIELDS:
+                dt = torch.float64
+            elif name == "keys":
+                dt = torch.int64  # uint64 bit pattern
+            elif name == "nc":
+                dt = torch.int32  # uint32 bit pattern
+            else:
+                dt = self.hydro_dtype
+            self.t[name] = torch.zeros(max(n, 1), dtype=dt, device=device)
+
+    @classmethod
+    def from_host(cls, arrays, names, mixed=False, device="cuda"):
+        import torch
+
+        n = len(arrays["x"])
+        ps = cls(n, names, mixed, device)
+        for name, a in arrays.items():
+            if name in ps.t and a is not None:
+                ps.t[name][:n] = torch.from_numpy(np.ascontiguousarray(a)).to(ps.t[name].dtype).to(device)
+        return ps
+
+    @property
+    def hydro_type(self):
+        return abi.SPHX_F32 if self.mixed else abi.SPHX_F64
+
+    def fields(self, exclude=()):
+        return abi.fields_from({k: v for k, v in self.t.items() if k not in exclude})
+
+    def host(self, name):
+        a = self.t[name][: self.n].cpu().numpy()
+        if name == "keys":
+            return a.view(np.uint64)
+        if name == "nc":
+            return a.view(np.uint32)
+        return a
+
+
+def std_step(ws, ps, box, params, bucket=64):
+    """One std-SPH pass; returns (converged, minDtCourant)."""
+    n = ps.n
+    f = ps.fields()
+    ws.domain_sync(box, f, params, n, bucket)
+    converged = ws.find_neighbors(box, f, params, 0, n, True)
+    ws.pair("sphx_density_std", box, f, params, 0, n)
+    ws.eos("sphx_eos_std", f, params, 0, n)
+    ws.pair("sphx_iad_std", box, f, params, 0, n)
+    dt = ws.momentum_energy_std(box, f, params, 0, n)
+    return converged, dt
+
+
+def ve_step(ws, ps, box, params, bucket=64, av_clean=False):
+    """One VE pass; returns (converged, minDtCourant)."""
+    n = ps.n
+    f = ps.fields()
+    ws.domain_sync(box, f, params, n, bucket)
+    converged = ws.find_neighbors(box, f, params, 0, n, True)
+    ws.pair("sphx_xmass", box, f, params, 0, n)
+    ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+    ws.eos("sphx_eos_ve", f, params, 0, n)
+    ws.pair("sphx_iad_divv_curlv", box, f, params, 0, n)
+    ws.pair("sphx_av_switches", box, f, params, 0, n)
+    dt = ws.momentum_energy_ve(box, f, params, 0, n, av_clean)
+    return converged, dt
