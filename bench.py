@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH hydro hot path on one MI355X.
+
+Workload (the headline configuration): subsonic turbulence box, VE formulation, 200^3 = 8,000,000
+jittered-lattice particles in the periodic unit box, rho = 1, u = 1 (temperature-based EOS),
+seeded solenoidal velocity field P(k) ~ k^-4 on 1 <= |k| <= 2 at rms Mach 0.3, ng0 = 100.
+
+One step = one full pass with all fields resident in HBM:
+  keys -> stable radix sort + permutation of every field -> cornerstone tree (bucket 64) ->
+  neighbour search with the h/nc iteration -> XMass -> VeDefGradh -> EOS -> IAD + div/curl v ->
+  AV switches -> momentum-energy + Courant time step.
+It runs in the mixed form (the headline value) and in the plain FP64 form; the JSON line reports
+particle-updates/s = N * steps / T for the mixed form, the FP64 value beside it, the roofline of
+the dominant kernel from HIP events on the launch stream, and the reference's own OpenMP CPU path
+(oracle/_ref) timed on a bounded sample of the same workload on this host.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--side 200]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SPH particle-updates/s (nbr search+density+IAD+force), mixed vs FP64, 1 GPU"
+
+# MI355X published peaks
+PEAK_FP64_VECTOR_TFLOPS = 78.6
+PEAK_FP32_VECTOR_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic flops per neighbour pair of the VE momentum-energy loop
+# (sph/include/sph/hydro_ve/momentum_energy_kern.hpp:121-212, avClean = false), counting
+# + - * as 1, division and square root as 1, each table lookup as 7 (index, two loads'
+# difference, slope, offset, fma): separation 3, fold/r2/dist 6, v_ij 3, 1/h_j 1, v1 v2 2,
+# h_j^-3 2, W_i W_j 2+14, IAD projections 2*(3*5+1) = 32, rho_j 2, rv 5, w_ij 1, viscosity 9,
+# signal 4, Atwood 4 + branch 2, a_mom 2, AV terms 2+2+3*4 = 16, AV energy 5, energy 7,
+# momentum 3*6 = 18, accumulations 4 -> 142.
+FLOPS_PER_PAIR_MOM_VE = 142
+# per-target part of the same kernel (loads aside): rho_i, 1/h_i, h_i^-3, eta_crit, epilogue
+FLOPS_PER_TARGET_MOM_VE = 40
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[1])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side", type=int, default=200, help="lattice side (N = side^3)")
+    ap.add_argument("--cpu-side", type=int, default=100, help="lattice side of the CPU-baseline sample")
+    ap.add_argument("--cpu-passes", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--forms", default="mixed,fp64")
+    return ap.parse_args()
+
+
+def make_ic(side, seed=42):
+    from sphexa_amd import ic
+    from sphexa_amd.particles import ideal_gas_cv
+
+    arr, spec = ic.turbulence_box(side, seed=seed)
+    arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
+    return arr, spec
+
+
+PHASES = ("domain::sync", "FindNeighbors", "XMass", "Normalization & Gradh", "EquationOfState",
+          "IadVelocityDivCurl", "AVswitches", "MomentumAndEnergy")
+
+
+def run_gpu(arr, spec, mixed, steps, warmup):
+    import torch
+
+    from sphexa_amd import abi
+    from sphexa_amd.lib import Workspace
+    from sphexa_amd.particles import VE_FIELDS, ParticleSet
+
+    ws = Workspace(0)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-4)
+    ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+    n = ps.n
+    f = ps.fields()
+    stream = torch.cuda.current_stream()
+
+    def one_step(ev):
+        ev[0].record(stream)
+        ws.domain_sync(box, f, params, n, 64)
+        ev[1].record(stream)
+        ws.find_neighbors(box, f, params, 0, n, True)
+        ev[2].record(stream)
+        ws.pair("sphx_xmass", box, f, params, 0, n)
+        ev[3].record(stream)
+        ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+        ev[4].record(stream)
+        ws.eos("sphx_eos_ve", f, params, 0, n)
+        ev[5].record(stream)
+        ws.pair("sphx_iad_divv_curlv", box, f, params, 0, n)
+        ev[6].record(stream)
+        ws.pair("sphx_av_switches", box, f, params, 0, n)
+        ev[7].record(stream)
+        dt = ws.momentum_energy_ve(box, f, params, 0, n, False)
+        ev[8].record(stream)
+        return dt
+
+    mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(9)]
+    for _ in range(warmup):
+        one_step(mk())
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev = mk()
+        one_step(ev)
+        events.append(ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    phase_ms = {p: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in events])) for i, p in enumerate(PHASES)}
+    pairs, maxnc = ws.neighbor_stats()
+    out = dict(n=n, wall_s=wall, ms_per_step=1e3 * wall / steps, phase_ms=phase_ms, pairs=int(pairs),
+               max_nc=int(maxnc), rate=n * steps / wall)
+    del ps, f
+    ws.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(side, passes):
+    from oracle import oracle as O
+    from oracle import pipeline
+    from sphexa_amd import abi
+
+    kind = "reference" if O.available("ref") else "port"
+    lib = O.load("ref" if kind == "reference" else "orc")
+    arr, spec = make_ic(side, seed=7)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    params = abi.Params.defaults(lib.kernel_K(6.0), minDt=1e-4)
+    times = []
+    for _ in range(passes):
+        t0 = time.perf_counter()
+        pipeline.run(lib, arr, box, params, form="ve")
+        times.append(time.perf_counter() - t0)
+    n = side ** 3
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n / float(np.median(times)), unit="particle-updates/s", cores=threads, kind=kind,
+                sample=f"VE FP64 pass (keys, sort, tree, findNeighbors+h-iteration, XMass, VeDefGradh, EOS, "
+                       f"IAD+divv/curlv, AVswitches, momentum-energy) of the same turbulence workload at "
+                       f"{side}^3 = {n} particles, median of {passes} passes, OpenMP threads = {threads}")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (multi-GPU sharding is a "
+              f"design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if world > 1:
+        print("bench.py is single-GPU; launch it without torch.distributed", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    arr, spec = make_ic(args.side)
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    for form in forms:
+        res[form] = run_gpu(arr, spec, form == "mixed", args.steps, args.warmup)
+        print(f"# {form}: {res[form]['ms_per_step']:.3f} ms/step, phases (ms) "
+              + ", ".join(f"{k}={v:.3f}" for k, v in res[form]["phase_ms"].items()), file=sys.stderr, flush=True)
+
+    head = res.get("mixed") or res[forms[0]]
+    mixed = "mixed" in res
+    mom_ms = head["phase_ms"]["MomentumAndEnergy"]
+    flops = FLOPS_PER_PAIR_MOM_VE * head["pairs"] + FLOPS_PER_TARGET_MOM_VE * head["n"]
+    achieved = flops / (mom_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_VECTOR_TFLOPS if mixed else PEAK_FP64_VECTOR_TFLOPS
+    roofline = dict(bound="valu", kernel="momentumVeKernel", achieved=round(achieved, 3), peak=peak,
+                    unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
+                    flops_per_launch=int(flops), avg_launch_ms=round(mom_ms, 4))
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_side, args.cpu_passes)
+    line = {
+        "metric": METRIC,
+        "value": round(head["rate"], 1),
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed" if mixed else "f64",
+        "data": "synthetic",
+        "config": {"workload": "turbulence-box VE pass, 200^3 = 8M particles, periodic unit box, Mach 0.3, ng0 100",
+                   "particles": head["n"], "formulation": "VE", "precision": "mixed" if mixed else "fp64",
+                   "bucket_size": 64, "ngmax": 150},
+        "fp64_value": round(res["fp64"]["rate"], 1) if "fp64" in res else None,
+        "mixed_over_fp64": round(res["mixed"]["rate"] / res["fp64"]["rate"], 3) if len(res) == 2 else None,
+        "phase_ms": {f: {k: round(v, 4) for k, v in r["phase_ms"].items()} for f, r in res.items()},
+        "neighbor_pairs": head["pairs"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+/*
+ * sphexa_amd C++ host interface: the reference's hydro-path function names and argument meaning
+ * (sph/include/sph/find_neighbors.hpp and the hydro_std, hydro_ve drivers) on top of the C ABI in
+ * sphexa_amd.h. A Dataset type provides device field pointers through `devData` and the scalar
+ * parameters under their ParticlesData names (sph/include/sph/particles_data.hpp:91-146);
+ * `DeviceDataset` below is a minimal ready-made one.
+ *
+ * Error behaviour follows the reference: a rejected argument or a HIP failure throws
+ * std::runtime_error; ng0 > ngmax throws "ng0 should be smaller than ngmax"
+ * (find_neighbors.hpp:52); a non-converged h/nc iteration throws "coupled nc/h-updated failed to
+ * converge" as the reference GPU path does (hydro_ve/xmass_gpu.cu:128).
+ */
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "sphexa_amd.h"
+
+namespace sphexa_amd
+{
+
+enum class BoundaryType : char
+{
+    open     = 0,
+    periodic = 1,
+    fixed    = 2
+};
+
+//! cstone::Box<double> (domain/include/cstone/sfc/box.hpp:95-174), the fields the path uses
+struct Box
+{
+    double       lo[3], hi[3];
+    BoundaryType boundary[3];
+
+    Box(double xyzMin, double xyzMax, BoundaryType b = BoundaryType::open)
+        : lo{xyzMin, xyzMin, xyzMin}
+        , hi{xyzMax, xyzMax, xyzMax}
+        , boundary{b, b, b}
+    {
+    }
+    Box(double xmin, double xmax, double ymin, double ymax, double zmin, double zmax,
+        BoundaryType bx = BoundaryType::open, BoundaryType by = BoundaryType::open,
+        BoundaryType bz = BoundaryType::open)
+        : lo{xmin, ymin, zmin}
+        , hi{xmax, ymax, zmax}
+        , boundary{bx, by, bz}
+    {
+    }
+
+    sphx_box_t c() const
+    {
+        sphx_box_t b;
+        for (int d = 0; d < 3; ++d)
+        {
+            b.lo[d] = lo[d], b.hi[d] = hi[d], b.boundary[d] = int32_t(boundary[d]);
+        }
+        return b;
+    }
+};
+
+//! cstone::GroupView (domain/include/cstone/traversal/groups.hpp:28-34); the device path treats the
+//! targets [firstBody, lastBody) as consecutive 64-particle groups, so groupStart/End may be null
+struct GroupView
+{
+    uint32_t        firstBody, lastBody;
+    uint32_t        numGroups;
+    const uint32_t* groupStart;
+    const uint32_t* groupEnd;
+};
+
+//! device field pointers, named and typed as DeviceParticlesData (sph/include/sph/particles_data_gpu.cuh)
+template<class HydroType>
+struct DeviceFields
+{
+    double *x{}, *y{}, *z{}, *u{}, *temp{}, *du{};
+    HydroType *h{}, *m{}, *vx{}, *vy{}, *vz{}, *rho{}, *p{}, *c{}, *prho{}, *tdpdTrho{};
+    HydroType *c11{}, *c12{}, *c13{}, *c22{}, *c23{}, *c33{}, *xm{}, *kx{}, *gradh{}, *divv{}, *curlv{}, *alpha{};
+    HydroType *ax{}, *ay{}, *az{}, *dV11{}, *dV12{}, *dV13{}, *dV22{}, *dV23{}, *dV33{};
+    uint64_t* keys{};
+    uint32_t* nc{};
+};
+
+//! A minimal Dataset: device fields + the ParticlesData scalars + the workspace (scratch owner)
+template<class HydroType>
+struct DeviceDataset
+{
+    using RealType = double;
+    using HydroT   = HydroType;
+
+    DeviceFields<HydroType> devData;
+    sphx_workspace_t*       workspace{nullptr};
+
+    unsigned ng0{100}, ngmax{150};
+    double   K{0}, Kcour{0.2}, gamma{5.0 / 3.0}, muiConst{10.0};
+    double   minDt{1e-12}, minDtCourant{INFINITY};
+    double   alphamin{0.05}, alphamax{1.0}, decay_constant{0.2};
+    double   Atmin{0.1}, Atmax{0.2}, ramp{1.0 / (0.2 - 0.1)};
+};
+
+namespace detail
+{
+
+inline void check(sphx_workspace_t* ws, int code)
+{
+    if (code == SPHX_OK) { return; }
+    throw std::runtime_error(std::string("sphexa_amd: ") + sphx_last_error(ws));
+}
+
+template<class Dataset>
+sphx_params_t params(const Dataset& d)
+{
+    using H = typename Dataset::HydroT;
+    sphx_params_t p{};
+    p.K = d.K, p.Kcour = d.Kcour, p.gamma = d.gamma, p.muiConst = d.muiConst, p.minDt = d.minDt;
+    p.alphamin = d.alphamin, p.alphamax = d.alphamax, p.decay_constant = d.decay_constant;
+    p.Atmin = d.Atmin, p.Atmax = d.Atmax, p.ramp = d.ramp;
+    p.ng0 = d.ng0, p.ngmax = d.ngmax;
+    p.hydro_type = sizeof(H) == 4 ? SPHX_F32 : SPHX_F64;
+    return p;
+}
+
+template<class Dataset>
+sphx_fields_t fields(Dataset& d)
+{
+    auto&         s = d.devData;
+    sphx_fields_t f{};
+    f.x = s.x, f.y = s.y, f.z = s.z, f.u = s.u, f.temp = s.temp, f.du = s.du;
+    f.h = s.h, f.m = s.m, f.vx = s.vx, f.vy = s.vy, f.vz = s.vz, f.rho = s.rho, f.p = s.p, f.c = s.c;
+    f.prho = s.prho, f.tdpdTrho = s.tdpdTrho, f.c11 = s.c11, f.c12 = s.c12, f.c13 = s.c13, f.c22 = s.c22;
+    f.c23 = s.c23, f.c33 = s.c33, f.xm = s.xm, f.kx = s.kx, f.gradh = s.gradh, f.divv = s.divv;
+    f.curlv = s.curlv, f.alpha = s.alpha, f.ax = s.ax, f.ay = s.ay, f.az = s.az;
+    f.dV11 = s.dV11, f.dV12 = s.dV12, f.dV13 = s.dV13, f.dV22 = s.dV22, f.dV23 = s.dV23, f.dV33 = s.dV33;
+    f.keys = s.keys, f.nc = s.nc;
+    return f;
+}
+
+} // namespace detail
+
+//! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234): keys, sort, permute, tree
+template<class Dataset>
+void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucketSize = 64)
+{
+    auto b = box.c();
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_domain_sync(d.workspace, &b, &f, &p, numParticles, bucketSize));
+}
+
+//! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56)
+template<class Dataset>
+void findNeighborsSfc(size_t startIndex, size_t endIndex, Dataset& d, const Box& box)
+{
+    if (d.ng0 > d.ngmax) { throw std::runtime_error("ng0 should be smaller than ngmax\n"); }
+    auto b    = box.c();
+    auto f    = detail::fields(d);
+    auto p    = detail::params(d);
+    int  code = sphx_find_neighbors(d.workspace, &b, &f, &p, uint32_t(startIndex), uint32_t(endIndex), 1);
+    if (code == SPHX_ERR_NOT_CONVERGED) { throw std::runtime_error("coupled nc/h-updated failed to converge"); }
+    detail::check(d.workspace, code);
+}
+
+#define SPHEXA_AMD_GROUP_PASS(name, cfn)                                                                               \
+    template<class Dataset>                                                                                            \
+    void name(const GroupView& grp, Dataset& d, const Box& box)                                                        \
+    {                                                                                                                  \
+        auto b = box.c();                                                                                              \
+        auto f = detail::fields(d);                                                                                    \
+        auto p = detail::params(d);                                                                                    \
+        detail::check(d.workspace, cfn(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody));                         \
+    }
+
+//! sph::computeDensity (hydro_std/density.hpp:56-60)
+SPHEXA_AMD_GROUP_PASS(computeDensity, sphx_density_std)
+//! sph::computeIAD (hydro_std/iad.hpp:75-79)
+SPHEXA_AMD_GROUP_PASS(computeIAD, sphx_iad_std)
+//! sph::computeXMass (hydro_ve/xmass.hpp:67-72)
+SPHEXA_AMD_GROUP_PASS(computeXMass, sphx_xmass)
+//! sph::computeVeDefGradh (hydro_ve/ve_def_gradh.hpp:79-84)
+SPHEXA_AMD_GROUP_PASS(computeVeDefGradh, sphx_ve_def_gradh)
+//! sph::computeIadDivvCurlv (hydro_ve/iad_divv_curlv.hpp:92-97)
+SPHEXA_AMD_GROUP_PASS(computeIadDivvCurlv, sphx_iad_divv_curlv)
+//! sph::computeAVswitches (hydro_ve/av_switches.hpp:80-85)
+SPHEXA_AMD_GROUP_PASS(computeAVswitches, sphx_av_switches)
+
+#undef SPHEXA_AMD_GROUP_PASS
+
+//! sph::computeEOS_HydroStd, idealGas (hydro_std/eos.hpp:147-153)
+template<class Dataset>
+void computeEOS_HydroStd(size_t startIndex, size_t endIndex, Dataset& d)
+{
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_eos_std(d.workspace, &f, &p, uint32_t(startIndex), uint32_t(endIndex)));
+}
+
+//! sph::computeEOS, idealGas (hydro_ve/eos.hpp:192-197)
+template<class Dataset>
+void computeEOS(size_t startIndex, size_t endIndex, Dataset& d)
+{
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_eos_ve(d.workspace, &f, &p, uint32_t(startIndex), uint32_t(endIndex)));
+}
+
+//! sph::computeMomentumEnergySTD (hydro_std/momentum_energy.hpp:97-101); sets d.minDtCourant
+template<class Dataset>
+void computeMomentumEnergySTD(const GroupView& grp, Dataset& d, const Box& box)
+{
+    auto   b  = box.c();
+    auto   f  = detail::fields(d);
+    auto   p  = detail::params(d);
+    double dt = 0;
+    detail::check(d.workspace,
+                  sphx_momentum_energy_std(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody, &dt));
+    d.minDtCourant = dt;
+}
+
+//! sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:106-115); sets d.minDtCourant.
+//! Per-group time steps (groupDt) belong to the block-time-step propagators and are not produced.
+template<bool avClean, class Dataset>
+void computeMomentumEnergy(const GroupView& grp, float* groupDt, Dataset& d, const Box& box)
+{
+    if (groupDt != nullptr) { throw std::runtime_error("per-group time steps are not supported"); }
+    auto   b  = box.c();
+    auto   f  = detail::fields(d);
+    auto   p  = detail::params(d);
+    double dt = 0;
+    detail::check(d.workspace, sphx_momentum_energy_ve(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody,
+                                                       avClean ? 1 : 0, &dt));
+    d.minDtCourant = dt;
+}
+
+} // namespace sphexa_amd

@@ -36,6 +36,21 @@ def compile_one(src):
     return obj
 
 
+CPP_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "dataset_api.cpp")
+CPP_DRIVER = os.path.join(ROOT, "tests", "cpp", "dataset_api")
+
+
+def build_cpp_driver():
+    """tests/cpp/dataset_api: the C++ interface (include/sphexa_amd.hpp) driven like the
+    reference propagators; run by the GPU tests."""
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", f"-I{os.path.join(ROOT, 'include')}",
+           CPP_DRIVER_SRC, f"-L{HERE}", "-lsphexa_amd", "-Wl,-rpath,$ORIGIN/../../sphexa_amd", "-o", CPP_DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ driver build failed:\n{r.stdout}\n{r.stderr}")
+    return CPP_DRIVER
+
+
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -48,8 +63,9 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_driver()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CPP_DRIVER}")
     return LIB
 
 

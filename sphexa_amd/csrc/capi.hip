@@ -123,7 +123,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     ws->wh64.release(), ws->whd64.release(), ws->wh32.release(), ws->whd32.release();
     ws->tree.releaseAll();
     ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release(), ws->idxTmp.release();
-    ws->fieldTmp.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
+    ws->fieldTmp.release(), ws->order.release(), ws->records.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
     delete ws;
     return SPHX_OK;
 }
@@ -234,8 +234,8 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         }
         fill64(ws, keys, 0, n);
         computeKeys(ws, b, f->x, f->y, f->z, keys, n);
-        DevBuf<uint32_t> order;
-        order.reserve(n);
+        ws->order.reserve(n);
+        DevBuf<uint32_t>& order = ws->order;
         sortKeys(ws, keys, order.get(), n);
 
         // permute every allocated field through one staging buffer
@@ -259,7 +259,6 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         }
         buildTree(ws, keys, n, bucketSize, b);
         SPHX_HIP(hipStreamSynchronize(ws->stream));
-        order.release();
         own.release();
         return SPHX_OK;
     });

@@ -93,6 +93,12 @@ struct TreeBuildScratch
     }
 };
 
+//! 16-byte storage unit of the neighbour-record buffer
+struct alignas(16) Chunk16
+{
+    uint64_t lo, hi;
+};
+
 struct Tree
 {
     int32_t              numLeafNodes = 0, numInternalNodes = 0, numNodes = 0;
@@ -142,6 +148,8 @@ struct sphx_workspace
     sphx::DevBuf<uint64_t> keyTmp;
     sphx::DevBuf<uint32_t> idxTmp;
     sphx::DevBuf<char>     fieldTmp;     // permutation staging
+    sphx::DevBuf<uint32_t> order;        // sort permutation of the domain sync
+    sphx::DevBuf<sphx::Chunk16> records; // packed neighbour records of the current pair loop
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions

@@ -1,8 +1,9 @@
 // Standard SPH pair loops: density (XMass), ideal-gas EOS, IAD, momentum-energy with the
 // Courant time step (sph/include/sph/hydro_std/*, sph/include/sph/hydro_ve/xmass_kern.hpp).
+// Neighbour data is read from packed per-particle records (records.hpp).
 #include <cstring>
 
-#include "pairs.hpp"
+#include "records.hpp"
 
 namespace sphx
 {
@@ -10,37 +11,58 @@ namespace sphx
 namespace
 {
 
-struct Common
-{
-    const double* x;
-    const double* y;
-    const double* z;
-    DevBox        box;
-    double        K;
-    Lists         L;
-};
+constexpr unsigned kPackThreads = 256;
 
-Common makeCommon(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
-                  uint32_t first, uint32_t last)
+template<class T>
+__global__ void __launch_bounds__(kPackThreads) packMass(const double* __restrict__ x, const double* __restrict__ y,
+                                                         const double* __restrict__ z, const T* __restrict__ m,
+                                                         RecMass<T>* __restrict__ rec, uint32_t n)
 {
-    SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "pair loop called before find_neighbors");
-    SPHX_REQUIRE(first >= ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
-                 "target range outside the searched range");
-    SPHX_REQUIRE(first == ws->nbFirst, SPHX_ERR_ARG, "target range must start at the searched first index");
-    SPHX_REQUIRE(p->ngmax == ws->nbNgmax, SPHX_ERR_ARG, "ngmax differs from the one used by find_neighbors");
-    SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
-    SPHX_REQUIRE(f->x && f->y && f->z && f->nc, SPHX_ERR_ARG, "x, y, z and nc are required");
-    Common c;
-    c.x = f->x, c.y = f->y, c.z = f->z, c.box = box, c.K = p->K;
-    c.L = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax};
-    return c;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    RecMass<T> r;
+    r.x = x[i], r.y = y[i], r.z = z[i], r.m = m[i];
+    rec[i] = r;
 }
 
 template<class T>
-const T* whTable(sphx_workspace* ws)
+__global__ void __launch_bounds__(kPackThreads) packVol(const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z, const T* __restrict__ m,
+                                                        const T* __restrict__ rho, RecVol<T>* __restrict__ rec,
+                                                        uint32_t n)
 {
-    if constexpr (isMixed<T>) { return ws->wh32.get(); }
-    else { return ws->wh64.get(); }
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    RecVol<T> r;
+    r.x = x[i], r.y = y[i], r.z = z[i];
+    r.vol  = fastDiv(m[i], rho[i]);
+    rec[i] = r;
+}
+
+struct MomStdFields
+{
+    const void *vx, *vy, *vz, *h, *m, *rho, *p, *c, *c11, *c12, *c13, *c22, *c23, *c33;
+    void *      ax, *ay, *az;
+    double*     du;
+};
+
+template<class T>
+__global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFields f, RecMomStd<T>* __restrict__ rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.numParticles) { return; }
+    RecMomStd<T> r;
+    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
+    T rho  = ld<T>(f.rho, i);
+    r.hInv = T(1) / ld<T>(f.h, i);
+    r.m    = ld<T>(f.m, i);
+    r.vol  = fastDiv(r.m, rho);
+    r.pro  = fastDiv(ld<T>(f.p, i), rho); // p_j / (rho_j gradh_j), gradh_j = 1
+    r.c    = ld<T>(f.c, i);
+    r.c11 = ld<T>(f.c11, i), r.c12 = ld<T>(f.c12, i), r.c13 = ld<T>(f.c13, i);
+    r.c22 = ld<T>(f.c22, i), r.c23 = ld<T>(f.c23, i), r.c33 = ld<T>(f.c33, i);
+    rec[i] = r;
 }
 
 /* xmassJLoop (sph/include/sph/hydro_ve/xmass_kern.hpp:51-79):
@@ -48,29 +70,30 @@ const T* whTable(sphx_workspace* ws)
  * toRho: computeDensity (hydro_std/density.hpp:42-53) stores rho = m / xm instead. */
 template<class T, bool toRho>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    xmassKernel(Common a, const T* __restrict__ h, const T* __restrict__ m, const T* __restrict__ wh, T* __restrict__ out)
+    xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, const T* __restrict__ wh,
+                T* __restrict__ out)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
-    const uint32_t i  = t.i;
-    double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    T              hi = h[i], mi = m[i];
-    T              hInv  = T(1) / hi;
-    T              h3Inv = hInv * hInv * hInv;
-    double         r2h   = 2.0 * double(hi);
-    Acc<T>         rho0(mi);
+    const uint32_t   i     = t.i;
+    const RecMass<T> ri    = rec[i];
+    T                hi    = h[i];
+    T                hInv  = T(1) / hi;
+    T                h3Inv = hInv * hInv * hInv;
+    double           r2h   = 2.0 * double(hi);
+    Acc<T>           rho0(ri.m);
     for (unsigned k = 0; k < t.n; ++k)
     {
-        uint32_t j = t.list[size_t(k) * kGroup];
-        T        rx, ry, rz;
-        separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+        const RecMass<T> rj = rec[t.list[size_t(k) * kGroup]];
+        T                rx, ry, rz;
+        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
         T dist = fastSqrt(rx * rx + ry * ry + rz * rz);
         T w    = lookup(wh, dist * hInv);
-        rho0.add(w * m[j]);
+        rho0.add(w * rj.m);
     }
     double rhoZero = double(rho0.value()) * a.K * double(h3Inv);
-    T      xmi     = T(double(mi) / rhoZero);
-    out[i]         = toRho ? T(mi / xmi) : xmi;
+    T      xmi     = T(double(ri.m) / rhoZero);
+    out[i]         = toRho ? T(ri.m / xmi) : xmi;
 }
 
 //! idealGasEOS_u (sph/include/sph/eos.hpp:35-44) or idealGasEOS with cv from mui (:47-51)
@@ -90,25 +113,25 @@ __global__ void __launch_bounds__(256) eosStdKernel(const double* __restrict__ u
 /* IADJLoopSTD (hydro_std/iad_kern.hpp:11-76): tau_ab = sum_j m_j/rho_j W_ij r_a r_b, inverted */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadStdKernel(Common a, const T* __restrict__ h, const T* __restrict__ m, const T* __restrict__ rho,
-                 const T* __restrict__ wh, T* c11, T* c12, T* c13, T* c22, T* c23, T* c33)
+    iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, const T* __restrict__ wh,
+                 T* c11, T* c12, T* c13, T* c22, T* c23, T* c33)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
-    const uint32_t i  = t.i;
-    double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    T              hi    = h[i];
-    T              hiInv = T(1) / hi;
-    double         r2h   = 2.0 * double(hi);
-    T              t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
+    const uint32_t  i     = t.i;
+    const RecVol<T> ri    = rec[i];
+    T               hi    = h[i];
+    T               hiInv = T(1) / hi;
+    double          r2h   = 2.0 * double(hi);
+    T               t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     for (unsigned k = 0; k < t.n; ++k)
     {
-        uint32_t j = t.list[size_t(k) * kGroup];
-        T        rx, ry, rz;
-        separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+        const RecVol<T> rj = rec[t.list[size_t(k) * kGroup]];
+        T               rx, ry, rz;
+        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
         T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
         T w     = lookup(wh, dist * hiInv);
-        T vol_w = fastDiv(m[j], rho[j]) * w;
+        T vol_w = rj.vol * w;
         t11 += rx * rx * vol_w;
         t12 += rx * ry * vol_w;
         t13 += rx * rz * vol_w;
@@ -122,93 +145,65 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     c22[i] = T(cij[3]), c23[i] = T(cij[4]), c33[i] = T(cij[5]);
 }
 
-struct MomStdFields
-{
-    const void *vx, *vy, *vz, *h, *m, *rho, *p, *c, *c11, *c12, *c13, *c22, *c23, *c33;
-    void *      ax, *ay, *az;
-    double*     du;
-};
-
 /* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumStdKernel(Common a, MomStdFields f, const T* __restrict__ wh, float Kcour, unsigned long long* minDt)
+    momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, const T* __restrict__ wh,
+                      float Kcour, unsigned long long* minDt)
 {
-    Lane t = laneTarget(a.L);
+    Lane   t      = laneTarget(a.L);
     double dtLane = INFINITY;
     if (t.valid)
     {
-        const uint32_t i = t.i;
-        const T* __restrict__ vx  = (const T*)f.vx;
-        const T* __restrict__ vy  = (const T*)f.vy;
-        const T* __restrict__ vz  = (const T*)f.vz;
-        const T* __restrict__ h   = (const T*)f.h;
-        const T* __restrict__ m   = (const T*)f.m;
-        const T* __restrict__ rho = (const T*)f.rho;
-        const T* __restrict__ p   = (const T*)f.p;
-        const T* __restrict__ c   = (const T*)f.c;
-        const T* __restrict__ c11 = (const T*)f.c11;
-        const T* __restrict__ c12 = (const T*)f.c12;
-        const T* __restrict__ c13 = (const T*)f.c13;
-        const T* __restrict__ c22 = (const T*)f.c22;
-        const T* __restrict__ c23 = (const T*)f.c23;
-        const T* __restrict__ c33 = (const T*)f.c33;
-
-        double xi = a.x[i], yi = a.y[i], zi = a.z[i];
-        T vxi = vx[i], vyi = vy[i], vzi = vz[i];
-        T hi = h[i], roi = rho[i], pri = p[i], ci = c[i];
-        T mi_roi = fastDiv(m[i], roi);
-        T hiInv  = T(1) / hi;
-        T hiInv3 = hiInv * hiInv * hiInv;
-        T c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
-        T pro_i = fastDiv(pri, roi * roi); // p_i / (gradh_i rho_i^2), gradh_i = 1
-        double r2h = 2.0 * double(hi);
+        const uint32_t     i      = t.i;
+        const RecMomStd<T> ri     = rec[i];
+        T                  hi     = ld<T>(f.h, i);
+        T                  roi    = ld<T>(f.rho, i);
+        T                  hiInv  = ri.hInv;
+        T                  hiInv3 = hiInv * hiInv * hiInv;
+        T                  pro_i  = fastDiv(ld<T>(f.p, i), roi * roi); // p_i / (gradh_i rho_i^2), gradh_i = 1
+        double             r2h    = 2.0 * double(hi);
 
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
         Acc<T> energy;
         for (unsigned k = 0; k < t.n; ++k)
         {
-            uint32_t j = t.list[size_t(k) * kGroup];
-            T        rx, ry, rz;
-            separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+            const RecMomStd<T> rj = rec[t.list[size_t(k) * kGroup]];
+            T                  rx, ry, rz;
+            separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
             T r2    = rx * rx + ry * ry + rz * rz;
             T dist  = fastSqrt(r2);
-            T vx_ij = vxi - vx[j], vy_ij = vyi - vy[j], vz_ij = vzi - vz[j];
-            T hjInv = T(1) / h[j];
+            T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
             T v1    = dist * hiInv;
-            T v2    = dist * hjInv;
+            T v2    = dist * rj.hInv;
             T rv    = rx * vx_ij + ry * vy_ij + rz * vz_ij;
             T Wi    = hiInv3 * lookup(wh, v1);
-            T Wj    = hjInv * hjInv * hjInv * lookup(wh, v2);
+            T Wj    = rj.hInv * rj.hInv * rj.hInv * lookup(wh, v2);
 
-            T tA1_i = c11i * rx + c12i * ry + c13i * rz;
-            T tA2_i = c12i * rx + c22i * ry + c23i * rz;
-            T tA3_i = c13i * rx + c23i * ry + c33i * rz;
-            T c11j = c11[j], c12j = c12[j], c13j = c13[j], c22j = c22[j], c23j = c23[j], c33j = c33[j];
-            T tA1_j = c11j * rx + c12j * ry + c13j * rz;
-            T tA2_j = c12j * rx + c22j * ry + c23j * rz;
-            T tA3_j = c13j * rx + c23j * ry + c33j * rz;
+            T tA1_i = ri.c11 * rx + ri.c12 * ry + ri.c13 * rz;
+            T tA2_i = ri.c12 * rx + ri.c22 * ry + ri.c23 * rz;
+            T tA3_i = ri.c13 * rx + ri.c23 * ry + ri.c33 * rz;
+            T tA1_j = rj.c11 * rx + rj.c12 * ry + rj.c13 * rz;
+            T tA2_j = rj.c12 * rx + rj.c22 * ry + rj.c23 * rz;
+            T tA3_j = rj.c13 * rx + rj.c23 * ry + rj.c33 * rz;
 
-            T roj  = rho[j];
-            T cj   = c[j];
             T wij  = fastDiv(rv, dist);
-            T visc = T(0.5) * artificialViscosity(T(1), T(1), ci, cj, wij);
-            T vsig = ci + cj - T(3) * wij;
+            T visc = T(0.5) * artificialViscosity(T(1), T(1), ri.c, rj.c, wij);
+            T vsig = ri.c + rj.c - T(3) * wij;
             maxvs  = vsig > maxvs ? vsig : maxvs;
 
-            T mj        = m[j];
-            T mj_roj_Wj = fastDiv(mj, roj) * Wj;
-            T mj_pro_i  = mj * pro_i;
+            T mj_roj_Wj = rj.vol * Wj;
+            T mj_pro_i  = rj.m * pro_i;
             {
-                T aa = Wi * (mj_pro_i + visc * mi_roi);
-                T bb = mj_roj_Wj * (fastDiv(p[j], roj) + visc);
+                T aa = Wi * (mj_pro_i + visc * ri.vol);
+                T bb = mj_roj_Wj * (rj.pro + visc);
                 momx += aa * tA1_i + bb * tA1_j;
                 momy += aa * tA2_i + bb * tA2_j;
                 momz += aa * tA3_i + bb * tA3_j;
             }
             {
-                T aa = Wi * (T(2) * mj_pro_i + visc * mi_roi);
+                T aa = Wi * (T(2) * mj_pro_i + visc * ri.vol);
                 T bb = visc * mj_roj_Wj;
                 energy.add(vx_ij * (aa * tA1_i + bb * tA1_j) + vy_ij * (aa * tA2_i + bb * tA2_j) +
                            vz_ij * (aa * tA3_i + bb * tA3_j));
@@ -218,22 +213,54 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         st<T>(f.ax, i, T(a.K * double(momx)));
         st<T>(f.ay, i, T(a.K * double(momy)));
         st<T>(f.az, i, T(a.K * double(momz)));
-        dtLane = courantDt(maxvs, hi, ci, Kcour);
+        dtLane = courantDt(maxvs, hi, ri.c, Kcour);
     }
     reduceMinDt<T>(minDt, dtLane);
 }
 
 template<class T>
-void launchDensity(sphx_workspace* ws, const Common& a, const sphx_fields_t* f, bool toRho, void* out)
+void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, bool toRho, void* out)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
+    RecMass<T>* rec = recordBuffer<RecMass<T>>(ws, a.numParticles);
+    packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.x, a.y, a.z, (const T*)f->m,
+                                                                                         rec, a.numParticles);
+    SPHX_CHECK_LAUNCH();
     if (toRho)
-        xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, (const T*)f->h, (const T*)f->m,
-                                                                            whTable<T>(ws), (T*)out);
+        xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+                                                                            (T*)out);
     else
-        xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, (const T*)f->h, (const T*)f->m,
-                                                                             whTable<T>(ws), (T*)out);
+        xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+                                                                             (T*)out);
+    SPHX_CHECK_LAUNCH();
+}
+
+template<class T>
+void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles);
+    packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
+        a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
+    SPHX_CHECK_LAUNCH();
+    iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+                                                                    (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
+                                                                    (T*)f->c23, (T*)f->c33);
+    SPHX_CHECK_LAUNCH();
+}
+
+template<class T>
+void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields& mf, float Kcour,
+                       unsigned long long* dt)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles);
+    packMomStd<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
+    SPHX_CHECK_LAUNCH();
+    momentumStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour, dt);
     SPHX_CHECK_LAUNCH();
 }
 
@@ -242,7 +269,7 @@ void launchDensity(sphx_workspace* ws, const Common& a, const sphx_fields_t* f, 
 void densityStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                 uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->rho, SPHX_ERR_ARG, "density needs h, m and rho");
     if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, true, f->rho);
     else launchDensity<double>(ws, a, f, true, f->rho);
@@ -251,7 +278,7 @@ void densityStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, c
 void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
              uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->xm, SPHX_ERR_ARG, "xmass needs h, m and xm");
     if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, false, f->xm);
     else launchDensity<double>(ws, a, f, false, f->xm);
@@ -281,44 +308,27 @@ void eosStd(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, 
 void iadStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
             uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->rho && f->c11 && f->c12 && f->c13 && f->c22 && f->c23 && f->c33, SPHX_ERR_ARG,
                  "IAD needs h, m, rho and c11..c33");
-    unsigned blocks = pairBlocks(first, last);
-    if (blocks == 0) { return; }
-#define LAUNCH(T)                                                                                                      \
-    iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(                                                    \
-        a, (const T*)f->h, (const T*)f->m, (const T*)f->rho, whTable<T>(ws), (T*)f->c11, (T*)f->c12, (T*)f->c13,      \
-        (T*)f->c22, (T*)f->c23, (T*)f->c33)
-    if (p->hydro_type == SPHX_F32) LAUNCH(float);
-    else LAUNCH(double);
-#undef LAUNCH
-    SPHX_CHECK_LAUNCH();
+    if (p->hydro_type == SPHX_F32) launchIadStd<float>(ws, a, f);
+    else launchIadStd<double>(ws, a, f);
 }
 
 double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                          uint32_t first, uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->vx && f->vy && f->vz && f->h && f->m && f->rho && f->p && f->c && f->c11 && f->c12 && f->c13 &&
                      f->c22 && f->c23 && f->c33 && f->ax && f->ay && f->az && f->du,
                  SPHX_ERR_ARG, "momentum-energy needs v, h, m, rho, p, c, c11..c33, ax, ay, az and du");
     ws->counters.reserve(8);
     unsigned long long* dt = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
     fill64(ws, reinterpret_cast<uint64_t*>(dt), 0x7ff0000000000000ull, 1); // +inf
-    MomStdFields mf{f->vx, f->vy, f->vz, f->h, f->m, f->rho, f->p, f->c, f->c11, f->c12, f->c13, f->c22, f->c23, f->c33,
-                    f->ax, f->ay, f->az, f->du};
-    unsigned blocks = pairBlocks(first, last);
-    if (blocks > 0)
-    {
-        if (p->hydro_type == SPHX_F32)
-            momentumStdKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, ws->wh32.get(),
-                                                                                  float(p->Kcour), dt);
-        else
-            momentumStdKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, ws->wh64.get(),
-                                                                                   float(p->Kcour), dt);
-        SPHX_CHECK_LAUNCH();
-    }
+    MomStdFields mf{f->vx,  f->vy,  f->vz,  f->h,   f->m,   f->rho, f->p,  f->c,  f->c11,
+                    f->c12, f->c13, f->c22, f->c23, f->c33, f->ax,  f->ay, f->az, f->du};
+    if (p->hydro_type == SPHX_F32) launchMomentumStd<float>(ws, a, mf, float(p->Kcour), dt);
+    else launchMomentumStd<double>(ws, a, mf, float(p->Kcour), dt);
     unsigned long long bits = 0;
     SPHX_HIP(hipMemcpyAsync(&bits, dt, 8, hipMemcpyDeviceToHost, ws->stream));
     SPHX_HIP(hipStreamSynchronize(ws->stream));

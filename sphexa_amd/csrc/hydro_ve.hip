@@ -1,9 +1,10 @@
 // Generalised-volume-element (VE) SPH pair loops (sph/include/sph/hydro_ve/*):
 // XMass (shared with std density, hydro_std.hip), VeDefGradh, EOS, IAD + div/curl v,
-// AV switches, momentum-energy with optional AV cleaning.
+// AV switches, momentum-energy with optional AV cleaning. Neighbour data is read from packed
+// per-particle records (records.hpp).
 #include <cstring>
 
-#include "pairs.hpp"
+#include "records.hpp"
 
 namespace sphx
 {
@@ -11,89 +12,54 @@ namespace sphx
 namespace
 {
 
-struct Common
-{
-    const double* x;
-    const double* y;
-    const double* z;
-    DevBox        box;
-    double        K;
-    Lists         L;
-};
-
-Common makeCommon(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
-                  uint32_t first, uint32_t last)
-{
-    SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "pair loop called before find_neighbors");
-    SPHX_REQUIRE(first == ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
-                 "target range must start at the searched first index and stay inside the searched range");
-    SPHX_REQUIRE(p->ngmax == ws->nbNgmax, SPHX_ERR_ARG, "ngmax differs from the one used by find_neighbors");
-    SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
-    SPHX_REQUIRE(f->x && f->y && f->z && f->nc, SPHX_ERR_ARG, "x, y, z and nc are required");
-    Common c;
-    c.x = f->x, c.y = f->y, c.z = f->z, c.box = box, c.K = p->K;
-    c.L = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax};
-    return c;
-}
+constexpr unsigned kPackThreads = 256;
 
 template<class T>
-const T* whTable(sphx_workspace* ws)
+__global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* __restrict__ m,
+                                                          const T* __restrict__ xm, RecGradh<T>* __restrict__ rec)
 {
-    if constexpr (isMixed<T>) { return ws->wh32.get(); }
-    else { return ws->wh64.get(); }
-}
-
-template<class T>
-const T* whdTable(sphx_workspace* ws)
-{
-    if constexpr (isMixed<T>) { return ws->whd32.get(); }
-    else { return ws->whd64.get(); }
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.numParticles) { return; }
+    RecGradh<T> r;
+    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i], r.m = m[i], r.xm = xm[i];
+    rec[i] = r;
 }
 
 /* veDefGradhJLoop (hydro_ve/ve_def_gradh_kern.hpp:44-90): kx and the grad-h term */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    veDefGradhKernel(Common a, const T* __restrict__ h, const T* __restrict__ m, const T* __restrict__ xm,
+    veDefGradhKernel(PairArgs a, const RecGradh<T>* __restrict__ rec, const T* __restrict__ h,
                      const T* __restrict__ wh, const T* __restrict__ whd, T* __restrict__ kx, T* __restrict__ gradh)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
-    const uint32_t i  = t.i;
-    double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    T              hi = h[i], mi = m[i], xmi = xm[i];
-    T              hInv = T(1) / hi;
-    double         r2h  = 2.0 * double(hi);
-    Acc<T>         kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
+    const uint32_t    i   = t.i;
+    const RecGradh<T> ri  = rec[i];
+    T                 hi  = h[i], mi = ri.m, xmi = ri.xm;
+    T                 hInv = T(1) / hi;
+    double            r2h  = 2.0 * double(hi);
+    Acc<T>            kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
     for (unsigned k = 0; k < t.n; ++k)
     {
-        uint32_t j = t.list[size_t(k) * kGroup];
-        T        rx, ry, rz;
-        separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+        const RecGradh<T> rj = rec[t.list[size_t(k) * kGroup]];
+        T                 rx, ry, rz;
+        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
         T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
         T vloc  = dist * hInv;
         T w     = lookup(wh, vloc);
         T dw    = lookup(whd, vloc);
         T dterh = -(T(3) * w + vloc * dw);
-        T xmj   = xm[j];
-        kxs.add(w * xmj);
-        wom.add(dterh * xmj);
-        wr0.add(dterh * m[j]);
+        kxs.add(w * rj.xm);
+        wom.add(dterh * rj.xm);
+        wr0.add(dterh * rj.m);
     }
-    // per-target epilogue in FP64, in the reference's order
-    double dhInv  = double(hInv);
-    double h3Inv  = dhInv * dhInv * dhInv;
-    double kxi    = double(kxs.value()) * (a.K * h3Inv);
-    double whom   = double(wom.value()) * (a.K * h3Inv * dhInv);
-    double wrho0  = double(wr0.value()) * (a.K * h3Inv * dhInv);
-    if constexpr (!isMixed<T>)
-    {
-        // keep the reference's T-typed rounding of hInv^3
-        T th3 = hInv * hInv * hInv;
-        kxi   = double(kxs.value()) * (a.K * double(th3));
-        whom  = double(wom.value()) * (a.K * double(th3) * dhInv);
-        wrho0 = double(wr0.value()) * (a.K * double(th3) * dhInv);
-        h3Inv = double(th3);
-    }
+    // per-target epilogue in FP64 following the reference's order; h^-3 is rounded in T as there
+    T      th3   = hInv * hInv * hInv;
+    double dhInv = double(hInv);
+    double h3Inv = double(th3);
+    double kxi   = double(kxs.value()) * (a.K * h3Inv);
+    double whom  = double(wom.value()) * (a.K * h3Inv * dhInv);
+    double wrho0 = double(wr0.value()) * (a.K * h3Inv * dhInv);
     whom          = whom * double(mi) / double(xmi) + (kxi - a.K * double(xmi) * h3Inv) * wrho0;
     double rhoi   = kxi * double(mi) / double(xmi);
     double dhdrho = -double(hi) / (rhoi * 3.0);
@@ -127,6 +93,19 @@ struct IadVeFields
     void *      c11, *c12, *c13, *c22, *c23, *c33, *divv, *curlv, *dV11, *dV12, *dV13, *dV22, *dV23, *dV33;
 };
 
+template<class T>
+__global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeFields f, RecIadVe<T>* __restrict__ rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.numParticles) { return; }
+    RecIadVe<T> r;
+    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.xm  = ld<T>(f.xm, i);
+    r.vol = fastDiv(r.xm, ld<T>(f.kx, i));
+    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
+    rec[i] = r;
+}
+
 /* IADJLoop (hydro_ve/iad_kern.hpp:44-109) + divV_curlVJLoop (hydro_ve/divv_curlv_kern.hpp:44-123)
  * in one neighbour pass. The velocity-gradient terms are linear in the target's IAD matrix C_i:
  *   dV_a[b] = sum_j (v_ja - v_ia) xm_j (-(C_i r_ij)_b W_ij) = -sum_c C_i[b][c] M[a][c],
@@ -134,41 +113,33 @@ struct IadVeFields
  * so the 9 moments M are accumulated alongside tau and combined with C_i after the inversion. */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadDivvCurlvKernel(Common a, IadVeFields f, const T* __restrict__ wh)
+    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, const T* __restrict__ wh)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
-    const T* __restrict__ h  = (const T*)f.h;
-    const T* __restrict__ xm = (const T*)f.xm;
-    const T* __restrict__ kx = (const T*)f.kx;
-    const T* __restrict__ vx = (const T*)f.vx;
-    const T* __restrict__ vy = (const T*)f.vy;
-    const T* __restrict__ vz = (const T*)f.vz;
-
-    const uint32_t i  = t.i;
-    double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    T              hi = h[i], vxi = vx[i], vyi = vy[i], vzi = vz[i];
-    T              hiInv = T(1) / hi;
-    double         r2h   = 2.0 * double(hi);
-    T              t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
-    T              M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const uint32_t    i     = t.i;
+    const RecIadVe<T> ri    = rec[i];
+    T                 hi    = ld<T>(f.h, i);
+    T                 hiInv = T(1) / hi;
+    double            r2h   = 2.0 * double(hi);
+    T                 t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
+    T                 M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (unsigned k = 0; k < t.n; ++k)
     {
-        uint32_t j = t.list[size_t(k) * kGroup];
-        T        rx, ry, rz;
-        separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+        const RecIadVe<T> rj = rec[t.list[size_t(k) * kGroup]];
+        T                 rx, ry, rz;
+        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
         T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
         T w     = lookup(wh, dist * hiInv);
-        T xmj   = xm[j];
-        T vol_w = fastDiv(xmj, kx[j]) * w;
+        T vol_w = rj.vol * w;
         t11 += rx * rx * vol_w;
         t12 += rx * ry * vol_w;
         t13 += rx * rz * vol_w;
         t22 += ry * ry * vol_w;
         t23 += ry * rz * vol_w;
         t33 += rz * rz * vol_w;
-        T xw  = xmj * w;
-        T dvx = (vx[j] - vxi) * xw, dvy = (vy[j] - vyi) * xw, dvz = (vz[j] - vzi) * xw;
+        T xw  = rj.xm * w;
+        T dvx = (rj.vx - ri.vx) * xw, dvy = (rj.vy - ri.vy) * xw, dvz = (rj.vz - ri.vz) * xw;
         M[0][0] += dvx * rx, M[0][1] += dvx * ry, M[0][2] += dvx * rz;
         M[1][0] += dvy * rx, M[1][1] += dvy * ry, M[1][2] += dvy * rz;
         M[2][0] += dvz * rx, M[2][1] += dvz * ry, M[2][2] += dvz * rz;
@@ -189,7 +160,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             g[aa][b] = -(C[b][0] * M[aa][0] + C[b][1] * M[aa][1] + C[b][2] * M[aa][2]);
 
     T hiInv3   = hiInv * hiInv * hiInv;
-    T norm_kxi = T(a.K * double(hiInv3) / double(kx[i]));
+    T norm_kxi = T(a.K * double(hiInv3) / double(ld<T>(f.kx, i)));
     st<T>(f.divv, i, norm_kxi * (g[0][0] + g[1][1] + g[2][2]));
     if (f.curlv)
     {
@@ -213,41 +184,47 @@ struct AvFields
     void*       alpha;
 };
 
+template<class T>
+__global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, RecAv<T>* __restrict__ rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.numParticles) { return; }
+    RecAv<T> r;
+    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.vol  = fastDiv(ld<T>(f.xm, i), ld<T>(f.kx, i));
+    r.c    = ld<T>(f.c, i);
+    r.divv = ld<T>(f.divv, i);
+    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
+    rec[i] = r;
+}
+
 /* AVswitchesJLoop (hydro_ve/av_switches_kern.hpp:44-137). grad(div v) is linear in C_i:
  * gd_b = -K/h_i^3 sum_c C_i[b][c] sum_j f_j W_ij r_ij,c with f_j = xm_j/kx_j (divv_i - divv_j). */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    avSwitchesKernel(Common a, AvFields f, const T* __restrict__ wh, double dt, T alphamin, T alphamax, T decay)
+    avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, const T* __restrict__ wh, double dt,
+                     T alphamin, T alphamax, T decay)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
-    const T* __restrict__ h    = (const T*)f.h;
-    const T* __restrict__ c    = (const T*)f.c;
-    const T* __restrict__ xm   = (const T*)f.xm;
-    const T* __restrict__ kx   = (const T*)f.kx;
-    const T* __restrict__ vx   = (const T*)f.vx;
-    const T* __restrict__ vy   = (const T*)f.vy;
-    const T* __restrict__ vz   = (const T*)f.vz;
-    const T* __restrict__ divv = (const T*)f.divv;
-
-    const uint32_t i  = t.i;
-    double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-    T              hi = h[i], ci = c[i], vxi = vx[i], vyi = vy[i], vzi = vz[i], divv_i = divv[i];
+    const uint32_t i     = t.i;
+    const RecAv<T> ri    = rec[i];
+    T              hi    = ld<T>(f.h, i);
     T              hiInv = T(1) / hi;
     double         r2h   = 2.0 * double(hi);
-    T              vsig  = T(1.e-40) * ci;
+    T              vsig  = T(1.e-40) * ri.c;
     T              S[3]  = {0, 0, 0};
     for (unsigned k = 0; k < t.n; ++k)
     {
-        uint32_t j = t.list[size_t(k) * kGroup];
-        T        rx, ry, rz;
-        separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+        const RecAv<T> rj = rec[t.list[size_t(k) * kGroup]];
+        T              rx, ry, rz;
+        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
         T dist = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T rv   = rx * (vxi - vx[j]) + ry * (vyi - vy[j]) + rz * (vzi - vz[j]);
-        T vs   = rv < T(0) ? ci + c[j] - T(3) * fastDiv(rv, dist) : T(0);
+        T rv   = rx * (ri.vx - rj.vx) + ry * (ri.vy - rj.vy) + rz * (ri.vz - rj.vz);
+        T vs   = rv < T(0) ? ri.c + rj.c - T(3) * fastDiv(rv, dist) : T(0);
         vsig   = vsig < vs ? vs : vsig;
         T w    = lookup(wh, dist * hiInv);
-        T fw   = fastDiv(xm[j], kx[j]) * (divv_i - divv[j]) * w;
+        T fw   = rj.vol * (ri.divv - rj.divv) * w;
         S[0] += fw * rx, S[1] += fw * ry, S[2] += fw * rz;
     }
     T c11i = ld<T>(f.c11, i), c12i = ld<T>(f.c12, i), c13i = ld<T>(f.c13, i);
@@ -258,12 +235,13 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T gz   = -(c13i * S[0] + c23i * S[1] + c33i * S[2]) * kh3;
     T graddivv = fastSqrt(gx * gx + gy * gy + gz * gz);
 
+    T divv_i   = ri.divv;
     T alpha_i  = ld<T>(f.alpha, i);
     T alphaloc = 0;
     if (divv_i < T(0))
     {
         T a_const = hi * hi * graddivv;
-        alphaloc  = alphamax * a_const / (a_const + hi * fabs(divv_i) + T(0.05) * ci);
+        alphaloc  = alphamax * a_const / (a_const + hi * fabs(divv_i) + T(0.05) * ri.c);
     }
     if (alphaloc >= alpha_i) { alpha_i = alphaloc; }
     else
@@ -282,6 +260,27 @@ struct MomVeFields
     void *      ax, *ay, *az;
     double*     du;
 };
+
+template<class T>
+__global__ void __launch_bounds__(kPackThreads) packMomVe(PairArgs a, MomVeFields f, RecMomVe<T>* __restrict__ rec)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.numParticles) { return; }
+    RecMomVe<T> r;
+    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
+    r.hInv  = T(1) / ld<T>(f.h, i);
+    r.m     = ld<T>(f.m, i);
+    r.c     = ld<T>(f.c, i);
+    r.xm    = ld<T>(f.xm, i);
+    r.rho   = ld<T>(f.kx, i) * r.m / r.xm;
+    r.vol   = fastDiv(r.m, r.rho);
+    r.alpha = ld<T>(f.alpha, i);
+    r.prho  = ld<T>(f.prho, i);
+    r.c11 = ld<T>(f.c11, i), r.c12 = ld<T>(f.c12, i), r.c13 = ld<T>(f.c13, i);
+    r.c22 = ld<T>(f.c22, i), r.c23 = ld<T>(f.c23, i), r.c33 = ld<T>(f.c33, i);
+    rec[i] = r;
+}
 
 //! avRvCorrection (hydro_ve/momentum_energy_kern.hpp:44-63) with the reference's symv layout
 template<class T>
@@ -312,40 +311,19 @@ __device__ __forceinline__ T avRvCorrection(T rx, T ry, T rz, T eta_ab, T eta_cr
 /* momentumAndEnergyJLoop<avClean> (hydro_ve/momentum_energy_kern.hpp:66-222) + tsKCourant */
 template<class T, bool avClean>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumVeKernel(Common a, MomVeFields f, const T* __restrict__ wh, T Atmin, T Atmax, T ramp, float Kcour,
-                     unsigned long long* minDt)
+    momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, const T* __restrict__ wh, T Atmin,
+                     T Atmax, T ramp, float Kcour, unsigned long long* minDt)
 {
     Lane   t      = laneTarget(a.L);
     double dtLane = INFINITY;
     if (t.valid)
     {
-        const T* __restrict__ vx    = (const T*)f.vx;
-        const T* __restrict__ vy    = (const T*)f.vy;
-        const T* __restrict__ vz    = (const T*)f.vz;
-        const T* __restrict__ h     = (const T*)f.h;
-        const T* __restrict__ m     = (const T*)f.m;
-        const T* __restrict__ prho  = (const T*)f.prho;
-        const T* __restrict__ c     = (const T*)f.c;
-        const T* __restrict__ c11   = (const T*)f.c11;
-        const T* __restrict__ c12   = (const T*)f.c12;
-        const T* __restrict__ c13   = (const T*)f.c13;
-        const T* __restrict__ c22   = (const T*)f.c22;
-        const T* __restrict__ c23   = (const T*)f.c23;
-        const T* __restrict__ c33   = (const T*)f.c33;
-        const T* __restrict__ kx    = (const T*)f.kx;
-        const T* __restrict__ xm    = (const T*)f.xm;
-        const T* __restrict__ alpha = (const T*)f.alpha;
-
-        const uint32_t i  = t.i;
-        double         xi = a.x[i], yi = a.y[i], zi = a.z[i];
-        T vxi = vx[i], vyi = vy[i], vzi = vz[i];
-        T hi = h[i], mi = m[i], ci = c[i], kxi = kx[i], alpha_i = alpha[i], xmassi = xm[i];
-        T rhoi   = kxi * mi / xmassi;
-        T prhoi  = prho[i];
-        T hiInv  = T(1) / hi;
-        T hiInv3 = hiInv * hiInv * hiInv;
-        T c11i = c11[i], c12i = c12[i], c13i = c13[i], c22i = c22[i], c23i = c23[i], c33i = c33[i];
-        T gradV_i[6] = {0, 0, 0, 0, 0, 0};
+        const uint32_t    i      = t.i;
+        const RecMomVe<T> ri     = rec[i];
+        T                 hi     = ld<T>(f.h, i);
+        T                 hiInv  = ri.hInv;
+        T                 hiInv3 = hiInv * hiInv * hiInv;
+        T                 gradV_i[6] = {0, 0, 0, 0, 0, 0};
         if constexpr (avClean)
         {
             gradV_i[0] = ld<T>(f.dV11, i), gradV_i[1] = ld<T>(f.dV12, i), gradV_i[2] = ld<T>(f.dV13, i);
@@ -359,28 +337,24 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         Acc<T> energy, avEnergy;
         for (unsigned k = 0; k < t.n; ++k)
         {
-            uint32_t j = t.list[size_t(k) * kGroup];
-            T        rx, ry, rz;
-            separation(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
+            const uint32_t    j  = t.list[size_t(k) * kGroup];
+            const RecMomVe<T> rj = rec[j];
+            T                 rx, ry, rz;
+            separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
             T r2    = rx * rx + ry * ry + rz * rz;
             T dist  = fastSqrt(r2);
-            T vx_ij = vxi - vx[j], vy_ij = vyi - vy[j], vz_ij = vzi - vz[j];
-            T hjInv = T(1) / h[j];
+            T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
             T v1    = dist * hiInv;
-            T v2    = dist * hjInv;
+            T v2    = dist * rj.hInv;
             T Wi    = hiInv3 * lookup(wh, v1);
-            T Wj    = hjInv * hjInv * hjInv * lookup(wh, v2);
+            T Wj    = rj.hInv * rj.hInv * rj.hInv * lookup(wh, v2);
 
-            T tA1_i = -(c11i * rx + c12i * ry + c13i * rz) * Wi;
-            T tA2_i = -(c12i * rx + c22i * ry + c23i * rz) * Wi;
-            T tA3_i = -(c13i * rx + c23i * ry + c33i * rz) * Wi;
-            T c11j = c11[j], c12j = c12[j], c13j = c13[j], c22j = c22[j], c23j = c23[j], c33j = c33[j];
-            T tA1_j = -(c11j * rx + c12j * ry + c13j * rz) * Wj;
-            T tA2_j = -(c12j * rx + c22j * ry + c23j * rz) * Wj;
-            T tA3_j = -(c13j * rx + c23j * ry + c33j * rz) * Wj;
-
-            T mj = m[j], cj = c[j], kxj = kx[j], xmassj = xm[j];
-            T rhoj = kxj * mj / xmassj;
+            T tA1_i = -(ri.c11 * rx + ri.c12 * ry + ri.c13 * rz) * Wi;
+            T tA2_i = -(ri.c12 * rx + ri.c22 * ry + ri.c23 * rz) * Wi;
+            T tA3_i = -(ri.c13 * rx + ri.c23 * ry + ri.c33 * rz) * Wi;
+            T tA1_j = -(rj.c11 * rx + rj.c12 * ry + rj.c13 * rz) * Wj;
+            T tA2_j = -(rj.c12 * rx + rj.c22 * ry + rj.c23 * rz) * Wj;
+            T tA3_j = -(rj.c13 * rx + rj.c23 * ry + rj.c33 * rz) * Wj;
 
             T rv = rx * vx_ij + ry * vy_ij + rz * vz_ij;
             if constexpr (avClean)
@@ -390,50 +364,50 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
                 rv += avRvCorrection(rx, ry, rz, v1 < v2 ? v1 : v2, eta_crit, gradV_i, gradV_j);
             }
             T wij  = fastDiv(rv, dist);
-            T visc = artificialViscosity(alpha_i, alpha[j], ci, cj, wij);
-            T vsig = T(0.5) * (ci + cj) - T(2) * wij;
+            T visc = artificialViscosity(ri.alpha, rj.alpha, ri.c, rj.c, wij);
+            T vsig = T(0.5) * (ri.c + rj.c) - T(2) * wij;
             maxvs  = vsig > maxvs ? vsig : maxvs;
 
             T a_mom, b_mom;
-            T Atwood = fastDiv(fabs(rhoi - rhoj), rhoi + rhoj);
+            T Atwood = fastDiv(fabs(ri.rho - rj.rho), ri.rho + rj.rho);
             if (Atwood < Atmin)
             {
-                a_mom = xmassi * xmassi;
-                b_mom = xmassj * xmassj;
+                a_mom = ri.xm * ri.xm;
+                b_mom = rj.xm * rj.xm;
             }
             else if (Atwood > Atmax)
             {
-                a_mom = xmassi * xmassj;
+                a_mom = ri.xm * rj.xm;
                 b_mom = a_mom;
             }
             else
             {
                 T sigma = ramp * (Atwood - Atmin);
-                a_mom   = pow(xmassi, T(2) - sigma) * pow(xmassj, sigma);
-                b_mom   = pow(xmassj, T(2) - sigma) * pow(xmassi, sigma);
+                a_mom   = pow(ri.xm, T(2) - sigma) * pow(rj.xm, sigma);
+                b_mom   = pow(rj.xm, T(2) - sigma) * pow(ri.xm, sigma);
             }
 
-            T a_visc   = fastDiv(mj, rhoi) * visc;
-            T b_visc   = fastDiv(mj, rhoj) * visc;
+            T a_visc   = fastDiv(rj.m, ri.rho) * visc;
+            T b_visc   = rj.vol * visc;
             T a_visc_x = T(0.5) * (a_visc * tA1_i + b_visc * tA1_j);
             T a_visc_y = T(0.5) * (a_visc * tA2_i + b_visc * tA2_j);
             T a_visc_z = T(0.5) * (a_visc * tA3_i + b_visc * tA3_j);
             avEnergy.add(a_visc_x * vx_ij + a_visc_y * vy_ij + a_visc_z * vz_ij);
-            energy.add(mj * a_mom * (vx_ij * tA1_i + vy_ij * tA2_i + vz_ij * tA3_i));
+            energy.add(rj.m * a_mom * (vx_ij * tA1_i + vy_ij * tA2_i + vz_ij * tA3_i));
 
-            T mom_i = mj * prhoi * a_mom;
-            T mom_j = mj * prho[j] * b_mom;
+            T mom_i = rj.m * ri.prho * a_mom;
+            T mom_j = rj.m * rj.prho * b_mom;
             momx += mom_i * tA1_i + mom_j * tA1_j + a_visc_x;
             momy += mom_i * tA2_i + mom_j * tA2_j + a_visc_y;
             momz += mom_i * tA3_i + mom_j * tA3_j + a_visc_z;
         }
         T avE    = avEnergy.value() > T(0) ? avEnergy.value() : T(0);
-        T eCoeff = f.tdpdTrho ? ld<T>(f.tdpdTrho, i) : prhoi;
+        T eCoeff = f.tdpdTrho ? ld<T>(f.tdpdTrho, i) : ri.prho;
         f.du[i]  = a.K * double(eCoeff * energy.value() + T(0.5) * avE);
         st<T>(f.ax, i, T(-a.K * double(momx)));
         st<T>(f.ay, i, T(-a.K * double(momy)));
         st<T>(f.az, i, T(-a.K * double(momz)));
-        dtLane = courantDt(maxvs, hi, ci, Kcour);
+        dtLane = courantDt(maxvs, hi, ri.c, Kcour);
     }
     reduceMinDt<T>(minDt, dtLane);
 }
@@ -441,7 +415,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 unsigned long long* resetDt(sphx_workspace* ws)
 {
     ws->counters.reserve(8);
-    unsigned long long* dt  = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
+    unsigned long long* dt = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
     fill64(ws, reinterpret_cast<uint64_t*>(dt), 0x7ff0000000000000ull, 1); // +inf
     return dt;
 }
@@ -456,22 +430,73 @@ double readDt(sphx_workspace* ws, unsigned long long* dt)
     return r;
 }
 
+template<class T>
+void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecGradh<T>* rec = recordBuffer<RecGradh<T>>(ws, a.numParticles);
+    packGradh<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, (const T*)f->m,
+                                                                                          (const T*)f->xm, rec);
+    SPHX_CHECK_LAUNCH();
+    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+                                                                        whdTable<T>(ws), (T*)f->kx, (T*)f->gradh);
+    SPHX_CHECK_LAUNCH();
+}
+
+template<class T>
+void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles);
+    packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
+    SPHX_CHECK_LAUNCH();
+    iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, whTable<T>(ws));
+    SPHX_CHECK_LAUNCH();
+}
+
+template<class T>
+void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const sphx_params_t* p)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecAv<T>* rec = recordBuffer<RecAv<T>>(ws, a.numParticles);
+    packAv<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, af, rec);
+    SPHX_CHECK_LAUNCH();
+    avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, af, rec, whTable<T>(ws), p->minDt,
+                                                                        T(p->alphamin), T(p->alphamax),
+                                                                        T(p->decay_constant));
+    SPHX_CHECK_LAUNCH();
+}
+
+template<class T>
+void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, const sphx_params_t* p, bool avClean,
+                 unsigned long long* dt)
+{
+    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    if (blocks == 0) { return; }
+    RecMomVe<T>* rec = recordBuffer<RecMomVe<T>>(ws, a.numParticles);
+    packMomVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
+    SPHX_CHECK_LAUNCH();
+    if (avClean)
+        momentumVeKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            a, mf, rec, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+    else
+        momentumVeKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            a, mf, rec, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+    SPHX_CHECK_LAUNCH();
+}
+
 } // namespace
 
 void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                 uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->xm && f->kx && f->gradh, SPHX_ERR_ARG, "VeDefGradh needs h, m, xm, kx, gradh");
-    unsigned blocks = pairBlocks(first, last);
-    if (blocks == 0) { return; }
-#define LAUNCH(T)                                                                                                      \
-    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(                                                \
-        a, (const T*)f->h, (const T*)f->m, (const T*)f->xm, whTable<T>(ws), whdTable<T>(ws), (T*)f->kx, (T*)f->gradh)
-    if (p->hydro_type == SPHX_F32) LAUNCH(float);
-    else LAUNCH(double);
-#undef LAUNCH
-    SPHX_CHECK_LAUNCH();
+    if (p->hydro_type == SPHX_F32) launchGradh<float>(ws, a, f);
+    else launchGradh<double>(ws, a, f);
 }
 
 void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
@@ -500,7 +525,7 @@ void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, u
 void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                   uint32_t first, uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->xm && f->kx && f->vx && f->vy && f->vz && f->c11 && f->c12 && f->c13 && f->c22 &&
                      f->c23 && f->c33 && f->divv,
                  SPHX_ERR_ARG, "IadDivvCurlv needs h, xm, kx, v, c11..c33 and divv");
@@ -509,39 +534,27 @@ void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
                  "dV11..dV33 must be all set or all NULL");
     IadVeFields vf{f->h,   f->xm,   f->kx,    f->vx,   f->vy,   f->vz,   f->c11,  f->c12,  f->c13,  f->c22,
                    f->c23, f->c33, f->divv, f->curlv, f->dV11, f->dV12, f->dV13, f->dV22, f->dV23, f->dV33};
-    unsigned    blocks = pairBlocks(first, last);
-    if (blocks == 0) { return; }
-    if (p->hydro_type == SPHX_F32)
-        iadDivvCurlvKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, ws->wh32.get());
-    else
-        iadDivvCurlvKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, ws->wh64.get());
-    SPHX_CHECK_LAUNCH();
+    if (p->hydro_type == SPHX_F32) launchIadVe<float>(ws, a, vf);
+    else launchIadVe<double>(ws, a, vf);
 }
 
 void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                 uint32_t last)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->c && f->xm && f->kx && f->vx && f->vy && f->vz && f->divv && f->c11 && f->c12 && f->c13 &&
                      f->c22 && f->c23 && f->c33 && f->alpha,
                  SPHX_ERR_ARG, "AVswitches needs h, c, xm, kx, v, divv, c11..c33 and alpha");
     AvFields af{f->h, f->c, f->xm, f->kx, f->vx, f->vy, f->vz, f->divv, f->c11, f->c12, f->c13, f->c22, f->c23, f->c33,
                 f->alpha};
-    unsigned blocks = pairBlocks(first, last);
-    if (blocks == 0) { return; }
-    if (p->hydro_type == SPHX_F32)
-        avSwitchesKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, af, ws->wh32.get(), p->minDt, float(p->alphamin), float(p->alphamax), float(p->decay_constant));
-    else
-        avSwitchesKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, af, ws->wh64.get(), p->minDt, p->alphamin, p->alphamax, p->decay_constant);
-    SPHX_CHECK_LAUNCH();
+    if (p->hydro_type == SPHX_F32) launchAv<float>(ws, a, af, p);
+    else launchAv<double>(ws, a, af, p);
 }
 
 double momentumEnergyVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                         uint32_t first, uint32_t last, bool avClean)
 {
-    Common a = makeCommon(ws, box, f, p, first, last);
+    PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->vx && f->vy && f->vz && f->h && f->m && f->prho && f->c && f->c11 && f->c12 && f->c13 && f->c22 &&
                      f->c23 && f->c33 && f->kx && f->xm && f->alpha && f->ax && f->ay && f->az && f->du,
                  SPHX_ERR_ARG, "VE momentum-energy needs v, h, m, prho, c, c11..c33, kx, xm, alpha, a and du");
@@ -550,26 +563,9 @@ double momentumEnergyVe(sphx_workspace* ws, const DevBox& box, const sphx_fields
     MomVeFields mf{f->vx,  f->vy,   f->vz,   f->h,    f->m,    f->prho, f->tdpdTrho, f->c,  f->c11, f->c12,
                    f->c13, f->c22,  f->c23,  f->c33,  f->kx,   f->xm,   f->alpha,    f->dV11, f->dV12, f->dV13,
                    f->dV22, f->dV23, f->dV33, f->ax,   f->ay,   f->az,   f->du};
-    unsigned long long* dt     = resetDt(ws);
-    unsigned            blocks = pairBlocks(first, last);
-    if (blocks > 0)
-    {
-#define LAUNCH(T, C)                                                                                                   \
-    momentumVeKernel<T, C><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(                                             \
-        a, mf, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt)
-        if (p->hydro_type == SPHX_F32)
-        {
-            if (avClean) LAUNCH(float, true);
-            else LAUNCH(float, false);
-        }
-        else
-        {
-            if (avClean) LAUNCH(double, true);
-            else LAUNCH(double, false);
-        }
-#undef LAUNCH
-        SPHX_CHECK_LAUNCH();
-    }
+    unsigned long long* dt = resetDt(ws);
+    if (p->hydro_type == SPHX_F32) launchMomVe<float>(ws, a, mf, p, avClean, dt);
+    else launchMomVe<double>(ws, a, mf, p, avClean, dt);
     return readDt(ws, dt);
 }
 

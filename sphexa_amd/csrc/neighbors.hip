@@ -4,14 +4,16 @@
 // Per group the wave walks the octree once with the group's bounding box grown by the largest
 // search radius 2*h_max (exact periodic box-box distance, so no cell holding a neighbour of any
 // target is skipped), collecting candidate leaves in LDS; internal nodes live on an LDS stack
-// popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Each candidate leaf's
-// particles are then streamed through the wave with uniform (scalar) loads while every lane
-// tests them against its own target with the reference predicate (domain/include/cstone/
-// findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with dx = x_j - x_i, periodic
-// images folded by one box length, unfused FP64 arithmetic, keep if d2 < Th(4)*h_i*h_i and
-// j != i. The kept set and count are therefore bit-identical to the CPU search; the first ngmax
-// hits are stored, group-interleaved as [group][k][lane] so that the pair loops read them with
-// one coalesced 256-byte load per neighbour slot.
+// popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Candidate leaves that no
+// target's sphere reaches are dropped with a per-lane test and a ballot. The surviving leaves are
+// streamed as tiles of up to 64 particles through a double-buffered LDS area: coalesced vector
+// loads fetch tile q+1 while the wave tests tile q, reading each candidate with one broadcast
+// LDS access, every lane against its own target with the reference predicate
+// (domain/include/cstone/findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with
+// dx = x_j - x_i, periodic images folded by one box length, unfused FP64 arithmetic, keep if
+// d2 < Th(4)*h_i*h_i and j != i. The kept set and count are therefore bit-identical to the CPU
+// search; the first ngmax hits are stored, group-interleaved as [group][k][lane] so that the pair
+// loops read them with one coalesced 256-byte load per neighbour slot.
 #include "common.hpp"
 
 namespace sphx
@@ -21,7 +23,7 @@ namespace
 {
 
 constexpr int kWavesPerBlock = 4;
-constexpr int kStackCap      = 1024; // LDS node stack per wave (bound argued in traverse())
+constexpr int kStackCap      = 1024; // LDS node stack per wave (bound argued at searchGroup)
 constexpr int kLeafCap       = 256;  // LDS candidate-leaf buffer per wave
 
 struct SearchArgs
@@ -35,7 +37,7 @@ struct SearchArgs
     const double*        centers;
     const double*        sizes;
     DevBox               box;
-    double               slack; // absolute enlargement of the group radius (rounding margin)
+    double               slack; // absolute enlargement of search radii (rounding margin)
     uint32_t             first, last, ng0, ngmax;
     int                  iterateH;
     uint32_t*            nbrs;
@@ -43,10 +45,20 @@ struct SearchArgs
     unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles
 };
 
-//! per-target state used by the leaf sweep
+//! per-wave LDS: traversal stack, candidate leaves, double-buffered candidate tile
+struct WaveLds
+{
+    int32_t stack[kStackCap];
+    int32_t leaves[kLeafCap];
+    double  tx[2][kWave], ty[2][kWave], tz[2][kWave];
+};
+
+//! per-target state of a lane
 struct Target
 {
-    double   x, y, z, r2;
+    double   x, y, z;
+    double   r2;     // exact predicate radius^2 (formed in h's type)
+    double   reach2; // (2h + slack)^2, conservative radius for cell tests
     uint32_t i;
     bool     active;
 };
@@ -60,90 +72,143 @@ __device__ __forceinline__ double uniformD(double v)
     return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
 
-//! Test all particles of one candidate leaf against the wave's targets; hits are appended to
-//! each lane's list. `shift` is the periodic image offset (multiple of the box length, exact)
-//! valid for every pair that can be a neighbour; `foldPerPair` selects the reference's rint fold
-//! per pair when the box is too small for a single image per leaf.
-__device__ __forceinline__ void sweepLeaf(const SearchArgs& a, const Target& t, uint32_t lo, uint32_t hi,
-                                          double sx, double sy, double sz, bool shifted, bool foldPerPair,
+enum class Fold
+{
+    none,    // all pairs of the tile use the direct image
+    shifted, // one periodic image (shift multiple of the box length) serves all pairs
+    perPair  // box too small for a single image: rint fold per pair, as the reference
+};
+
+//! test the `n` staged candidates (particles lo..lo+n-1) against the lane's target
+template<Fold mode>
+__device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, const double* tx, const double* ty,
+                                          const double* tz, uint32_t lo, uint32_t n, double sx, double sy, double sz,
                                           unsigned& count, uint32_t* myList)
 {
 #pragma clang fp contract(off)
+    const DevBox&  b     = a.box;
     const unsigned ngmax = a.ngmax;
-    if (!shifted && !foldPerPair)
+#pragma unroll 4
+    for (uint32_t c = 0; c < n; ++c)
     {
-        for (uint32_t j = lo; j < hi; ++j)
+        double dx = tx[c] - t.x, dy = ty[c] - t.y, dz = tz[c] - t.z;
+        if constexpr (mode == Fold::shifted) { dx = dx - sx, dy = dy - sy, dz = dz - sz; }
+        if constexpr (mode == Fold::perPair)
         {
-            double dx = a.x[j] - t.x, dy = a.y[j] - t.y, dz = a.z[j] - t.z;
-            double d2 = dx * dx + dy * dy + dz * dz;
-            if (t.active && d2 < t.r2 && j != t.i)
-            {
-                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
-                count++;
-            }
+            dx = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
+            dy = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
+            dz = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
         }
-    }
-    else if (!foldPerPair)
-    {
-        for (uint32_t j = lo; j < hi; ++j)
+        double   d2 = dx * dx + dy * dy + dz * dz;
+        uint32_t j  = lo + c;
+        if (t.active && d2 < t.r2 && j != t.i)
         {
-            double dx = (a.x[j] - t.x) - sx, dy = (a.y[j] - t.y) - sy, dz = (a.z[j] - t.z) - sz;
-            double d2 = dx * dx + dy * dy + dz * dz;
-            if (t.active && d2 < t.r2 && j != t.i)
-            {
-                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
-                count++;
-            }
-        }
-    }
-    else
-    {
-        const DevBox& b = a.box;
-        for (uint32_t j = lo; j < hi; ++j)
-        {
-            double dx = a.x[j] - t.x, dy = a.y[j] - t.y, dz = a.z[j] - t.z;
-            dx        = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
-            dy        = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
-            dz        = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
-            double d2 = dx * dx + dy * dy + dz * dz;
-            if (t.active && d2 < t.r2 && j != t.i)
-            {
-                if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
-                count++;
-            }
+            if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
+            count++;
         }
     }
 }
 
-//! Sweep the buffered candidate leaves (wave-uniform loop).
-__device__ void flushLeaves(const SearchArgs& a, const Target& t, const int32_t* leafBuf, int nLeaf,
-                            const double gc[3], const double ge[3], double reach, unsigned& count, uint32_t* myList)
+//! Prune the buffered leaves against every lane's sphere, then stream the survivors through LDS.
+__device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
+                            const double ge[3], double reach, unsigned& count, uint32_t* myList)
 {
-    const DevBox& b = a.box;
+#pragma clang fp contract(off)
+    const DevBox&  b    = a.box;
+    const unsigned lane = laneId();
+
+    // 1. keep the leaves that at least one active target can reach (order preserved)
+    int kept = 0;
     for (int q = 0; q < nLeaf; ++q)
     {
-        int32_t  node = int32_t(uniform(uint32_t(leafBuf[q])));
-        uint32_t leaf = uniform(uint32_t(a.internalToLeaf[node]));
-        uint32_t lo = uniform(a.layout[leaf]), hi = uniform(a.layout[leaf + 1]);
-        if (lo == hi) { continue; }
-        double s[3], c[3], sh[3];
-        bool   shifted = false, fold = false;
+        int32_t node = int32_t(uniform(uint32_t(lds.leaves[q])));
+        double  p[3] = {t.x, t.y, t.z};
+        double  d2   = 0.0;
         for (int d = 0; d < 3; ++d)
         {
-            c[d]  = uniformD(a.centers[3 * size_t(node) + d]);
-            s[d]  = uniformD(a.sizes[3 * size_t(node) + d]);
-            sh[d] = 0.0;
-            if (b.pbc[d])
-            {
-                double k = rint((c[d] - gc[d]) * b.ilen[d]);
-                sh[d]    = k * b.len[d];
-                shifted  = shifted || (k != 0.0);
-                // a single image serves all pairs only if group + leaf + radius span < L/4
-                fold = fold || (ge[d] + s[d] + reach > 0.25 * b.len[d]);
-            }
+            double v = uniformD(a.centers[3 * size_t(node) + d]) - p[d];
+            if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
+            v = fabs(v) - uniformD(a.sizes[3 * size_t(node) + d]);
+            v = v > 0.0 ? v : 0.0;
+            d2 += v * v;
         }
-        sweepLeaf(a, t, lo, hi, sh[0], sh[1], sh[2], shifted, fold, count, myList);
+        if (ballot(t.active && d2 < t.reach2))
+        {
+            if (lane == 0) { lds.leaves[kept] = node; }
+            kept++;
+        }
     }
+    __builtin_amdgcn_wave_barrier();
+
+    // 2. tiles of <= 64 particles per kept leaf, software pipelined through two LDS buffers
+    int      q = 0;   // leaf being tiled
+    uint32_t off = 0; // offset of the next tile inside that leaf
+    uint32_t lo = 0, hi = 0;
+    auto     leafRange = [&](int qq, uint32_t& l, uint32_t& h, int32_t& node)
+    {
+        node          = int32_t(uniform(uint32_t(lds.leaves[qq])));
+        uint32_t leaf = uniform(uint32_t(a.internalToLeaf[node]));
+        l             = uniform(a.layout[leaf]);
+        h             = uniform(a.layout[leaf + 1]);
+    };
+    // advance (q, off) to the next non-empty tile; returns false at the end
+    auto nextTile = [&](int32_t& node, uint32_t& tlo, uint32_t& tn) -> bool
+    {
+        while (q < kept)
+        {
+            leafRange(q, lo, hi, node);
+            if (lo + off < hi)
+            {
+                tlo = lo + off;
+                tn  = min(hi - tlo, uint32_t(kWave));
+                off += kWave;
+                if (lo + off >= hi) { q++, off = 0; }
+                return true;
+            }
+            q++, off = 0;
+        }
+        return false;
+    };
+
+    int32_t  node;
+    uint32_t tlo, tn;
+    bool     have = nextTile(node, tlo, tn);
+    double   px = 0, py = 0, pz = 0;
+    if (have && lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
+    int buf = 0;
+    while (have)
+    {
+        if (lane < tn) { lds.tx[buf][lane] = px, lds.ty[buf][lane] = py, lds.tz[buf][lane] = pz; }
+        __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
+
+        int32_t  curNode = node;
+        uint32_t curLo = tlo, curN = tn;
+        have = nextTile(node, tlo, tn);
+        if (have && lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
+
+        // periodic image of the current leaf relative to the group
+        double sh[3] = {0, 0, 0};
+        bool   shifted = false, perPair = false;
+        for (int d = 0; d < 3; ++d)
+        {
+            if (!b.pbc[d]) { continue; }
+            double c = uniformD(a.centers[3 * size_t(curNode) + d]);
+            double s = uniformD(a.sizes[3 * size_t(curNode) + d]);
+            double k = rint((c - gc[d]) * b.ilen[d]);
+            sh[d]    = k * b.len[d];
+            shifted  = shifted || (k != 0.0);
+            perPair  = perPair || (ge[d] + s + reach > 0.25 * b.len[d]);
+        }
+        if (perPair)
+            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, myList);
+        else if (shifted)
+            sweepTile<Fold::shifted>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, sh[0], sh[1], sh[2],
+                                     count, myList);
+        else
+            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, myList);
+        buf ^= 1;
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 /* Collect all leaves whose (periodic) distance to the active targets' bounding box is below the
@@ -152,9 +217,8 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, const int32_t*
  * Stack bound: nodes are popped 64 at a time only while top + 7*64 <= kStackCap - 160, else one
  * at a time. A one-at-a-time pop replaces a node by at most 8 children and the next pop takes
  * the last child, i.e. it proceeds depth first, which adds at most 7 entries per tree level
- * (<= 7*21 = 147 < 160) on top of the stack it started from. */
-__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, int32_t* stack, int32_t* leafBuf,
-                                uint32_t* myList)
+ * below the popped node (<= 7*20 = 140 < 160) on top of the stack it started from. */
+__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList)
 {
 #pragma clang fp contract(off)
     const unsigned lane = laneId();
@@ -175,14 +239,14 @@ __device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double rad
     unsigned count = 0;
     int      top   = 1;
     int      nLeaf = 0;
-    if (lane == 0) { stack[0] = 0; }
+    if (lane == 0) { lds.stack[0] = 0; }
     __builtin_amdgcn_wave_barrier();
 
     while (top > 0)
     {
         int k = (top + 7 * kWave <= kStackCap - 160) ? min(kWave, top) : 1;
         int node = -1;
-        if (int(lane) < k) { node = stack[top - k + int(lane)]; }
+        if (int(lane) < k) { node = lds.stack[top - k + int(lane)]; }
         top -= k;
         __builtin_amdgcn_wave_barrier();
 
@@ -208,21 +272,21 @@ __device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double rad
         int      nl    = __popcll(mLeaf);
         if (nLeaf + nl > kLeafCap)
         {
-            flushLeaves(a, t, leafBuf, nLeaf, gc, ge, reach, count, myList);
+            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, myList);
             nLeaf = 0;
         }
-        if (ov && isLeaf) { leafBuf[nLeaf + laneRank(mLeaf)] = node; }
+        if (ov && isLeaf) { lds.leaves[nLeaf + laneRank(mLeaf)] = node; }
         if (ov && !isLeaf)
         {
             int pos = top + 8 * int(laneRank(mInt));
             for (int c = 0; c < 8; ++c)
-                stack[pos + c] = child + c;
+                lds.stack[pos + c] = child + c;
         }
         nLeaf += nl;
         top += 8 * __popcll(mInt);
         __builtin_amdgcn_wave_barrier();
     }
-    flushLeaves(a, t, leafBuf, nLeaf, gc, ge, reach, count, myList);
+    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, myList);
     return count;
 }
 
@@ -237,16 +301,26 @@ __device__ __forceinline__ Th updateH(unsigned ng0, unsigned nc, Th h)
 }
 
 template<class Th>
+__device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
+{
+#pragma clang fp contract(off)
+    Th r2      = Th(4.0) * hi * hi; // radius in h's type, as the reference
+    t.r2       = double(r2);
+    double rr  = 2.0 * double(hi) + slack;
+    t.reach2   = rr * rr;
+}
+
+template<class Th>
 __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArgs a, Th* __restrict__ h)
 {
-    __shared__ int32_t stackMem[kWavesPerBlock][kStackCap];
-    __shared__ int32_t leafMem[kWavesPerBlock][kLeafCap];
+    __shared__ WaveLds ldsMem[kWavesPerBlock];
 
     const unsigned wid  = threadIdx.x / kWave;
     const unsigned lane = laneId();
     const uint32_t g    = blockIdx.x * kWavesPerBlock + wid;
     const uint32_t base = a.first + g * kGroup;
     if (base >= a.last) { return; }
+    WaveLds& lds = ldsMem[wid];
 
     Target t;
     t.i      = base + lane;
@@ -255,15 +329,11 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
     t.x      = t.active ? a.x[t.i] : 0.0;
     t.y      = t.active ? a.y[t.i] : 0.0;
     t.z      = t.active ? a.z[t.i] : 0.0;
-    {
-#pragma clang fp contract(off)
-        Th r2 = Th(4.0) * hi * hi; // radius in h's type, as the reference
-        t.r2  = double(r2);
-    }
+    setRadius(t, hi, a.slack);
     uint32_t* myList = a.nbrs + size_t(g) * a.ngmax * kGroup + lane;
 
     bool     valid = t.active;
-    unsigned count = searchGroup(a, t, double(2) * double(hi), stackMem[wid], leafMem[wid], myList);
+    unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList);
     unsigned ncSph = 1 + count;
 
     int      updates = 0;
@@ -278,14 +348,10 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
             {
                 updates++;
                 hi = updateH(a.ng0, ncSph, hi);
-                {
-#pragma clang fp contract(off)
-                    Th r2 = Th(4.0) * hi * hi;
-                    t.r2  = double(r2);
-                }
+                setRadius(t, hi, a.slack);
             }
             t.active       = repeat;
-            unsigned again = searchGroup(a, t, double(2) * double(hi), stackMem[wid], leafMem[wid], myList);
+            unsigned again = searchGroup(a, t, double(2) * double(hi), lds, myList);
             if (repeat) { ncSph = 1 + again; }
         }
     }

@@ -185,6 +185,48 @@ __device__ __forceinline__ T artificialViscosity(T alpha_i, T alpha_j, T c_i, T 
     return visc;
 }
 
+//! arguments shared by all pair kernels
+struct PairArgs
+{
+    const double* x;
+    const double* y;
+    const double* z;
+    DevBox        box;
+    double        K;
+    Lists         L;
+    uint32_t      numParticles; // all particles that can appear as neighbours
+};
+
+inline PairArgs makePairArgs(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                             uint32_t first, uint32_t last)
+{
+    SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "pair loop called before find_neighbors");
+    SPHX_REQUIRE(first == ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
+                 "target range must start at the searched first index and stay inside the searched range");
+    SPHX_REQUIRE(p->ngmax == ws->nbNgmax, SPHX_ERR_ARG, "ngmax differs from the one used by find_neighbors");
+    SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
+    SPHX_REQUIRE(f->x && f->y && f->z && f->nc, SPHX_ERR_ARG, "x, y, z and nc are required");
+    PairArgs c;
+    c.x = f->x, c.y = f->y, c.z = f->z, c.box = box, c.K = p->K;
+    c.L            = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax};
+    c.numParticles = uint32_t(ws->tree.numParticles);
+    return c;
+}
+
+template<class T>
+inline const T* whTable(sphx_workspace* ws)
+{
+    if constexpr (isMixed<T>) { return ws->wh32.get(); }
+    else { return ws->wh64.get(); }
+}
+
+template<class T>
+inline const T* whdTable(sphx_workspace* ws)
+{
+    if constexpr (isMixed<T>) { return ws->whd32.get(); }
+    else { return ws->whd64.get(); }
+}
+
 //! wave minimum then one atomic per wave
 template<class T>
 __device__ __forceinline__ void reduceMinDt(unsigned long long* dst, double dt)

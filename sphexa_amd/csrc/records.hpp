@@ -1,0 +1,82 @@
+// Per-particle neighbour records for the pair loops.
+//
+// A pair loop reads, for every neighbour j of its lane's target, a handful of fields of j.
+// Read from the SoA Dataset arrays, every field is a separate 64-lane gather touching tens of
+// cache lines; the loops were bound by the vector-memory address path, not by ALU or HBM.
+// Before each pair loop a coalesced pack kernel therefore writes exactly the fields that loop
+// needs for j into one 16-byte aligned record per particle (workspace memory, SFC order), so a
+// neighbour costs a few 16-byte loads from one or two cache lines. Derived per-particle
+// quantities (1/h, m/rho, xm/kx, kx m/xm) are formed in the pack with the same expression the
+// pair loop would evaluate, so results are unchanged.
+#pragma once
+
+#include "pairs.hpp"
+
+namespace sphx
+{
+
+//! XMass / std density: position and mass
+template<class T>
+struct alignas(16) RecMass
+{
+    double x, y, z;
+    T      m;
+};
+
+//! VeDefGradh: position, mass, volume-element mass xm
+template<class T>
+struct alignas(16) RecGradh
+{
+    double x, y, z;
+    T      m, xm;
+};
+
+//! std IAD: position and volume m/rho
+template<class T>
+struct alignas(16) RecVol
+{
+    double x, y, z;
+    T      vol;
+};
+
+//! VE IAD + div/curl v: position, volume xm/kx, xm, velocity
+template<class T>
+struct alignas(16) RecIadVe
+{
+    double x, y, z;
+    T      vol, xm, vx, vy, vz;
+};
+
+//! AV switches: position, volume xm/kx, sound speed, div v, velocity
+template<class T>
+struct alignas(16) RecAv
+{
+    double x, y, z;
+    T      vol, c, divv, vx, vy, vz;
+};
+
+//! std momentum-energy: vol = m/rho, pro = p/rho
+template<class T>
+struct alignas(16) RecMomStd
+{
+    double x, y, z;
+    T      vx, vy, vz, hInv, m, vol, pro, c, c11, c12, c13, c22, c23, c33;
+};
+
+//! VE momentum-energy: rho = kx m/xm, vol = m/rho (the AV-cleaning gradients stay in SoA)
+template<class T>
+struct alignas(16) RecMomVe
+{
+    double x, y, z;
+    T      vx, vy, vz, hInv, m, c, rho, vol, xm, alpha, prho, c11, c12, c13, c22, c23, c33;
+};
+
+//! workspace buffer for `n` records of type R
+template<class R>
+R* recordBuffer(sphx_workspace* ws, size_t n)
+{
+    ws->records.reserve((n * sizeof(R) + 15) / 16);
+    return reinterpret_cast<R*>(ws->records.get());
+}
+
+} // namespace sphx

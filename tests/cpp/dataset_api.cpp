@@ -1,0 +1,126 @@
+// Drives one std or VE hydro pass through the C++ interface (include/sphexa_amd.hpp) the way the
+// reference propagators call the hydro functions (main/src/propagator/std_hydro.hpp:124-168,
+// main/src/propagator/ve_hydro.hpp:130-204), reading the initial conditions from a binary file
+// and writing the SFC-ordered results for comparison with the CPU oracle.
+//
+//   dataset_api <in.bin> <out.bin> std|ve
+//
+// in.bin : uint64 n, uint32 periodic, double lo, double hi, then n doubles each of
+//          x y z h m u vx vy vz alpha (u is the temperature for the VE pass)
+// out.bin: n doubles each of x, then the pass outputs (std: rho p c c11..c33 ax ay az du;
+//          VE: xm kx gradh prho c c11..c33 divv curlv alpha ax ay az du), n uint32 nc, double minDt
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sphexa_amd.hpp"
+
+using namespace sphexa_amd;
+
+#define HIPCHECK(x)                                                                                                    \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if ((x) != hipSuccess)                                                                                         \
+        {                                                                                                              \
+            std::fprintf(stderr, "HIP error %s at %d\n", #x, __LINE__);                                                 \
+            return 2;                                                                                                  \
+        }                                                                                                              \
+    } while (0)
+
+int main(int argc, char** argv)
+{
+    if (argc != 4) { return 1; }
+    std::string form = argv[3];
+    FILE*       in   = std::fopen(argv[1], "rb");
+    if (!in) { return 1; }
+    uint64_t n;
+    uint32_t periodic;
+    double   lo, hi;
+    if (std::fread(&n, 8, 1, in) != 1 || std::fread(&periodic, 4, 1, in) != 1 || std::fread(&lo, 8, 1, in) != 1 ||
+        std::fread(&hi, 8, 1, in) != 1)
+        return 1;
+    std::vector<std::vector<double>> ic(10, std::vector<double>(n));
+    for (auto& v : ic)
+        if (std::fread(v.data(), 8, n, in) != n) return 1;
+    std::fclose(in);
+
+    DeviceDataset<double> d;
+    if (sphx_workspace_create(&d.workspace, 0, nullptr) != SPHX_OK) { return 2; }
+    if (sphx_workspace_set_kernel(d.workspace, 6.0, &d.K) != SPHX_OK) { return 2; }
+    d.minDt = 1e-3;
+
+    std::vector<double*> owned;
+    auto alloc = [&](double*& p) {
+        if (hipMalloc(&p, n * 8) != hipSuccess) throw std::runtime_error("hipMalloc");
+        if (hipMemset(p, 0, n * 8) != hipSuccess) throw std::runtime_error("hipMemset");
+        owned.push_back(p);
+    };
+    auto& s = d.devData;
+    bool  ve = form == "ve";
+    for (double** p : {&s.x, &s.y, &s.z, &s.du, &s.h, &s.m, &s.vx, &s.vy, &s.vz, &s.c, &s.c11, &s.c12, &s.c13,
+                       &s.c22, &s.c23, &s.c33, &s.ax, &s.ay, &s.az})
+        alloc(*p);
+    if (ve)
+        for (double** p : {&s.temp, &s.prho, &s.xm, &s.kx, &s.gradh, &s.divv, &s.curlv, &s.alpha}) alloc(*p);
+    else
+        for (double** p : {&s.u, &s.rho, &s.p}) alloc(*p);
+    HIPCHECK(hipMalloc(&s.keys, n * 8));
+    HIPCHECK(hipMalloc(&s.nc, n * 4));
+    double* icDst[10] = {s.x, s.y, s.z, s.h, s.m, ve ? s.temp : s.u, s.vx, s.vy, s.vz, ve ? s.alpha : nullptr};
+    for (int k = 0; k < 10; ++k)
+        if (icDst[k]) HIPCHECK(hipMemcpy(icDst[k], ic[k].data(), n * 8, hipMemcpyHostToDevice));
+
+    Box       box(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
+    GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
+    try
+    {
+        syncDomain(n, d, box);
+        findNeighborsSfc(0, n, d, box);
+        if (ve)
+        {
+            computeXMass(grp, d, box);
+            computeVeDefGradh(grp, d, box);
+            computeEOS(0, n, d);
+            computeIadDivvCurlv(grp, d, box);
+            computeAVswitches(grp, d, box);
+            computeMomentumEnergy<false>(grp, nullptr, d, box);
+        }
+        else
+        {
+            computeDensity(grp, d, box);
+            computeEOS_HydroStd(0, n, d);
+            computeIAD(grp, d, box);
+            computeMomentumEnergySTD(grp, d, box);
+        }
+    }
+    catch (const std::exception& e)
+    {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 3;
+    }
+
+    std::vector<double*> outs = ve ? std::vector<double*>{s.x, s.xm, s.kx, s.gradh, s.prho, s.c, s.c11, s.c12,
+                                                          s.c13, s.c22, s.c23, s.c33, s.divv, s.curlv, s.alpha,
+                                                          s.ax, s.ay, s.az, s.du}
+                                   : std::vector<double*>{s.x, s.rho, s.p, s.c, s.c11, s.c12, s.c13, s.c22, s.c23,
+                                                          s.c33, s.ax, s.ay, s.az, s.du};
+    FILE* out = std::fopen(argv[2], "wb");
+    std::vector<double> buf(n);
+    for (double* p : outs)
+    {
+        HIPCHECK(hipMemcpy(buf.data(), p, n * 8, hipMemcpyDeviceToHost));
+        std::fwrite(buf.data(), 8, n, out);
+    }
+    std::vector<uint32_t> nc(n);
+    HIPCHECK(hipMemcpy(nc.data(), s.nc, n * 4, hipMemcpyDeviceToHost));
+    std::fwrite(nc.data(), 4, n, out);
+    std::fwrite(&d.minDtCourant, 8, 1, out);
+    std::fclose(out);
+    for (double* p : owned) (void)hipFree(p);
+    (void)hipFree(s.keys), (void)hipFree(s.nc);
+    sphx_workspace_destroy(d.workspace);
+    return 0;
+}

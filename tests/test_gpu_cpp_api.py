@@ -1,0 +1,64 @@
+"""The C++ host interface (include/sphexa_amd.hpp) driven like the reference propagators
+(tests/cpp/dataset_api.cpp) reproduces the CPU oracle's pass."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from sphexa_amd import abi, ic
+from sphexa_amd.particles import ideal_gas_cv
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "dataset_api")
+
+STD = ("rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az", "du")
+VE = ("xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "c33", "divv", "curlv", "alpha", "ax", "ay",
+      "az", "du")
+
+
+@pytest.mark.parametrize("form", ["std", "ve"])
+def test_cpp_interface_pass(tmp_path, orc, form):
+    from oracle import pipeline
+
+    assert os.path.exists(DRIVER), "tests/cpp/dataset_api not built (python -m sphexa_amd.build)"
+    if form == "std":
+        arr, spec = ic.sedov_lattice(16, seed=21)
+        arr["vx"] = 0.1 * np.sin(6.0 * arr["x"])
+        arr["vy"] = 0.05 * np.cos(5.0 * arr["y"])
+        arr["vz"] = 0.02 * arr["z"]
+        third = arr["u"]
+    else:
+        arr, spec = ic.turbulence_box(16, seed=22)
+        arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
+        arr["alpha"] = np.random.default_rng(2).uniform(0.05, 1.0, arr["x"].size)
+        third = arr["temp"]
+    n = arr["x"].size
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([n], np.uint64).tobytes())
+        fh.write(np.array([1], np.uint32).tobytes())
+        fh.write(np.array([spec["lo"], spec["hi"]], np.float64).tobytes())
+        for col in (arr["x"], arr["y"], arr["z"], arr["h"], arr["m"], third, arr["vx"], arr["vy"], arr["vz"],
+                    arr["alpha"]):
+            fh.write(np.ascontiguousarray(col, np.float64).tobytes())
+    r = subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), form], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(tmp_path / "out.bin", "rb").read()
+    names = ("x",) + (VE if form == "ve" else STD)
+    k = len(names) * n * 8
+    vals = np.frombuffer(raw[:k], np.float64).reshape(len(names), n)
+    nc = np.frombuffer(raw[k:k + 4 * n], np.uint32)
+    dt = float(np.frombuffer(raw[k + 4 * n:], np.float64)[0])
+
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    o = pipeline.run(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0), minDt=1e-3), form=form)
+    assert np.array_equal(vals[0], o["a"]["x"])
+    assert np.array_equal(nc, o["a"]["nc"])
+    for i, name in enumerate(names[1:], 1):
+        ref = o["a"][name]
+        err = np.max(np.abs(vals[i] - ref)) / max(np.max(np.abs(ref)), 1e-300)
+        assert err < 1e-10, (name, err)
+    assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
