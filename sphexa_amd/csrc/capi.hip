@@ -84,6 +84,31 @@ std::vector<T> tabulate(F&& f)
     return t;
 }
 
+//! interval k holds the table values at both of its ends; interval kTableSize-1, past the
+//! support, holds zeros (the value lookups return there)
+template<class T>
+std::vector<TabPair<T>> intervals(const std::vector<T>& t)
+{
+    std::vector<TabPair<T>> p(kTableSize, TabPair<T>{T(0), T(0)});
+    for (int k = 0; k + 1 < kTableSize; ++k) { p[k] = TabPair<T>{t[k], t[k + 1]}; }
+    return p;
+}
+
+template<class T>
+std::vector<TabQuad<T>> intervals(const std::vector<T>& w, const std::vector<T>& d)
+{
+    std::vector<TabQuad<T>> p(kTableSize, TabQuad<T>{T(0), T(0), T(0), T(0)});
+    for (int k = 0; k + 1 < kTableSize; ++k) { p[k] = TabQuad<T>{w[k], w[k + 1], d[k], d[k + 1]}; }
+    return p;
+}
+
+template<class E>
+void upload(sphx_workspace* ws, DevBuf<E>& buf, const std::vector<E>& host)
+{
+    buf.reserve(host.size());
+    SPHX_HIP(hipMemcpyAsync(buf.get(), host.data(), host.size() * sizeof(E), hipMemcpyHostToDevice, ws->stream));
+}
+
 void requireTables(const sphx_workspace* ws)
 {
     SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
@@ -121,6 +146,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     (void)hipSetDevice(ws->device);
     (void)hipStreamSynchronize(ws->stream);
     ws->wh64.release(), ws->whd64.release(), ws->wh32.release(), ws->whd32.release();
+    ws->whq64.release(), ws->whq32.release();
     ws->tree.releaseAll();
     ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release(), ws->idxTmp.release();
     ws->fieldTmp.release(), ws->order.release(), ws->records.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
@@ -138,12 +164,12 @@ int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K)
         auto wd64 = tabulate<double>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
         auto w32  = tabulate<float>([sincIndex](double v) { return sincPow(v, sincIndex); });
         auto wd32 = tabulate<float>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
-        ws->wh64.reserve(kTableSize), ws->whd64.reserve(kTableSize);
-        ws->wh32.reserve(kTableSize), ws->whd32.reserve(kTableSize);
-        SPHX_HIP(hipMemcpyAsync(ws->wh64.get(), w64.data(), kTableSize * 8, hipMemcpyHostToDevice, ws->stream));
-        SPHX_HIP(hipMemcpyAsync(ws->whd64.get(), wd64.data(), kTableSize * 8, hipMemcpyHostToDevice, ws->stream));
-        SPHX_HIP(hipMemcpyAsync(ws->wh32.get(), w32.data(), kTableSize * 4, hipMemcpyHostToDevice, ws->stream));
-        SPHX_HIP(hipMemcpyAsync(ws->whd32.get(), wd32.data(), kTableSize * 4, hipMemcpyHostToDevice, ws->stream));
+        upload(ws, ws->wh64, intervals(w64));
+        upload(ws, ws->whd64, intervals(wd64));
+        upload(ws, ws->wh32, intervals(w32));
+        upload(ws, ws->whd32, intervals(wd32));
+        upload(ws, ws->whq64, intervals(w64, wd64));
+        upload(ws, ws->whq32, intervals(w32, wd32));
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         ws->K           = normalization(sincIndex);
         ws->sincIndex   = sincIndex;

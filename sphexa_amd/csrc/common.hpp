@@ -93,6 +93,20 @@ struct TreeBuildScratch
     }
 };
 
+//! one interpolation interval of a kernel table, values at both ends, fetched as one vector load
+template<class T>
+struct alignas(2 * sizeof(T)) TabPair
+{
+    T t0, t1;
+};
+
+//! W and dW/dv over one interval, for loops that evaluate both at the same argument
+template<class T>
+struct alignas(4 * sizeof(T)) TabQuad
+{
+    T w0, w1, d0, d1;
+};
+
 //! 16-byte storage unit of the neighbour-record buffer
 struct alignas(16) Chunk16
 {
@@ -127,10 +141,12 @@ struct sphx_workspace
     hipStream_t stream = nullptr;
     std::string lastError;
 
-    // kernel tables: W and dW/dv on [0,2], 20000 points, in both precisions
-    double                 K = 0, sincIndex = 0;
-    sphx::DevBuf<double>   wh64, whd64;
-    sphx::DevBuf<float>    wh32, whd32;
+    // kernel tables: W and dW/dv on [0,2], 20000 points, in both precisions, stored per interval
+    double                              K = 0, sincIndex = 0;
+    sphx::DevBuf<sphx::TabPair<double>> wh64, whd64;
+    sphx::DevBuf<sphx::TabPair<float>>  wh32, whd32;
+    sphx::DevBuf<sphx::TabQuad<double>> whq64;
+    sphx::DevBuf<sphx::TabQuad<float>>  whq32;
     bool                   tablesValid = false;
 
     sphx::Tree tree;
@@ -166,6 +182,19 @@ inline void* scratchBytes(sphx_workspace* ws, size_t bytes)
 }
 
 inline unsigned ceilDiv(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
+
+constexpr unsigned kNumXcd = 8; // MI355X: 8 accelerator complex dies, one L2 each
+
+//! The hardware deals the blocks of a launch round-robin over the XCDs (block b runs on XCD b % 8).
+//! Renumber them so that each XCD works through one contiguous run of logical blocks: groups that are
+//! adjacent in SFC order, and so share most of their neighbours, then run under the same L2.
+//! A bijection of [0, nb) for every nb.
+__host__ __device__ inline uint32_t xcdBlock(uint32_t b, uint32_t nb)
+{
+    uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    uint32_t xcd = b % kNumXcd, idx = b / kNumXcd;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
 
 /* ---------------------------------------------------------------------------------------------
  * bit helpers (placeholder-bit keys, cf. domain/include/cstone/sfc/common.hpp)

@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
  * toRho: computeDensity (hydro_std/density.hpp:42-53) stores rho = m / xm instead. */
 template<class T, bool toRho>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, const T* __restrict__ wh,
+    xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, const TabPair<T>* __restrict__ wh,
                 T* __restrict__ out)
 {
     Lane t = laneTarget(a.L);
@@ -82,15 +82,21 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T                h3Inv = hInv * hInv * hInv;
     double           r2h   = 2.0 * double(hi);
     Acc<T>           rho0(ri.m);
-    for (unsigned k = 0; k < t.n; ++k)
-    {
-        const RecMass<T> rj = rec[t.list[size_t(k) * kGroup]];
-        T                rx, ry, rz;
-        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-        T dist = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T w    = lookup(wh, dist * hInv);
-        rho0.add(w * rj.m);
-    }
+    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        constexpr int                U = 4;
+        NeighborBatch<U, RecMass<T>> nb(t, rec);
+        for (unsigned k = 0; k < t.n; k += U)
+        {
+            nb.load(k);
+            TableFetch<T, TabPair<T>> w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                w[u] = fetch(wh, pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                rho0.add(nb.live(k, u, value(w[u])) * nb.r[u].m);
+        }
+    });
     double rhoZero = double(rho0.value()) * a.K * double(h3Inv);
     T      xmi     = T(double(ri.m) / rhoZero);
     out[i]         = toRho ? T(ri.m / xmi) : xmi;
@@ -113,7 +119,7 @@ __global__ void __launch_bounds__(256) eosStdKernel(const double* __restrict__ u
 /* IADJLoopSTD (hydro_std/iad_kern.hpp:11-76): tau_ab = sum_j m_j/rho_j W_ij r_a r_b, inverted */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, const T* __restrict__ wh,
+    iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, const TabPair<T>* __restrict__ wh,
                  T* c11, T* c12, T* c13, T* c22, T* c23, T* c33)
 {
     Lane t = laneTarget(a.L);
@@ -124,21 +130,34 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T               hiInv = T(1) / hi;
     double          r2h   = 2.0 * double(hi);
     T               t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
-    for (unsigned k = 0; k < t.n; ++k)
-    {
-        const RecVol<T> rj = rec[t.list[size_t(k) * kGroup]];
-        T               rx, ry, rz;
-        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-        T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T w     = lookup(wh, dist * hiInv);
-        T vol_w = rj.vol * w;
-        t11 += rx * rx * vol_w;
-        t12 += rx * ry * vol_w;
-        t13 += rx * rz * vol_w;
-        t22 += ry * ry * vol_w;
-        t23 += ry * rz * vol_w;
-        t33 += rz * rz * vol_w;
-    }
+    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        constexpr int               U = isMixed<T> ? 4 : 2;
+        NeighborBatch<U, RecVol<T>> nb(t, rec);
+        for (unsigned k = 0; k < t.n; k += U)
+        {
+            nb.load(k);
+            Sep<T>                    r[U];
+            TableFetch<T, TabPair<T>> w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
+                w[u] = fetch(wh, r[u].d * hiInv);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                T rx = r[u].x, ry = r[u].y, rz = r[u].z;
+                T vol_w = nb.r[u].vol * nb.live(k, u, value(w[u]));
+                t11 += rx * rx * vol_w;
+                t12 += rx * ry * vol_w;
+                t13 += rx * rz * vol_w;
+                t22 += ry * ry * vol_w;
+                t23 += ry * rz * vol_w;
+                t33 += rz * rz * vol_w;
+            }
+        }
+    });
     double cij[6];
     iadInvert(t11, t12, t13, t22, t23, t33, double(hi), a.K, cij);
     c11[i] = T(cij[0]), c12[i] = T(cij[1]), c13[i] = T(cij[2]);
@@ -148,7 +167,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 /* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, const T* __restrict__ wh,
+    momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh,
                       float Kcour, unsigned long long* minDt)
 {
     Lane   t      = laneTarget(a.L);
@@ -166,49 +185,62 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
-        Acc<T> energy;
-        for (unsigned k = 0; k < t.n; ++k)
-        {
-            const RecMomStd<T> rj = rec[t.list[size_t(k) * kGroup]];
-            T                  rx, ry, rz;
-            separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-            T r2    = rx * rx + ry * ry + rz * rz;
-            T dist  = fastSqrt(r2);
-            T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
-            T v1    = dist * hiInv;
-            T v2    = dist * rj.hInv;
-            T rv    = rx * vx_ij + ry * vy_ij + rz * vz_ij;
-            T Wi    = hiInv3 * lookup(wh, v1);
-            T Wj    = rj.hInv * rj.hInv * rj.hInv * lookup(wh, v2);
-
-            T tA1_i = ri.c11 * rx + ri.c12 * ry + ri.c13 * rz;
-            T tA2_i = ri.c12 * rx + ri.c22 * ry + ri.c23 * rz;
-            T tA3_i = ri.c13 * rx + ri.c23 * ry + ri.c33 * rz;
-            T tA1_j = rj.c11 * rx + rj.c12 * ry + rj.c13 * rz;
-            T tA2_j = rj.c12 * rx + rj.c22 * ry + rj.c23 * rz;
-            T tA3_j = rj.c13 * rx + rj.c23 * ry + rj.c33 * rz;
-
-            T wij  = fastDiv(rv, dist);
-            T visc = T(0.5) * artificialViscosity(T(1), T(1), ri.c, rj.c, wij);
-            T vsig = ri.c + rj.c - T(3) * wij;
-            maxvs  = vsig > maxvs ? vsig : maxvs;
-
-            T mj_roj_Wj = rj.vol * Wj;
-            T mj_pro_i  = rj.m * pro_i;
+        Acc<T>       energy;
+        withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+            constexpr int                  U = isMixed<T> ? 2 : 1;
+            NeighborBatch<U, RecMomStd<T>> nb(t, rec);
+            for (unsigned k = 0; k < t.n; k += U)
             {
-                T aa = Wi * (mj_pro_i + visc * ri.vol);
-                T bb = mj_roj_Wj * (rj.pro + visc);
-                momx += aa * tA1_i + bb * tA1_j;
-                momy += aa * tA2_i + bb * tA2_j;
-                momz += aa * tA3_i + bb * tA3_j;
+                nb.load(k);
+                Sep<T>                    r[U];
+                TableFetch<T, TabPair<T>> wi[U], wj[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    r[u]  = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
+                    wi[u] = fetch(wh, r[u].d * hiInv);
+                    wj[u] = fetch(wh, r[u].d * nb.r[u].hInv);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    const RecMomStd<T>& rj = nb.r[u];
+                    T rx = r[u].x, ry = r[u].y, rz = r[u].z, dist = r[u].d;
+                    T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
+                    T rv    = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+                    T Wi    = hiInv3 * nb.live(k, u, value(wi[u]));
+                    T Wj    = rj.hInv * rj.hInv * rj.hInv * nb.live(k, u, value(wj[u]));
+
+                    T tA1_i = ri.c11 * rx + ri.c12 * ry + ri.c13 * rz;
+                    T tA2_i = ri.c12 * rx + ri.c22 * ry + ri.c23 * rz;
+                    T tA3_i = ri.c13 * rx + ri.c23 * ry + ri.c33 * rz;
+                    T tA1_j = rj.c11 * rx + rj.c12 * ry + rj.c13 * rz;
+                    T tA2_j = rj.c12 * rx + rj.c22 * ry + rj.c23 * rz;
+                    T tA3_j = rj.c13 * rx + rj.c23 * ry + rj.c33 * rz;
+
+                    T wij  = fastDiv(rv, dist);
+                    T visc = T(0.5) * artificialViscosity(T(1), T(1), ri.c, rj.c, wij);
+                    T vsig = ri.c + rj.c - T(3) * wij;
+                    maxvs  = nb.in(k, u) && vsig > maxvs ? vsig : maxvs;
+
+                    T mj_roj_Wj = rj.vol * Wj;
+                    T mj_pro_i  = rj.m * pro_i;
+                    {
+                        T aa = Wi * (mj_pro_i + visc * ri.vol);
+                        T bb = mj_roj_Wj * (rj.pro + visc);
+                        momx += aa * tA1_i + bb * tA1_j;
+                        momy += aa * tA2_i + bb * tA2_j;
+                        momz += aa * tA3_i + bb * tA3_j;
+                    }
+                    {
+                        T aa = Wi * (T(2) * mj_pro_i + visc * ri.vol);
+                        T bb = visc * mj_roj_Wj;
+                        energy.add(vx_ij * (aa * tA1_i + bb * tA1_j) + vy_ij * (aa * tA2_i + bb * tA2_j) +
+                                   vz_ij * (aa * tA3_i + bb * tA3_j));
+                    }
+                }
             }
-            {
-                T aa = Wi * (T(2) * mj_pro_i + visc * ri.vol);
-                T bb = visc * mj_roj_Wj;
-                energy.add(vx_ij * (aa * tA1_i + bb * tA1_j) + vy_ij * (aa * tA2_i + bb * tA2_j) +
-                           vz_ij * (aa * tA3_i + bb * tA3_j));
-            }
-        }
+        });
         f.du[i] = -a.K * 0.5 * double(energy.value());
         st<T>(f.ax, i, T(a.K * double(momx)));
         st<T>(f.ay, i, T(a.K * double(momy)));

@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
     veDefGradhKernel(PairArgs a, const RecGradh<T>* __restrict__ rec, const T* __restrict__ h,
-                     const T* __restrict__ wh, const T* __restrict__ whd, T* __restrict__ kx, T* __restrict__ gradh)
+                     const TabQuad<T>* __restrict__ whq, T* __restrict__ kx, T* __restrict__ gradh)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -39,20 +39,29 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T                 hInv = T(1) / hi;
     double            r2h  = 2.0 * double(hi);
     Acc<T>            kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
-    for (unsigned k = 0; k < t.n; ++k)
-    {
-        const RecGradh<T> rj = rec[t.list[size_t(k) * kGroup]];
-        T                 rx, ry, rz;
-        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-        T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T vloc  = dist * hInv;
-        T w     = lookup(wh, vloc);
-        T dw    = lookup(whd, vloc);
-        T dterh = -(T(3) * w + vloc * dw);
-        kxs.add(w * rj.xm);
-        wom.add(dterh * rj.xm);
-        wr0.add(dterh * rj.m);
-    }
+    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        constexpr int                 U = isMixed<T> ? 4 : 2;
+        NeighborBatch<U, RecGradh<T>> nb(t, rec);
+        for (unsigned k = 0; k < t.n; k += U)
+        {
+            nb.load(k);
+            TableFetch<T, TabQuad<T>> w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                w[u] = fetch(whq, pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                T wv, dw;
+                values(w[u], wv, dw);
+                wv = nb.live(k, u, wv), dw = nb.live(k, u, dw);
+                T dterh = -(T(3) * wv + w[u].v * dw);
+                kxs.add(wv * nb.r[u].xm);
+                wom.add(dterh * nb.r[u].xm);
+                wr0.add(dterh * nb.r[u].m);
+            }
+        }
+    });
     // per-target epilogue in FP64 following the reference's order; h^-3 is rounded in T as there
     T      th3   = hInv * hInv * hInv;
     double dhInv = double(hInv);
@@ -113,7 +122,7 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
  * so the 9 moments M are accumulated alongside tau and combined with C_i after the inversion. */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, const T* __restrict__ wh)
+    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -124,26 +133,41 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     double            r2h   = 2.0 * double(hi);
     T                 t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     T                 M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (unsigned k = 0; k < t.n; ++k)
-    {
-        const RecIadVe<T> rj = rec[t.list[size_t(k) * kGroup]];
-        T                 rx, ry, rz;
-        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-        T dist  = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T w     = lookup(wh, dist * hiInv);
-        T vol_w = rj.vol * w;
-        t11 += rx * rx * vol_w;
-        t12 += rx * ry * vol_w;
-        t13 += rx * rz * vol_w;
-        t22 += ry * ry * vol_w;
-        t23 += ry * rz * vol_w;
-        t33 += rz * rz * vol_w;
-        T xw  = rj.xm * w;
-        T dvx = (rj.vx - ri.vx) * xw, dvy = (rj.vy - ri.vy) * xw, dvz = (rj.vz - ri.vz) * xw;
-        M[0][0] += dvx * rx, M[0][1] += dvx * ry, M[0][2] += dvx * rz;
-        M[1][0] += dvy * rx, M[1][1] += dvy * ry, M[1][2] += dvy * rz;
-        M[2][0] += dvz * rx, M[2][1] += dvz * ry, M[2][2] += dvz * rz;
-    }
+    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        constexpr int                 U = isMixed<T> ? 4 : 2;
+        NeighborBatch<U, RecIadVe<T>> nb(t, rec);
+        for (unsigned k = 0; k < t.n; k += U)
+        {
+            nb.load(k);
+            Sep<T>                    r[U];
+            TableFetch<T, TabPair<T>> w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
+                w[u] = fetch(wh, r[u].d * hiInv);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const RecIadVe<T>& rj = nb.r[u];
+                T rx = r[u].x, ry = r[u].y, rz = r[u].z;
+                T wv    = nb.live(k, u, value(w[u]));
+                T vol_w = rj.vol * wv;
+                t11 += rx * rx * vol_w;
+                t12 += rx * ry * vol_w;
+                t13 += rx * rz * vol_w;
+                t22 += ry * ry * vol_w;
+                t23 += ry * rz * vol_w;
+                t33 += rz * rz * vol_w;
+                T xw  = rj.xm * wv;
+                T dvx = (rj.vx - ri.vx) * xw, dvy = (rj.vy - ri.vy) * xw, dvz = (rj.vz - ri.vz) * xw;
+                M[0][0] += dvx * rx, M[0][1] += dvx * ry, M[0][2] += dvx * rz;
+                M[1][0] += dvy * rx, M[1][1] += dvy * ry, M[1][2] += dvy * rz;
+                M[2][0] += dvz * rx, M[2][1] += dvz * ry, M[2][2] += dvz * rz;
+            }
+        }
+    });
     double cij[6];
     iadInvert(t11, t12, t13, t22, t23, t33, double(hi), a.K, cij);
     T c[6];
@@ -202,7 +226,7 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
  * gd_b = -K/h_i^3 sum_c C_i[b][c] sum_j f_j W_ij r_ij,c with f_j = xm_j/kx_j (divv_i - divv_j). */
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, const T* __restrict__ wh, double dt,
+    avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh, double dt,
                      T alphamin, T alphamax, T decay)
 {
     Lane t = laneTarget(a.L);
@@ -214,19 +238,33 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     double         r2h   = 2.0 * double(hi);
     T              vsig  = T(1.e-40) * ri.c;
     T              S[3]  = {0, 0, 0};
-    for (unsigned k = 0; k < t.n; ++k)
-    {
-        const RecAv<T> rj = rec[t.list[size_t(k) * kGroup]];
-        T              rx, ry, rz;
-        separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-        T dist = fastSqrt(rx * rx + ry * ry + rz * rz);
-        T rv   = rx * (ri.vx - rj.vx) + ry * (ri.vy - rj.vy) + rz * (ri.vz - rj.vz);
-        T vs   = rv < T(0) ? ri.c + rj.c - T(3) * fastDiv(rv, dist) : T(0);
-        vsig   = vsig < vs ? vs : vsig;
-        T w    = lookup(wh, dist * hiInv);
-        T fw   = rj.vol * (ri.divv - rj.divv) * w;
-        S[0] += fw * rx, S[1] += fw * ry, S[2] += fw * rz;
-    }
+    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        constexpr int              U = isMixed<T> ? 4 : 2;
+        NeighborBatch<U, RecAv<T>> nb(t, rec);
+        for (unsigned k = 0; k < t.n; k += U)
+        {
+            nb.load(k);
+            Sep<T>                    r[U];
+            TableFetch<T, TabPair<T>> w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
+                w[u] = fetch(wh, r[u].d * hiInv);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const RecAv<T>& rj = nb.r[u];
+                T rx = r[u].x, ry = r[u].y, rz = r[u].z;
+                T rv = rx * (ri.vx - rj.vx) + ry * (ri.vy - rj.vy) + rz * (ri.vz - rj.vz);
+                T vs = nb.in(k, u) && rv < T(0) ? ri.c + rj.c - T(3) * fastDiv(rv, r[u].d) : T(0);
+                vsig = vsig < vs ? vs : vsig;
+                T fw = rj.vol * (ri.divv - rj.divv) * nb.live(k, u, value(w[u]));
+                S[0] += fw * rx, S[1] += fw * ry, S[2] += fw * rz;
+            }
+        }
+    });
     T c11i = ld<T>(f.c11, i), c12i = ld<T>(f.c12, i), c13i = ld<T>(f.c13, i);
     T c22i = ld<T>(f.c22, i), c23i = ld<T>(f.c23, i), c33i = ld<T>(f.c33, i);
     T kh3  = T(a.K * double(hiInv * hiInv * hiInv));
@@ -311,7 +349,7 @@ __device__ __forceinline__ T avRvCorrection(T rx, T ry, T rz, T eta_ab, T eta_cr
 /* momentumAndEnergyJLoop<avClean> (hydro_ve/momentum_energy_kern.hpp:66-222) + tsKCourant */
 template<class T, bool avClean>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, const T* __restrict__ wh, T Atmin,
+    momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh, T Atmin,
                      T Atmax, T ramp, float Kcour, unsigned long long* minDt)
 {
     Lane   t      = laneTarget(a.L);
@@ -334,73 +372,89 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
-        Acc<T> energy, avEnergy;
-        for (unsigned k = 0; k < t.n; ++k)
-        {
-            const uint32_t    j  = t.list[size_t(k) * kGroup];
-            const RecMomVe<T> rj = rec[j];
-            T                 rx, ry, rz;
-            separation(a.box, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, rx, ry, rz);
-            T r2    = rx * rx + ry * ry + rz * rz;
-            T dist  = fastSqrt(r2);
-            T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
-            T v1    = dist * hiInv;
-            T v2    = dist * rj.hInv;
-            T Wi    = hiInv3 * lookup(wh, v1);
-            T Wj    = rj.hInv * rj.hInv * rj.hInv * lookup(wh, v2);
-
-            T tA1_i = -(ri.c11 * rx + ri.c12 * ry + ri.c13 * rz) * Wi;
-            T tA2_i = -(ri.c12 * rx + ri.c22 * ry + ri.c23 * rz) * Wi;
-            T tA3_i = -(ri.c13 * rx + ri.c23 * ry + ri.c33 * rz) * Wi;
-            T tA1_j = -(rj.c11 * rx + rj.c12 * ry + rj.c13 * rz) * Wj;
-            T tA2_j = -(rj.c12 * rx + rj.c22 * ry + rj.c23 * rz) * Wj;
-            T tA3_j = -(rj.c13 * rx + rj.c23 * ry + rj.c33 * rz) * Wj;
-
-            T rv = rx * vx_ij + ry * vy_ij + rz * vz_ij;
-            if constexpr (avClean)
+        Acc<T>       energy, avEnergy;
+        withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+            constexpr int                 U = isMixed<T> ? 2 : 1;
+            NeighborBatch<U, RecMomVe<T>> nb(t, rec);
+            for (unsigned k = 0; k < t.n; k += U)
             {
-                T gradV_j[6] = {ld<T>(f.dV11, j), ld<T>(f.dV12, j), ld<T>(f.dV13, j),
-                                ld<T>(f.dV22, j), ld<T>(f.dV23, j), ld<T>(f.dV33, j)};
-                rv += avRvCorrection(rx, ry, rz, v1 < v2 ? v1 : v2, eta_crit, gradV_i, gradV_j);
-            }
-            T wij  = fastDiv(rv, dist);
-            T visc = artificialViscosity(ri.alpha, rj.alpha, ri.c, rj.c, wij);
-            T vsig = T(0.5) * (ri.c + rj.c) - T(2) * wij;
-            maxvs  = vsig > maxvs ? vsig : maxvs;
+              nb.load(k);
+              Sep<T>                    r[U];
+              TableFetch<T, TabPair<T>> wi[U], wj[U];
+#pragma unroll
+              for (int u = 0; u < U; ++u)
+              {
+                  r[u]  = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
+                  wi[u] = fetch(wh, r[u].d * hiInv);
+                  wj[u] = fetch(wh, r[u].d * nb.r[u].hInv);
+              }
+#pragma unroll
+              for (int u = 0; u < U; ++u)
+              {
+                const uint32_t     j  = nb.j[u];
+                const RecMomVe<T>& rj = nb.r[u];
+                T rx = r[u].x, ry = r[u].y, rz = r[u].z, dist = r[u].d;
+                T vx_ij = ri.vx - rj.vx, vy_ij = ri.vy - rj.vy, vz_ij = ri.vz - rj.vz;
+                T v1    = wi[u].v;
+                T v2    = wj[u].v;
+                T Wi    = hiInv3 * nb.live(k, u, value(wi[u]));
+                T Wj    = rj.hInv * rj.hInv * rj.hInv * nb.live(k, u, value(wj[u]));
 
-            T a_mom, b_mom;
-            T Atwood = fastDiv(fabs(ri.rho - rj.rho), ri.rho + rj.rho);
-            if (Atwood < Atmin)
-            {
-                a_mom = ri.xm * ri.xm;
-                b_mom = rj.xm * rj.xm;
-            }
-            else if (Atwood > Atmax)
-            {
-                a_mom = ri.xm * rj.xm;
-                b_mom = a_mom;
-            }
-            else
-            {
-                T sigma = ramp * (Atwood - Atmin);
-                a_mom   = pow(ri.xm, T(2) - sigma) * pow(rj.xm, sigma);
-                b_mom   = pow(rj.xm, T(2) - sigma) * pow(ri.xm, sigma);
-            }
+                T tA1_i = -(ri.c11 * rx + ri.c12 * ry + ri.c13 * rz) * Wi;
+                T tA2_i = -(ri.c12 * rx + ri.c22 * ry + ri.c23 * rz) * Wi;
+                T tA3_i = -(ri.c13 * rx + ri.c23 * ry + ri.c33 * rz) * Wi;
+                T tA1_j = -(rj.c11 * rx + rj.c12 * ry + rj.c13 * rz) * Wj;
+                T tA2_j = -(rj.c12 * rx + rj.c22 * ry + rj.c23 * rz) * Wj;
+                T tA3_j = -(rj.c13 * rx + rj.c23 * ry + rj.c33 * rz) * Wj;
 
-            T a_visc   = fastDiv(rj.m, ri.rho) * visc;
-            T b_visc   = rj.vol * visc;
-            T a_visc_x = T(0.5) * (a_visc * tA1_i + b_visc * tA1_j);
-            T a_visc_y = T(0.5) * (a_visc * tA2_i + b_visc * tA2_j);
-            T a_visc_z = T(0.5) * (a_visc * tA3_i + b_visc * tA3_j);
-            avEnergy.add(a_visc_x * vx_ij + a_visc_y * vy_ij + a_visc_z * vz_ij);
-            energy.add(rj.m * a_mom * (vx_ij * tA1_i + vy_ij * tA2_i + vz_ij * tA3_i));
+                T rv = rx * vx_ij + ry * vy_ij + rz * vz_ij;
+                if constexpr (avClean)
+                {
+                    T gradV_j[6] = {ld<T>(f.dV11, j), ld<T>(f.dV12, j), ld<T>(f.dV13, j),
+                                    ld<T>(f.dV22, j), ld<T>(f.dV23, j), ld<T>(f.dV33, j)};
+                    rv += avRvCorrection(rx, ry, rz, v1 < v2 ? v1 : v2, eta_crit, gradV_i, gradV_j);
+                }
+                T wij  = fastDiv(rv, dist);
+                T visc = artificialViscosity(ri.alpha, rj.alpha, ri.c, rj.c, wij);
+                T vsig = T(0.5) * (ri.c + rj.c) - T(2) * wij;
+                maxvs  = nb.in(k, u) && vsig > maxvs ? vsig : maxvs;
 
-            T mom_i = rj.m * ri.prho * a_mom;
-            T mom_j = rj.m * rj.prho * b_mom;
-            momx += mom_i * tA1_i + mom_j * tA1_j + a_visc_x;
-            momy += mom_i * tA2_i + mom_j * tA2_j + a_visc_y;
-            momz += mom_i * tA3_i + mom_j * tA3_j + a_visc_z;
-        }
+                T a_mom, b_mom;
+                T dRho   = ri.rho - rj.rho;
+                T Atwood = fastDiv(dRho < T(0) ? -dRho : dRho, ri.rho + rj.rho);
+                if (Atwood < Atmin)
+                {
+                    a_mom = ri.xm * ri.xm;
+                    b_mom = rj.xm * rj.xm;
+                }
+                else if (Atwood > Atmax)
+                {
+                    a_mom = ri.xm * rj.xm;
+                    b_mom = a_mom;
+                }
+                else
+                {
+                    T sigma = ramp * (Atwood - Atmin);
+                    a_mom   = pow(ri.xm, T(2) - sigma) * pow(rj.xm, sigma);
+                    b_mom   = pow(rj.xm, T(2) - sigma) * pow(ri.xm, sigma);
+                }
+
+                T a_visc   = fastDiv(rj.m, ri.rho) * visc;
+                T b_visc   = rj.vol * visc;
+                T a_visc_x = T(0.5) * (a_visc * tA1_i + b_visc * tA1_j);
+                T a_visc_y = T(0.5) * (a_visc * tA2_i + b_visc * tA2_j);
+                T a_visc_z = T(0.5) * (a_visc * tA3_i + b_visc * tA3_j);
+                avEnergy.add(a_visc_x * vx_ij + a_visc_y * vy_ij + a_visc_z * vz_ij);
+                energy.add(rj.m * a_mom * (vx_ij * tA1_i + vy_ij * tA2_i + vz_ij * tA3_i));
+
+                T mom_i = rj.m * ri.prho * a_mom;
+                T mom_j = rj.m * rj.prho * b_mom;
+                momx += mom_i * tA1_i + mom_j * tA1_j + a_visc_x;
+                momy += mom_i * tA2_i + mom_j * tA2_j + a_visc_y;
+                momz += mom_i * tA3_i + mom_j * tA3_j + a_visc_z;
+              }
+            }
+        });
         T avE    = avEnergy.value() > T(0) ? avEnergy.value() : T(0);
         T eCoeff = f.tdpdTrho ? ld<T>(f.tdpdTrho, i) : ri.prho;
         f.du[i]  = a.K * double(eCoeff * energy.value() + T(0.5) * avE);
@@ -439,8 +493,8 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
     packGradh<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, (const T*)f->m,
                                                                                           (const T*)f->xm, rec);
     SPHX_CHECK_LAUNCH();
-    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
-                                                                        whdTable<T>(ws), (T*)f->kx, (T*)f->gradh);
+    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whqTable<T>(ws),
+                                                                        (T*)f->kx, (T*)f->gradh);
     SPHX_CHECK_LAUNCH();
 }
 

@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
 
     const unsigned wid  = threadIdx.x / kWave;
     const unsigned lane = laneId();
-    const uint32_t g    = blockIdx.x * kWavesPerBlock + wid;
+    const uint32_t g    = xcdBlock(blockIdx.x, gridDim.x) * kWavesPerBlock + wid;
     const uint32_t base = a.first + g * kGroup;
     if (base >= a.last) { return; }
     WaveLds& lds = ldsMem[wid];
@@ -359,6 +359,8 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
     {
         if (updates > 0) { h[t.i] = hi; }
         a.nc[t.i] = ncSph;
+        // slot 0 always names a particle: the pair loops read one slot ahead without a branch
+        if (ncSph == 1) { myList[0] = t.i; }
     }
 
     unsigned stored = valid ? min(ncSph - 1, a.ngmax) : 0;

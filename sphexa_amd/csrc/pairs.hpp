@@ -1,7 +1,8 @@
 // Helpers shared by the SPH pair-loop kernels (std and VE).
 //
 // Launch shape: one wavefront per group of 64 consecutive targets (the neighbour-search group),
-// 4 waves per block. Lane l of wave g owns target first + 64 g + l and walks its own list
+// 4 waves per block, blocks renumbered per XCD (xcdBlock) so that neighbouring groups share an
+// L2. Lane l of wave g owns target first + 64 g + l and walks its own list
 // nbrs[(g*ngmax + k)*64 + l], k < min(nc-1, ngmax): one coalesced 256-byte load per slot.
 //
 // Two arithmetic forms share each kernel template:
@@ -43,7 +44,7 @@ __device__ __forceinline__ Lane laneTarget(const Lists& L)
 {
     Lane           t;
     const unsigned lane = threadIdx.x % kWave;
-    const uint32_t g    = blockIdx.x * kPairWaves + threadIdx.x / kWave;
+    const uint32_t g    = xcdBlock(blockIdx.x, gridDim.x) * kPairWaves + threadIdx.x / kWave;
     t.i                 = L.first + g * kGroup + lane;
     t.valid             = t.i < L.last;
     t.n                 = t.valid ? min(L.nc[t.i] - 1, L.ngmax) : 0u;
@@ -57,38 +58,45 @@ inline unsigned pairBlocks(uint32_t first, uint32_t last)
 }
 
 //! lt::lookup (sph/include/sph/table_lookup.hpp:10-26): linear interpolation of a 20000-point
-//! table on [0,2], zero beyond the support
+//! table on [0,2], zero beyond the support. The table is stored per interval (both end values
+//! adjacent), so one lookup is one vector gather instead of two scalar ones. Interval 19999, one
+//! past the last, holds zeros: arguments at or beyond the support clamp onto it and interpolate
+//! to +0, the reference's value there, without a branch around the load. Arguments are >= 0.
 template<class T>
-__device__ __forceinline__ T lookup(const T* __restrict__ table, T v)
+__device__ __forceinline__ T interpolate(T t0, T t1, T v, int idx)
+{
+    constexpr T dx         = T(2.0) / (kTableSize - 1);
+    constexpr T invDx      = T(1) / dx;
+    T           derivative = (t1 - t0) * invDx;
+    return t0 + derivative * (v - T(idx) * dx);
+}
+
+template<class T>
+__device__ __forceinline__ int tableInterval(T v)
 {
     constexpr int numIntervals = kTableSize - 1;
     constexpr T   dx           = T(2.0) / numIntervals;
     constexpr T   invDx        = T(1) / dx;
-    int           idx          = int(v * invDx);
-    bool          inside       = idx < numIntervals;
-    int           k            = inside ? max(idx, 0) : 0;
-    T             t0 = table[k], t1 = table[k + 1];
-    T             derivative = (t1 - t0) * invDx;
-    return inside ? t0 + derivative * (v - T(idx) * dx) : T(0);
+    T             vc           = v < T(4) ? v : T(4); // keeps the conversion in range
+    return min(int(vc * invDx), numIntervals);
 }
 
-//! legacy minimum image of a separation with cut-off r (sfc/box.hpp:218-238)
 template<class T>
-__device__ __forceinline__ void foldLegacy(const DevBox& b, T r, T& xx, T& yy, T& zz)
+__device__ __forceinline__ T lookup(const TabPair<T>* __restrict__ table, T v)
 {
-    if (b.pbc[0]) { xx = xx > r ? xx - T(b.len[0]) : (xx < -r ? xx + T(b.len[0]) : xx); }
-    if (b.pbc[1]) { yy = yy > r ? yy - T(b.len[1]) : (yy < -r ? yy + T(b.len[1]) : yy); }
-    if (b.pbc[2]) { zz = zz > r ? zz - T(b.len[2]) : (zz < -r ? zz + T(b.len[2]) : zz); }
+    int              idx = tableInterval(v);
+    const TabPair<T> e   = table[idx];
+    return interpolate(e.t0, e.t1, v, idx);
 }
 
-//! separation x_i - x_j in the pair type: folded in FP64, then rounded (exact for T = double)
+//! W(v) and dW/dv(v) from one interval record
 template<class T>
-__device__ __forceinline__ void separation(const DevBox& b, double hi2, double xi, double yi, double zi, double xj,
-                                           double yj, double zj, T& rx, T& ry, T& rz)
+__device__ __forceinline__ void lookupBoth(const TabQuad<T>* __restrict__ table, T v, T& w, T& dw)
 {
-    double dx = xi - xj, dy = yi - yj, dz = zi - zj;
-    foldLegacy(b, hi2, dx, dy, dz);
-    rx = T(dx), ry = T(dy), rz = T(dz);
+    int              idx = tableInterval(v);
+    const TabQuad<T> e   = table[idx];
+    w                    = interpolate(e.w0, e.w1, v, idx);
+    dw                   = interpolate(e.d0, e.d1, v, idx);
 }
 
 //! fast reciprocal / square root for the FP32 form, IEEE operations for FP64
@@ -104,6 +112,159 @@ __device__ __forceinline__ T fastSqrt(T a)
 {
     if constexpr (isMixed<T>) { return __builtin_amdgcn_sqrtf(a); }
     else { return sqrt(a); }
+}
+
+/*! Batched walk over one lane's neighbour list (slot k at list[k * 64]).
+ *
+ * A pair loop visits its neighbours U at a time: load(k) makes the records of slots k..k+U-1
+ * current (their list entries were requested one batch earlier) and requests the list entries of
+ * the following batch. The loop then forms the U separations and issues the U table fetches
+ * before it consumes any of them, so one batch costs two memory round trips (records, tables)
+ * instead of three per neighbour. Slots beyond the lane's count are clamped to its last stored
+ * one and their pair terms are skipped (in(k, u)). Slot 0 of a valid target always holds a valid
+ * particle index (the search writes the target itself there when it has no neighbours), so no
+ * load reads past the stored list or an unwritten slot. Valid targets only. */
+template<int U, class R>
+struct NeighborBatch
+{
+    const uint32_t* list;
+    const R*        rec;
+    unsigned        n, lastSlot;
+    uint32_t        next[U]; // list entries of the next batch
+    uint32_t        j[U];    // particle indices of the current batch
+    R               r[U];    // records of the current batch
+
+    __device__ __forceinline__ NeighborBatch(const Lane& t, const R* __restrict__ records)
+        : list(t.list), rec(records), n(t.n), lastSlot(t.n ? t.n - 1 : 0)
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            next[u] = slot(u);
+            // an opaque definition: otherwise the compiler merges this load with the look-ahead
+            // load of the loop into one load at the top of each batch, undoing the look-ahead
+            asm volatile("" : "+v"(next[u]));
+        }
+    }
+
+    __device__ __forceinline__ uint32_t slot(unsigned k) const { return list[size_t(min(k, lastSlot)) * kGroup]; }
+
+    __device__ __forceinline__ void load(unsigned k)
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            j[u] = next[u];
+            r[u] = rec[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            next[u] = slot(k + U + u);
+    }
+
+    //! whether member u of the batch starting at slot k is a real neighbour
+    __device__ __forceinline__ bool in(unsigned k, int u) const { return k + u < n; }
+
+    //! a kernel weight of member u, zeroed for a clamped slot. Every pair term of the loops is
+    //! proportional to a kernel weight, so a clamped slot contributes exactly zero without a branch
+    //! (a branch would let the compiler sink that member's loads behind it).
+    template<class T>
+    __device__ __forceinline__ T live(unsigned k, int u, T w) const
+    {
+        return in(k, u) ? w : T(0);
+    }
+};
+
+//! table entry of argument v, fetched ahead of its interpolation
+template<class T, class E>
+struct TableFetch
+{
+    E   e;
+    T   v;
+    int idx;
+};
+
+template<class T, class E>
+__device__ __forceinline__ TableFetch<T, E> fetch(const E* __restrict__ table, T v)
+{
+    TableFetch<T, E> f;
+    f.v   = v;
+    f.idx = tableInterval(v);
+    f.e   = table[f.idx];
+    return f;
+}
+
+template<class T>
+__device__ __forceinline__ T value(const TableFetch<T, TabPair<T>>& f)
+{
+    return interpolate(f.e.t0, f.e.t1, f.v, f.idx);
+}
+
+template<class T>
+__device__ __forceinline__ void values(const TableFetch<T, TabQuad<T>>& f, T& w, T& dw)
+{
+    w  = interpolate(f.e.w0, f.e.w1, f.v, f.idx);
+    dw = interpolate(f.e.d0, f.e.d1, f.v, f.idx);
+}
+
+/*! Whether a wave must apply the legacy fold (sfc/box.hpp:218-238). The fold changes a separation
+ * only when |x_i - x_j| exceeds the cut-off 2h_i. For a listed neighbour (minimum-image distance
+ * below 2h_i) that needs the raw difference to wrap across a periodic face, which puts the target
+ * within 2h_i of that face. Waves without such a target run the loop without the fold. */
+__device__ __forceinline__ bool waveNeedsFold(const DevBox& b, double x, double y, double z, double r2h)
+{
+    const double p[3] = {x, y, z};
+    const double r    = r2h * (1.0 + 1e-6);
+    bool         near = false;
+    for (int d = 0; d < 3; ++d)
+        near |= b.pbc[d] && (p[d] - b.lo[d] <= r || b.hi[d] - p[d] <= r);
+    return ballot(near) != 0;
+}
+
+/*! separation x_i - x_j in the pair type, formed in FP64 (exact for T = double, then rounded for
+ * float). With Fold, the legacy minimum image with cut-off r (sfc/box.hpp:218-238): per periodic
+ * dimension subtract L if d > r, add L if d < -r. Written with selects, d - 0 == d exactly, so the
+ * result is that of the reference's branches. */
+template<bool Fold, class T>
+__device__ __forceinline__ void separation(const DevBox& b, double r, double xi, double yi, double zi, double xj,
+                                           double yj, double zj, T& rx, T& ry, T& rz)
+{
+    double d[3] = {xi - xj, yi - yj, zi - zj};
+    if constexpr (Fold)
+    {
+        for (int q = 0; q < 3; ++q)
+        {
+            bool   up   = b.pbc[q] && d[q] > r;
+            bool   down = b.pbc[q] && d[q] < -r;
+            double s    = up ? b.len[q] : (down ? -b.len[q] : 0.0);
+            d[q]        = d[q] - s;
+        }
+    }
+    rx = T(d[0]), ry = T(d[1]), rz = T(d[2]);
+}
+
+//! separation of a pair and its length
+template<class T>
+struct Sep
+{
+    T x, y, z, d;
+};
+
+template<bool Fold, class T, class Ri, class Rj>
+__device__ __forceinline__ Sep<T> pairSeparation(const DevBox& b, double r2h, const Ri& ri, const Rj& rj)
+{
+    Sep<T> s;
+    separation<Fold>(b, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, s.x, s.y, s.z);
+    s.d = fastSqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+    return s;
+}
+
+//! run `loop` with the fold compiled in or out, chosen once per wave
+template<class F>
+__device__ __forceinline__ void withFold(bool fold, F&& loop)
+{
+    if (fold) { loop(std::true_type{}); }
+    else { loop(std::false_type{}); }
 }
 
 //! compensated accumulator (Kahan) for the FP32 form, plain sum for FP64
@@ -214,17 +375,24 @@ inline PairArgs makePairArgs(sphx_workspace* ws, const DevBox& box, const sphx_f
 }
 
 template<class T>
-inline const T* whTable(sphx_workspace* ws)
+inline const TabPair<T>* whTable(sphx_workspace* ws)
 {
     if constexpr (isMixed<T>) { return ws->wh32.get(); }
     else { return ws->wh64.get(); }
 }
 
 template<class T>
-inline const T* whdTable(sphx_workspace* ws)
+inline const TabPair<T>* whdTable(sphx_workspace* ws)
 {
     if constexpr (isMixed<T>) { return ws->whd32.get(); }
     else { return ws->whd64.get(); }
+}
+
+template<class T>
+inline const TabQuad<T>* whqTable(sphx_workspace* ws)
+{
+    if constexpr (isMixed<T>) { return ws->whq32.get(); }
+    else { return ws->whq64.get(); }
 }
 
 //! wave minimum then one atomic per wave
