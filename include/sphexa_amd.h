@@ -118,8 +118,24 @@ typedef struct sphx_params
     double   Atmin, Atmax, ramp;
     uint32_t ng0, ngmax;
     int32_t  hydro_type;     /* sphx_dtype of the HydroType fields */
-    int32_t  reserved;
+    int32_t  flags;          /* SPHX_FLAG_* bits, 0 = reference behaviour */
 } sphx_params_t;
+
+/* std momentum-energy with a per-particle AV coefficient from fields.alpha instead of the
+ * reference's constant alpha = 1 (hydro_std/momentum_energy_kern.hpp:98); the coefficient the
+ * std adjoint differentiates against. */
+#define SPHX_FLAG_STD_ALPHA 1
+
+/* Cotangents and gradients of the adjoint entry points: one HydroType array per named quantity
+ * (du included). An adjoint reads the cotangents of its forward outputs from `in` and adds
+ * (+=) the gradients of its differentiable inputs into `out`, so callers zero `out` first; the
+ * members each entry point uses are listed with it. */
+typedef struct sphx_adjoint
+{
+    void *x, *y, *z, *h, *rho, *p, *alpha;
+    void *c11, *c12, *c13, *c22, *c23, *c33;
+    void *ax, *ay, *az, *du;
+} sphx_adjoint_t;
 
 /* OctreeNsView<double, uint64_t> (domain/include/cstone/tree/octree.hpp:280-300) plus the
  * cornerstone leaf counts; all device pointers into workspace-owned memory. */
@@ -260,6 +276,33 @@ int sphx_av_switches(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fie
 int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                             const sphx_params_t* prm, uint32_t first, uint32_t last, int avClean,
                             double* minDtCourant);
+
+/* ---- adjoints (no counterpart in the reference) ---------------------------------------------
+ * Vector-Jacobian products of the std density, IAD and momentum-energy passes with respect to
+ * positions, smoothing lengths and a per-particle AV coefficient, for gradient-based calibration
+ * and inverse problems. Each recomputes its pair terms from the neighbour lists of the last
+ * sphx_find_neighbors (the lists are a checkpoint: they are held fixed, as is the h of the
+ * forward). Kernel derivatives use the tabulated dW/dv interpolated like W. Gradients to
+ * neighbours are scattered with atomic adds, so FP results can vary in the last bits. */
+
+/* rho = K h^-3 (m_i + sum_j m_j W(r_ij/h_i)) (sphx_density_std).
+ * in: rho. out: x, y, z, h. fields: x, y, z, h, m, nc. */
+int sphx_density_std_vjp(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                         const sphx_params_t* prm, uint32_t first, uint32_t last, const sphx_adjoint_t* in,
+                         sphx_adjoint_t* out);
+
+/* c_ij = h^3/K tau^-1, tau = sum_j m_j/rho_j W(r_ij/h_i) r_ij r_ij^T (sphx_iad_std).
+ * in: c11..c33. out: x, y, z, h, rho. fields: x, y, z, h, m, rho, c11..c33 (forward output), nc. */
+int sphx_iad_std_vjp(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                     const sphx_params_t* prm, uint32_t first, uint32_t last, const sphx_adjoint_t* in,
+                     sphx_adjoint_t* out);
+
+/* ax, ay, az, du of sphx_momentum_energy_std, with the AV coefficient taken from fields.alpha when
+ * prm->flags has SPHX_FLAG_STD_ALPHA, else 1. in: ax, ay, az, du (du may be NULL).
+ * out: x, y, z, h, rho, p, alpha, c11..c33. fields: as the forward. */
+int sphx_momentum_energy_std_vjp(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                                 const sphx_params_t* prm, uint32_t first, uint32_t last,
+                                 const sphx_adjoint_t* in, sphx_adjoint_t* out);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,7 @@ _U32 = C.c_uint32
 _BOXP = C.POINTER(abi.Box)
 _FP = C.POINTER(abi.Fields)
 _PP = C.POINTER(abi.Params)
+_AP = C.POINTER(abi.Adjoint)
 
 _SIGS = {
     "sfc_keys": (None, [_BOXP, _P, _P, _P, _P, _SZ]),
@@ -50,6 +51,9 @@ _SIGS = {
     "av_switches": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "divv_curlv": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "momentum_energy_ve": (C.c_double, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, C.c_int, _P]),
+    "density_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
+    "iad_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
+    "momentum_energy_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
 }
 
 _cache = {}
@@ -156,6 +160,15 @@ def sorted_lists(nb, nc, ngmax):
     return out
 
 
+def adjoint_from(arrays):
+    """abi.Adjoint over host arrays (missing names stay NULL)."""
+    a = abi.Adjoint()
+    for name in abi.ADJOINT_FIELDS:
+        if name in arrays and arrays[name] is not None:
+            setattr(a, name, arrays[name].ctypes.data)
+    return a
+
+
 class CpuPass:
     """One std or VE hydro pass on host arrays through the oracle (or the reference wrapper)."""
 
@@ -203,6 +216,13 @@ class CpuPass:
     def momentum_energy_ve(self, av_clean=False, maxvs=None):
         return self.lib.momentum_energy_ve(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first,
                                            self.last, ptr(self.nb), ptr(self.wh), int(av_clean), ptr(maxvs))
+
+    def vjp(self, name, cot, grad):
+        """Adjoint `name` (density_std_vjp, iad_std_vjp, momentum_energy_std_vjp): cotangent arrays
+        `cot` and gradient accumulators `grad` (dicts of host arrays, added into)."""
+        getattr(self.lib, name)(C.byref(self.box), C.byref(self._f()), C.byref(self.p), self.first, self.last,
+                                ptr(self.nb), ptr(self.wh), ptr(self.whd), C.byref(adjoint_from(cot)),
+                                C.byref(adjoint_from(grad)))
 
     def run_std(self):
         self.density_std()

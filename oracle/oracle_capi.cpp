@@ -129,12 +129,67 @@ double momentumEnergyStd(const Box& box, const sphx_fields_t* f, const sphx_para
         momentumEnergyStdLoop(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->vx),
                               F<T>(f->vy), F<T>(f->vz), h, F<T>(f->m), F<T>(f->rho), F<T>(f->p), c,
                               F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33),
-                              wh, F<T>(f->ax), F<T>(f->ay), F<T>(f->az), f->du, &maxvs);
+                              wh, F<T>(f->ax), F<T>(f->ay), F<T>(f->az), f->du, &maxvs,
+                              (p->flags & SPHX_FLAG_STD_ALPHA) ? F<T>(f->alpha) : (const T*)nullptr);
         if (maxvsOut) { maxvsOut[i] = maxvs; }
         T dt_i = courantDt(maxvs, h[i], c[i], float(p->Kcour));
         minDt  = std::min(minDt, dt_i);
     }
     return minDt;
+}
+
+/* Adjoints: serial over targets (neighbour gradients are scattered), lists as above. */
+template<class T>
+void densityStdVjpAll(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first,
+                      LocalIndex last, const LocalIndex* nb, const T* wh, const T* whd, const sphx_adjoint_t* in,
+                      sphx_adjoint_t* out)
+{
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        densityStdVjp(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->h), F<T>(f->m),
+                      wh, whd, F<T>(in->rho)[i], F<T>(out->x), F<T>(out->y), F<T>(out->z), F<T>(out->h));
+    }
+}
+
+template<class T>
+void iadStdVjpAll(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first, LocalIndex last,
+                  const LocalIndex* nb, const T* wh, const T* whd, const sphx_adjoint_t* in, sphx_adjoint_t* out)
+{
+    const T* c[6]  = {F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33)};
+    const T* gc[6] = {F<T>(in->c11), F<T>(in->c12), F<T>(in->c13), F<T>(in->c22), F<T>(in->c23), F<T>(in->c33)};
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn = capped<T>(f, i, p->ngmax);
+        T        C6[6], G6[6];
+        for (int q = 0; q < 6; ++q)
+            C6[q] = c[q][i], G6[q] = gc[q][i];
+        iadStdVjp(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->h), F<T>(f->m),
+                  F<T>(f->rho), C6, G6, wh, whd, F<T>(out->x), F<T>(out->y), F<T>(out->z), F<T>(out->h),
+                  F<T>(out->rho));
+    }
+}
+
+template<class T>
+void momentumEnergyStdVjpAll(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, LocalIndex first,
+                             LocalIndex last, const LocalIndex* nb, const T* wh, const T* whd,
+                             const sphx_adjoint_t* in, sphx_adjoint_t* out)
+{
+    const T*       C6[6] = {F<T>(f->c11), F<T>(f->c12), F<T>(f->c13), F<T>(f->c22), F<T>(f->c23), F<T>(f->c33)};
+    const T*       alpha = (p->flags & SPHX_FLAG_STD_ALPHA) ? F<T>(f->alpha) : nullptr;
+    MomStdGrads<T> g{F<T>(out->x),   F<T>(out->y), F<T>(out->z), F<T>(out->h),
+                     F<T>(out->rho), F<T>(out->p), F<T>(out->alpha),
+                     {F<T>(out->c11), F<T>(out->c12), F<T>(out->c13), F<T>(out->c22), F<T>(out->c23),
+                      F<T>(out->c33)}};
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        unsigned nn    = capped<T>(f, i, p->ngmax);
+        T        ga[3] = {F<T>(in->ax)[i], F<T>(in->ay)[i], F<T>(in->az)[i]};
+        T        gdu   = in->du ? F<T>(in->du)[i] : T(0);
+        momentumEnergyStdVjp(i, p->K, box, nb + size_t(i - first) * p->ngmax, nn, f->x, f->y, f->z, F<T>(f->vx),
+                             F<T>(f->vy), F<T>(f->vz), F<T>(f->h), F<T>(f->m), F<T>(f->rho), F<T>(f->p),
+                             F<T>(f->c), C6, alpha, wh, whd, ga, gdu, g);
+    }
 }
 
 //! computeVeDefGradhImpl, hydro_ve/ve_def_gradh.hpp:40-77
@@ -398,6 +453,33 @@ double orc_momentum_energy_std(const sphx_box_t* b, const sphx_fields_t* f, cons
     if (p->hydro_type == SPHX_F32)
         return momentumEnergyStd<float>(toBox(b), f, p, first, last, nb, (const float*)wh, (float*)maxvs);
     return momentumEnergyStd<double>(toBox(b), f, p, first, last, nb, (const double*)wh, (double*)maxvs);
+}
+
+void orc_density_std_vjp(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                         uint32_t last, const uint32_t* nb, const void* wh, const void* whd, const sphx_adjoint_t* in,
+                         sphx_adjoint_t* out)
+{
+#define C(T) densityStdVjpAll<T>(toBox(b), f, p, first, last, nb, (const T*)wh, (const T*)whd, in, out)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_iad_std_vjp(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                     uint32_t last, const uint32_t* nb, const void* wh, const void* whd, const sphx_adjoint_t* in,
+                     sphx_adjoint_t* out)
+{
+#define C(T) iadStdVjpAll<T>(toBox(b), f, p, first, last, nb, (const T*)wh, (const T*)whd, in, out)
+    ORC_DISPATCH(C)
+#undef C
+}
+
+void orc_momentum_energy_std_vjp(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p,
+                                 uint32_t first, uint32_t last, const uint32_t* nb, const void* wh, const void* whd,
+                                 const sphx_adjoint_t* in, sphx_adjoint_t* out)
+{
+#define C(T) momentumEnergyStdVjpAll<T>(toBox(b), f, p, first, last, nb, (const T*)wh, (const T*)whd, in, out)
+    ORC_DISPATCH(C)
+#undef C
 }
 
 void orc_xmass(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,

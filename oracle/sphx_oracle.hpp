@@ -752,13 +752,15 @@ inline void iadLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb
     c33[i]   = (tau11 * tau22 - tau12 * tau12) * factor;
 }
 
-//! momentumAndEnergyJLoop (std), hydro_std/momentum_energy_kern.hpp:11-134
+//! momentumAndEnergyJLoop (std), hydro_std/momentum_energy_kern.hpp:11-134. `alpha` (may be null)
+//! replaces the loop's constant AV coefficient 1 by a per-particle one; null is the reference loop.
 template<class T, class Tm>
 inline void momentumEnergyStdLoop(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn,
                                   const double* x, const double* y, const double* z, const T* vx, const T* vy,
                                   const T* vz, const T* h, const Tm* m, const T* rho, const T* p, const T* c,
                                   const T* c11, const T* c12, const T* c13, const T* c22, const T* c23, const T* c33,
-                                  const T* wh, T* ax, T* ay, T* az, double* du, T* maxvsignal)
+                                  const T* wh, T* ax, T* ay, T* az, double* du, T* maxvsignal,
+                                  const T* alpha = nullptr)
 {
     constexpr T gradh_i = 1.0, gradh_j = 1.0;
     auto        xi = x[i], yi = y[i], zi = z[i];
@@ -799,8 +801,8 @@ inline void momentumEnergyStdLoop(LocalIndex i, double K, const Box& box, const 
         auto        roj   = rho[j];
         auto        cj    = c[j];
         T           wij   = rv / dist;
-        constexpr T alpha = T(1);
-        T           visc  = T(0.5) * avisc(alpha, alpha, ci, cj, wij);
+        T           alpha_i = alpha ? alpha[i] : T(1), alpha_j = alpha ? alpha[j] : T(1);
+        T           visc  = T(0.5) * avisc(alpha_i, alpha_j, ci, cj, wij);
         T           vsig  = ci + cj - T(3) * wij;
         maxvs             = (vsig > maxvs) ? vsig : maxvs;
 
@@ -826,6 +828,251 @@ inline void momentumEnergyStdLoop(LocalIndex i, double K, const Box& box, const 
     ay[i]       = K * momy;
     az[i]       = K * momz;
     *maxvsignal = maxvs;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Adjoints of the std density, IAD and momentum-energy loops (no reference counterpart).
+ *
+ * Vector-Jacobian products of the functions above with the neighbour lists held fixed: given the
+ * cotangent of a loop's outputs for target i, they add dL/d(input) for i and for its neighbours j.
+ * The kernel derivative dW/dv is the tabulated whd interpolated like W (the derivative of the
+ * smooth kernel, which central differences of the interpolated forward reproduce to O(dx^2)).
+ * The periodic fold is a constant shift, so d r_ij / d x_i = -d r_ij / d x_j = identity.
+ * Derivations (r = x_i - x_j after the fold, d = |r|, v = d/h_i):
+ *   density  rho_i = K h_i^-3 (m_i + sum_j m_j W(v))
+ *            d/dr = K h_i^-4 m_j W'(v) r/d,  d/dh_i = -K h_i^-4 (3 rho0_i + sum_j m_j v W'(v))
+ *   IAD      C_i = h_i^3/K tau_i^-1,  tau_i = sum_j V_j W(v) r r^T,  V_j = m_j/rho_j
+ *            with G the matrix cotangent of C_i (off-diagonal stored entries count twice):
+ *            tauBar = -(K/h_i^3) C_i G C_i,  dL/dh_i (direct) = 3/h_i tr(G C_i),
+ *            per pair q = r^T tauBar r:  d/dr = V_j (W'(v) q r/(h_i d) + 2 W(v) tauBar r),
+ *            d/dh_i = -V_j W'(v) q d/h_i^2,  d/drho_j = -m_j/rho_j^2 W(v) q
+ *   momentum-energy: reverse mode through the loop body of momentumEnergyStdLoop, term by term.
+ * ----------------------------------------------------------------------------------------------*/
+
+template<class T>
+inline void foldedSeparation(const Box& box, T hi, double xi, double yi, double zi, double xj, double yj, double zj,
+                             T& rx, T& ry, T& rz)
+{
+    rx = xi - xj, ry = yi - yj, rz = zi - zj;
+    foldLegacy(box, T(2) * hi, rx, ry, rz);
+}
+
+//! adjoint of the std density rho_i = m_i / xmassLoop(i): adds into gx, gy, gz (i and j) and gh (i)
+template<class T, class Tm>
+inline void densityStdVjp(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn,
+                          const double* x, const double* y, const double* z, const T* h, const Tm* m, const T* wh,
+                          const T* whd, T gRho, T* gx, T* gy, T* gz, T* gh)
+{
+    T hi = h[i], hInv = T(1) / hi;
+    T coef = gRho * T(K) * hInv * hInv * hInv * hInv;
+    T rho0 = m[i], sumVdW = 0, gxi = 0, gyi = 0, gzi = 0;
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j = nb[k];
+        T          rx, ry, rz;
+        foldedSeparation(box, hi, x[i], y[i], z[i], x[j], y[j], z[j], rx, ry, rz);
+        T d  = std::sqrt(rx * rx + ry * ry + rz * rz);
+        T v  = d * hInv;
+        T w  = lookup(wh, v);
+        T dw = lookup(whd, v);
+        rho0 += m[j] * w;
+        sumVdW += m[j] * v * dw;
+        if (d > T(0))
+        {
+            T f = coef * m[j] * dw / d;
+            gxi += f * rx, gyi += f * ry, gzi += f * rz;
+            gx[j] -= f * rx, gy[j] -= f * ry, gz[j] -= f * rz;
+        }
+    }
+    gx[i] += gxi, gy[i] += gyi, gz[i] += gzi;
+    gh[i] += -coef * (T(3) * rho0 + sumVdW);
+}
+
+//! adjoint of IADJLoopSTD: cotangent gC (c11..c33 of target i); adds into gx, gy, gz, gh, gRho
+template<class T, class Tm>
+inline void iadStdVjp(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn, const double* x,
+                      const double* y, const double* z, const T* h, const Tm* m, const T* rho, const T C6[6],
+                      const T G6[6], const T* wh, const T* whd, T* gx, T* gy, T* gz, T* gh, T* gRho)
+{
+    T hi = h[i], hInv = T(1) / hi;
+    // symmetric matrices from the 6 stored entries (c11 c12 c13 c22 c23 c33)
+    T C[3][3] = {{C6[0], C6[1], C6[2]}, {C6[1], C6[3], C6[4]}, {C6[2], C6[4], C6[5]}};
+    T G[3][3] = {{G6[0], G6[1] / 2, G6[2] / 2}, {G6[1] / 2, G6[3], G6[4] / 2}, {G6[2] / 2, G6[4] / 2, G6[5]}};
+    T CG[3][3], B[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            CG[a][b] = C[a][0] * G[0][b] + C[a][1] * G[1][b] + C[a][2] * G[2][b];
+    T s = -T(K) * hInv * hInv * hInv;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            B[a][b] = s * (CG[a][0] * C[0][b] + CG[a][1] * C[1][b] + CG[a][2] * C[2][b]);
+    T trGC = G6[0] * C6[0] + G6[1] * C6[1] + G6[2] * C6[2] + G6[3] * C6[3] + G6[4] * C6[4] + G6[5] * C6[5];
+    T ghi  = T(3) * hInv * trGC;
+    T gxi = 0, gyi = 0, gzi = 0;
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j = nb[k];
+        T          r[3];
+        foldedSeparation(box, hi, x[i], y[i], z[i], x[j], y[j], z[j], r[0], r[1], r[2]);
+        T d  = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        T v  = d * hInv;
+        T w  = lookup(wh, v);
+        T dw = lookup(whd, v);
+        T Vj = m[j] / rho[j];
+        T Br[3];
+        for (int a = 0; a < 3; ++a)
+            Br[a] = B[a][0] * r[0] + B[a][1] * r[1] + B[a][2] * r[2];
+        T q = r[0] * Br[0] + r[1] * Br[1] + r[2] * Br[2];
+        T radial = d > T(0) ? Vj * dw * q * hInv / d : T(0);
+        T dr[3];
+        for (int a = 0; a < 3; ++a)
+            dr[a] = radial * r[a] + T(2) * Vj * w * Br[a];
+        gxi += dr[0], gyi += dr[1], gzi += dr[2];
+        gx[j] -= dr[0], gy[j] -= dr[1], gz[j] -= dr[2];
+        ghi += -Vj * dw * q * d * hInv * hInv;
+        gRho[j] += -m[j] / (rho[j] * rho[j]) * w * q;
+    }
+    gx[i] += gxi, gy[i] += gyi, gz[i] += gzi;
+    gh[i] += ghi;
+}
+
+//! gradient accumulators of the momentum-energy adjoint
+template<class T>
+struct MomStdGrads
+{
+    T *x, *y, *z, *h, *rho, *p, *alpha, *c[6];
+};
+
+//! adjoint of momentumEnergyStdLoop with per-particle alpha (null = 1): cotangents ga (ax, ay, az)
+//! and gdu of target i
+template<class T, class Tm>
+inline void momentumEnergyStdVjp(LocalIndex i, double K, const Box& box, const LocalIndex* nb, unsigned nn,
+                                 const double* x, const double* y, const double* z, const T* vx, const T* vy,
+                                 const T* vz, const T* h, const Tm* m, const T* rho, const T* p, const T* c,
+                                 const T* const* C6, const T* alpha, const T* wh, const T* whd, const T ga[3], T gdu,
+                                 MomStdGrads<T> g)
+{
+    auto load6 = [C6](LocalIndex k, T out[3][3])
+    {
+        T c11 = C6[0][k], c12 = C6[1][k], c13 = C6[2][k], c22 = C6[3][k], c23 = C6[4][k], c33 = C6[5][k];
+        T M[3][3] = {{c11, c12, c13}, {c12, c22, c23}, {c13, c23, c33}};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                out[a][b] = M[a][b];
+    };
+    // scatter a matrix cotangent of C (outer product u r^T, symmetric use) into the 6 stored entries
+    auto addC = [](T* const* gc, LocalIndex k, const T u[3], const T r[3], T sign)
+    {
+        gc[0][k] += sign * u[0] * r[0];
+        gc[1][k] += sign * (u[0] * r[1] + u[1] * r[0]);
+        gc[2][k] += sign * (u[0] * r[2] + u[2] * r[0]);
+        gc[3][k] += sign * u[1] * r[1];
+        gc[4][k] += sign * (u[1] * r[2] + u[2] * r[1]);
+        gc[5][k] += sign * u[2] * r[2];
+    };
+
+    T hi = h[i], hiInv = T(1) / hi, hiInv3 = hiInv * hiInv * hiInv;
+    T roi = rho[i], pri = p[i], ci = c[i], mi = m[i];
+    T vol_i = mi / roi, pro_i = pri / (roi * roi);
+    T alpha_i = alpha ? alpha[i] : T(1);
+    T Ci[3][3];
+    load6(i, Ci);
+    T Ga[3] = {T(K) * ga[0], T(K) * ga[1], T(K) * ga[2]};
+    T ge    = -T(K) * T(0.5) * gdu;
+
+    T gx_i = 0, gy_i = 0, gz_i = 0, gh_i = 0, grho_i = 0, gp_i = 0, galpha_i = 0;
+    T gC_i[6] = {0, 0, 0, 0, 0, 0};
+    T* gCi[6] = {&gC_i[0], &gC_i[1], &gC_i[2], &gC_i[3], &gC_i[4], &gC_i[5]};
+
+    for (unsigned k = 0; k < nn; ++k)
+    {
+        LocalIndex j = nb[k];
+        T          r[3];
+        foldedSeparation(box, hi, x[i], y[i], z[i], x[j], y[j], z[j], r[0], r[1], r[2]);
+        T d     = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        T vij[3] = {vx[i] - vx[j], vy[i] - vy[j], vz[i] - vz[j]};
+        T hj = h[j], hjInv = T(1) / hj, hjInv3 = hjInv * hjInv * hjInv;
+        T v1 = d * hiInv, v2 = d * hjInv;
+        T W1 = lookup(wh, v1), dW1 = lookup(whd, v1), W2 = lookup(wh, v2), dW2 = lookup(whd, v2);
+        T Wi = hiInv3 * W1, Wj = hjInv3 * W2;
+        T Cj[3][3];
+        load6(j, Cj);
+        T tAi[3], tAj[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            tAi[a] = Ci[a][0] * r[0] + Ci[a][1] * r[1] + Ci[a][2] * r[2];
+            tAj[a] = Cj[a][0] * r[0] + Cj[a][1] * r[1] + Cj[a][2] * r[2];
+        }
+        T rv      = r[0] * vij[0] + r[1] * vij[1] + r[2] * vij[2];
+        T w       = d > T(0) ? rv / d : T(0);
+        T alpha_j = alpha ? alpha[j] : T(1);
+        T cj      = c[j];
+        T visc = 0, dvisc_dw = 0, dvisc_dalpha = 0;
+        if (w < T(0))
+        {
+            T a4      = (alpha_i + alpha_j) / T(4) * (ci + cj);
+            visc      = T(0.5) * (-(a4 - T(2) * w) * w);
+            dvisc_dw  = T(0.5) * (-a4 + T(4) * w);
+            dvisc_dalpha = T(0.5) * (-(ci + cj) / T(4) * w);
+        }
+        T mj = m[j], roj = rho[j], vol_j = mj / roj, prj = p[j] / roj;
+
+        T aa = Wi * (mj * pro_i + visc * vol_i);
+        T bb = vol_j * Wj * (prj + visc);
+        T ae = Wi * (T(2) * mj * pro_i + visc * vol_i);
+        T be = visc * vol_j * Wj;
+
+        T s_i = Ga[0] * tAi[0] + Ga[1] * tAi[1] + Ga[2] * tAi[2];
+        T s_j = Ga[0] * tAj[0] + Ga[1] * tAj[1] + Ga[2] * tAj[2];
+        T e_i = vij[0] * tAi[0] + vij[1] * tAi[1] + vij[2] * tAi[2];
+        T e_j = vij[0] * tAj[0] + vij[1] * tAj[1] + vij[2] * tAj[2];
+
+        T aaBar = s_i, bbBar = s_j, aeBar = ge * e_i, beBar = ge * e_j;
+        T tAiBar[3], tAjBar[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            tAiBar[a] = aa * Ga[a] + ge * ae * vij[a];
+            tAjBar[a] = bb * Ga[a] + ge * be * vij[a];
+        }
+        T rBar[3];
+        for (int a = 0; a < 3; ++a)
+            rBar[a] = Ci[0][a] * tAiBar[0] + Ci[1][a] * tAiBar[1] + Ci[2][a] * tAiBar[2] + Cj[0][a] * tAjBar[0] +
+                      Cj[1][a] * tAjBar[1] + Cj[2][a] * tAjBar[2];
+        addC(gCi, 0, tAiBar, r, T(1));
+        addC(g.c, j, tAjBar, r, T(1));
+
+        T WiBar = aaBar * (mj * pro_i + visc * vol_i) + aeBar * (T(2) * mj * pro_i + visc * vol_i);
+        T WjBar = bbBar * vol_j * (prj + visc) + beBar * visc * vol_j;
+        T proBar_i = (aaBar + T(2) * aeBar) * Wi * mj;
+        T viscBar  = (aaBar + aeBar) * Wi * vol_i + (bbBar + beBar) * vol_j * Wj;
+        T volBar_i = (aaBar + aeBar) * Wi * visc;
+        T volBar_j = bbBar * Wj * (prj + visc) + beBar * visc * Wj;
+        T prjBar   = bbBar * vol_j * Wj;
+
+        T wBar = viscBar * dvisc_dw;
+        galpha_i += viscBar * dvisc_dalpha;
+        g.alpha[j] += viscBar * dvisc_dalpha;
+        if (d > T(0))
+        {
+            T dBar = WiBar * hiInv3 * hiInv * dW1 + WjBar * hjInv3 * hjInv * dW2;
+            for (int a = 0; a < 3; ++a)
+                rBar[a] += dBar * r[a] / d + wBar * (vij[a] / d - w * r[a] / (d * d));
+        }
+        gh_i += -WiBar * hiInv3 * hiInv * (T(3) * W1 + v1 * dW1);
+        g.h[j] += -WjBar * hjInv3 * hjInv * (T(3) * W2 + v2 * dW2);
+
+        gp_i += proBar_i / (roi * roi);
+        grho_i += -T(2) * proBar_i * pri / (roi * roi * roi) - volBar_i * mi / (roi * roi);
+        g.p[j] += prjBar / roj;
+        g.rho[j] += -volBar_j * mj / (roj * roj) - prjBar * p[j] / (roj * roj);
+
+        gx_i += rBar[0], gy_i += rBar[1], gz_i += rBar[2];
+        g.x[j] -= rBar[0], g.y[j] -= rBar[1], g.z[j] -= rBar[2];
+    }
+    g.x[i] += gx_i, g.y[i] += gy_i, g.z[i] += gz_i, g.h[i] += gh_i;
+    g.rho[i] += grho_i, g.p[i] += gp_i, g.alpha[i] += galpha_i;
+    for (int q = 0; q < 6; ++q)
+        g.c[q][i] += gC_i[q];
 }
 
 //! veDefGradhJLoop, hydro_ve/ve_def_gradh_kern.hpp:44-90 -> (kx, gradh)

@@ -63,7 +63,7 @@ class Params(C.Structure):
                 ("minDt", C.c_double), ("alphamin", C.c_double), ("alphamax", C.c_double),
                 ("decay_constant", C.c_double), ("Atmin", C.c_double), ("Atmax", C.c_double),
                 ("ramp", C.c_double), ("ng0", C.c_uint32), ("ngmax", C.c_uint32),
-                ("hydro_type", C.c_int32), ("reserved", C.c_int32)]
+                ("hydro_type", C.c_int32), ("flags", C.c_int32)]
 
     @classmethod
     def defaults(cls, K, hydro_type=SPHX_F64, **kw):
@@ -71,10 +71,21 @@ class Params(C.Structure):
         alphamin 0.05, alphamax 1, decay 0.2, Atmin 0.1, Atmax 0.2, ramp 1/(Atmax-Atmin)."""
         p = cls(K=K, Kcour=0.2, gamma=5.0 / 3.0, muiConst=10.0, minDt=1e-12, alphamin=0.05, alphamax=1.0,
                 decay_constant=0.2, Atmin=0.1, Atmax=0.2, ramp=1.0 / (0.2 - 0.1), ng0=100, ngmax=150,
-                hydro_type=hydro_type, reserved=0)
+                hydro_type=hydro_type, flags=0)
         for k, v in kw.items():
             setattr(p, k, v)
         return p
+
+
+SPHX_FLAG_STD_ALPHA = 1
+
+ADJOINT_FIELDS = ("x", "y", "z", "h", "rho", "p", "alpha", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az",
+                  "du")
+
+
+class Adjoint(C.Structure):
+    """sphx_adjoint_t: cotangent / gradient arrays of the adjoint entry points."""
+    _fields_ = [(n, C.c_void_p) for n in ADJOINT_FIELDS]
 
 
 class TreeView(C.Structure):

@@ -367,6 +367,22 @@ int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const 
     });
 }
 
+#define SPHX_VJP_ENTRY(name, fn)                                                                                        \
+    int name(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_params_t* prm,           \
+             uint32_t first, uint32_t last, const sphx_adjoint_t* in, sphx_adjoint_t* out)                             \
+    {                                                                                                                  \
+        return guarded(ws, [&] {                                                                                       \
+            SPHX_REQUIRE(box && f && prm && in && out, SPHX_ERR_ARG, "null argument");                                 \
+            requireTables(ws);                                                                                         \
+            fn(ws, makeDevBox(box), f, prm, first, last, in, out);                                                     \
+            return SPHX_OK;                                                                                            \
+        });                                                                                                            \
+    }
+
+SPHX_VJP_ENTRY(sphx_density_std_vjp, densityStdVjp)
+SPHX_VJP_ENTRY(sphx_iad_std_vjp, iadStdVjp)
+SPHX_VJP_ENTRY(sphx_momentum_energy_std_vjp, momentumEnergyStdVjp)
+
 int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                             const sphx_params_t* prm, uint32_t first, uint32_t last, int avClean,
                             double* minDtCourant)

@@ -333,6 +333,13 @@ void iadStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const
 double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                          uint32_t first, uint32_t last);
 
+void densityStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                   uint32_t first, uint32_t last, const sphx_adjoint_t* in, sphx_adjoint_t* out);
+void iadStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+               uint32_t last, const sphx_adjoint_t* in, sphx_adjoint_t* out);
+void momentumEnergyStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
+                          uint32_t first, uint32_t last, const sphx_adjoint_t* in, sphx_adjoint_t* out);
+
 void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
              uint32_t last);
 void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,

@@ -44,6 +44,7 @@ struct MomStdFields
     const void *vx, *vy, *vz, *h, *m, *rho, *p, *c, *c11, *c12, *c13, *c22, *c23, *c33;
     void *      ax, *ay, *az;
     double*     du;
+    const void* alpha; // per-particle AV coefficient (SPHX_FLAG_STD_ALPHA), else the reference's 1
 };
 
 template<class T>
@@ -164,8 +165,9 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     c22[i] = T(cij[3]), c23[i] = T(cij[4]), c33[i] = T(cij[5]);
 }
 
-/* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant */
-template<class T>
+/* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant.
+ * PerAlpha: AV coefficients alpha_i, alpha_j from f.alpha instead of the constant 1. */
+template<class T, bool PerAlpha>
 __global__ void __launch_bounds__(kPairWaves * kWave)
     momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh,
                       float Kcour, unsigned long long* minDt)
@@ -182,6 +184,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         T                  hiInv3 = hiInv * hiInv * hiInv;
         T                  pro_i  = fastDiv(ld<T>(f.p, i), roi * roi); // p_i / (gradh_i rho_i^2), gradh_i = 1
         double             r2h    = 2.0 * double(hi);
+        const T            alpha_i = PerAlpha ? ld<T>(f.alpha, i) : T(1);
 
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
@@ -219,7 +222,8 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
                     T tA3_j = rj.c13 * rx + rj.c23 * ry + rj.c33 * rz;
 
                     T wij  = fastDiv(rv, dist);
-                    T visc = T(0.5) * artificialViscosity(T(1), T(1), ri.c, rj.c, wij);
+                    T alpha_j = PerAlpha ? ld<T>(f.alpha, nb.j[u]) : T(1);
+                    T visc    = T(0.5) * artificialViscosity(alpha_i, alpha_j, ri.c, rj.c, wij);
                     T vsig = ri.c + rj.c - T(3) * wij;
                     maxvs  = nb.in(k, u) && vsig > maxvs ? vsig : maxvs;
 
@@ -285,14 +289,18 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
 
 template<class T>
 void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields& mf, float Kcour,
-                       unsigned long long* dt)
+                       unsigned long long* dt, bool perAlpha)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
     RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles);
     packMomStd<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
     SPHX_CHECK_LAUNCH();
-    momentumStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour, dt);
+    if (perAlpha)
+        momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour, dt);
+    else
+        momentumStdKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour,
+                                                                                    dt);
     SPHX_CHECK_LAUNCH();
 }
 
@@ -357,10 +365,12 @@ double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_field
     ws->counters.reserve(8);
     unsigned long long* dt = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
     fill64(ws, reinterpret_cast<uint64_t*>(dt), 0x7ff0000000000000ull, 1); // +inf
-    MomStdFields mf{f->vx,  f->vy,  f->vz,  f->h,   f->m,   f->rho, f->p,  f->c,  f->c11,
-                    f->c12, f->c13, f->c22, f->c23, f->c33, f->ax,  f->ay, f->az, f->du};
-    if (p->hydro_type == SPHX_F32) launchMomentumStd<float>(ws, a, mf, float(p->Kcour), dt);
-    else launchMomentumStd<double>(ws, a, mf, float(p->Kcour), dt);
+    const bool perAlpha = (p->flags & SPHX_FLAG_STD_ALPHA) != 0;
+    SPHX_REQUIRE(!perAlpha || f->alpha, SPHX_ERR_ARG, "SPHX_FLAG_STD_ALPHA set but fields.alpha is NULL");
+    MomStdFields mf{f->vx,  f->vy,  f->vz,  f->h,  f->m,  f->rho, f->p,  f->c,  f->c11, f->c12,
+                    f->c13, f->c22, f->c23, f->c33, f->ax, f->ay,  f->az, f->du, f->alpha};
+    if (p->hydro_type == SPHX_F32) launchMomentumStd<float>(ws, a, mf, float(p->Kcour), dt, perAlpha);
+    else launchMomentumStd<double>(ws, a, mf, float(p->Kcour), dt, perAlpha);
     unsigned long long bits = 0;
     SPHX_HIP(hipMemcpyAsync(&bits, dt, 8, hipMemcpyDeviceToHost, ws->stream));
     SPHX_HIP(hipStreamSynchronize(ws->stream));

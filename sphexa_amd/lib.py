@@ -17,6 +17,7 @@ _P = C.c_void_p
 _BOXP = C.POINTER(abi.Box)
 _FP = C.POINTER(abi.Fields)
 _PP = C.POINTER(abi.Params)
+_AP = C.POINTER(abi.Adjoint)
 _U32 = C.c_uint32
 _SZ = C.c_size_t
 
@@ -43,6 +44,9 @@ _SIGS = {
     "sphx_iad_divv_curlv": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_av_switches": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_momentum_energy_ve": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.POINTER(C.c_double)],
+    "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
+    "sphx_iad_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
+    "sphx_momentum_energy_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
 }
 
 EXPORTED = tuple(_SIGS) + ("sphx_last_error",)
@@ -156,6 +160,10 @@ class Workspace:
         self.call("sphx_momentum_energy_std", C.byref(box), C.byref(fields), C.byref(params), first, last,
                   C.byref(dt))
         return dt.value
+
+    def vjp(self, name, box, fields, params, first, last, cot, grad):
+        """Adjoint entry point `name`: cotangents `cot`, gradients added into `grad` (abi.Adjoint)."""
+        self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last, C.byref(cot), C.byref(grad))
 
     def momentum_energy_ve(self, box, fields, params, first, last, av_clean=False):
         dt = C.c_double()

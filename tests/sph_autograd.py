@@ -1,0 +1,123 @@
+"""torch.autograd wrappers of the std density, IAD and momentum-energy passes (test harness only).
+
+Forward and backward both run in the HIP library: the forward through the pass entry points,
+the backward through their adjoints (sphx_*_vjp). The neighbour lists are those of the last
+sphx_find_neighbors on the workspace and stay fixed while the graph is evaluated, so finite
+differences of `std_loss` see the same function the adjoints differentiate.
+"""
+import torch
+
+from sphexa_amd import abi
+
+C6 = ("c11", "c12", "c13", "c22", "c23", "c33")
+
+
+class StdGraph:
+    """Context of a differentiable std pass: workspace, box, params (flag SPHX_FLAG_STD_ALPHA set),
+    the non-differentiable fields (m, u, vx, vy, vz, c, nc) and the particle count."""
+
+    def __init__(self, ws, box, params, const, n, hydro_dtype):
+        self.ws, self.box, self.n, self.dtype = ws, box, n, hydro_dtype
+        self.params = abi.Params.defaults(params.K)
+        for name, _ in abi.Params._fields_:
+            setattr(self.params, name, getattr(params, name))
+        self.params.flags = params.flags | abi.SPHX_FLAG_STD_ALPHA
+        self.const = dict(const)
+
+    def fields(self, **tensors):
+        d = dict(self.const)
+        d.update(tensors)
+        return abi.fields_from(d)
+
+    def zeros(self):
+        return torch.zeros(self.n, dtype=self.dtype, device=self.const["m"].device)
+
+
+def _cot(g, t):
+    """cotangent as a contiguous HydroType tensor (zeros if the output was unused)."""
+    if t is None:
+        return g.zeros()
+    return t.to(g.dtype).contiguous()
+
+
+class DensityFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, g, x, y, z, h):
+        rho = g.zeros()
+        g.ws.pair("sphx_density_std", g.box, g.fields(x=x, y=y, z=z, h=h, rho=rho), g.params, 0, g.n)
+        ctx.g = g
+        ctx.save_for_backward(x, y, z, h)
+        return rho
+
+    @staticmethod
+    def backward(ctx, grho):
+        g = ctx.g
+        x, y, z, h = ctx.saved_tensors
+        cot = {"rho": _cot(g, grho)}
+        out = {k: g.zeros() for k in ("x", "y", "z", "h")}
+        g.ws.vjp("sphx_density_std_vjp", g.box, g.fields(x=x, y=y, z=z, h=h), g.params, 0, g.n, adjoint(cot),
+                 adjoint(out))
+        return None, out["x"].to(x.dtype), out["y"].to(y.dtype), out["z"].to(z.dtype), out["h"]
+
+
+class IadFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, g, x, y, z, h, rho):
+        c = {k: g.zeros() for k in C6}
+        g.ws.pair("sphx_iad_std", g.box, g.fields(x=x, y=y, z=z, h=h, rho=rho, **c), g.params, 0, g.n)
+        ctx.g = g
+        ctx.save_for_backward(x, y, z, h, rho, *(c[k] for k in C6))
+        return tuple(c[k] for k in C6)
+
+    @staticmethod
+    def backward(ctx, *gc):
+        g = ctx.g
+        x, y, z, h, rho, *c = ctx.saved_tensors
+        cot = {k: _cot(g, t) for k, t in zip(C6, gc)}
+        out = {k: g.zeros() for k in ("x", "y", "z", "h", "rho")}
+        f = g.fields(x=x, y=y, z=z, h=h, rho=rho, **dict(zip(C6, c)))
+        g.ws.vjp("sphx_iad_std_vjp", g.box, f, g.params, 0, g.n, adjoint(cot), adjoint(out))
+        return None, out["x"].to(x.dtype), out["y"].to(y.dtype), out["z"].to(z.dtype), out["h"], out["rho"]
+
+
+class MomentumFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, g, x, y, z, h, rho, p, c11, c12, c13, c22, c23, c33, alpha):
+        res = {k: g.zeros() for k in ("ax", "ay", "az")}
+        du = torch.zeros(g.n, dtype=torch.float64, device=x.device)
+        c = dict(zip(C6, (c11, c12, c13, c22, c23, c33)))
+        f = g.fields(x=x, y=y, z=z, h=h, rho=rho, p=p, alpha=alpha, du=du, **c, **res)
+        g.ws.momentum_energy_std(g.box, f, g.params, 0, g.n)
+        ctx.g = g
+        ctx.save_for_backward(x, y, z, h, rho, p, c11, c12, c13, c22, c23, c33, alpha)
+        return res["ax"], res["ay"], res["az"], du
+
+    @staticmethod
+    def backward(ctx, gax, gay, gaz, gdu):
+        g = ctx.g
+        x, y, z, h, rho, p, c11, c12, c13, c22, c23, c33, alpha = ctx.saved_tensors
+        cot = {"ax": _cot(g, gax), "ay": _cot(g, gay), "az": _cot(g, gaz), "du": _cot(g, gdu)}
+        names = ("x", "y", "z", "h", "rho", "p", "alpha") + C6
+        out = {k: g.zeros() for k in names}
+        c = dict(zip(C6, (c11, c12, c13, c22, c23, c33)))
+        f = g.fields(x=x, y=y, z=z, h=h, rho=rho, p=p, alpha=alpha, **c)
+        g.ws.vjp("sphx_momentum_energy_std_vjp", g.box, f, g.params, 0, g.n, adjoint(cot), adjoint(out))
+        return (None, out["x"].to(x.dtype), out["y"].to(y.dtype), out["z"].to(z.dtype), out["h"], out["rho"],
+                out["p"]) + tuple(out[k] for k in C6) + (out["alpha"],)
+
+
+def adjoint(tensors):
+    a = abi.Adjoint()
+    for k, t in tensors.items():
+        setattr(a, k, t.data_ptr())
+    return a
+
+
+def std_loss(g, x, y, z, h, alpha, u, gamma, rho_target, w):
+    """L = sum (rho - rho_target)^2 + w sum |a|^2 through density, ideal-gas EOS (in torch, from
+    the fixed u), IAD and momentum-energy."""
+    rho = DensityFn.apply(g, x, y, z, h)
+    p = (rho.double() * (u * (gamma - 1.0))).to(g.dtype)
+    c = IadFn.apply(g, x, y, z, h, rho)
+    ax, ay, az, _ = MomentumFn.apply(g, x, y, z, h, rho, p, *c, alpha)
+    return ((rho.double() - rho_target) ** 2).sum() + w * (ax.double() ** 2 + ay.double() ** 2 + az.double() ** 2).sum()

@@ -39,6 +39,11 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.Box) == 6 * 8 + 3 * 4 + 4  # padded to 8
     assert C.sizeof(abi.Params) == 11 * 8 + 4 * 4
     assert C.sizeof(abi.TreeView) == 4 * 4 + 10 * 8
+    body = text[text.index("typedef struct sphx_adjoint"):text.index("} sphx_adjoint_t;")]
+    names = tuple(re.findall(r"\*?(\w+)[,;]", body.split("{", 1)[1]))
+    assert names == abi.ADJOINT_FIELDS
+    assert C.sizeof(abi.Adjoint) == 8 * len(abi.ADJOINT_FIELDS)
+    assert "int32_t  flags;" in text and abi.Params.flags.offset == 11 * 8 + 3 * 4
 
 
 def test_null_workspace_is_rejected_without_touching_the_gpu():
