@@ -113,11 +113,16 @@ def adjoint(tensors):
     return a
 
 
-def std_loss(g, x, y, z, h, alpha, u, gamma, rho_target, w):
-    """L = sum (rho - rho_target)^2 + w sum |a|^2 through density, ideal-gas EOS (in torch, from
-    the fixed u), IAD and momentum-energy."""
+def std_loss_terms(g, x, y, z, h, alpha, u, gamma, rho_target, w):
+    """Per-particle terms (rho_i - rho_target_i)^2 + w |a_i|^2 through density, ideal-gas EOS (in
+    torch, from the fixed u), IAD and momentum-energy."""
     rho = DensityFn.apply(g, x, y, z, h)
     p = (rho.double() * (u * (gamma - 1.0))).to(g.dtype)
     c = IadFn.apply(g, x, y, z, h, rho)
     ax, ay, az, _ = MomentumFn.apply(g, x, y, z, h, rho, p, *c, alpha)
-    return ((rho.double() - rho_target) ** 2).sum() + w * (ax.double() ** 2 + ay.double() ** 2 + az.double() ** 2).sum()
+    return (rho.double() - rho_target) ** 2 + w * (ax.double() ** 2 + ay.double() ** 2 + az.double() ** 2)
+
+
+def std_loss(*args):
+    """L = sum (rho - rho_target)^2 + w sum |a|^2 (arguments of std_loss_terms)."""
+    return std_loss_terms(*args).sum()
