@@ -68,6 +68,31 @@ def make_ic(side, seed=42):
     return arr, spec
 
 
+# main kernels timed with HIP events on the workspace stream (sphx_kernel_ms)
+KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
+
+# outputs of the VE pass compared between the mixed and FP64 forms (rho = kx m / xm added);
+# positive quantities pointwise, signed ones relative to the field's largest magnitude
+COMPARED = ("h", "xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "c33", "divv", "curlv",
+            "alpha", "ax", "ay", "az", "du")
+POINTWISE = ("h", "xm", "kx", "gradh", "prho", "c", "rho")
+
+
+def field_errors(mixed, ref):
+    """Max and RMS relative error of each field of the mixed form against the FP64 form."""
+    out = {}
+    for k in sorted(ref):
+        d = np.abs(mixed[k] - ref[k])
+        if k in POINTWISE:
+            r = d / np.abs(ref[k])
+            norm = "pointwise"
+        else:
+            r = d / max(float(np.max(np.abs(ref[k]))), 1e-300)
+            norm = "field max"
+        out[k] = {"max": float(f"{np.max(r):.3e}"), "rms": float(f"{np.sqrt(np.mean(r * r)):.3e}"), "norm": norm}
+    return out
+
+
 PHASES = ("domain::sync", "FindNeighbors", "XMass", "Normalization & Gradh", "EquationOfState",
           "IadVelocityDivCurl", "AVswitches", "MomentumAndEnergy")
 
@@ -86,6 +111,9 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     n = ps.n
     f = ps.fields()
     stream = torch.cuda.current_stream()
+
+    ws.set_profiling(True)
+    kernel_ms = {k: [] for k in KERNELS}
 
     def one_step(ev):
         ev[0].record(stream)
@@ -107,6 +135,12 @@ def run_gpu(arr, spec, mixed, steps, warmup):
         ev[8].record(stream)
         return dt
 
+    def read_kernels():
+        # the search and momentum-energy entry points end with a stream synchronisation, so every
+        # event of the step has completed and these reads do not stall
+        for k in KERNELS:
+            kernel_ms[k].append(ws.kernel_ms(k))
+
     mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(9)]
     for _ in range(warmup):
         one_step(mk())
@@ -116,13 +150,18 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     for _ in range(steps):
         ev = mk()
         one_step(ev)
+        read_kernels()
         events.append(ev)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     phase_ms = {p: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in events])) for i, p in enumerate(PHASES)}
     pairs, maxnc = ws.neighbor_stats()
     out = dict(n=n, wall_s=wall, ms_per_step=1e3 * wall / steps, phase_ms=phase_ms, pairs=int(pairs),
-               max_nc=int(maxnc), rate=n * steps / wall)
+               max_nc=int(maxnc), rate=n * steps / wall,
+               kernel_ms={k: float(np.mean(v)) for k, v in kernel_ms.items()},
+               fields={k: ps.host(k).astype(np.float64) for k in COMPARED},
+               nc=ps.host("nc").copy())
+    out["fields"]["rho"] = out["fields"]["kx"] * ps.host("m").astype(np.float64) / out["fields"]["xm"]
     del ps, f
     ws.close()
     torch.cuda.empty_cache()
@@ -175,7 +214,7 @@ def main():
 
     head = res.get("mixed") or res[forms[0]]
     mixed = "mixed" in res
-    mom_ms = head["phase_ms"]["MomentumAndEnergy"]
+    mom_ms = head["kernel_ms"]["momentum_ve"]
     flops = FLOPS_PER_PAIR_MOM_VE * head["pairs"] + FLOPS_PER_TARGET_MOM_VE * head["n"]
     achieved = flops / (mom_ms * 1e-3) / 1e12
     peak = PEAK_FP32_VECTOR_TFLOPS if mixed else PEAK_FP64_VECTOR_TFLOPS
@@ -203,6 +242,11 @@ def main():
         "fp64_value": round(res["fp64"]["rate"], 1) if "fp64" in res else None,
         "mixed_over_fp64": round(res["mixed"]["rate"] / res["fp64"]["rate"], 3) if len(res) == 2 else None,
         "phase_ms": {f: {k: round(v, 4) for k, v in r["phase_ms"].items()} for f, r in res.items()},
+        "kernel_ms": {f: {k: round(v, 4) for k, v in r["kernel_ms"].items()} for f, r in res.items()},
+        "mixed_error": ({"reference": "fp64 form of this pass", "fields": field_errors(res["mixed"]["fields"],
+                                                                                       res["fp64"]["fields"]),
+                         "nc_equal_fraction": float(np.mean(res["mixed"]["nc"] == res["fp64"]["nc"]))}
+                        if len(res) == 2 else None),
         "neighbor_pairs": head["pairs"],
         "roofline": roofline,
         "cpu_baseline": cpu,

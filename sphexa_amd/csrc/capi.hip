@@ -147,6 +147,11 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     (void)hipStreamSynchronize(ws->stream);
     ws->wh64.release(), ws->whd64.release(), ws->wh32.release(), ws->whd32.release();
     ws->whq64.release(), ws->whq32.release();
+    for (auto& kv : ws->kernelEvents)
+    {
+        if (kv.second.start) { (void)hipEventDestroy(kv.second.start); }
+        if (kv.second.stop) { (void)hipEventDestroy(kv.second.stop); }
+    }
     ws->tree.releaseAll();
     ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release(), ws->idxTmp.release();
     ws->fieldTmp.release(), ws->order.release(), ws->records.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
@@ -155,6 +160,31 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
 }
 
 const char* sphx_last_error(const sphx_workspace_t* ws) { return ws ? ws->lastError.c_str() : "null workspace"; }
+
+int sphx_workspace_set_profiling(sphx_workspace_t* ws, int on)
+{
+    return guarded(ws, [&] {
+        ws->profile = on != 0;
+        for (auto& kv : ws->kernelEvents)
+            kv.second.recorded = false;
+        return SPHX_OK;
+    });
+}
+
+int sphx_kernel_ms(sphx_workspace_t* ws, const char* kernel, double* ms)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(kernel && ms, SPHX_ERR_ARG, "null argument");
+        auto it = ws->kernelEvents.find(kernel);
+        SPHX_REQUIRE(it != ws->kernelEvents.end() && it->second.recorded, SPHX_ERR_STATE,
+                     std::string("no timed launch of ") + kernel);
+        SPHX_HIP(hipEventSynchronize(it->second.stop));
+        float t = 0;
+        SPHX_HIP(hipEventElapsedTime(&t, it->second.start, it->second.stop));
+        *ms = t;
+        return SPHX_OK;
+    });
+}
 
 int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K)
 {

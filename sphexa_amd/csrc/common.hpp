@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -169,7 +170,47 @@ struct sphx_workspace
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions
+
+    // optional per-kernel timing: HIP events around the main kernel of each pass
+    struct KernelEvents
+    {
+        hipEvent_t start = nullptr, stop = nullptr;
+        bool       recorded = false;
+    };
+    bool                                profile = false;
+    std::map<std::string, KernelEvents> kernelEvents;
 };
+
+namespace sphx
+{
+//! records HIP events on the workspace stream around the enclosed launch when profiling is on
+class KernelScope
+{
+public:
+    KernelScope(sphx_workspace* ws, const char* name)
+        : ws_(ws)
+    {
+        if (!ws->profile) { return; }
+        ev_ = &ws->kernelEvents[name];
+        if (!ev_->start)
+        {
+            SPHX_HIP(hipEventCreate(&ev_->start));
+            SPHX_HIP(hipEventCreate(&ev_->stop));
+        }
+        SPHX_HIP(hipEventRecord(ev_->start, ws->stream));
+    }
+    ~KernelScope()
+    {
+        if (ev_ && hipEventRecord(ev_->stop, ws_->stream) == hipSuccess) { ev_->recorded = true; }
+    }
+    KernelScope(const KernelScope&)            = delete;
+    KernelScope& operator=(const KernelScope&) = delete;
+
+private:
+    sphx_workspace*                ws_;
+    sphx_workspace::KernelEvents* ev_ = nullptr;
+};
+} // namespace sphx
 
 namespace sphx
 {

@@ -263,6 +263,7 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
     packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.x, a.y, a.z, (const T*)f->m,
                                                                                          rec, a.numParticles);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, toRho ? "density_std" : "xmass");
     if (toRho)
         xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
                                                                             (T*)out);
@@ -281,6 +282,7 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
     packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
         a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "iad_std");
     iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
                                                                     (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
                                                                     (T*)f->c23, (T*)f->c33);
@@ -296,6 +298,7 @@ void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields
     RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles);
     packMomStd<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "momentum_std");
     if (perAlpha)
         momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour, dt);
     else

@@ -493,6 +493,7 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
     packGradh<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, (const T*)f->m,
                                                                                           (const T*)f->xm, rec);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "ve_def_gradh");
     veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whqTable<T>(ws),
                                                                         (T*)f->kx, (T*)f->gradh);
     SPHX_CHECK_LAUNCH();
@@ -506,6 +507,7 @@ void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf)
     RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles);
     packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "iad_divv_curlv");
     iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, whTable<T>(ws));
     SPHX_CHECK_LAUNCH();
 }
@@ -518,6 +520,7 @@ void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const s
     RecAv<T>* rec = recordBuffer<RecAv<T>>(ws, a.numParticles);
     packAv<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, af, rec);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "av_switches");
     avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, af, rec, whTable<T>(ws), p->minDt,
                                                                         T(p->alphamin), T(p->alphamax),
                                                                         T(p->decay_constant));
@@ -533,6 +536,7 @@ void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, c
     RecMomVe<T>* rec = recordBuffer<RecMomVe<T>>(ws, a.numParticles);
     packMomVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
     SPHX_CHECK_LAUNCH();
+    KernelScope timer(ws, "momentum_ve");
     if (avClean)
         momentumVeKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
             a, mf, rec, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);

@@ -425,7 +425,8 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
 
     if (groups > 0)
     {
-        unsigned blocks = ceilDiv(groups, kWavesPerBlock);
+        unsigned    blocks = ceilDiv(groups, kWavesPerBlock);
+        KernelScope timer(ws, "search");
         if (p->hydro_type == SPHX_F32)
             searchKernel<float><<<blocks, kWavesPerBlock * kWave, 0, ws->stream>>>(a, static_cast<float*>(f->h));
         else

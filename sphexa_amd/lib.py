@@ -44,6 +44,8 @@ _SIGS = {
     "sphx_iad_divv_curlv": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_av_switches": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_momentum_energy_ve": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.POINTER(C.c_double)],
+    "sphx_workspace_set_profiling": [_P, C.c_int],
+    "sphx_kernel_ms": [_P, C.c_char_p, C.POINTER(C.c_double)],
     "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_iad_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_momentum_energy_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
@@ -160,6 +162,15 @@ class Workspace:
         self.call("sphx_momentum_energy_std", C.byref(box), C.byref(fields), C.byref(params), first, last,
                   C.byref(dt))
         return dt.value
+
+    def set_profiling(self, on=True):
+        self.call("sphx_workspace_set_profiling", int(on))
+
+    def kernel_ms(self, kernel):
+        """Duration (ms) of the last timed launch of `kernel` (see sphx_kernel_ms)."""
+        ms = C.c_double()
+        self.call("sphx_kernel_ms", kernel.encode(), C.byref(ms))
+        return ms.value
 
     def vjp(self, name, box, fields, params, first, last, cot, grad):
         """Adjoint entry point `name`: cotangents `cot`, gradients added into `grad` (abi.Adjoint)."""
