@@ -45,6 +45,12 @@ FLOPS_PER_PAIR_MOM_VE = 142
 # per-target part of the same kernel (loads aside): rho_i, 1/h_i, h_i^-3, eta_crit, epilogue
 FLOPS_PER_TARGET_MOM_VE = 40
 
+# HBM bytes per launch of the momentum-energy kernel on this workload (200^3), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE counter runs scaled by a plain copy kernel of known size (read x 2.0,
+# write x 1.0); see profiles/ve_turbulence_8M_counters.md. Counters cannot be read from inside the
+# timed run, so the measured figure is carried here for the default workload only.
+MEASURED_TRAFFIC_MOM_VE = {(200, "mixed"): 5.242e9, (200, "fp64"): 7.636e9}
+
 
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[1])
@@ -56,7 +62,23 @@ def parse():
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (counter calibration under rocprofv3)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    """A plain copy kernel of known size: 2^28 float32 elements read and written (1 GiB each way)."""
+    import torch
+
+    n = 1 << 28
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}), flush=True)
 
 
 def make_ic(side, seed=42):
@@ -201,6 +223,9 @@ def main():
     if world > 1:
         print("bench.py is single-GPU; launch it without torch.distributed", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import torch
 
@@ -218,8 +243,11 @@ def main():
     flops = FLOPS_PER_PAIR_MOM_VE * head["pairs"] + FLOPS_PER_TARGET_MOM_VE * head["n"]
     achieved = flops / (mom_ms * 1e-3) / 1e12
     peak = PEAK_FP32_VECTOR_TFLOPS if mixed else PEAK_FP64_VECTOR_TFLOPS
+    traffic = MEASURED_TRAFFIC_MOM_VE.get((args.side, "mixed" if mixed else "fp64"))
     roofline = dict(bound="valu", kernel="momentumVeKernel", achieved=round(achieved, 3), peak=peak,
-                    unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=None,
+                    unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic,
+                    traffic_gbs=round(traffic / (mom_ms * 1e-3) / 1e9, 1) if traffic else None,
+                    hbm_peak_gbs=PEAK_HBM_GBS,
                     flops_per_launch=int(flops), avg_launch_ms=round(mom_ms, 4))
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_side, args.cpu_passes)

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     withFold(waveNeedsFold(a.box, xi, yi, zi, r2h), [&](auto fold) {
         for (unsigned k = 0; k < t.n; ++k)
         {
-            const uint32_t j = t.list[size_t(k) * kGroup];
+            const uint32_t j = t.list[laneSlot(k)];
             T              rx, ry, rz;
             separation<decltype(fold)::value>(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
             T d  = sqrt(rx * rx + ry * ry + rz * rz);
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     withFold(waveNeedsFold(a.box, xi, yi, zi, r2h), [&](auto fold) {
         for (unsigned k = 0; k < t.n; ++k)
         {
-            const uint32_t j = t.list[size_t(k) * kGroup];
+            const uint32_t j = t.list[laneSlot(k)];
             T              r[3];
             separation<decltype(fold)::value>(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], r[0], r[1], r[2]);
             T d    = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     withFold(waveNeedsFold(a.box, xi, yi, zi, r2h), [&](auto fold) {
         for (unsigned k = 0; k < t.n; ++k)
         {
-            const uint32_t j = t.list[size_t(k) * kGroup];
+            const uint32_t j = t.list[laneSlot(k)];
             T              r[3];
             separation<decltype(fold)::value>(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], r[0], r[1], r[2]);
             T d      = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);

@@ -23,6 +23,22 @@ constexpr int      kTableSize = 20000;
 constexpr int      kWave      = 64; // gfx950 wavefront
 constexpr int      kGroup     = 64; // targets per neighbour-search group (one wavefront)
 
+/* Neighbour-list layout. Group g of 64 consecutive targets owns ngmax4 = ngmax rounded up to 4
+ * slots per lane; slot k of lane l is at
+ *     (g * ngmax4 + (k & ~3)) * 64 + 4 l + (k & 3),
+ * so each lane's slots come in 16-byte quads and quad q of all 64 lanes is one contiguous
+ * kilobyte: the search stores a lane's entries a quad at a time, and a pair loop reads four
+ * neighbour indices of all its lanes with one coalesced 16-byte-per-lane load. */
+__host__ __device__ inline uint32_t paddedNgmax(uint32_t ngmax) { return (ngmax + 3u) & ~3u; }
+
+__host__ __device__ inline size_t listOffset(uint32_t g, uint32_t ngmax4, uint32_t k, uint32_t lane)
+{
+    return (size_t(g) * ngmax4 + (k & ~3u)) * kGroup + 4u * lane + (k & 3u);
+}
+
+//! slot k relative to a lane's slot 0
+__host__ __device__ inline size_t laneSlot(uint32_t k) { return size_t(k & ~3u) * kGroup + (k & 3u); }
+
 /* ---------------------------------------------------------------------------------------------
  * error handling: every entry point returns an sphx_status and records a message
  * -------------------------------------------------------------------------------------------*/

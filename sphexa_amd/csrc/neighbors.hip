@@ -12,8 +12,9 @@
 // (domain/include/cstone/findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with
 // dx = x_j - x_i, periodic images folded by one box length, unfused FP64 arithmetic, keep if
 // d2 < Th(4)*h_i*h_i and j != i. The kept set and count are therefore bit-identical to the CPU
-// search; the first ngmax hits are stored, group-interleaved as [group][k][lane] so that the pair
-// loops read them with one coalesced 256-byte load per neighbour slot.
+// search; the first ngmax hits are stored in the group layout of common.hpp (listOffset): a lane
+// collects its entries in a register quad and stores 16 bytes at a time, and the pair loops read
+// four slots of all 64 lanes with one coalesced load.
 #include "common.hpp"
 
 namespace sphx
@@ -38,7 +39,7 @@ struct SearchArgs
     const double*        sizes;
     DevBox               box;
     double               slack; // absolute enlargement of search radii (rounding margin)
-    uint32_t             first, last, ng0, ngmax;
+    uint32_t             first, last, ng0, ngmax, ngmax4;
     int                  iterateH;
     uint32_t*            nbrs;
     uint32_t*            nc;
@@ -63,6 +64,31 @@ struct Target
     bool     active;
 };
 
+//! A lane's appending writer into its neighbour list: entries collect in a register quad that is
+//! stored (16 bytes) when full and, for the incomplete last quad, by finish(). Slots past the
+//! stored count inside that quad receive don't-care values; readers never use them.
+struct ListWriter
+{
+    uint32_t* base; // the lane's slot 0
+    uint32_t  q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+
+    __device__ __forceinline__ void put(unsigned k, uint32_t j)
+    {
+        const unsigned s = k & 3u;
+        q0 = s == 0 ? j : q0, q1 = s == 1 ? j : q1, q2 = s == 2 ? j : q2, q3 = s == 3 ? j : q3;
+        if (s == 3) { store(k); }
+    }
+    __device__ __forceinline__ void store(unsigned k)
+    {
+        *reinterpret_cast<uint4*>(base + laneSlot(k & ~3u)) = make_uint4(q0, q1, q2, q3);
+    }
+    //! flush after `stored` entries
+    __device__ __forceinline__ void finish(unsigned stored)
+    {
+        if (stored & 3u) { store(stored - 1); }
+    }
+};
+
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ double uniformD(double v)
@@ -83,7 +109,7 @@ enum class Fold
 template<Fold mode>
 __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, const double* tx, const double* ty,
                                           const double* tz, uint32_t lo, uint32_t n, double sx, double sy, double sz,
-                                          unsigned& count, uint32_t* myList)
+                                          unsigned& count, ListWriter& out)
 {
 #pragma clang fp contract(off)
     const DevBox&  b     = a.box;
@@ -103,7 +129,7 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
         uint32_t j  = lo + c;
         if (t.active && d2 < t.r2 && j != t.i)
         {
-            if (count < ngmax) { myList[size_t(count) * kGroup] = j; }
+            if (count < ngmax) { out.put(count, j); }
             count++;
         }
     }
@@ -111,7 +137,7 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
 
 //! Prune the buffered leaves against every lane's sphere, then stream the survivors through LDS.
 __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
-                            const double ge[3], double reach, unsigned& count, uint32_t* myList)
+                            const double ge[3], double reach, unsigned& count, ListWriter& out)
 {
 #pragma clang fp contract(off)
     const DevBox&  b    = a.box;
@@ -200,12 +226,12 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
             perPair  = perPair || (ge[d] + s + reach > 0.25 * b.len[d]);
         }
         if (perPair)
-            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, myList);
+            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, out);
         else if (shifted)
             sweepTile<Fold::shifted>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, sh[0], sh[1], sh[2],
-                                     count, myList);
+                                     count, out);
         else
-            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, myList);
+            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, out);
         buf ^= 1;
     }
     __builtin_amdgcn_wave_barrier();
@@ -218,7 +244,7 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
  * at a time. A one-at-a-time pop replaces a node by at most 8 children and the next pop takes
  * the last child, i.e. it proceeds depth first, which adds at most 7 entries per tree level
  * below the popped node (<= 7*20 = 140 < 160) on top of the stack it started from. */
-__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList)
+__device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out)
 {
 #pragma clang fp contract(off)
     const unsigned lane = laneId();
@@ -272,7 +298,7 @@ __device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double rad
         int      nl    = __popcll(mLeaf);
         if (nLeaf + nl > kLeafCap)
         {
-            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, myList);
+            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out);
             nLeaf = 0;
         }
         if (ov && isLeaf) { lds.leaves[nLeaf + laneRank(mLeaf)] = node; }
@@ -286,7 +312,16 @@ __device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double rad
         top += 8 * __popcll(mInt);
         __builtin_amdgcn_wave_barrier();
     }
-    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, myList);
+    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out);
+    return count;
+}
+
+//! search of the active lanes' targets into their lists (slot 0 at myList); returns the counts
+__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList)
+{
+    ListWriter out{myList};
+    unsigned   count = searchGroupInto(a, t, radius, lds, out);
+    if (t.active) { out.finish(min(count, a.ngmax)); }
     return count;
 }
 
@@ -330,7 +365,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
     t.y      = t.active ? a.y[t.i] : 0.0;
     t.z      = t.active ? a.z[t.i] : 0.0;
     setRadius(t, hi, a.slack);
-    uint32_t* myList = a.nbrs + size_t(g) * a.ngmax * kGroup + lane;
+    uint32_t* myList = a.nbrs + listOffset(g, a.ngmax4, 0, lane);
 
     bool     valid = t.active;
     unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList);
@@ -383,7 +418,7 @@ __global__ void copyListsKernel(const uint32_t* __restrict__ nbrs, const uint32_
     uint32_t r = i - first, g = r / kGroup, lane = r % kGroup;
     uint32_t n = min(nc[i] - 1, ngmax);
     for (uint32_t k = 0; k < n; ++k)
-        out[size_t(r) * ngmax + k] = nbrs[(size_t(g) * ngmax + k) * kGroup + lane];
+        out[size_t(r) * ngmax + k] = nbrs[listOffset(g, paddedNgmax(ngmax), k, lane)];
 }
 
 } // namespace
@@ -400,7 +435,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     const Tree& t       = ws->tree;
     uint32_t    numTgt  = last - first;
     uint32_t    groups  = ceilDiv(numTgt, kGroup);
-    ws->nbrs.reserve(std::max<size_t>(1, size_t(groups) * p->ngmax * kGroup));
+    ws->nbrs.reserve(std::max<size_t>(1, size_t(groups) * paddedNgmax(p->ngmax) * kGroup));
     ws->counters.reserve(8);
     fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 3);
 
@@ -417,7 +452,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.sizes          = t.sizes.get();
     a.box            = box;
     a.slack          = 1e-12 * scale;
-    a.first = first, a.last = last, a.ng0 = p->ng0, a.ngmax = p->ngmax;
+    a.first = first, a.last = last, a.ng0 = p->ng0, a.ngmax = p->ngmax, a.ngmax4 = paddedNgmax(p->ngmax);
     a.iterateH = iterateH ? 1 : 0;
     a.nbrs     = ws->nbrs.get();
     a.nc       = f->nc;

@@ -2,8 +2,8 @@
 //
 // Launch shape: one wavefront per group of 64 consecutive targets (the neighbour-search group),
 // 4 waves per block, blocks renumbered per XCD (xcdBlock) so that neighbouring groups share an
-// L2. Lane l of wave g owns target first + 64 g + l and walks its own list
-// nbrs[(g*ngmax + k)*64 + l], k < min(nc-1, ngmax): one coalesced 256-byte load per slot.
+// L2. Lane l of wave g owns target first + 64 g + l and walks its own list slots
+// k < min(nc-1, ngmax) in the quad layout of common.hpp (listOffset).
 //
 // Two arithmetic forms share each kernel template:
 //  * T = double ("plain" FP64): the reference JLoop formulas in FP64.
@@ -29,7 +29,7 @@ struct Lists
 {
     const uint32_t* nbrs;
     const uint32_t* nc;
-    uint32_t        first, last, ngmax;
+    uint32_t        first, last, ngmax, ngmax4;
 };
 
 struct Lane
@@ -37,7 +37,7 @@ struct Lane
     uint32_t        i;     // target index
     bool            valid; // lane holds a target
     unsigned        n;     // neighbours to visit
-    const uint32_t* list;  // this lane's first slot; slot k at list[k * 64]
+    const uint32_t* list;  // this lane's slot 0; slot k at list[laneSlot(k)]
 };
 
 __device__ __forceinline__ Lane laneTarget(const Lists& L)
@@ -48,7 +48,7 @@ __device__ __forceinline__ Lane laneTarget(const Lists& L)
     t.i                 = L.first + g * kGroup + lane;
     t.valid             = t.i < L.last;
     t.n                 = t.valid ? min(L.nc[t.i] - 1, L.ngmax) : 0u;
-    t.list              = L.nbrs + size_t(g) * L.ngmax * kGroup + lane;
+    t.list              = L.nbrs + listOffset(g, L.ngmax4, 0, lane);
     return t;
 }
 
@@ -114,19 +114,22 @@ __device__ __forceinline__ T fastSqrt(T a)
     else { return sqrt(a); }
 }
 
-/*! Batched walk over one lane's neighbour list (slot k at list[k * 64]).
+/*! Batched walk over one lane's neighbour list.
  *
- * A pair loop visits its neighbours U at a time: load(k) makes the records of slots k..k+U-1
- * current (their list entries were requested one batch earlier) and requests the list entries of
- * the following batch. The loop then forms the U separations and issues the U table fetches
- * before it consumes any of them, so one batch costs two memory round trips (records, tables)
- * instead of three per neighbour. Slots beyond the lane's count are clamped to its last stored
- * one and their pair terms are skipped (in(k, u)). Slot 0 of a valid target always holds a valid
- * particle index (the search writes the target itself there when it has no neighbours), so no
- * load reads past the stored list or an unwritten slot. Valid targets only. */
+ * A pair loop visits its neighbours U at a time (U = 1, 2 or 4): load(k) makes the records of
+ * slots k..k+U-1 current (their list entries were requested one batch earlier, U slots with one
+ * aligned vector load from the lane's quad) and requests the list entries of the following batch.
+ * The loop then forms the U separations and issues the U table fetches before it consumes any of
+ * them, so one batch costs two memory round trips (records, tables) instead of three per
+ * neighbour. Slots beyond the lane's count take the batch's first valid entry and their pair
+ * terms are zeroed (live). Slot 0 of a valid target always holds a valid particle index (the
+ * search writes the target itself there when it has no neighbours), so every index used for a
+ * record load names a particle. Valid targets only. */
 template<int U, class R>
 struct NeighborBatch
 {
+    static_assert(U == 1 || U == 2 || U == 4, "batch of 1, 2 or 4 slots");
+
     const uint32_t* list;
     const R*        rec;
     unsigned        n, lastSlot;
@@ -137,17 +140,37 @@ struct NeighborBatch
     __device__ __forceinline__ NeighborBatch(const Lane& t, const R* __restrict__ records)
         : list(t.list), rec(records), n(t.n), lastSlot(t.n ? t.n - 1 : 0)
     {
+        slots(0, next);
 #pragma unroll
         for (int u = 0; u < U; ++u)
         {
-            next[u] = slot(u);
             // an opaque definition: otherwise the compiler merges this load with the look-ahead
             // load of the loop into one load at the top of each batch, undoing the look-ahead
             asm volatile("" : "+v"(next[u]));
         }
     }
 
-    __device__ __forceinline__ uint32_t slot(unsigned k) const { return list[size_t(min(k, lastSlot)) * kGroup]; }
+    //! list entries of slots k..k+U-1 (clamped as described above)
+    __device__ __forceinline__ void slots(unsigned k, uint32_t out[U]) const
+    {
+        const unsigned  kq = min(k, lastSlot) & ~unsigned(U - 1);
+        const uint32_t* p  = list + laneSlot(kq);
+        uint32_t        v[U];
+        if constexpr (U == 4)
+        {
+            const uint4 q = *reinterpret_cast<const uint4*>(p);
+            v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+        }
+        else if constexpr (U == 2)
+        {
+            const uint2 q = *reinterpret_cast<const uint2*>(p);
+            v[0] = q.x, v[1] = q.y;
+        }
+        else { v[0] = p[0]; }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            out[u] = kq + u <= lastSlot ? v[u] : v[0];
+    }
 
     __device__ __forceinline__ void load(unsigned k)
     {
@@ -157,9 +180,7 @@ struct NeighborBatch
             j[u] = next[u];
             r[u] = rec[j[u]];
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            next[u] = slot(k + U + u);
+        slots(k + U, next);
     }
 
     //! whether member u of the batch starting at slot k is a real neighbour
@@ -369,7 +390,7 @@ inline PairArgs makePairArgs(sphx_workspace* ws, const DevBox& box, const sphx_f
     SPHX_REQUIRE(f->x && f->y && f->z && f->nc, SPHX_ERR_ARG, "x, y, z and nc are required");
     PairArgs c;
     c.x = f->x, c.y = f->y, c.z = f->z, c.box = box, c.K = p->K;
-    c.L            = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax};
+    c.L            = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax, paddedNgmax(p->ngmax)};
     c.numParticles = uint32_t(ws->tree.numParticles);
     return c;
 }
