@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     double           r2h   = 2.0 * double(hi);
     Acc<T>           rho0(ri.m);
     withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-        constexpr int                U = 4;
+        constexpr int                U = kBatchLight<T>;
         NeighborBatch<U, RecMass<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
         {
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     double          r2h   = 2.0 * double(hi);
     T               t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-        constexpr int               U = isMixed<T> ? 4 : 2;
+        constexpr int               U = kBatchLight<T>;
         NeighborBatch<U, RecVol<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
         {
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy;
         withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-            constexpr int                  U = isMixed<T> ? 2 : 1;
+            constexpr int                  U = kBatchMom<T>;
             NeighborBatch<U, RecMomStd<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)
             {

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     double            r2h  = 2.0 * double(hi);
     Acc<T>            kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
     withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-        constexpr int                 U = isMixed<T> ? 4 : 2;
+        constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecGradh<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
         {
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T                 t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     T                 M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-        constexpr int                 U = isMixed<T> ? 4 : 2;
+        constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecIadVe<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
         {
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
     T              vsig  = T(1.e-40) * ri.c;
     T              S[3]  = {0, 0, 0};
     withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-        constexpr int              U = isMixed<T> ? 4 : 2;
+        constexpr int              U = kBatchLight<T>;
         NeighborBatch<U, RecAv<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
         {
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy, avEnergy;
         withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
-            constexpr int                 U = isMixed<T> ? 2 : 1;
+            constexpr int                 U = kBatchMom<T>;
             NeighborBatch<U, RecMomVe<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)
             {

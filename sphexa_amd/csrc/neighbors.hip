@@ -51,7 +51,8 @@ struct WaveLds
 {
     int32_t stack[kStackCap];
     int32_t leaves[kLeafCap];
-    double  tx[2][kWave], ty[2][kWave], tz[2][kWave];
+    double   tx[2][kWave], ty[2][kWave], tz[2][kWave];
+    uint32_t tj[2][kWave]; // particle index of each staged candidate
 };
 
 //! per-target state of a lane
@@ -105,35 +106,55 @@ enum class Fold
     perPair  // box too small for a single image: rint fold per pair, as the reference
 };
 
-//! test the `n` staged candidates (particles lo..lo+n-1) against the lane's target
+//! Test the `n` staged candidates against the lane's target. Hits are recorded as bits of a
+//! per-lane mask over chunks of 32 candidates and appended afterwards in candidate order, so the
+//! list keeps the traversal order while the test loop stays free of the divergent append.
 template<Fold mode>
 __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, const double* tx, const double* ty,
-                                          const double* tz, uint32_t lo, uint32_t n, double sx, double sy, double sz,
-                                          unsigned& count, ListWriter& out)
+                                          const double* tz, const uint32_t* tj, uint32_t n, double sx, double sy,
+                                          double sz, unsigned& count, ListWriter& out)
 {
 #pragma clang fp contract(off)
     const DevBox&  b     = a.box;
     const unsigned ngmax = a.ngmax;
-#pragma unroll 4
-    for (uint32_t c = 0; c < n; ++c)
+    for (uint32_t c0 = 0; c0 < n; c0 += 32)
     {
-        double dx = tx[c] - t.x, dy = ty[c] - t.y, dz = tz[c] - t.z;
-        if constexpr (mode == Fold::shifted) { dx = dx - sx, dy = dy - sy, dz = dz - sz; }
-        if constexpr (mode == Fold::perPair)
+        const uint32_t cn = min(n - c0, 32u);
+        uint32_t       m  = 0;
+#pragma unroll 4
+        for (uint32_t c = 0; c < cn; ++c)
         {
-            dx = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
-            dy = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
-            dz = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
+            double dx = tx[c0 + c] - t.x, dy = ty[c0 + c] - t.y, dz = tz[c0 + c] - t.z;
+            if constexpr (mode == Fold::shifted) { dx = dx - sx, dy = dy - sy, dz = dz - sz; }
+            if constexpr (mode == Fold::perPair)
+            {
+                dx = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
+                dy = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
+                dz = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
+            }
+            double d2 = dx * dx + dy * dy + dz * dz;
+            m |= d2 < t.r2 ? (1u << c) : 0u;
         }
-        double   d2 = dx * dx + dy * dy + dz * dz;
-        uint32_t j  = lo + c;
-        if (t.active && d2 < t.r2 && j != t.i)
+        if (!t.active) { m = 0; }
+        while (m)
         {
-            if (count < ngmax) { out.put(count, j); }
-            count++;
+            const uint32_t j = tj[c0 + __builtin_ctz(m)];
+            m &= m - 1;
+            if (j != t.i)
+            {
+                if (count < ngmax) { out.put(count, j); }
+                count++;
+            }
         }
     }
 }
+
+//! periodic image of a candidate leaf relative to the group centre (see flushLeaves)
+struct LeafImage
+{
+    double sh[3];
+    bool   shifted, perPair;
+};
 
 //! Prune the buffered leaves against every lane's sphere, then stream the survivors through LDS.
 __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
@@ -196,42 +217,80 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
         return false;
     };
 
-    int32_t  node;
-    uint32_t tlo, tn;
-    bool     have = nextTile(node, tlo, tn);
-    double   px = 0, py = 0, pz = 0;
-    if (have && lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
-    int buf = 0;
-    while (have)
+    // image of a leaf: one shift by whole box lengths serves all its pairs unless the group, the
+    // leaf and the reach together exceed a quarter box (then the per-pair fold of the reference)
+    auto image = [&](int32_t nd)
     {
-        if (lane < tn) { lds.tx[buf][lane] = px, lds.ty[buf][lane] = py, lds.tz[buf][lane] = pz; }
-        __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
-
-        int32_t  curNode = node;
-        uint32_t curLo = tlo, curN = tn;
-        have = nextTile(node, tlo, tn);
-        if (have && lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
-
-        // periodic image of the current leaf relative to the group
-        double sh[3] = {0, 0, 0};
-        bool   shifted = false, perPair = false;
+        LeafImage im{{0, 0, 0}, false, false};
         for (int d = 0; d < 3; ++d)
         {
             if (!b.pbc[d]) { continue; }
-            double c = uniformD(a.centers[3 * size_t(curNode) + d]);
-            double s = uniformD(a.sizes[3 * size_t(curNode) + d]);
-            double k = rint((c - gc[d]) * b.ilen[d]);
-            sh[d]    = k * b.len[d];
-            shifted  = shifted || (k != 0.0);
-            perPair  = perPair || (ge[d] + s + reach > 0.25 * b.len[d]);
+            double c  = uniformD(a.centers[3 * size_t(nd) + d]);
+            double s  = uniformD(a.sizes[3 * size_t(nd) + d]);
+            double k  = rint((c - gc[d]) * b.ilen[d]);
+            im.sh[d]  = k * b.len[d];
+            im.shifted = im.shifted || (k != 0.0);
+            im.perPair = im.perPair || (ge[d] + s + reach > 0.25 * b.len[d]);
         }
-        if (perPair)
-            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, out);
-        else if (shifted)
-            sweepTile<Fold::shifted>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, sh[0], sh[1], sh[2],
-                                     count, out);
+        return im;
+    };
+    const double reach2 = reach * reach;
+
+    int32_t   node;
+    uint32_t  tlo, tn;
+    bool      have = nextTile(node, tlo, tn);
+    double    px = 0, py = 0, pz = 0;
+    LeafImage img{};
+    if (have)
+    {
+        img = image(node);
+        if (lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
+    }
+    int buf = 0;
+    while (have)
+    {
+        // stage the candidates that lie within reach of the group box (order kept); a candidate
+        // within 2h_i of a target i of the group is within reach of the box under the leaf's image
+        bool keep = lane < tn;
+        if (keep && !img.perPair)
+        {
+            const double q[3] = {px - img.sh[0], py - img.sh[1], pz - img.sh[2]};
+            double       d2   = 0.0;
+            for (int d = 0; d < 3; ++d)
+            {
+                double v = fabs(q[d] - gc[d]) - ge[d];
+                v        = v > 0.0 ? v : 0.0;
+                d2 += v * v;
+            }
+            keep = d2 < reach2;
+        }
+        const uint64_t kmask = ballot(keep);
+        if (keep)
+        {
+            const unsigned slot = laneRank(kmask);
+            lds.tx[buf][slot] = px, lds.ty[buf][slot] = py, lds.tz[buf][slot] = pz;
+            lds.tj[buf][slot] = tlo + lane;
+        }
+        __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
+
+        const LeafImage cur  = img;
+        const uint32_t  curN = uint32_t(__popcll(kmask));
+        have                 = nextTile(node, tlo, tn);
+        if (have)
+        {
+            img = image(node);
+            if (lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
+        }
+
+        if (cur.perPair)
+            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, 0, 0, 0, count,
+                                     out);
+        else if (cur.shifted)
+            sweepTile<Fold::shifted>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, cur.sh[0],
+                                     cur.sh[1], cur.sh[2], count, out);
         else
-            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], curLo, curN, 0, 0, 0, count, out);
+            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, 0, 0, 0, count,
+                                  out);
         buf ^= 1;
     }
     __builtin_amdgcn_wave_barrier();

@@ -25,6 +25,25 @@ constexpr int kPairWaves = 4;
 template<class T>
 constexpr bool isMixed = std::is_same_v<T, float>;
 
+// neighbours per batch (NeighborBatch) of the light loops (XMass, grad-h, IAD, AV switches) and of
+// the momentum-energy loops, per form; larger batches hide more latency and cost registers
+#ifndef SPHX_BATCH_LIGHT_F32
+#define SPHX_BATCH_LIGHT_F32 8
+#endif
+#ifndef SPHX_BATCH_LIGHT_F64
+#define SPHX_BATCH_LIGHT_F64 8
+#endif
+#ifndef SPHX_BATCH_MOM_F32
+#define SPHX_BATCH_MOM_F32 2
+#endif
+#ifndef SPHX_BATCH_MOM_F64
+#define SPHX_BATCH_MOM_F64 2
+#endif
+template<class T>
+constexpr int kBatchLight = isMixed<T> ? SPHX_BATCH_LIGHT_F32 : SPHX_BATCH_LIGHT_F64;
+template<class T>
+constexpr int kBatchMom = isMixed<T> ? SPHX_BATCH_MOM_F32 : SPHX_BATCH_MOM_F64;
+
 struct Lists
 {
     const uint32_t* nbrs;
@@ -116,9 +135,9 @@ __device__ __forceinline__ T fastSqrt(T a)
 
 /*! Batched walk over one lane's neighbour list.
  *
- * A pair loop visits its neighbours U at a time (U = 1, 2 or 4): load(k) makes the records of
- * slots k..k+U-1 current (their list entries were requested one batch earlier, U slots with one
- * aligned vector load from the lane's quad) and requests the list entries of the following batch.
+ * A pair loop visits its neighbours U at a time (U = 1, 2 or a multiple of 4): load(k) makes the records of
+ * slots k..k+U-1 current (their list entries were requested one batch earlier with aligned vector
+ * loads from the lane's quads) and requests the list entries of the following batch.
  * The loop then forms the U separations and issues the U table fetches before it consumes any of
  * them, so one batch costs two memory round trips (records, tables) instead of three per
  * neighbour. Slots beyond the lane's count take the batch's first valid entry and their pair
@@ -128,7 +147,7 @@ __device__ __forceinline__ T fastSqrt(T a)
 template<int U, class R>
 struct NeighborBatch
 {
-    static_assert(U == 1 || U == 2 || U == 4, "batch of 1, 2 or 4 slots");
+    static_assert(U == 1 || U == 2 || U % 4 == 0, "batch of 1, 2 or a multiple of 4 slots");
 
     const uint32_t* list;
     const R*        rec;
@@ -156,10 +175,14 @@ struct NeighborBatch
         const unsigned  kq = min(k, lastSlot) & ~unsigned(U - 1);
         const uint32_t* p  = list + laneSlot(kq);
         uint32_t        v[U];
-        if constexpr (U == 4)
+        if constexpr (U % 4 == 0)
         {
-            const uint4 q = *reinterpret_cast<const uint4*>(p);
-            v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+#pragma unroll
+            for (int c = 0; c < U; c += 4)
+            {
+                const uint4 q = *reinterpret_cast<const uint4*>(list + laneSlot(kq + c));
+                v[c] = q.x, v[c + 1] = q.y, v[c + 2] = q.z, v[c + 3] = q.w;
+            }
         }
         else if constexpr (U == 2)
         {

@@ -9,7 +9,7 @@ import os
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsphexa_amd.so")
+LIB_PATH = os.environ.get("SPHEXA_AMD_LIB") or os.path.join(HERE, "libsphexa_amd.so")
 
 _lib = None
 
