@@ -228,6 +228,18 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
 int sphx_find_neighbors(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                         const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH);
 
+/* sphx_find_neighbors fused with the first pair loop, as the reference GPU path runs them
+ * (the h/nc iteration inside computeXMass, sph/include/sph/hydro_ve/xmass_gpu.cu:57-129): the
+ * final search of each target also forms m_i + sum_j m_j W(r_ij/h_i) over its stored neighbours
+ * in list order and writes xm = m_i / (K h_i^-3 (...)) (out = SPHX_XMASS_XM, xmassJLoop,
+ * hydro_ve/xmass_kern.hpp:51-79) or rho = m / xm (out = SPHX_XMASS_RHO, computeDensity,
+ * hydro_std/density.hpp:42-60). Equivalent to sphx_find_neighbors followed by sphx_xmass or
+ * sphx_density_std, without their second pass over the lists. Needs the kernel tables. */
+#define SPHX_XMASS_XM 1
+#define SPHX_XMASS_RHO 2
+int sphx_find_neighbors_xmass(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                              const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH, int out);
+
 /* Copy the stored neighbour list of targets [first,last) into `out` (device or host memory,
  * row-major [last-first][ngmax], entries beyond min(nc-1, ngmax) unspecified). Test/inspection
  * helper matching the CPU Dataset's d.neighbors layout (particles_data.hpp:250-251). */

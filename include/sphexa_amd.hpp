@@ -163,7 +163,38 @@ void findNeighborsSfc(size_t startIndex, size_t endIndex, Dataset& d, const Box&
     detail::check(d.workspace, code);
 }
 
-#define SPHEXA_AMD_GROUP_PASS(name, cfn)                                                                               \
+namespace detail {
+template<class Dataset>
+void searchXMass(const GroupView& grp, Dataset& d, const Box& box, int out)
+{
+    if (d.ng0 > d.ngmax) { throw std::runtime_error("ng0 should be smaller than ngmax\n"); }
+    auto b    = box.c();
+    auto f    = detail::fields(d);
+    auto p    = detail::params(d);
+    int  code = sphx_find_neighbors_xmass(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody, 1, out);
+    if (code == SPHX_ERR_NOT_CONVERGED) { throw std::runtime_error("coupled nc/h-updated failed to converge"); }
+    detail::check(d.workspace, code);
+}
+} // namespace detail
+
+//! GPU sph::computeXMass (sph/include/sph/hydro_ve/xmass_gpu.cu:57-131): neighbour search with the
+//! h/nc iteration and the XMass sum in one pass; replaces findNeighborsSfc + computeXMass
+template<class Dataset>
+void findNeighborsXMass(const GroupView& grp, Dataset& d, const Box& box)
+{
+    detail::searchXMass(grp, d, box, SPHX_XMASS_XM);
+}
+
+//! GPU sph::computeDensity (sph/include/sph/hydro_ve/xmass_gpu.cu:150-165, rho = m / xm): neighbour
+//! search with the h/nc iteration and the std density in one pass; replaces findNeighborsSfc +
+//! computeDensity
+template<class Dataset>
+void findNeighborsDensity(const GroupView& grp, Dataset& d, const Box& box)
+{
+    detail::searchXMass(grp, d, box, SPHX_XMASS_RHO);
+}
+
+#define SPHEXA_AMD_GROUP_PASS(name, cfn)                                                                           \
     template<class Dataset>                                                                                            \
     void name(const GroupView& grp, Dataset& d, const Box& box)                                                        \
     {                                                                                                                  \

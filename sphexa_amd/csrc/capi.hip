@@ -329,6 +329,18 @@ int sphx_find_neighbors(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_
     });
 }
 
+int sphx_find_neighbors_xmass(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                              const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH, int out)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
+        SPHX_REQUIRE(out == SPHX_XMASS_XM || out == SPHX_XMASS_RHO, SPHX_ERR_ARG,
+                     "out must be SPHX_XMASS_XM or SPHX_XMASS_RHO");
+        return findNeighbors(ws, makeDevBox(box), f, prm, first, last, iterateH != 0,
+                             out == SPHX_XMASS_RHO ? kSearchRho : kSearchXm);
+    });
+}
+
 int sphx_copy_neighbors(sphx_workspace_t* ws, uint32_t* out, uint32_t first, uint32_t last)
 {
     return guarded(ws, [&] {

@@ -378,8 +378,11 @@ void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
 void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
 void buildTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);
 
+//! what findNeighbors computes besides the lists: nothing, or the fused XMass sum as xm or as rho
+constexpr int kSearchOnly = 0, kSearchXm = 1, kSearchRho = 2;
+
 int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
-                  uint32_t first, uint32_t last, bool iterateH);
+                  uint32_t first, uint32_t last, bool iterateH, int xmassOut = kSearchOnly);
 void copyNeighbors(sphx_workspace* ws, uint32_t* out, uint32_t first, uint32_t last);
 
 void densityStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,

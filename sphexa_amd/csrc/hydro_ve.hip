@@ -346,6 +346,21 @@ __device__ __forceinline__ T avRvCorrection(T rx, T ry, T rz, T eta_ab, T eta_cr
     return -phi_ab * (dmy1 + dmy2);
 }
 
+//! log2 / exp2 of the Atwood-ramp weights: the hardware transcendentals in the FP32 form
+template<class T>
+__device__ __forceinline__ T ramLog2(T v)
+{
+    if constexpr (isMixed<T>) { return __log2f(v); }
+    else { return log2(v); }
+}
+
+template<class T>
+__device__ __forceinline__ T ramExp2(T v)
+{
+    if constexpr (isMixed<T>) { return exp2f(v); }
+    else { return exp2(v); }
+}
+
 /* momentumAndEnergyJLoop<avClean> (hydro_ve/momentum_energy_kern.hpp:66-222) + tsKCourant */
 template<class T, bool avClean>
 __global__ void __launch_bounds__(kPairWaves * kWave)
@@ -434,9 +449,12 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
                 }
                 else
                 {
+                    // xm_i^(2-s) xm_j^s = xm_i^2 2^(s log2(xm_j/xm_i)), and symmetrically for b:
+                    // one logarithm and two exponentials instead of four powers
                     T sigma = ramp * (Atwood - Atmin);
-                    a_mom   = pow(ri.xm, T(2) - sigma) * pow(rj.xm, sigma);
-                    b_mom   = pow(rj.xm, T(2) - sigma) * pow(ri.xm, sigma);
+                    T lr    = ramLog2(fastDiv(rj.xm, ri.xm));
+                    a_mom   = ri.xm * ri.xm * ramExp2(sigma * lr);
+                    b_mom   = rj.xm * rj.xm * ramExp2(-sigma * lr);
                 }
 
                 T a_visc   = fastDiv(rj.m, ri.rho) * visc;

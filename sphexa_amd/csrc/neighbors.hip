@@ -15,7 +15,7 @@
 // search; the first ngmax hits are stored in the group layout of common.hpp (listOffset): a lane
 // collects its entries in a register quad and stores 16 bytes at a time, and the pair loops read
 // four slots of all 64 lanes with one coalesced load.
-#include "common.hpp"
+#include "pairs.hpp"
 
 namespace sphx
 {
@@ -44,6 +44,12 @@ struct SearchArgs
     uint32_t*            nbrs;
     uint32_t*            nc;
     unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles
+    // fused XMass: masses, kernel table (TabPair of h's type), normalisation, output (xm or rho)
+    const void*          m;
+    const void*          wh;
+    double               K;
+    void*                xmOut;
+    int                  toRho;
 };
 
 //! per-wave LDS: traversal stack, candidate leaves, double-buffered candidate tile
@@ -53,6 +59,7 @@ struct WaveLds
     int32_t leaves[kLeafCap];
     double   tx[2][kWave], ty[2][kWave], tz[2][kWave];
     uint32_t tj[2][kWave]; // particle index of each staged candidate
+    double   tm[2][kWave]; // its mass (fused XMass only)
 };
 
 //! per-target state of a lane
@@ -90,6 +97,50 @@ struct ListWriter
     }
 };
 
+//! what the search does with each stored hit besides listing it: nothing ...
+struct ListOnly
+{
+    static constexpr bool kStageMass = false;
+    static constexpr int  kBatch     = 1;
+    __device__ __forceinline__ void stage(int, bool, double, double, double, double) {}
+    __device__ __forceinline__ void flush() {}
+};
+
+//! ... or the XMass sum of xmassJLoop (hydro_ve/xmass_kern.hpp:51-79): m_i + sum_j m_j W(r_ij / h_i)
+//! over the stored neighbours in list order, r_ij from the same separation as the pair loop
+//! (minimum image, then rounded to h's type), compensated for the FP32 form. Hits are staged
+//! kBatch at a time: their table entries are fetched together, then accumulated in list order,
+//! slots without a stored hit leaving the sum untouched (bitwise the one-at-a-time sum).
+template<class Th>
+struct XMassSum
+{
+    static constexpr bool kStageMass = true;
+    static constexpr int  kBatch     = isMixed<Th> ? 4 : 2;
+    const TabPair<Th>*    wh;
+    Th                    hInv;
+    Acc<Th>               rho0;
+    TableFetch<Th, TabPair<Th>> w[kBatch];
+    Th                          mj[kBatch];
+    bool                        on[kBatch];
+
+    __device__ __forceinline__ void stage(int u, bool stored, double dx, double dy, double dz, double m)
+    {
+        Th rx = Th(dx), ry = Th(dy), rz = Th(dz);
+        Th dist = fastSqrt(rx * rx + ry * ry + rz * rz);
+        on[u]   = stored;
+        mj[u]   = Th(m);
+        w[u]    = fetch(wh, stored ? dist * hInv : Th(4)); // 4: past the support, a valid zero entry
+    }
+    __device__ __forceinline__ void flush()
+    {
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+        {
+            rho0.addIf(on[u], value(w[u]) * mj[u]);
+        }
+    }
+};
+
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ double uniformD(double v)
@@ -109,10 +160,11 @@ enum class Fold
 //! Test the `n` staged candidates against the lane's target. Hits are recorded as bits of a
 //! per-lane mask over chunks of 32 candidates and appended afterwards in candidate order, so the
 //! list keeps the traversal order while the test loop stays free of the divergent append.
-template<Fold mode>
+template<Fold mode, class Sink>
 __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, const double* tx, const double* ty,
-                                          const double* tz, const uint32_t* tj, uint32_t n, double sx, double sy,
-                                          double sz, unsigned& count, ListWriter& out)
+                                          const double* tz, const uint32_t* tj, const double* tm, uint32_t n,
+                                          double sx, double sy, double sz, unsigned& count, ListWriter& out,
+                                          Sink& sink)
 {
 #pragma clang fp contract(off)
     const DevBox&  b     = a.box;
@@ -138,13 +190,31 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
         if (!t.active) { m = 0; }
         while (m)
         {
-            const uint32_t j = tj[c0 + __builtin_ctz(m)];
-            m &= m - 1;
-            if (j != t.i)
+#pragma unroll
+            for (int u = 0; u < Sink::kBatch; ++u)
             {
-                if (count < ngmax) { out.put(count, j); }
-                count++;
+                const bool     have  = m != 0;
+                const uint32_t c     = c0 + (have ? uint32_t(__builtin_ctz(m)) : 0u);
+                const uint32_t j     = tj[c];
+                const bool     keep  = have && j != t.i;
+                const bool     store = keep && count < ngmax;
+                m &= m - 1;
+                if (store) { out.put(count, j); }
+                count += keep ? 1u : 0u;
+                if constexpr (Sink::kStageMass)
+                {
+                    double dx = tx[c] - t.x, dy = ty[c] - t.y, dz = tz[c] - t.z;
+                    if constexpr (mode == Fold::shifted) { dx = dx - sx, dy = dy - sy, dz = dz - sz; }
+                    if constexpr (mode == Fold::perPair)
+                    {
+                        dx = dx - double(b.pbc[0]) * b.len[0] * rint(dx * b.ilen[0]);
+                        dy = dy - double(b.pbc[1]) * b.len[1] * rint(dy * b.ilen[1]);
+                        dz = dz - double(b.pbc[2]) * b.len[2] * rint(dz * b.ilen[2]);
+                    }
+                    sink.stage(u, store, dx, dy, dz, tm[c]);
+                }
             }
+            sink.flush();
         }
     }
 }
@@ -157,8 +227,9 @@ struct LeafImage
 };
 
 //! Prune the buffered leaves against every lane's sphere, then stream the survivors through LDS.
+template<class Sink>
 __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
-                            const double ge[3], double reach, unsigned& count, ListWriter& out)
+                            const double ge[3], double reach, unsigned& count, ListWriter& out, Sink& sink)
 {
 #pragma clang fp contract(off)
     const DevBox&  b    = a.box;
@@ -239,13 +310,18 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
     int32_t   node;
     uint32_t  tlo, tn;
     bool      have = nextTile(node, tlo, tn);
-    double    px = 0, py = 0, pz = 0;
+    double    px = 0, py = 0, pz = 0, pm = 0;
     LeafImage img{};
-    if (have)
+    auto      fetchTile = [&]()
     {
         img = image(node);
-        if (lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
-    }
+        if (lane < tn)
+        {
+            px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane];
+            if constexpr (Sink::kStageMass) { pm = double(static_cast<const decltype(sink.hInv)*>(a.m)[tlo + lane]); }
+        }
+    };
+    if (have) { fetchTile(); }
     int buf = 0;
     while (have)
     {
@@ -270,27 +346,25 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
             const unsigned slot = laneRank(kmask);
             lds.tx[buf][slot] = px, lds.ty[buf][slot] = py, lds.tz[buf][slot] = pz;
             lds.tj[buf][slot] = tlo + lane;
+            if constexpr (Sink::kStageMass) { lds.tm[buf][slot] = pm; }
         }
         __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
 
         const LeafImage cur  = img;
         const uint32_t  curN = uint32_t(__popcll(kmask));
         have                 = nextTile(node, tlo, tn);
-        if (have)
-        {
-            img = image(node);
-            if (lane < tn) { px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane]; }
-        }
+        if (have) { fetchTile(); }
 
+        const double* tx = lds.tx[buf];
+        const double* ty = lds.ty[buf];
+        const double* tz = lds.tz[buf];
         if (cur.perPair)
-            sweepTile<Fold::perPair>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, 0, 0, 0, count,
-                                     out);
+            sweepTile<Fold::perPair>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, 0, 0, 0, count, out, sink);
         else if (cur.shifted)
-            sweepTile<Fold::shifted>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, cur.sh[0],
-                                     cur.sh[1], cur.sh[2], count, out);
+            sweepTile<Fold::shifted>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, cur.sh[0], cur.sh[1],
+                                     cur.sh[2], count, out, sink);
         else
-            sweepTile<Fold::none>(a, t, lds.tx[buf], lds.ty[buf], lds.tz[buf], lds.tj[buf], curN, 0, 0, 0, count,
-                                  out);
+            sweepTile<Fold::none>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, 0, 0, 0, count, out, sink);
         buf ^= 1;
     }
     __builtin_amdgcn_wave_barrier();
@@ -303,7 +377,9 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
  * at a time. A one-at-a-time pop replaces a node by at most 8 children and the next pop takes
  * the last child, i.e. it proceeds depth first, which adds at most 7 entries per tree level
  * below the popped node (<= 7*20 = 140 < 160) on top of the stack it started from. */
-__device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out)
+template<class Sink>
+__device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out,
+                                    Sink& sink)
 {
 #pragma clang fp contract(off)
     const unsigned lane = laneId();
@@ -357,7 +433,7 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
         int      nl    = __popcll(mLeaf);
         if (nLeaf + nl > kLeafCap)
         {
-            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out);
+            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out, sink);
             nLeaf = 0;
         }
         if (ov && isLeaf) { lds.leaves[nLeaf + laneRank(mLeaf)] = node; }
@@ -371,15 +447,17 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
         top += 8 * __popcll(mInt);
         __builtin_amdgcn_wave_barrier();
     }
-    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out);
+    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out, sink);
     return count;
 }
 
 //! search of the active lanes' targets into their lists (slot 0 at myList); returns the counts
-__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList)
+template<class Sink>
+__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList,
+                                Sink& sink)
 {
     ListWriter out{myList};
-    unsigned   count = searchGroupInto(a, t, radius, lds, out);
+    unsigned   count = searchGroupInto(a, t, radius, lds, out, sink);
     if (t.active) { out.finish(min(count, a.ngmax)); }
     return count;
 }
@@ -404,8 +482,12 @@ __device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
     t.reach2   = rr * rr;
 }
 
-template<class Th>
-__global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArgs a, Th* __restrict__ h)
+/* One wave per group: search, h/nc iteration, and with XMass the fused density sum of the final
+ * search, stored as xm = m / rho0 (xmassJLoop) or rho = m / xm (computeDensity,
+ * hydro_std/density.hpp:42-53), with the per-target epilogue of the XMass kernel. */
+template<class Th, bool XMass>
+__global__ void __launch_bounds__(kWavesPerBlock * kWave) __attribute__((amdgpu_waves_per_eu(isMixed<Th> ? 4 : 3)))
+searchKernel(SearchArgs a, Th* __restrict__ h)
 {
     __shared__ WaveLds ldsMem[kWavesPerBlock];
 
@@ -426,8 +508,22 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
     setRadius(t, hi, a.slack);
     uint32_t* myList = a.nbrs + listOffset(g, a.ngmax4, 0, lane);
 
+    using Sink = std::conditional_t<XMass, XMassSum<Th>, ListOnly>;
+    Sink      sink;
+    const Th  mi = XMass && t.active ? static_cast<const Th*>(a.m)[t.i] : Th(0);
+    auto      resetSink = [&]()
+    {
+        if constexpr (XMass)
+        {
+            sink.wh   = static_cast<const TabPair<Th>*>(a.wh);
+            sink.hInv = Th(1) / hi;
+            sink.rho0 = Acc<Th>(mi);
+        }
+    };
+    resetSink();
+
     bool     valid = t.active;
-    unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList);
+    unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList, sink);
     unsigned ncSph = 1 + count;
 
     int      updates = 0;
@@ -443,9 +539,10 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
                 updates++;
                 hi = updateH(a.ng0, ncSph, hi);
                 setRadius(t, hi, a.slack);
+                resetSink();
             }
             t.active       = repeat;
-            unsigned again = searchGroup(a, t, double(2) * double(hi), lds, myList);
+            unsigned again = searchGroup(a, t, double(2) * double(hi), lds, myList, sink);
             if (repeat) { ncSph = 1 + again; }
         }
     }
@@ -455,6 +552,15 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave) searchKernel(SearchArg
         a.nc[t.i] = ncSph;
         // slot 0 always names a particle: the pair loops read one slot ahead without a branch
         if (ncSph == 1) { myList[0] = t.i; }
+        if constexpr (XMass)
+        {
+            // epilogue of xmassKernel: FP64 normalisation with h^-3 rounded in h's type
+            const Th hInv    = Th(1) / hi;
+            const Th h3Inv   = hInv * hInv * hInv;
+            double   rhoZero = double(sink.rho0.value()) * a.K * double(h3Inv);
+            Th       xmi     = Th(double(mi) / rhoZero);
+            static_cast<Th*>(a.xmOut)[t.i] = a.toRho ? Th(mi / xmi) : xmi;
+        }
     }
 
     unsigned stored = valid ? min(ncSph - 1, a.ngmax) : 0;
@@ -483,7 +589,7 @@ __global__ void copyListsKernel(const uint32_t* __restrict__ nbrs, const uint32_
 } // namespace
 
 int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
-                  uint32_t first, uint32_t last, bool iterateH)
+                  uint32_t first, uint32_t last, bool iterateH, int xmassOut)
 {
     SPHX_REQUIRE(ws->tree.valid, SPHX_ERR_STATE, "find_neighbors called before the tree was built");
     SPHX_REQUIRE(f->x && f->y && f->z && f->h && f->nc, SPHX_ERR_ARG, "find_neighbors needs x, y, z, h and nc");
@@ -516,15 +622,35 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.nbrs     = ws->nbrs.get();
     a.nc       = f->nc;
     a.stats    = reinterpret_cast<unsigned long long*>(ws->counters.get());
+    a.m = nullptr, a.wh = nullptr, a.K = 0, a.xmOut = nullptr, a.toRho = 0;
+    const bool f32 = p->hydro_type == SPHX_F32;
+    if (xmassOut != kSearchOnly)
+    {
+        SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
+        void* outField = xmassOut == kSearchRho ? f->rho : f->xm;
+        SPHX_REQUIRE(f->m && outField, SPHX_ERR_ARG, "the fused XMass needs m and its output field (xm or rho)");
+        a.m     = f->m;
+        a.wh    = f32 ? static_cast<const void*>(ws->wh32.get()) : static_cast<const void*>(ws->wh64.get());
+        a.K     = p->K;
+        a.xmOut = outField;
+        a.toRho = xmassOut == kSearchRho;
+    }
 
     if (groups > 0)
     {
         unsigned    blocks = ceilDiv(groups, kWavesPerBlock);
         KernelScope timer(ws, "search");
-        if (p->hydro_type == SPHX_F32)
-            searchKernel<float><<<blocks, kWavesPerBlock * kWave, 0, ws->stream>>>(a, static_cast<float*>(f->h));
+        const dim3  grid(blocks), block(kWavesPerBlock * kWave);
+        if (xmassOut == kSearchOnly)
+        {
+            if (f32) searchKernel<float, false><<<grid, block, 0, ws->stream>>>(a, static_cast<float*>(f->h));
+            else searchKernel<double, false><<<grid, block, 0, ws->stream>>>(a, static_cast<double*>(f->h));
+        }
         else
-            searchKernel<double><<<blocks, kWavesPerBlock * kWave, 0, ws->stream>>>(a, static_cast<double*>(f->h));
+        {
+            if (f32) searchKernel<float, true><<<grid, block, 0, ws->stream>>>(a, static_cast<float*>(f->h));
+            else searchKernel<double, true><<<grid, block, 0, ws->stream>>>(a, static_cast<double*>(f->h));
+        }
         SPHX_CHECK_LAUNCH();
     }
     unsigned long long stats[3];

@@ -328,6 +328,19 @@ struct Acc
         }
         else { s += v; }
     }
+    //! add(v) where p holds, the state untouched elsewhere (branch-free)
+    __device__ __forceinline__ void addIf(bool p, T v)
+    {
+        if constexpr (isMixed<T>)
+        {
+            T y  = v - c;
+            T t2 = s + y;
+            T c2 = (t2 - s) - y;
+            s    = p ? t2 : s;
+            c    = p ? c2 : c;
+        }
+        else { s = p ? s + v : s; }
+    }
     __device__ __forceinline__ T value() const { return s; }
 };
 

@@ -32,6 +32,7 @@ _SIGS = {
     "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
     "sphx_domain_sync": [_P, _BOXP, _FP, _PP, _SZ, _U32],
     "sphx_find_neighbors": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int],
+    "sphx_find_neighbors_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.c_int],
     "sphx_copy_neighbors": [_P, _P, _U32, _U32],
     "sphx_neighbor_stats": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
     "sphx_density_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
@@ -141,6 +142,12 @@ class Workspace:
         """Returns True when every particle's h/nc iteration converged."""
         code = self.call("sphx_find_neighbors", C.byref(box), C.byref(fields), C.byref(params), first, last,
                          int(iterate_h), allow=(abi.SPHX_ERR_NOT_CONVERGED,))
+        return code == abi.SPHX_OK
+
+    def find_neighbors_xmass(self, box, fields, params, first, last, iterate_h=True, rho=False):
+        """Search fused with XMass (xm) or, with rho=True, the std density; True when converged."""
+        code = self.call("sphx_find_neighbors_xmass", C.byref(box), C.byref(fields), C.byref(params), first, last,
+                         int(iterate_h), 2 if rho else 1, allow=(abi.SPHX_ERR_NOT_CONVERGED,))
         return code == abi.SPHX_OK
 
     def copy_neighbors(self, out, first, last):

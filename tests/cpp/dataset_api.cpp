@@ -3,7 +3,11 @@
 // main/src/propagator/ve_hydro.hpp:130-204), reading the initial conditions from a binary file
 // and writing the SFC-ordered results for comparison with the CPU oracle.
 //
-//   dataset_api <in.bin> <out.bin> std|ve
+//   dataset_api <in.bin> <out.bin> std|ve [cpu|gpu]
+//
+// "cpu" (the default) calls findNeighborsSfc and then computeDensity / computeXMass, as the
+// reference's CPU build does; "gpu" calls the fused findNeighborsDensity / findNeighborsXMass,
+// which is how the reference's GPU build runs the search (inside computeDensity / computeXMass).
 //
 // in.bin : uint64 n, uint32 periodic, double lo, double hi, then n doubles each of
 //          x y z h m u vx vy vz alpha (u is the temperature for the VE pass)
@@ -32,8 +36,9 @@ using namespace sphexa_amd;
 
 int main(int argc, char** argv)
 {
-    if (argc != 4) { return 1; }
-    std::string form = argv[3];
+    if (argc != 4 && argc != 5) { return 1; }
+    std::string form  = argv[3];
+    bool        fused = argc == 5 && std::string(argv[4]) == "gpu";
     FILE*       in   = std::fopen(argv[1], "rb");
     if (!in) { return 1; }
     uint64_t n;
@@ -78,10 +83,11 @@ int main(int argc, char** argv)
     try
     {
         syncDomain(n, d, box);
-        findNeighborsSfc(0, n, d, box);
+        if (!fused) { findNeighborsSfc(0, n, d, box); }
         if (ve)
         {
-            computeXMass(grp, d, box);
+            if (fused) { findNeighborsXMass(grp, d, box); }
+            else { computeXMass(grp, d, box); }
             computeVeDefGradh(grp, d, box);
             computeEOS(0, n, d);
             computeIadDivvCurlv(grp, d, box);
@@ -90,7 +96,8 @@ int main(int argc, char** argv)
         }
         else
         {
-            computeDensity(grp, d, box);
+            if (fused) { findNeighborsDensity(grp, d, box); }
+            else { computeDensity(grp, d, box); }
             computeEOS_HydroStd(0, n, d);
             computeIAD(grp, d, box);
             computeMomentumEnergySTD(grp, d, box);

@@ -19,8 +19,11 @@ VE = ("xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "c33"
       "az", "du")
 
 
+@pytest.mark.parametrize("calls", ["cpu", "gpu"])
 @pytest.mark.parametrize("form", ["std", "ve"])
-def test_cpp_interface_pass(tmp_path, orc, form):
+def test_cpp_interface_pass(tmp_path, orc, form, calls):
+    """calls="cpu": findNeighborsSfc then computeDensity / computeXMass; calls="gpu": the fused
+    search + density / XMass, as the reference GPU build runs them"""
     from oracle import pipeline
 
     assert os.path.exists(DRIVER), "tests/cpp/dataset_api not built (python -m sphexa_amd.build)"
@@ -43,7 +46,7 @@ def test_cpp_interface_pass(tmp_path, orc, form):
         for col in (arr["x"], arr["y"], arr["z"], arr["h"], arr["m"], third, arr["vx"], arr["vy"], arr["vz"],
                     arr["alpha"]):
             fh.write(np.ascontiguousarray(col, np.float64).tobytes())
-    r = subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), form], capture_output=True,
+    r = subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), form, calls], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = open(tmp_path / "out.bin", "rb").read()

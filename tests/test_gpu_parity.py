@@ -226,3 +226,33 @@ def test_ve_mixed_accuracy(ws, orc):
     assert max_rel(rho_g, rho_o) <= 1e-5
     for k in ("ax", "ay", "az"):
         assert rel_to_max(ps.host(k), o["a"][k]) <= 1e-4, k
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("form", ["std", "ve"])
+def test_fused_search_xmass_matches_separate_passes(ws, mixed, form):
+    """sphx_find_neighbors_xmass == sphx_find_neighbors + sphx_density_std / sphx_xmass: same lists,
+    counts and h, and the density (xm) within rounding (the fused sum forms r_ij without FMA)."""
+    if form == "std":
+        arr, spec = ic.sedov_lattice(16, seed=3)
+        names, out = STD_FIELDS, "rho"
+    else:
+        arr, spec = _ve_ic(16, 5)
+        names, out = VE_FIELDS, "xm"
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+    res = []
+    for fused in (False, True):
+        ps = ParticleSet.from_host(arr, names, mixed=mixed)
+        f = ps.fields()
+        ws.domain_sync(box, f, params, ps.n, 64)
+        if fused:
+            assert ws.find_neighbors_xmass(box, f, params, 0, ps.n, True, rho=(form == "std"))
+        else:
+            assert ws.find_neighbors(box, f, params, 0, ps.n, True)
+            ws.pair("sphx_density_std" if form == "std" else "sphx_xmass", box, f, params, 0, ps.n)
+        res.append((ps.host("nc"), ps.host("h"), ps.host(out), gpu_lists(ws, ps, params.ngmax)))
+    (nc0, h0, v0, l0), (nc1, h1, v1, l1) = res
+    assert np.array_equal(nc0, nc1) and np.array_equal(h0, h1) and np.array_equal(l0, l1)
+    tol = 1e-6 if mixed else 1e-13
+    assert max_rel(v1, v0) <= tol
