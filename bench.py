@@ -7,8 +7,7 @@ seeded solenoidal velocity field P(k) ~ k^-4 on 1 <= |k| <= 2 at rms Mach 0.3, n
 
 One step = one full pass with all fields resident in HBM:
   keys -> stable radix sort + permutation of every field -> cornerstone tree (bucket 64) ->
-  neighbour search with the h/nc iteration, and XMass (fused with the search in the mixed form,
-  a kernel of its own in FP64) -> VeDefGradh -> EOS ->
+  neighbour search with the h/nc iteration -> XMass -> VeDefGradh -> EOS ->
   IAD + div/curl v -> AV switches -> momentum-energy + Courant time step.
 It runs in the mixed form (the headline value) and in the plain FP64 form; the JSON line reports
 particle-updates/s = N * steps / T for the mixed form, the FP64 value beside it, the roofline of
@@ -91,9 +90,8 @@ def make_ic(side, seed=42):
     return arr, spec
 
 
-# main kernels timed with HIP events on the workspace stream (sphx_kernel_ms); "search" carries
-# the fused XMass sum in the mixed form, the FP64 form runs "xmass" as its own kernel
-KERNELS = ("search", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
+# main kernels timed with HIP events on the workspace stream (sphx_kernel_ms)
+KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
 
 # outputs of the VE pass compared between the mixed and FP64 forms (rho = kx m / xm added);
 # positive quantities pointwise, signed ones relative to the field's largest magnitude
@@ -137,7 +135,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     stream = torch.cuda.current_stream()
 
     ws.set_profiling(True)
-    kernels = KERNELS if mixed else KERNELS[:1] + ("xmass",) + KERNELS[1:]
+    kernels = KERNELS
     kernel_ms = {k: [] for k in kernels}
 
     def one_step(ev):

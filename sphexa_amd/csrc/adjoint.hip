@@ -47,8 +47,7 @@ struct DensityVjpArgs
 
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    densityStdVjpKernel(PairArgs a, DensityVjpArgs f, const TabPair<T>* __restrict__ wh,
-                        const TabPair<T>* __restrict__ whd)
+    densityStdVjpKernel(PairArgs a, DensityVjpArgs f, SincKernel<T> kern)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -67,8 +66,8 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             separation<decltype(fold)::value>(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], rx, ry, rz);
             T d  = sqrt(rx * rx + ry * ry + rz * rz);
             T v  = d * hInv;
-            T w  = lookup(wh, v);
-            T dw = lookup(whd, v);
+            T w  = kern.W(v);
+            T dw = kern.dW(v);
             T mj = ld<T>(f.m, j);
             rho0 += mj * w;
             sumVdW += mj * v * dw;
@@ -92,7 +91,7 @@ struct IadVjpArgs
 
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadStdVjpKernel(PairArgs a, IadVjpArgs f, const TabPair<T>* __restrict__ wh, const TabPair<T>* __restrict__ whd)
+    iadStdVjpKernel(PairArgs a, IadVjpArgs f, SincKernel<T> kern)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -127,8 +126,8 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             separation<decltype(fold)::value>(a.box, r2h, xi, yi, zi, a.x[j], a.y[j], a.z[j], r[0], r[1], r[2]);
             T d    = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
             T v    = d * hInv;
-            T w    = lookup(wh, v);
-            T dw   = lookup(whd, v);
+            T w    = kern.W(v);
+            T dw   = kern.dW(v);
             T mj   = ld<T>(f.m, j);
             T rhoj = ld<T>(f.rho, j);
             T Vj   = mj / rhoj;
@@ -181,8 +180,7 @@ __device__ __forceinline__ void symOuter(const T u[3], const T r[3], T out[6])
 
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumStdVjpKernel(PairArgs a, MomVjpArgs f, const TabPair<T>* __restrict__ wh,
-                         const TabPair<T>* __restrict__ whd)
+    momentumStdVjpKernel(PairArgs a, MomVjpArgs f, SincKernel<T> kern)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -215,7 +213,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             T hjInv  = T(1) / ld<T>(f.h, j);
             T hjInv3 = hjInv * hjInv * hjInv;
             T v1 = d * hiInv, v2 = d * hjInv;
-            T W1 = lookup(wh, v1), dW1 = lookup(whd, v1), W2 = lookup(wh, v2), dW2 = lookup(whd, v2);
+            T W1 = kern.W(v1), dW1 = kern.dW(v1), W2 = kern.W(v2), dW2 = kern.dW(v2);
             T Wi = hiInv3 * W1, Wj = hjInv3 * W2;
             T Cj[3][3];
             loadC<T>(f, j, Cj);
@@ -321,9 +319,9 @@ void densityStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f
     unsigned       blocks = pairBlocks(first, last);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
-        densityStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh32.get(), ws->whd32.get());
+        densityStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
     else
-        densityStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh64.get(), ws->whd64.get());
+        densityStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
     SPHX_CHECK_LAUNCH();
 }
 
@@ -348,9 +346,9 @@ void iadStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, co
     unsigned blocks = pairBlocks(first, last);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
-        iadStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh32.get(), ws->whd32.get());
+        iadStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
     else
-        iadStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh64.get(), ws->whd64.get());
+        iadStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
     SPHX_CHECK_LAUNCH();
 }
 
@@ -391,11 +389,9 @@ void momentumEnergyStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fiel
     unsigned blocks = pairBlocks(first, last);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
-        momentumStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh32.get(),
-                                                                                     ws->whd32.get());
+        momentumStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
     else
-        momentumStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, ws->wh64.get(),
-                                                                                      ws->whd64.get());
+        momentumStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
     SPHX_CHECK_LAUNCH();
 }
 

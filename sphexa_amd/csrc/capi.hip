@@ -145,8 +145,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     if (ws == nullptr) { return SPHX_OK; }
     (void)hipSetDevice(ws->device);
     (void)hipStreamSynchronize(ws->stream);
-    ws->wh64.release(), ws->whd64.release(), ws->wh32.release(), ws->whd32.release();
-    ws->whq64.release(), ws->whq32.release();
+    ws->wh64.release(), ws->whd64.release(), ws->whq64.release();
     for (auto& kv : ws->kernelEvents)
     {
         if (kv.second.start) { (void)hipEventDestroy(kv.second.start); }
@@ -192,14 +191,9 @@ int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K)
         SPHX_REQUIRE(sincIndex > 0, SPHX_ERR_ARG, "sinc index must be positive");
         auto w64  = tabulate<double>([sincIndex](double v) { return sincPow(v, sincIndex); });
         auto wd64 = tabulate<double>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
-        auto w32  = tabulate<float>([sincIndex](double v) { return sincPow(v, sincIndex); });
-        auto wd32 = tabulate<float>([sincIndex](double v) { return sincPowDerivative(v, sincIndex); });
         upload(ws, ws->wh64, intervals(w64));
         upload(ws, ws->whd64, intervals(wd64));
-        upload(ws, ws->wh32, intervals(w32));
-        upload(ws, ws->whd32, intervals(wd32));
         upload(ws, ws->whq64, intervals(w64, wd64));
-        upload(ws, ws->whq32, intervals(w32, wd32));
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         ws->K           = normalization(sincIndex);
         ws->sincIndex   = sincIndex;

@@ -160,10 +160,9 @@ struct sphx_workspace
 
     // kernel tables: W and dW/dv on [0,2], 20000 points, in both precisions, stored per interval
     double                              K = 0, sincIndex = 0;
+    // the reference's W and dW/dv tables for the FP64 form (the FP32 form evaluates W directly)
     sphx::DevBuf<sphx::TabPair<double>> wh64, whd64;
-    sphx::DevBuf<sphx::TabPair<float>>  wh32, whd32;
     sphx::DevBuf<sphx::TabQuad<double>> whq64;
-    sphx::DevBuf<sphx::TabQuad<float>>  whq32;
     bool                   tablesValid = false;
 
     sphx::Tree tree;

@@ -70,8 +70,8 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
  *   xm_i = m_i / (K / h_i^3 * (m_i + sum_j m_j W(r_ij / h_i)))
  * toRho: computeDensity (hydro_std/density.hpp:42-53) stores rho = m / xm instead. */
 template<class T, bool toRho>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, const TabPair<T>* __restrict__ wh,
+__global__ void SPHX_LIGHT_LOOP(T)
+    xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
                 T* __restrict__ out)
 {
     Lane t = laneTarget(a.L);
@@ -89,10 +89,10 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         for (unsigned k = 0; k < t.n; k += U)
         {
             nb.load(k);
-            TableFetch<T, TabPair<T>> w[U];
+            typename SincKernel<T>::FetchW w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                w[u] = fetch(wh, pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
+                w[u] = kern.fetchW(pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 rho0.add(nb.live(k, u, value(w[u])) * nb.r[u].m);
@@ -119,8 +119,8 @@ __global__ void __launch_bounds__(256) eosStdKernel(const double* __restrict__ u
 
 /* IADJLoopSTD (hydro_std/iad_kern.hpp:11-76): tau_ab = sum_j m_j/rho_j W_ij r_a r_b, inverted */
 template<class T>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, const TabPair<T>* __restrict__ wh,
+__global__ void SPHX_LIGHT_LOOP(T)
+    iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
                  T* c11, T* c12, T* c13, T* c22, T* c23, T* c33)
 {
     Lane t = laneTarget(a.L);
@@ -138,12 +138,12 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         {
             nb.load(k);
             Sep<T>                    r[U];
-            TableFetch<T, TabPair<T>> w[U];
+            typename SincKernel<T>::FetchW w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
             {
                 r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
-                w[u] = fetch(wh, r[u].d * hiInv);
+                w[u] = kern.fetchW(r[u].d * hiInv);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -168,8 +168,8 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 /* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant.
  * PerAlpha: AV coefficients alpha_i, alpha_j from f.alpha instead of the constant 1. */
 template<class T, bool PerAlpha>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh,
+__global__ void SPHX_MOM_LOOP(T)
+    momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, SincKernel<T> kern,
                       float Kcour, unsigned long long* minDt)
 {
     Lane   t      = laneTarget(a.L);
@@ -196,13 +196,13 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             {
                 nb.load(k);
                 Sep<T>                    r[U];
-                TableFetch<T, TabPair<T>> wi[U], wj[U];
+                typename SincKernel<T>::FetchW wi[U], wj[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                 {
                     r[u]  = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
-                    wi[u] = fetch(wh, r[u].d * hiInv);
-                    wj[u] = fetch(wh, r[u].d * nb.r[u].hInv);
+                    wi[u] = kern.fetchW(r[u].d * hiInv);
+                    wj[u] = kern.fetchW(r[u].d * nb.r[u].hInv);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
@@ -265,10 +265,10 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, toRho ? "density_std" : "xmass");
     if (toRho)
-        xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+        xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
                                                                             (T*)out);
     else
-        xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+        xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
                                                                              (T*)out);
     SPHX_CHECK_LAUNCH();
 }
@@ -283,7 +283,7 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
         a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "iad_std");
-    iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whTable<T>(ws),
+    iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
                                                                     (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
                                                                     (T*)f->c23, (T*)f->c33);
     SPHX_CHECK_LAUNCH();
@@ -300,9 +300,9 @@ void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "momentum_std");
     if (perAlpha)
-        momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour, dt);
+        momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws), Kcour, dt);
     else
-        momentumStdKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, whTable<T>(ws), Kcour,
+        momentumStdKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws), Kcour,
                                                                                     dt);
     SPHX_CHECK_LAUNCH();
 }

@@ -27,9 +27,9 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
 
 /* veDefGradhJLoop (hydro_ve/ve_def_gradh_kern.hpp:44-90): kx and the grad-h term */
 template<class T>
-__global__ void __launch_bounds__(kPairWaves * kWave)
+__global__ void SPHX_LIGHT_LOOP(T)
     veDefGradhKernel(PairArgs a, const RecGradh<T>* __restrict__ rec, const T* __restrict__ h,
-                     const TabQuad<T>* __restrict__ whq, T* __restrict__ kx, T* __restrict__ gradh)
+                     SincKernel<T> kern, T* __restrict__ kx, T* __restrict__ gradh)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -45,10 +45,10 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         for (unsigned k = 0; k < t.n; k += U)
         {
             nb.load(k);
-            TableFetch<T, TabQuad<T>> w[U];
+            typename SincKernel<T>::FetchWD w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                w[u] = fetch(whq, pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
+                w[u] = kern.fetchWD(pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]).d * hInv);
 #pragma unroll
             for (int u = 0; u < U; ++u)
             {
@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
  *   M[a][c] = sum_j (v_ja - v_ia) xm_j W_ij r_ij,c,
  * so the 9 moments M are accumulated alongside tau and combined with C_i after the inversion. */
 template<class T>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh)
+__global__ void SPHX_LIGHT_LOOP(T)
+    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, SincKernel<T> kern)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -140,12 +140,12 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         {
             nb.load(k);
             Sep<T>                    r[U];
-            TableFetch<T, TabPair<T>> w[U];
+            typename SincKernel<T>::FetchW w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
             {
                 r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
-                w[u] = fetch(wh, r[u].d * hiInv);
+                w[u] = kern.fetchW(r[u].d * hiInv);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -225,8 +225,8 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
 /* AVswitchesJLoop (hydro_ve/av_switches_kern.hpp:44-137). grad(div v) is linear in C_i:
  * gd_b = -K/h_i^3 sum_c C_i[b][c] sum_j f_j W_ij r_ij,c with f_j = xm_j/kx_j (divv_i - divv_j). */
 template<class T>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh, double dt,
+__global__ void SPHX_LIGHT_LOOP(T)
+    avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, SincKernel<T> kern, double dt,
                      T alphamin, T alphamax, T decay)
 {
     Lane t = laneTarget(a.L);
@@ -245,12 +245,12 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
         {
             nb.load(k);
             Sep<T>                    r[U];
-            TableFetch<T, TabPair<T>> w[U];
+            typename SincKernel<T>::FetchW w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
             {
                 r[u] = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
-                w[u] = fetch(wh, r[u].d * hiInv);
+                w[u] = kern.fetchW(r[u].d * hiInv);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u)
@@ -363,8 +363,8 @@ __device__ __forceinline__ T ramExp2(T v)
 
 /* momentumAndEnergyJLoop<avClean> (hydro_ve/momentum_energy_kern.hpp:66-222) + tsKCourant */
 template<class T, bool avClean>
-__global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, const TabPair<T>* __restrict__ wh, T Atmin,
+__global__ void SPHX_MOM_LOOP(T)
+    momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, SincKernel<T> kern, T Atmin,
                      T Atmax, T ramp, float Kcour, unsigned long long* minDt)
 {
     Lane   t      = laneTarget(a.L);
@@ -395,13 +395,13 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             {
               nb.load(k);
               Sep<T>                    r[U];
-              TableFetch<T, TabPair<T>> wi[U], wj[U];
+              typename SincKernel<T>::FetchW wi[U], wj[U];
 #pragma unroll
               for (int u = 0; u < U; ++u)
               {
                   r[u]  = pairSeparation<decltype(fold)::value, T>(a.box, r2h, ri, nb.r[u]);
-                  wi[u] = fetch(wh, r[u].d * hiInv);
-                  wj[u] = fetch(wh, r[u].d * nb.r[u].hInv);
+                  wi[u] = kern.fetchW(r[u].d * hiInv);
+                  wj[u] = kern.fetchW(r[u].d * nb.r[u].hInv);
               }
 #pragma unroll
               for (int u = 0; u < U; ++u)
@@ -512,7 +512,7 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
                                                                                           (const T*)f->xm, rec);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "ve_def_gradh");
-    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, whqTable<T>(ws),
+    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
                                                                         (T*)f->kx, (T*)f->gradh);
     SPHX_CHECK_LAUNCH();
 }
@@ -526,7 +526,7 @@ void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf)
     packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "iad_divv_curlv");
-    iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, whTable<T>(ws));
+    iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, sincKernel<T>(ws));
     SPHX_CHECK_LAUNCH();
 }
 
@@ -539,7 +539,7 @@ void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const s
     packAv<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, af, rec);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "av_switches");
-    avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, af, rec, whTable<T>(ws), p->minDt,
+    avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, af, rec, sincKernel<T>(ws), p->minDt,
                                                                         T(p->alphamin), T(p->alphamax),
                                                                         T(p->decay_constant));
     SPHX_CHECK_LAUNCH();
@@ -557,10 +557,10 @@ void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, c
     KernelScope timer(ws, "momentum_ve");
     if (avClean)
         momentumVeKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, mf, rec, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+            a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
     else
         momentumVeKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, mf, rec, whTable<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+            a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
     SPHX_CHECK_LAUNCH();
 }
 

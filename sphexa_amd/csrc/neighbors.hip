@@ -46,7 +46,9 @@ struct SearchArgs
     unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles
     // fused XMass: masses, kernel table (TabPair of h's type), normalisation, output (xm or rho)
     const void*          m;
-    const void*          wh;
+    const void*          wh;    // FP64 form: the W table
+    float                sincN; // FP32 form: the sinc exponent (W evaluated directly) ...
+    int                  sincInt; // ... and that exponent when it is an integer, else -1
     double               K;
     void*                xmOut;
     int                  toRho;
@@ -116,10 +118,10 @@ struct XMassSum
 {
     static constexpr bool kStageMass = true;
     static constexpr int  kBatch     = isMixed<Th> ? 4 : 2;
-    const TabPair<Th>*    wh;
+    SincKernel<Th>        kern;
     Th                    hInv;
     Acc<Th>               rho0;
-    TableFetch<Th, TabPair<Th>> w[kBatch];
+    typename SincKernel<Th>::FetchW w[kBatch];
     Th                          mj[kBatch];
     bool                        on[kBatch];
 
@@ -129,7 +131,7 @@ struct XMassSum
         Th dist = fastSqrt(rx * rx + ry * ry + rz * rz);
         on[u]   = stored;
         mj[u]   = Th(m);
-        w[u]    = fetch(wh, stored ? dist * hInv : Th(4)); // 4: past the support, a valid zero entry
+        w[u]    = kern.fetchW(stored ? dist * hInv : Th(4)); // 4: past the support, W = 0
     }
     __device__ __forceinline__ void flush()
     {
@@ -485,8 +487,17 @@ __device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
 /* One wave per group: search, h/nc iteration, and with XMass the fused density sum of the final
  * search, stored as xm = m / rho0 (xmassJLoop) or rho = m / xm (computeDensity,
  * hydro_std/density.hpp:42-53), with the per-target epilogue of the XMass kernel. */
+// waves per SIMD the search kernel's register allocation must leave room for, per form
+#ifndef SPHX_WAVES_SEARCH_F32
+#define SPHX_WAVES_SEARCH_F32 4
+#endif
+#ifndef SPHX_WAVES_SEARCH_F64
+#define SPHX_WAVES_SEARCH_F64 3
+#endif
+
 template<class Th, bool XMass>
-__global__ void __launch_bounds__(kWavesPerBlock * kWave) __attribute__((amdgpu_waves_per_eu(isMixed<Th> ? 4 : 3)))
+__global__ void __launch_bounds__(kWavesPerBlock * kWave)
+__attribute__((amdgpu_waves_per_eu(isMixed<Th> ? SPHX_WAVES_SEARCH_F32 : SPHX_WAVES_SEARCH_F64)))
 searchKernel(SearchArgs a, Th* __restrict__ h)
 {
     __shared__ WaveLds ldsMem[kWavesPerBlock];
@@ -515,7 +526,8 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
     {
         if constexpr (XMass)
         {
-            sink.wh   = static_cast<const TabPair<Th>*>(a.wh);
+            if constexpr (isMixed<Th>) { sink.kern.n = a.sincN, sink.kern.ni = a.sincInt; }
+            else { sink.kern.w = static_cast<const TabPair<double>*>(a.wh); }
             sink.hInv = Th(1) / hi;
             sink.rho0 = Acc<Th>(mi);
         }
@@ -622,7 +634,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.nbrs     = ws->nbrs.get();
     a.nc       = f->nc;
     a.stats    = reinterpret_cast<unsigned long long*>(ws->counters.get());
-    a.m = nullptr, a.wh = nullptr, a.K = 0, a.xmOut = nullptr, a.toRho = 0;
+    a.m = nullptr, a.wh = nullptr, a.sincN = 0, a.sincInt = -1, a.K = 0, a.xmOut = nullptr, a.toRho = 0;
     const bool f32 = p->hydro_type == SPHX_F32;
     if (xmassOut != kSearchOnly)
     {
@@ -630,7 +642,9 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
         void* outField = xmassOut == kSearchRho ? f->rho : f->xm;
         SPHX_REQUIRE(f->m && outField, SPHX_ERR_ARG, "the fused XMass needs m and its output field (xm or rho)");
         a.m     = f->m;
-        a.wh    = f32 ? static_cast<const void*>(ws->wh32.get()) : static_cast<const void*>(ws->wh64.get());
+        a.wh    = ws->wh64.get();
+        a.sincN   = sincKernel<float>(ws).n;
+        a.sincInt = sincKernel<float>(ws).ni;
         a.K     = p->K;
         a.xmOut = outField;
         a.toRho = xmassOut == kSearchRho;

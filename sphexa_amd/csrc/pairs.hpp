@@ -28,7 +28,7 @@ constexpr bool isMixed = std::is_same_v<T, float>;
 // neighbours per batch (NeighborBatch) of the light loops (XMass, grad-h, IAD, AV switches) and of
 // the momentum-energy loops, per form; larger batches hide more latency and cost registers
 #ifndef SPHX_BATCH_LIGHT_F32
-#define SPHX_BATCH_LIGHT_F32 8
+#define SPHX_BATCH_LIGHT_F32 4
 #endif
 #ifndef SPHX_BATCH_LIGHT_F64
 #define SPHX_BATCH_LIGHT_F64 8
@@ -39,6 +39,29 @@ constexpr bool isMixed = std::is_same_v<T, float>;
 #ifndef SPHX_BATCH_MOM_F64
 #define SPHX_BATCH_MOM_F64 2
 #endif
+
+// waves per SIMD the register allocator must leave room for (amdgpu_waves_per_eu) in the light
+// loops and in momentum-energy, per form; 1 leaves the choice to the compiler
+#ifndef SPHX_WAVES_LIGHT_F32
+#define SPHX_WAVES_LIGHT_F32 1
+#endif
+#ifndef SPHX_WAVES_LIGHT_F64
+#define SPHX_WAVES_LIGHT_F64 1
+#endif
+#ifndef SPHX_WAVES_MOM_F32
+#define SPHX_WAVES_MOM_F32 1
+#endif
+#ifndef SPHX_WAVES_MOM_F64
+#define SPHX_WAVES_MOM_F64 1
+#endif
+template<class T>
+constexpr int kWavesLight = isMixed<T> ? SPHX_WAVES_LIGHT_F32 : SPHX_WAVES_LIGHT_F64;
+template<class T>
+constexpr int kWavesMom = isMixed<T> ? SPHX_WAVES_MOM_F32 : SPHX_WAVES_MOM_F64;
+
+//! launch attributes of the pair-loop kernels (T: the HydroType template parameter)
+#define SPHX_LIGHT_LOOP(T) __launch_bounds__(kPairWaves * kWave) __attribute__((amdgpu_waves_per_eu(kWavesLight<T>)))
+#define SPHX_MOM_LOOP(T) __launch_bounds__(kPairWaves * kWave) __attribute__((amdgpu_waves_per_eu(kWavesMom<T>)))
 template<class T>
 constexpr int kBatchLight = isMixed<T> ? SPHX_BATCH_LIGHT_F32 : SPHX_BATCH_LIGHT_F64;
 template<class T>
@@ -251,6 +274,142 @@ __device__ __forceinline__ void values(const TableFetch<T, TabQuad<T>>& f, T& w,
     dw = interpolate(f.e.d0, f.e.d1, f.v, f.idx);
 }
 
+/*! The SPH kernel W(v) = sinc(pi v / 2)^n and dW/dv on [0, 2), zero from v = 2 on.
+ *
+ * The FP64 form interpolates the reference's 20000-point tables (lookup), so its values are the
+ * reference's. The FP32 form evaluates the function itself. A table lookup is a 64-lane gather
+ * that touches about one cache line per lane, and the pair loops are bound by that gather
+ * traffic, while the ALUs are mostly idle. The evaluation runs in FP64 arithmetic, which on this
+ * GPU issues at the FP32 rate (FP32 ops are not packed here), and is rounded once to FP32: its
+ * error is the final rounding, below that of the FP32 table (values rounded to FP32, then
+ * interpolated in FP32), and its difference from the FP64 table's interpolant is the
+ * interpolation error, about 6e-9 of W(0) for n = 6.
+ *
+ * With x = pi v / 2: sin x from its Taylor polynomial on [0, pi/2] (to t^13; the remainder is
+ * below 7e-10), reached for v > 1 through sin x = sin y, cos x = -cos y, y = (2 - v) pi/2 formed
+ * from the exact 2 - v, so sinc keeps its relative accuracy up to the edge of the support;
+ * sinc = sin x / x and d sinc/dx = (cos x - sinc) / x (1 and 0 at x = 0), with 1/x from the
+ * hardware reciprocal and one Newton step; W = sinc^n by repeated multiplication for an integer
+ * n (the reference's sinc indices), else through FP32 log2 / exp2; dW/dv = n sinc^(n-1)
+ * d sinc/dx pi/2. */
+struct DirectW
+{
+    float w, v;
+};
+struct DirectWD
+{
+    float w, dw, v;
+};
+
+__device__ __forceinline__ float value(const DirectW& f) { return f.w; }
+
+__device__ __forceinline__ void values(const DirectWD& f, float& w, float& dw)
+{
+    w  = f.w;
+    dw = f.dw;
+}
+
+//! x^k for a small integer k >= 0 (wave-uniform loop), or 2^(k log2 x) when k is not an integer
+__device__ __forceinline__ double sincPower(double x, float k, int ki)
+{
+    if (ki < 0) { return double(__builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(float(x)))); }
+    double r = 1.0, b = x;
+    for (int e = ki; e > 0; e >>= 1)
+    {
+        if (e & 1) { r *= b; }
+        b *= b;
+    }
+    return r;
+}
+
+template<bool Derivative>
+__device__ __forceinline__ void sincDirect(float vf, float n, int ni, float& w, float& dw)
+{
+    constexpr double halfPi = 1.57079632679489661923;
+    const double     v      = vf;
+    const bool       upper  = v > 1.0;
+    const double     x      = v * halfPi;
+    const double     y      = upper ? (2.0 - v) * halfPi : x; // in [0, pi/2] on the support
+    const double     y2     = y * y;
+    // sin y = y (1 - y^2/3! + ... - y^12/13!)
+    double sy = -1.0 / 6227020800.0;
+    sy        = sy * y2 + 1.0 / 39916800.0;
+    sy        = sy * y2 - 1.0 / 362880.0;
+    sy        = sy * y2 + 1.0 / 5040.0;
+    sy        = sy * y2 - 1.0 / 120.0;
+    sy        = sy * y2 + 1.0 / 6.0;
+    sy        = y - y * y2 * sy;
+    double xInv = double(__builtin_amdgcn_rcpf(float(x)));
+    xInv        = xInv + xInv * (1.0 - x * xInv);
+    const bool   zero   = vf == 0.0f;
+    const double sinc   = zero ? 1.0 : sy * xInv;
+    const bool   inside = vf < 2.0f;
+    const double pm1    = sincPower(sinc, n - 1.0f, ni - 1); // sinc^(n-1)
+    w                   = inside ? float(pm1 * sinc) : 0.0f;
+    if constexpr (Derivative)
+    {
+        // cos y = 1 - y^2/2! + ... + y^14/14!
+        double cy = -1.0 / 87178291200.0;
+        cy        = cy * y2 + 1.0 / 479001600.0;
+        cy        = cy * y2 - 1.0 / 3628800.0;
+        cy        = cy * y2 + 1.0 / 40320.0;
+        cy        = cy * y2 - 1.0 / 720.0;
+        cy        = cy * y2 + 1.0 / 24.0;
+        cy        = cy * y2 - 0.5;
+        cy        = 1.0 + y2 * cy;
+        const double cx = upper ? -cy : cy;
+        const double ds = zero ? 0.0 : (cx - sinc) * xInv;
+        dw              = inside ? float(double(n) * pm1 * ds * halfPi) : 0.0f;
+    }
+}
+
+template<class T>
+struct SincKernel;
+
+//! FP64: the reference's tables (W, dW/dv, and both per interval)
+template<>
+struct SincKernel<double>
+{
+    const TabPair<double>* w;
+    const TabPair<double>* dw;
+    const TabQuad<double>* wq;
+    using FetchW  = TableFetch<double, TabPair<double>>;
+    using FetchWD = TableFetch<double, TabQuad<double>>;
+
+    __device__ __forceinline__ FetchW  fetchW(double v) const { return fetch(w, v); }
+    __device__ __forceinline__ FetchWD fetchWD(double v) const { return fetch(wq, v); }
+    __device__ __forceinline__ double  W(double v) const { return lookup(w, v); }
+    __device__ __forceinline__ double  dW(double v) const { return lookup(dw, v); }
+};
+
+//! FP32: evaluated directly (sincDirect)
+template<>
+struct SincKernel<float>
+{
+    float n;
+    int   ni; // n when it is an integer, else -1
+    using FetchW  = DirectW;
+    using FetchWD = DirectWD;
+
+    __device__ __forceinline__ FetchW fetchW(float v) const
+    {
+        FetchW f;
+        float  unused;
+        f.v = v;
+        sincDirect<false>(v, n, ni, f.w, unused);
+        return f;
+    }
+    __device__ __forceinline__ FetchWD fetchWD(float v) const
+    {
+        FetchWD f;
+        f.v = v;
+        sincDirect<true>(v, n, ni, f.w, f.dw);
+        return f;
+    }
+    __device__ __forceinline__ float W(float v) const { return fetchW(v).w; }
+    __device__ __forceinline__ float dW(float v) const { return fetchWD(v).dw; }
+};
+
 /*! Whether a wave must apply the legacy fold (sfc/box.hpp:218-238). The fold changes a separation
  * only when |x_i - x_j| exceeds the cut-off 2h_i. For a listed neighbour (minimum-image distance
  * below 2h_i) that needs the raw difference to wrap across a periodic face, which puts the target
@@ -431,25 +590,23 @@ inline PairArgs makePairArgs(sphx_workspace* ws, const DevBox& box, const sphx_f
     return c;
 }
 
+//! the kernel function of form T for the device (the workspace's tables, or its sinc index)
 template<class T>
-inline const TabPair<T>* whTable(sphx_workspace* ws)
+inline SincKernel<T> sincKernel(sphx_workspace* ws)
 {
-    if constexpr (isMixed<T>) { return ws->wh32.get(); }
-    else { return ws->wh64.get(); }
-}
-
-template<class T>
-inline const TabPair<T>* whdTable(sphx_workspace* ws)
-{
-    if constexpr (isMixed<T>) { return ws->whd32.get(); }
-    else { return ws->whd64.get(); }
-}
-
-template<class T>
-inline const TabQuad<T>* whqTable(sphx_workspace* ws)
-{
-    if constexpr (isMixed<T>) { return ws->whq32.get(); }
-    else { return ws->whq64.get(); }
+    SincKernel<T> k;
+    if constexpr (isMixed<T>)
+    {
+        k.n  = float(ws->sincIndex);
+        k.ni = ws->sincIndex == double(int(ws->sincIndex)) && ws->sincIndex < 64 ? int(ws->sincIndex) : -1;
+    }
+    else
+    {
+        k.w  = ws->wh64.get();
+        k.dw = ws->whd64.get();
+        k.wq = ws->whq64.get();
+    }
+    return k;
 }
 
 //! wave minimum then one atomic per wave

@@ -5,8 +5,7 @@ sync): x, y, z, u, temp, du in float64 (RealType); the HydroType fields in float
 plain form or float32 for the mixed form (the reference's SphTypes layout,
 sph/include/sph/types.hpp:39-46); keys uint64; nc uint32.
 
-The drivers issue the phases in the order of the reference propagators (the neighbour search and
-the first density sum fused in the mixed form, see search_first_sum):
+The drivers issue the phases in the order of the reference propagators:
   std: HydroProp::computeForces   (main/src/propagator/std_hydro.hpp:124-168)
   VE:  HydroVeProp::computeForces (main/src/propagator/ve_hydro.hpp:130-204)
 with the one-rank Domain::sync in front and no halo exchanges.
@@ -78,13 +77,11 @@ class ParticleSet:
 
 def search_first_sum(ws, box, f, params, n, rho):
     """Neighbour search with the h/nc iteration, then the first density sum: rho (std,
-    computeDensity) or xm (VE, computeXMass). The mixed form runs both in one kernel
-    (sphx_find_neighbors_xmass), as the reference GPU computeXMass does
-    (sph/include/sph/hydro_ve/xmass_gpu.cu:57-131). The FP64 form runs them as two kernels:
-    the FP64 search kernel holds 3 waves per SIMD and hides the latency of the sum's table
-    gathers worse than the standalone pair loop, so there the split is the faster one."""
-    if params.hydro_type == abi.SPHX_F32:
-        return ws.find_neighbors_xmass(box, f, params, 0, n, True, rho=rho)
+    computeDensity) or xm (VE, computeXMass), as two kernels. sphx_find_neighbors_xmass runs
+    both in one kernel, as the reference GPU computeXMass does
+    (sph/include/sph/hydro_ve/xmass_gpu.cu:57-131), and gives the same lists, counts and h; it
+    is slower here in both forms, because the search kernel holds fewer waves per SIMD than the
+    standalone pair loop and evaluates its hits one lane at a time."""
     converged = ws.find_neighbors(box, f, params, 0, n, True)
     ws.pair("sphx_density_std" if rho else "sphx_xmass", box, f, params, 0, n)
     return converged
