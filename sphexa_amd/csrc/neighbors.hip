@@ -5,10 +5,11 @@
 // search radius 2*h_max (exact periodic box-box distance, so no cell holding a neighbour of any
 // target is skipped), collecting candidate leaves in LDS; internal nodes live on an LDS stack
 // popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Candidate leaves that no
-// target's sphere reaches are dropped with a per-lane test and a ballot. The surviving leaves are
-// streamed as tiles of up to 64 particles through a double-buffered LDS area: coalesced vector
-// loads fetch tile q+1 while the wave tests tile q, reading each candidate with one broadcast
-// LDS access, every lane against its own target with the reference predicate
+// target's sphere reaches are dropped (one leaf per lane, boxes broadcast, one ballot per leaf).
+// The particles of the surviving leaves within reach of the group box are compacted into full
+// tiles of 64 in a double-buffered LDS area (coalesced loads of the next window overlap the
+// sweep), and the wave tests each tile reading every candidate with one broadcast LDS access,
+// every lane against its own target with the reference predicate
 // (domain/include/cstone/findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with
 // dx = x_j - x_i, periodic images folded by one box length, unfused FP64 arithmetic, keep if
 // d2 < Th(4)*h_i*h_i and j != i. The kept set and count are therefore bit-identical to the CPU
@@ -25,7 +26,9 @@ namespace
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kStackCap      = 1024; // LDS node stack per wave (bound argued at searchGroup)
-constexpr int kLeafCap       = 256;  // LDS candidate-leaf buffer per wave
+constexpr int kLeafCap       = 64;   // LDS candidate-leaf buffer per wave (one leaf per lane)
+constexpr int kPerPairImage  = 64;   // image code of a leaf swept with the per-pair fold
+constexpr int kNoShiftImage  = 13;   // image code of shift (0, 0, 0)
 
 struct SearchArgs
 {
@@ -54,11 +57,15 @@ struct SearchArgs
     int                  toRho;
 };
 
-//! per-wave LDS: traversal stack, candidate leaves, double-buffered candidate tile
+//! per-wave LDS: traversal stack, candidate leaves, the kept leaves' particle ranges, and the
+//! double-buffered candidate tile
 struct WaveLds
 {
-    int32_t stack[kStackCap];
-    int32_t leaves[kLeafCap];
+    int32_t  stack[kStackCap];
+    int32_t  leaves[kLeafCap];      // candidate leaf nodes, traversal order
+    uint32_t leafLo[kLeafCap];      // kept leaves: first particle
+    uint32_t leafPre[kLeafCap + 1]; // kept leaves: exclusive prefix of particle counts, then the total
+    int32_t  leafImg[kLeafCap];     // kept leaves: periodic image code (imageShift)
     double   tx[2][kWave], ty[2][kWave], tz[2][kWave];
     uint32_t tj[2][kWave]; // particle index of each staged candidate
     double   tm[2][kWave]; // its mass (fused XMass only)
@@ -111,13 +118,13 @@ struct ListOnly
 //! ... or the XMass sum of xmassJLoop (hydro_ve/xmass_kern.hpp:51-79): m_i + sum_j m_j W(r_ij / h_i)
 //! over the stored neighbours in list order, r_ij from the same separation as the pair loop
 //! (minimum image, then rounded to h's type), compensated for the FP32 form. Hits are staged
-//! kBatch at a time: their table entries are fetched together, then accumulated in list order,
+//! kBatch at a time: their kernel values are fetched together, then accumulated in list order,
 //! slots without a stored hit leaving the sum untouched (bitwise the one-at-a-time sum).
 template<class Th>
 struct XMassSum
 {
     static constexpr bool kStageMass = true;
-    static constexpr int  kBatch     = isMixed<Th> ? 4 : 2;
+    static constexpr int  kBatch     = isMixed<Th> ? 1 : 2; // FP32 evaluates W, no gather to overlap
     SincKernel<Th>        kern;
     Th                    hInv;
     Acc<Th>               rho0;
@@ -221,14 +228,84 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
     }
 }
 
-//! periodic image of a candidate leaf relative to the group centre (see flushLeaves)
-struct LeafImage
+__device__ __forceinline__ double readlaneD(double v, int q)
 {
-    double sh[3];
-    bool   shifted, perPair;
-};
+    uint64_t b  = __double_as_longlong(v);
+    uint32_t lo = __builtin_amdgcn_readlane(uint32_t(b), q), hi = __builtin_amdgcn_readlane(uint32_t(b >> 32), q);
+    return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
 
-//! Prune the buffered leaves against every lane's sphere, then stream the survivors through LDS.
+//! inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t waveInclusiveSum(uint32_t v)
+{
+    const unsigned lane = laneId();
+    for (int o = 1; o < kWave; o <<= 1)
+    {
+        uint32_t w = __shfl_up(v, o, kWave);
+        v += lane >= unsigned(o) ? w : 0u;
+    }
+    return v;
+}
+
+/* Periodic image of a candidate leaf relative to the group centre: the shift by whole box lengths
+ * k_d = rint((c_d - g_d) / L_d), k_d in {-1, 0, 1}, coded (k_x+1) + 3 (k_y+1) + 9 (k_z+1), which
+ * serves all pairs of the leaf; kPerPairImage when the group, the leaf and the reach together
+ * exceed a quarter box in a periodic dimension (then the per-pair fold of the reference). */
+__device__ __forceinline__ int imageCode(const DevBox& b, const double c[3], const double s[3], const double gc[3],
+                                         const double ge[3], double reach)
+{
+    int  code    = 0;
+    bool perPair = false;
+    for (int d = 2; d >= 0; --d)
+    {
+        int k = 0;
+        if (b.pbc[d])
+        {
+            k       = int(rint((c[d] - gc[d]) * b.ilen[d]));
+            perPair = perPair || (ge[d] + s[d] + reach > 0.25 * b.len[d]);
+        }
+        code = 3 * code + (k + 1);
+    }
+    return perPair ? kPerPairImage : code;
+}
+
+__device__ __forceinline__ double imageShift(const DevBox& b, int code, int d)
+{
+    const int k = (d == 0 ? code : d == 1 ? code / 3 : code / 9) % 3 - 1;
+    return double(k) * b.len[d];
+}
+
+//! sweep the first n candidates of tile buffer `buf`, all of image `code`
+template<class Sink>
+__device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t, WaveLds& lds, int buf, uint32_t n,
+                                            int code, unsigned& count, ListWriter& out, Sink& sink)
+{
+    __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
+    const double* tx = lds.tx[buf];
+    const double* ty = lds.ty[buf];
+    const double* tz = lds.tz[buf];
+    if (code == kPerPairImage)
+        sweepTile<Fold::perPair>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], n, 0, 0, 0, count, out, sink);
+    else if (code != kNoShiftImage)
+        sweepTile<Fold::shifted>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], n, imageShift(a.box, code, 0),
+                                 imageShift(a.box, code, 1), imageShift(a.box, code, 2), count, out, sink);
+    else
+        sweepTile<Fold::none>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], n, 0, 0, 0, count, out, sink);
+    __builtin_amdgcn_wave_barrier();
+}
+
+/* Sweep the buffered candidate leaves (nLeaf <= 64, traversal order).
+ *
+ * 1. Lane q takes leaf q: its box, particle range and periodic image load in parallel. The leaf
+ *    is kept if some active target's sphere reaches its box (the box is broadcast to the wave
+ *    lane by lane). Kept leaves get their first particle, image and the exclusive prefix of
+ *    their particle counts in LDS, in traversal order.
+ * 2. Their particles are read as one stream, 64 per window (a window never spans two images).
+ *    Each lane loads one particle and keeps it if it lies within reach of the group box under
+ *    its leaf's image. Kept particles are appended, in stream order, to the tile being filled.
+ *    A full tile of 64 (or the last one, or one whose image ends) is swept, while the next
+ *    window's loads are in flight. The candidate order is therefore the traversal order, as in
+ *    the reference. */
 template<class Sink>
 __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
                             const double ge[3], double reach, unsigned& count, ListWriter& out, Sink& sink)
@@ -237,102 +314,114 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
     const DevBox&  b    = a.box;
     const unsigned lane = laneId();
 
-    // 1. keep the leaves that at least one active target can reach (order preserved)
-    int kept = 0;
-    for (int q = 0; q < nLeaf; ++q)
+    // 1. leaf boxes, ranges and images in parallel; keep test with the box broadcast per leaf
+    double   c[3] = {0, 0, 0}, sz[3] = {0, 0, 0};
+    uint32_t lo = 0, cnt = 0;
+    int      code = kNoShiftImage;
+    if (int(lane) < nLeaf)
     {
-        int32_t node = int32_t(uniform(uint32_t(lds.leaves[q])));
-        double  p[3] = {t.x, t.y, t.z};
-        double  d2   = 0.0;
+        const int32_t node = lds.leaves[lane];
         for (int d = 0; d < 3; ++d)
         {
-            double v = uniformD(a.centers[3 * size_t(node) + d]) - p[d];
-            if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
-            v = fabs(v) - uniformD(a.sizes[3 * size_t(node) + d]);
-            v = v > 0.0 ? v : 0.0;
-            d2 += v * v;
+            c[d]  = a.centers[3 * size_t(node) + d];
+            sz[d] = a.sizes[3 * size_t(node) + d];
         }
-        if (ballot(t.active && d2 < t.reach2))
+        const uint32_t leaf = uint32_t(a.internalToLeaf[node]);
+        lo                  = a.layout[leaf];
+        cnt                 = a.layout[leaf + 1] - lo;
+        code                = imageCode(b, c, sz, gc, ge, reach);
+    }
+    uint64_t keepMask = 0;
+    {
+        const double p[3] = {t.x, t.y, t.z};
+        for (int q = 0; q < nLeaf; ++q)
         {
-            if (lane == 0) { lds.leaves[kept] = node; }
-            kept++;
+            double d2 = 0.0;
+            for (int d = 0; d < 3; ++d)
+            {
+                double v = readlaneD(c[d], q) - p[d];
+                if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
+                v = fabs(v) - readlaneD(sz[d], q);
+                v = v > 0.0 ? v : 0.0;
+                d2 += v * v;
+            }
+            keepMask |= ballot(t.active && d2 < t.reach2) ? (uint64_t(1) << q) : uint64_t(0);
         }
     }
+    const bool     kept   = (keepMask >> lane) & 1u;
+    const int      nKept  = __popcll(keepMask);
+    const uint32_t incl   = waveInclusiveSum(kept ? cnt : 0u);
+    if (kept)
+    {
+        const unsigned r = laneRank(keepMask);
+        lds.leafLo[r]    = lo;
+        lds.leafImg[r]   = code;
+        lds.leafPre[r]   = incl - cnt;
+    }
+    const uint32_t total = uniform(__shfl(incl, kWave - 1, kWave));
+    if (lane == 0) { lds.leafPre[nKept] = total; }
     __builtin_amdgcn_wave_barrier();
+    if (total == 0) { return; }
 
-    // 2. tiles of <= 64 particles per kept leaf, software pipelined through two LDS buffers
-    int      q = 0;   // leaf being tiled
-    uint32_t off = 0; // offset of the next tile inside that leaf
-    uint32_t lo = 0, hi = 0;
-    auto     leafRange = [&](int qq, uint32_t& l, uint32_t& h, int32_t& node)
-    {
-        node          = int32_t(uniform(uint32_t(lds.leaves[qq])));
-        uint32_t leaf = uniform(uint32_t(a.internalToLeaf[node]));
-        l             = uniform(a.layout[leaf]);
-        h             = uniform(a.layout[leaf + 1]);
-    };
-    // advance (q, off) to the next non-empty tile; returns false at the end
-    auto nextTile = [&](int32_t& node, uint32_t& tlo, uint32_t& tn) -> bool
-    {
-        while (q < kept)
-        {
-            leafRange(q, lo, hi, node);
-            if (lo + off < hi)
-            {
-                tlo = lo + off;
-                tn  = min(hi - tlo, uint32_t(kWave));
-                off += kWave;
-                if (lo + off >= hi) { q++, off = 0; }
-                return true;
-            }
-            q++, off = 0;
-        }
-        return false;
-    };
-
-    // image of a leaf: one shift by whole box lengths serves all its pairs unless the group, the
-    // leaf and the reach together exceed a quarter box (then the per-pair fold of the reference)
-    auto image = [&](int32_t nd)
-    {
-        LeafImage im{{0, 0, 0}, false, false};
-        for (int d = 0; d < 3; ++d)
-        {
-            if (!b.pbc[d]) { continue; }
-            double c  = uniformD(a.centers[3 * size_t(nd) + d]);
-            double s  = uniformD(a.sizes[3 * size_t(nd) + d]);
-            double k  = rint((c - gc[d]) * b.ilen[d]);
-            im.sh[d]  = k * b.len[d];
-            im.shifted = im.shifted || (k != 0.0);
-            im.perPair = im.perPair || (ge[d] + s + reach > 0.25 * b.len[d]);
-        }
-        return im;
-    };
+    // 2. the kept leaves' particles as one stream of windows, compacted into full tiles
     const double reach2 = reach * reach;
-
-    int32_t   node;
-    uint32_t  tlo, tn;
-    bool      have = nextTile(node, tlo, tn);
-    double    px = 0, py = 0, pz = 0, pm = 0;
-    LeafImage img{};
-    auto      fetchTile = [&]()
-    {
-        img = image(node);
-        if (lane < tn)
+    uint32_t     S      = 0;  // stream position of the current window
+    int          qs     = 0;  // kept leaf holding S
+    uint32_t     runEnd = 0;  // stream end of the run of leaves with the current window's image
+    int          runImg = kNoShiftImage;
+    // plan the window at S: its end, and this lane's particle (valid, index)
+    auto plan = [&](uint32_t& E, bool& valid, uint32_t& idx) {
+        while (qs + 1 < nKept && lds.leafPre[qs + 1] <= S) { qs++; }
+        if (S >= runEnd)
         {
-            px = a.x[tlo + lane], py = a.y[tlo + lane], pz = a.z[tlo + lane];
-            if constexpr (Sink::kStageMass) { pm = double(static_cast<const decltype(sink.hInv)*>(a.m)[tlo + lane]); }
+            runImg = lds.leafImg[qs];
+            int q  = qs + 1;
+            while (q < nKept && lds.leafImg[q] == runImg) { q++; }
+            runEnd = lds.leafPre[q];
+        }
+        E                = min(S + uint32_t(kWave), runEnd);
+        const uint32_t p = S + lane;
+        valid            = p < E;
+        idx              = 0;
+        if (valid)
+        {
+            int q = qs;
+            while (q + 1 < nKept && lds.leafPre[q + 1] <= p) { q++; }
+            idx = lds.leafLo[q] + (p - lds.leafPre[q]);
         }
     };
-    if (have) { fetchTile(); }
-    int buf = 0;
-    while (have)
-    {
-        // stage the candidates that lie within reach of the group box (order kept); a candidate
-        // within 2h_i of a target i of the group is within reach of the box under the leaf's image
-        bool keep = lane < tn;
-        if (keep && !img.perPair)
+    double   px = 0, py = 0, pz = 0, pm = 0;
+    auto     load = [&](bool valid, uint32_t idx) {
+        if (valid)
         {
-            const double q[3] = {px - img.sh[0], py - img.sh[1], pz - img.sh[2]};
+            px = a.x[idx], py = a.y[idx], pz = a.z[idx];
+            if constexpr (Sink::kStageMass) { pm = double(static_cast<const decltype(sink.hInv)*>(a.m)[idx]); }
+        }
+    };
+
+    uint32_t E;
+    bool     valid;
+    uint32_t idx;
+    plan(E, valid, idx);
+    load(valid, idx);
+    int      cur     = 0;             // tile buffer being filled
+    uint32_t acc     = 0;             // candidates in it
+    int      tileImg = runImg;        // their image
+    while (true)
+    {
+        const int      img = runImg;  // image of the loaded window
+        const uint32_t end = E;
+        if (img != tileImg && acc > 0) // a tile holds one image
+        {
+            sweepBuffer(a, t, lds, cur, acc, tileImg, count, out, sink);
+            acc = 0, cur ^= 1;
+        }
+        tileImg = img;
+        // keep the window's particles within reach of the group box under its image
+        bool keep = valid;
+        if (keep && img != kPerPairImage)
+        {
+            const double q[3] = {px - imageShift(b, img, 0), py - imageShift(b, img, 1), pz - imageShift(b, img, 2)};
             double       d2   = 0.0;
             for (int d = 0; d < 3; ++d)
             {
@@ -345,31 +434,30 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
         const uint64_t kmask = ballot(keep);
         if (keep)
         {
-            const unsigned slot = laneRank(kmask);
-            lds.tx[buf][slot] = px, lds.ty[buf][slot] = py, lds.tz[buf][slot] = pz;
-            lds.tj[buf][slot] = tlo + lane;
-            if constexpr (Sink::kStageMass) { lds.tm[buf][slot] = pm; }
+            const uint32_t slot = acc + laneRank(kmask);
+            const int      bb   = slot < uint32_t(kWave) ? cur : cur ^ 1;
+            const uint32_t sl   = slot < uint32_t(kWave) ? slot : slot - kWave;
+            lds.tx[bb][sl] = px, lds.ty[bb][sl] = py, lds.tz[bb][sl] = pz;
+            lds.tj[bb][sl] = idx;
+            if constexpr (Sink::kStageMass) { lds.tm[bb][sl] = pm; }
         }
-        __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
-
-        const LeafImage cur  = img;
-        const uint32_t  curN = uint32_t(__popcll(kmask));
-        have                 = nextTile(node, tlo, tn);
-        if (have) { fetchTile(); }
-
-        const double* tx = lds.tx[buf];
-        const double* ty = lds.ty[buf];
-        const double* tz = lds.tz[buf];
-        if (cur.perPair)
-            sweepTile<Fold::perPair>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, 0, 0, 0, count, out, sink);
-        else if (cur.shifted)
-            sweepTile<Fold::shifted>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, cur.sh[0], cur.sh[1],
-                                     cur.sh[2], count, out, sink);
-        else
-            sweepTile<Fold::none>(a, t, tx, ty, tz, lds.tj[buf], lds.tm[buf], curN, 0, 0, 0, count, out, sink);
-        buf ^= 1;
+        acc += uint32_t(__popcll(kmask));
+        // next window's loads go out before the sweep
+        S                = end;
+        const bool more  = S < total;
+        if (more)
+        {
+            plan(E, valid, idx);
+            load(valid, idx);
+        }
+        if (acc >= uint32_t(kWave))
+        {
+            sweepBuffer(a, t, lds, cur, kWave, tileImg, count, out, sink);
+            acc -= kWave, cur ^= 1;
+        }
+        if (!more) { break; }
     }
-    __builtin_amdgcn_wave_barrier();
+    if (acc > 0) { sweepBuffer(a, t, lds, cur, acc, tileImg, count, out, sink); }
 }
 
 /* Collect all leaves whose (periodic) distance to the active targets' bounding box is below the
@@ -487,7 +575,8 @@ __device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
 /* One wave per group: search, h/nc iteration, and with XMass the fused density sum of the final
  * search, stored as xm = m / rho0 (xmassJLoop) or rho = m / xm (computeDensity,
  * hydro_std/density.hpp:42-53), with the per-target epilogue of the XMass kernel. */
-// waves per SIMD the search kernel's register allocation must leave room for, per form
+// waves per SIMD the search kernel's register allocation must leave room for, per form (the
+// fused XMass variant is left to the compiler)
 #ifndef SPHX_WAVES_SEARCH_F32
 #define SPHX_WAVES_SEARCH_F32 4
 #endif
@@ -497,7 +586,7 @@ __device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
 
 template<class Th, bool XMass>
 __global__ void __launch_bounds__(kWavesPerBlock * kWave)
-__attribute__((amdgpu_waves_per_eu(isMixed<Th> ? SPHX_WAVES_SEARCH_F32 : SPHX_WAVES_SEARCH_F64)))
+__attribute__((amdgpu_waves_per_eu(XMass ? 1 : isMixed<Th> ? SPHX_WAVES_SEARCH_F32 : SPHX_WAVES_SEARCH_F64)))
 searchKernel(SearchArgs a, Th* __restrict__ h)
 {
     __shared__ WaveLds ldsMem[kWavesPerBlock];
