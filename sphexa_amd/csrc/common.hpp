@@ -349,6 +349,9 @@ struct DevBox
     double lo[3], hi[3], len[3], ilen[3];
     int    pbc[3];
     int    anyPbc;
+    // 32-bit fixed-point coordinates of the FP32 pair loops: u = rint((x - lo) 2^32 / len) mod 2^32
+    double fixScale[3]; // 2^32 / len
+    float  fixStep[3];  // len / 2^32
 };
 
 inline DevBox makeDevBox(const sphx_box_t* b)
@@ -363,6 +366,8 @@ inline DevBox makeDevBox(const sphx_box_t* b)
         d.ilen[k] = 1.0 / (b->hi[k] - b->lo[k]);
         d.pbc[k]  = b->boundary[k] == 1;
         d.anyPbc |= d.pbc[k];
+        d.fixScale[k] = 4294967296.0 / d.len[k];
+        d.fixStep[k]  = float(d.len[k] / 4294967296.0);
     }
     return d;
 }

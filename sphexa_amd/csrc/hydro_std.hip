@@ -14,27 +14,28 @@ namespace
 constexpr unsigned kPackThreads = 256;
 
 template<class T>
-__global__ void __launch_bounds__(kPackThreads) packMass(const double* __restrict__ x, const double* __restrict__ y,
-                                                         const double* __restrict__ z, const T* __restrict__ m,
-                                                         RecMass<T>* __restrict__ rec, uint32_t n)
+__global__ void __launch_bounds__(kPackThreads) packMass(DevBox b, const double* __restrict__ x,
+                                                         const double* __restrict__ y, const double* __restrict__ z,
+                                                         const T* __restrict__ m, RecMass<T>* __restrict__ rec,
+                                                         uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) { return; }
     RecMass<T> r;
-    r.x = x[i], r.y = y[i], r.z = z[i], r.m = m[i];
+    r.x = coordOf<T>(b, 0, x[i]), r.y = coordOf<T>(b, 1, y[i]), r.z = coordOf<T>(b, 2, z[i]), r.m = m[i];
     rec[i] = r;
 }
 
 template<class T>
-__global__ void __launch_bounds__(kPackThreads) packVol(const double* __restrict__ x, const double* __restrict__ y,
-                                                        const double* __restrict__ z, const T* __restrict__ m,
-                                                        const T* __restrict__ rho, RecVol<T>* __restrict__ rec,
-                                                        uint32_t n)
+__global__ void __launch_bounds__(kPackThreads) packVol(DevBox b, const double* __restrict__ x,
+                                                        const double* __restrict__ y, const double* __restrict__ z,
+                                                        const T* __restrict__ m, const T* __restrict__ rho,
+                                                        RecVol<T>* __restrict__ rec, uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) { return; }
     RecVol<T> r;
-    r.x = x[i], r.y = y[i], r.z = z[i];
+    r.x = coordOf<T>(b, 0, x[i]), r.y = coordOf<T>(b, 1, y[i]), r.z = coordOf<T>(b, 2, z[i]);
     r.vol  = fastDiv(m[i], rho[i]);
     rec[i] = r;
 }
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
     RecMomStd<T> r;
-    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
     r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
     T rho  = ld<T>(f.rho, i);
     r.hInv = T(1) / ld<T>(f.h, i);
@@ -83,7 +84,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T                h3Inv = hInv * hInv * hInv;
     double           r2h   = 2.0 * double(hi);
     Acc<T>           rho0(ri.m);
-    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                U = kBatchLight<T>;
         NeighborBatch<U, RecMass<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -131,7 +132,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T               hiInv = T(1) / hi;
     double          r2h   = 2.0 * double(hi);
     T               t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
-    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
         constexpr int               U = kBatchLight<T>;
         NeighborBatch<U, RecVol<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -189,7 +190,7 @@ __global__ void SPHX_MOM_LOOP(T)
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy;
-        withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
             constexpr int                  U = kBatchMom<T>;
             NeighborBatch<U, RecMomStd<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)
@@ -260,8 +261,9 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
     RecMass<T>* rec = recordBuffer<RecMass<T>>(ws, a.numParticles);
-    packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.x, a.y, a.z, (const T*)f->m,
-                                                                                         rec, a.numParticles);
+    packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.box, a.x, a.y, a.z,
+                                                                                         (const T*)f->m, rec,
+                                                                                         a.numParticles);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, toRho ? "density_std" : "xmass");
     if (toRho)
@@ -280,7 +282,7 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
     if (blocks == 0) { return; }
     RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles);
     packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
-        a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
+        a.box, a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
     SPHX_CHECK_LAUNCH();
     KernelScope timer(ws, "iad_std");
     iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),

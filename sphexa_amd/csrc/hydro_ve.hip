@@ -21,7 +21,8 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
     RecGradh<T> r;
-    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i], r.m = m[i], r.xm = xm[i];
+    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
+    r.m = m[i], r.xm = xm[i];
     rec[i] = r;
 }
 
@@ -39,7 +40,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T                 hInv = T(1) / hi;
     double            r2h  = 2.0 * double(hi);
     Acc<T>            kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
-    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecGradh<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
     RecIadVe<T> r;
-    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
     r.xm  = ld<T>(f.xm, i);
     r.vol = fastDiv(r.xm, ld<T>(f.kx, i));
     r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
@@ -133,7 +134,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     double            r2h   = 2.0 * double(hi);
     T                 t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     T                 M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecIadVe<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
     RecAv<T> r;
-    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
     r.vol  = fastDiv(ld<T>(f.xm, i), ld<T>(f.kx, i));
     r.c    = ld<T>(f.c, i);
     r.divv = ld<T>(f.divv, i);
@@ -238,7 +239,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     double         r2h   = 2.0 * double(hi);
     T              vsig  = T(1.e-40) * ri.c;
     T              S[3]  = {0, 0, 0};
-    withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
         constexpr int              U = kBatchLight<T>;
         NeighborBatch<U, RecAv<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomVe(PairArgs a, MomVeField
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
     RecMomVe<T> r;
-    r.x = a.x[i], r.y = a.y[i], r.z = a.z[i];
+    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
     r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
     r.hInv  = T(1) / ld<T>(f.h, i);
     r.m     = ld<T>(f.m, i);
@@ -388,7 +389,7 @@ __global__ void SPHX_MOM_LOOP(T)
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy, avEnergy;
-        withFold(waveNeedsFold(a.box, ri.x, ri.y, ri.z, r2h), [&](auto fold) {
+        withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
             constexpr int                 U = kBatchMom<T>;
             NeighborBatch<U, RecMomVe<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)

@@ -453,13 +453,49 @@ struct Sep
     T x, y, z, d;
 };
 
+/*! Position of a particle in the neighbour records: FP64 in the plain form; in the FP32 form
+ * 32-bit fixed point over the box, u = rint((x - lo) 2^32 / L) mod 2^32 per dimension (L the box
+ * length). There x_i - x_j is the exact wrapping difference u_i - u_j read as a signed 32-bit
+ * integer, the minimum image in a periodic dimension, which is what the reference's fold gives
+ * for a listed pair, and harmless in an open one (listed pairs are far closer than L/2). It is
+ * then converted to FP32 and scaled by L / 2^32 (about one ulp in all), from coordinates quantized
+ * to L / 2^32 (2.3e-10 in a unit box). A record position is 12 bytes instead of 24. */
+template<class T>
+using Coord = std::conditional_t<isMixed<T>, uint32_t, double>;
+
+__device__ __forceinline__ uint32_t fixedCoord(const DevBox& b, int d, double x)
+{
+    return uint32_t(uint64_t(int64_t(rint((x - b.lo[d]) * b.fixScale[d]))));
+}
+
+template<class T>
+__device__ __forceinline__ Coord<T> coordOf(const DevBox& b, int d, double x)
+{
+    if constexpr (isMixed<T>) { return fixedCoord(b, d, x); }
+    else { return x; }
+}
+
 template<bool Fold, class T, class Ri, class Rj>
 __device__ __forceinline__ Sep<T> pairSeparation(const DevBox& b, double r2h, const Ri& ri, const Rj& rj)
 {
     Sep<T> s;
-    separation<Fold>(b, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, s.x, s.y, s.z);
+    if constexpr (isMixed<T>)
+    {
+        s.x = float(int32_t(ri.x - rj.x)) * b.fixStep[0];
+        s.y = float(int32_t(ri.y - rj.y)) * b.fixStep[1];
+        s.z = float(int32_t(ri.z - rj.z)) * b.fixStep[2];
+    }
+    else { separation<Fold>(b, r2h, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, s.x, s.y, s.z); }
     s.d = fastSqrt(s.x * s.x + s.y * s.y + s.z * s.z);
     return s;
+}
+
+//! whether the wave of target record ri must fold its FP64 separations (the FP32 form never does)
+template<class T, class R>
+__device__ __forceinline__ bool recordNeedsFold(const DevBox& b, const R& ri, double r2h)
+{
+    if constexpr (isMixed<T>) { return false; }
+    else { return waveNeedsFold(b, ri.x, ri.y, ri.z, r2h); }
 }
 
 //! run `loop` with the fold compiled in or out, chosen once per wave

@@ -5,7 +5,8 @@
 // cache lines; the loops were bound by the vector-memory address path, not by ALU or HBM.
 // Before each pair loop a coalesced pack kernel therefore writes exactly the fields that loop
 // needs for j into one 16-byte aligned record per particle (workspace memory, SFC order), so a
-// neighbour costs a few 16-byte loads from one or two cache lines. Derived per-particle
+// neighbour costs a few 16-byte loads from one or two cache lines. Positions are FP64 in the plain
+// form and 32-bit fixed point in the FP32 form (Coord, pairs.hpp). Derived per-particle
 // quantities (1/h, m/rho, xm/kx, kx m/xm) are formed in the pack with the same expression the
 // pair loop would evaluate, so results are unchanged.
 #pragma once
@@ -19,7 +20,7 @@ namespace sphx
 template<class T>
 struct alignas(16) RecMass
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      m;
 };
 
@@ -27,7 +28,7 @@ struct alignas(16) RecMass
 template<class T>
 struct alignas(16) RecGradh
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      m, xm;
 };
 
@@ -35,7 +36,7 @@ struct alignas(16) RecGradh
 template<class T>
 struct alignas(16) RecVol
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      vol;
 };
 
@@ -43,7 +44,7 @@ struct alignas(16) RecVol
 template<class T>
 struct alignas(16) RecIadVe
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      vol, xm, vx, vy, vz;
 };
 
@@ -51,7 +52,7 @@ struct alignas(16) RecIadVe
 template<class T>
 struct alignas(16) RecAv
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      vol, c, divv, vx, vy, vz;
 };
 
@@ -59,7 +60,7 @@ struct alignas(16) RecAv
 template<class T>
 struct alignas(16) RecMomStd
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      vx, vy, vz, hInv, m, vol, pro, c, c11, c12, c13, c22, c23, c33;
 };
 
@@ -67,9 +68,15 @@ struct alignas(16) RecMomStd
 template<class T>
 struct alignas(16) RecMomVe
 {
-    double x, y, z;
+    Coord<T> x, y, z;
     T      vx, vy, vz, hInv, m, c, rho, vol, xm, alpha, prho, c11, c12, c13, c22, c23, c33;
 };
+
+// 16-byte loads per neighbour, FP32 form: 1 (mass, volume), 2 (grad-h, VE IAD), 3 (AV), 5 (momentum)
+static_assert(sizeof(RecMass<float>) == 16 && sizeof(RecVol<float>) == 16, "record layout");
+static_assert(sizeof(RecGradh<float>) == 32 && sizeof(RecIadVe<float>) == 32 && sizeof(RecAv<float>) == 48,
+              "record layout");
+static_assert(sizeof(RecMomStd<float>) == 80 && sizeof(RecMomVe<float>) == 80, "record layout");
 
 //! workspace buffer for `n` records of type R
 template<class R>
