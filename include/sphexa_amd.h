@@ -50,7 +50,9 @@ typedef enum sphx_dtype
     SPHX_F32 = 1
 } sphx_dtype;
 
-/* cstone::Box<double> (domain/include/cstone/sfc/box.hpp:95-174); boundary: 0 open, 1 periodic */
+/* cstone::Box<double> (domain/include/cstone/sfc/box.hpp:95-174); boundary: 0 open, 1 periodic,
+ * 2 fixed (BoundaryType, box.hpp:80-85: open for the search; the integrator leaves particles at rest
+ * within 2h of a fixed face) */
 typedef struct sphx_box
 {
     double  lo[3];
@@ -125,6 +127,30 @@ typedef struct sphx_params
  * reference's constant alpha = 1 (hydro_std/momentum_energy_kern.hpp:98); the coefficient the
  * std adjoint differentiates against. */
 #define SPHX_FLAG_STD_ALPHA 1
+
+/* Previous-step fields of the time integrator (particles_data.hpp:221,233,238). x_m1, y_m1, z_m1
+ * (X_n - X_n-1) and du_m1 are XM1Type, which is the HydroType here (double in the plain form,
+ * float in the mixed form, as SphTypes); mui is HydroType, NULL for the constant muiConst. */
+typedef struct sphx_history
+{
+    void* x_m1;
+    void* y_m1;
+    void* z_m1;
+    void* du_m1;
+    void* mui;
+} sphx_history_t;
+
+/* Global time-step state, the Dataset scalars of particles_data.hpp:96-141 (defaults in
+ * parentheses): minDt, minDt_m1 (1e-12), ttot (0), minDtCourant and minDtRho (INFINITY),
+ * maxDtIncrease (1.1), g (0: no acceleration limit), etaAcc (0.2), eps (0.005), Krho (0.06). */
+typedef struct sphx_timestep
+{
+    double minDt, minDt_m1, ttot;
+    double minDtCourant, minDtRho;
+    double maxDtIncrease;
+    double g, etaAcc, eps;
+    double Krho;
+} sphx_timestep_t;
 
 /* Cotangents and gradients of the adjoint entry points: one HydroType array per named quantity
  * (du included). An adjoint reads the cotangents of its forward outputs from `in` and adds
@@ -216,6 +242,12 @@ int sphx_tree_view(const sphx_workspace_t* ws, sphx_tree_view_t* out);
 int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t* f, const sphx_params_t* prm,
                      size_t n, uint32_t bucketSize);
 
+/* Reorder one more per-particle array (elemBytes = 4 or 8) by the permutation of the last
+ * sphx_domain_sync over the same n. For conserved fields outside sphx_fields_t (x_m1, y_m1, z_m1,
+ * du_m1, mui, ...), which Domain::sync reorders together with the others
+ * (domain/include/cstone/domain/domain.hpp:182-234). */
+int sphx_domain_reorder(sphx_workspace_t* ws, void* field, size_t n, int elemBytes);
+
 /* ---- neighbour search ---------------------------------------------------------------------- */
 
 /* Neighbour lists of targets [first,last) with the coupled h / nc iteration (at most 10 updates,
@@ -298,13 +330,46 @@ int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const s
                             const sphx_params_t* prm, uint32_t first, uint32_t last, int avClean,
                             double* minDtCourant);
 
+/* ---- time integration ------------------------------------------------------------------------
+ * One rank's share of the step's integrate phase (main/src/propagator/std_hydro.hpp:170-181,
+ * ve_hydro.hpp:206-217): time step, positions and energy, smoothing lengths. FP64 arithmetic on
+ * FP64 positions and energies in both forms, as the reference's mixed types keep them. */
+
+/* ts->minDtRho = Krho / |max_i divv_i| over [first, last). Replaces rhoTimestep
+ * (sph/include/sph/ts_global.hpp:70-93). fields: divv. */
+int sphx_rho_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                      uint32_t last, sphx_timestep_t* ts);
+
+/* New global step min(dtAcc, minDtCourant, minDtRho, maxDtIncrease * minDt), dtAcc =
+ * etaAcc sqrt(eps / |a|max) when g != 0 (else infinite); then ttot += step, minDt_m1 = minDt,
+ * minDt = step. Replaces computeTimestep (ts_global.hpp:95-113) on one rank (the MPI_Allreduce
+ * of the minimum is the identity). fields: ax, ay, az when g != 0. */
+int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                          uint32_t last, sphx_timestep_t* ts);
+
+/* Press second-order position update and Adams-Bashforth energy update over [first, last):
+ * x, y, z (put back into a periodic box), x_m1..z_m1, vx, vy, vz, and temp (when fields.temp is
+ * set, through cv of mui or muiConst) or u, with du_m1 = du. dt and dt_m1 are the step and the
+ * previous step (the reference passes them as float). Replaces computePositions / the GPU
+ * computePositionsKernel (sph/include/sph/positions.hpp:177-200, positions_gpu.cu:116-168).
+ * fields: x, y, z, vx, vy, vz, ax, ay, az, du, temp or u, h (fixed boundaries only). */
+int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                           const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first, uint32_t last,
+                           double dt, double dt_m1);
+
+/* h = updateH(ng0, nc, h) over [first, last). Replaces updateSmoothingLength
+ * (sph/include/sph/update_h.hpp:26-35). fields: h, nc. */
+int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                                 uint32_t first, uint32_t last);
+
 /* ---- adjoints (no counterpart in the reference) ---------------------------------------------
  * Vector-Jacobian products of the std density, IAD and momentum-energy passes with respect to
  * positions, smoothing lengths and a per-particle AV coefficient, for gradient-based calibration
  * and inverse problems. Each recomputes its pair terms from the neighbour lists of the last
  * sphx_find_neighbors (the lists are a checkpoint: they are held fixed, as is the h of the
- * forward). Kernel derivatives use the tabulated dW/dv interpolated like W. Gradients to
- * neighbours are scattered with atomic adds, so FP results can vary in the last bits. */
+ * forward). Kernel derivatives use the tabulated dW/dv interpolated like W in the FP64 form and
+ * are evaluated like W in the FP32 form, as the forward loops do. Gradients to neighbours are
+ * scattered with atomic adds, so FP results can vary in the last bits. */
 
 /* rho = K h^-3 (m_i + sum_j m_j W(r_ij/h_i)) (sphx_density_std).
  * in: rho. out: x, y, z, h. fields: x, y, z, h, m, nc. */

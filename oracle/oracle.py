@@ -22,6 +22,8 @@ _BOXP = C.POINTER(abi.Box)
 _FP = C.POINTER(abi.Fields)
 _PP = C.POINTER(abi.Params)
 _AP = C.POINTER(abi.Adjoint)
+_HP = C.POINTER(abi.History)
+_TSP = C.POINTER(abi.Timestep)
 
 _SIGS = {
     "sfc_keys": (None, [_BOXP, _P, _P, _P, _P, _SZ]),
@@ -51,6 +53,13 @@ _SIGS = {
     "av_switches": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "divv_curlv": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P]),
     "momentum_energy_ve": (C.c_double, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, C.c_int, _P]),
+    "energy_update": (C.c_double, [C.c_double] * 5),
+    "position_update": (None, [C.c_double, C.c_double, _P, _P, _P, _BOXP, _P, _P, _P]),
+    "fbc_check_f32": (C.c_int, [C.c_double, C.c_float, C.c_double, C.c_double, C.c_int]),
+    "compute_positions": (None, [_BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double]),
+    "update_smoothing_length": (None, [_FP, _PP, _U32, _U32]),
+    "compute_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
+    "rho_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
     "density_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "iad_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "momentum_energy_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),

@@ -552,4 +552,68 @@ double orc_momentum_energy_ve(const sphx_box_t* b, const sphx_fields_t* f, const
                    : momentumEnergyVe<false, double>(box, f, p, first, last, nb, (const double*)wh, (double*)maxvs);
 }
 
+// ---- time integration ------------------------------------------------------------------------
+
+double orc_energy_update(double u_old, double dt, double dt_m1, double du, double du_m1)
+{
+    return energyUpdate(u_old, dt, dt_m1, du, du_m1);
+}
+
+void orc_position_update(double dt, double dt_m1, const double* X, const double* A, const double* dX,
+                         const sphx_box_t* b, double* Xo, double* Vo, double* dXo)
+{
+    positionUpdate(dt, dt_m1, X, A, dX, toBox(b), Xo, Vo, dXo);
+}
+
+//! computePositions (positions_gpu.cu:116-168) on host arrays; hydro_type selects T
+void orc_compute_positions(const sphx_box_t* b, const sphx_fields_t* f, const sphx_history_t* hs,
+                           const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1)
+{
+    Box box = toBox(b);
+    auto run = [&](auto tag) {
+        using T          = decltype(tag);
+        double constCv   = hs->mui ? -1.0 : double(idealGasCv(T(p->muiConst), p->gamma));
+        double* u        = f->temp ? nullptr : f->u;
+        computePositions<T>(box, dt, dt_m1, f->x, f->y, f->z, (T*)f->vx, (T*)f->vy, (T*)f->vz, (T*)hs->x_m1,
+                            (T*)hs->y_m1, (T*)hs->z_m1, (const T*)f->ax, (const T*)f->ay, (const T*)f->az, f->temp, u,
+                            f->du, (T*)hs->du_m1, (const T*)f->h, (const T*)hs->mui, p->gamma, constCv, first, last);
+    };
+    if (p->hydro_type == SPHX_F32) { run(float{}); }
+    else { run(double{}); }
+}
+
+void orc_update_smoothing_length(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
+{
+    if (p->hydro_type == SPHX_F32) { updateSmoothingLength(first, last, p->ng0, f->nc, (float*)f->h); }
+    else { updateSmoothingLength(first, last, p->ng0, f->nc, (double*)f->h); }
+}
+
+//! computeTimestep (ts_global.hpp:95-113) with accelerationTimestep's maximum (:46-67)
+void orc_compute_timestep(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                          sphx_timestep_t* ts)
+{
+    double maxAccSq = 0;
+    if (ts->g != 0.0)
+    {
+        maxAccSq = p->hydro_type == SPHX_F32
+                       ? maxNormSquare((const float*)f->ax, (const float*)f->ay, (const float*)f->az, first, last)
+                       : maxNormSquare((const double*)f->ax, (const double*)f->ay, (const double*)f->az, first, last);
+    }
+    computeTimestep(maxAccSq, ts->minDt, ts->minDt_m1, ts->ttot, ts->minDtCourant, ts->minDtRho, ts->maxDtIncrease,
+                    ts->g, ts->etaAcc, ts->eps);
+}
+
+//! rhoTimestep (ts_global.hpp:70-93): Krho / |max divv|
+void orc_rho_timestep(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                      sphx_timestep_t* ts)
+{
+    double m = -INFINITY;
+    for (uint32_t i = first; i < last; ++i)
+    {
+        double v = p->hydro_type == SPHX_F32 ? double(((const float*)f->divv)[i]) : ((const double*)f->divv)[i];
+        m        = std::max(m, v);
+    }
+    ts->minDtRho = ts->Krho / std::abs(m);
+}
+
 } // extern "C"

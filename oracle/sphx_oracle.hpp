@@ -44,6 +44,7 @@ struct Box
     double lo[3], hi[3];
     double len[3], ilen[3];
     int    pbc[3]; // 1 = periodic
+    int    fbc[3]; // 2 = fixed
 
     Box() = default;
     Box(const double* limits6, const int* bc3)
@@ -55,6 +56,7 @@ struct Box
             len[d]  = hi[d] - lo[d];
             ilen[d] = 1.0 / (hi[d] - lo[d]);
             pbc[d]  = bc3[d] == 1;
+            fbc[d]  = bc3[d] == 2;
         }
     }
     bool anyPbc() const { return pbc[0] || pbc[1] || pbc[2]; }
@@ -1392,6 +1394,127 @@ inline void idealGasFromU(T1 u, T2 rho, T3 gamma, std::common_type_t<T1, T2, T3>
     Tc tmp   = u * (gamma - Tc(1));
     p        = rho * tmp;
     c        = std::sqrt(gamma * tmp);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Time integration (sph/include/sph/positions.hpp, positions_gpu.cu, update_h.hpp, ts_global.hpp).
+ * Follows the reference's GPU kernel and its instantiated types (positions_gpu.cu:116-193):
+ * positions, temperature / internal energy, du in double; velocities, accelerations, h, mui and
+ * the previous-step fields x_m1, y_m1, z_m1, du_m1 in the HydroType T.
+ * ----------------------------------------------------------------------------------------------*/
+
+//! energyUpdate, positions.hpp:54-61 (Adams-Bashforth 2nd order)
+template<class TU>
+inline TU energyUpdate(TU u_old, double dt, double dt_m1, double du, double du_m1)
+{
+    TU u_new = u_old + du * dt + 0.5 * (du - du_m1) / dt_m1 * std::abs(dt) * dt;
+    if (u_new < 0.) { u_new = u_old * std::exp(u_new * dt / u_old); }
+    return u_new;
+}
+
+//! putInBox, domain/include/cstone/sfc/box.hpp:193-214
+inline void putInBox(double X[3], const Box& box)
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        if (box.pbc[d] && X[d] > box.hi[d]) { X[d] -= box.len[d]; }
+        else if (box.pbc[d] && X[d] < box.lo[d]) { X[d] += box.len[d]; }
+    }
+}
+
+//! positionUpdate, positions.hpp:63-85 (Press 2nd order): (Xn, An, dXn) -> (Xn+1, Vn+1, dXn+1)
+inline void positionUpdate(double dt, double dt_m1, const double Xn[3], const double An[3], const double dXn[3],
+                           const Box& box, double Xnp1[3], double Vnp1[3], double dXnp1[3])
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        double Vnmhalf = dXn[d] * (1.0 / dt_m1);
+        double Vn      = Vnmhalf + 0.5 * dt_m1 * An[d];
+        Vnp1[d]        = Vn + An[d] * dt;
+        dXnp1[d]       = (Vn + 0.5 * An[d] * std::abs(dt)) * dt;
+        Xnp1[d]        = Xn[d] + dXnp1[d];
+    }
+    putInBox(Xnp1, box);
+}
+
+//! fbcCheck, positions.hpp:46-51
+template<class Th>
+inline bool fbcCheck(double coord, Th h, double top, double bottom, bool fbc)
+{
+    return fbc && (std::abs(top - coord) < Th(2) * h || std::abs(bottom - coord) < Th(2) * h);
+}
+
+/*! computePositionsKernel, positions_gpu.cu:116-168, over [first, last). constCv < 0 selects the
+ * per-particle idealGasCv(mui[i], gamma). temp (when non-null) takes precedence over u. */
+template<class T>
+inline void computePositions(const Box& box, double dt, double dt_m1, double* x, double* y, double* z, T* vx, T* vy,
+                             T* vz, T* x_m1, T* y_m1, T* z_m1, const T* ax, const T* ay, const T* az, double* temp,
+                             double* u, const double* du, T* du_m1, const T* h, const T* mui, double gamma,
+                             double constCv, size_t first, size_t last)
+{
+    bool anyFBC = box.fbc[0] || box.fbc[1] || box.fbc[2];
+    for (size_t i = first; i < last; ++i)
+    {
+        if (anyFBC && vx[i] == T(0) && vy[i] == T(0) && vz[i] == T(0))
+        {
+            if (fbcCheck(x[i], h[i], box.hi[0], box.lo[0], box.fbc[0]) ||
+                fbcCheck(y[i], h[i], box.hi[1], box.lo[1], box.fbc[1]) ||
+                fbcCheck(z[i], h[i], box.hi[2], box.lo[2], box.fbc[2]))
+            {
+                continue;
+            }
+        }
+        double A[3]  = {double(ax[i]), double(ay[i]), double(az[i])};
+        double X[3]  = {x[i], y[i], z[i]};
+        double dX[3] = {double(x_m1[i]), double(y_m1[i]), double(z_m1[i])};
+        double Xn1[3], V[3], dXn1[3];
+        positionUpdate(dt, dt_m1, X, A, dX, box, Xn1, V, dXn1);
+        x[i] = Xn1[0], y[i] = Xn1[1], z[i] = Xn1[2];
+        x_m1[i] = T(dXn1[0]), y_m1[i] = T(dXn1[1]), z_m1[i] = T(dXn1[2]);
+        vx[i] = T(V[0]), vy[i] = T(V[1]), vz[i] = T(V[2]);
+
+        if (temp != nullptr)
+        {
+            T      cv    = (constCv < 0) ? idealGasCv(mui[i], gamma) : constCv;
+            double u_old = temp[i] * cv;
+            temp[i]      = energyUpdate(u_old, dt, dt_m1, du[i], du_m1[i]) / cv;
+            du_m1[i]     = du[i];
+        }
+        else if (u != nullptr)
+        {
+            u[i]     = energyUpdate(u[i], dt, dt_m1, du[i], du_m1[i]);
+            du_m1[i] = du[i];
+        }
+    }
+}
+
+//! updateSmoothingLengthCpu, update_h.hpp:12-24
+template<class T>
+inline void updateSmoothingLength(size_t first, size_t last, unsigned ng0, const uint32_t* nc, T* h)
+{
+    for (size_t i = first; i < last; ++i)
+        h[i] = updateH(ng0, nc[i], h[i]);
+}
+
+//! maxNormSquareGpu's element (primitives_gpu.cu:179-185): x*x + y*y + z*z in T
+template<class T>
+inline double maxNormSquare(const T* x, const T* y, const T* z, size_t first, size_t last)
+{
+    T m = 0;
+    for (size_t i = first; i < last; ++i)
+        m = std::max(m, T(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]));
+    return double(m);
+}
+
+//! computeTimestep, ts_global.hpp:95-113, on one rank; dtAcc from accelerationTimestep (:46-67)
+inline void computeTimestep(double maxAccSq, double& minDt, double& minDt_m1, double& ttot, double minDtCourant,
+                            double minDtRho, double maxDtIncrease, double g, double etaAcc, double eps)
+{
+    double minDtAcc = (g != 0.0) ? etaAcc * std::sqrt(eps / std::sqrt(maxAccSq)) : INFINITY;
+    double dt       = std::min({minDtAcc, minDtCourant, minDtRho, maxDtIncrease * minDt});
+    ttot += dt;
+    minDt_m1 = minDt;
+    minDt    = dt;
 }
 
 } // namespace orc

@@ -79,6 +79,31 @@ class Params(C.Structure):
 
 SPHX_FLAG_STD_ALPHA = 1
 
+FIXED = 2  # cstone::BoundaryType::fixed (box.hpp:80-85)
+
+HISTORY_FIELDS = ("x_m1", "y_m1", "z_m1", "du_m1", "mui")
+
+
+class History(C.Structure):
+    """sphx_history_t: previous-step fields of the integrator (XM1Type = HydroType; mui optional)."""
+    _fields_ = [(n, C.c_void_p) for n in HISTORY_FIELDS]
+
+
+class Timestep(C.Structure):
+    """sphx_timestep_t: the global time-step scalars (particles_data.hpp:96-141)."""
+    _fields_ = [(n, C.c_double) for n in ("minDt", "minDt_m1", "ttot", "minDtCourant", "minDtRho", "maxDtIncrease",
+                                          "g", "etaAcc", "eps", "Krho")]
+
+    @classmethod
+    def defaults(cls, **kw):
+        """Reference defaults: minDt = minDt_m1 = 1e-12, ttot 0, minDtCourant = minDtRho = inf,
+        maxDtIncrease 1.1, g 0, etaAcc 0.2, eps 0.005, Krho 0.06."""
+        t = cls(minDt=1e-12, minDt_m1=1e-12, ttot=0.0, minDtCourant=float("inf"), minDtRho=float("inf"),
+                maxDtIncrease=1.1, g=0.0, etaAcc=0.2, eps=0.005, Krho=0.06)
+        for k, v in kw.items():
+            setattr(t, k, v)
+        return t
+
 ADJOINT_FIELDS = ("x", "y", "z", "h", "rho", "p", "alpha", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az",
                   "du")
 

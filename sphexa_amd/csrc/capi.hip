@@ -310,6 +310,22 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         buildTree(ws, keys, n, bucketSize, b);
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         own.release();
+        ws->syncN = n;
+        return SPHX_OK;
+    });
+}
+
+int sphx_domain_reorder(sphx_workspace_t* ws, void* field, size_t n, int elemBytes)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(field || n == 0, SPHX_ERR_ARG, "null argument");
+        SPHX_REQUIRE(elemBytes == 4 || elemBytes == 8, SPHX_ERR_ARG, "element size must be 4 or 8 bytes");
+        SPHX_REQUIRE(n == ws->syncN, SPHX_ERR_STATE, "no sphx_domain_sync over the same particle count");
+        if (n == 0) { return SPHX_OK; }
+        ws->fieldTmp.reserve(n * 8);
+        gather(ws, ws->order.get(), field, ws->fieldTmp.get(), n, elemBytes);
+        SPHX_HIP(hipMemcpyAsync(field, ws->fieldTmp.get(), n * size_t(elemBytes), hipMemcpyDeviceToDevice,
+                                ws->stream));
         return SPHX_OK;
     });
 }
@@ -387,6 +403,47 @@ int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_
     return guarded(ws, [&] {
         SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
         eosVe(ws, f, prm, first, last);
+        return SPHX_OK;
+    });
+}
+
+int sphx_rho_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                      uint32_t last, sphx_timestep_t* ts)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm && ts, SPHX_ERR_ARG, "null argument");
+        rhoTimestep(ws, f, prm, first, last, ts);
+        return SPHX_OK;
+    });
+}
+
+int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
+                          uint32_t last, sphx_timestep_t* ts)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm && ts, SPHX_ERR_ARG, "null argument");
+        computeTimestep(ws, f, prm, first, last, ts);
+        return SPHX_OK;
+    });
+}
+
+int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                           const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first, uint32_t last,
+                           double dt, double dt_m1)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && hist && prm, SPHX_ERR_ARG, "null argument");
+        computePositions(ws, box, f, hist, prm, first, last, dt, dt_m1);
+        return SPHX_OK;
+    });
+}
+
+int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                                 uint32_t first, uint32_t last)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
+        updateSmoothingLength(ws, f, prm, first, last);
         return SPHX_OK;
     });
 }

@@ -181,6 +181,7 @@ struct sphx_workspace
     sphx::DevBuf<uint32_t> idxTmp;
     sphx::DevBuf<char>     fieldTmp;     // permutation staging
     sphx::DevBuf<uint32_t> order;        // sort permutation of the domain sync
+    size_t                 syncN = 0;    // particles of that sync (0: none yet)
     sphx::DevBuf<sphx::Chunk16> records; // packed neighbour records of the current pair loop
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
@@ -409,6 +410,14 @@ void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, cons
 void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                 uint32_t first, uint32_t last);
 void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last);
+void rhoTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                 sphx_timestep_t* ts);
+void computeTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                     uint32_t last, sphx_timestep_t* ts);
+void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
+                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1);
+void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                           uint32_t last);
 void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                   uint32_t first, uint32_t last);
 void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,

@@ -21,6 +21,8 @@ _AP = C.POINTER(abi.Adjoint)
 _U32 = C.c_uint32
 _SZ = C.c_size_t
 
+_TSP = C.POINTER(abi.Timestep)
+_HP = C.POINTER(abi.History)
 _SIGS = {
     "sphx_workspace_create": [C.POINTER(_P), C.c_int, _P],
     "sphx_workspace_destroy": [_P],
@@ -31,6 +33,7 @@ _SIGS = {
     "sphx_build_tree": [_P, _P, _SZ, _U32, _BOXP],
     "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
     "sphx_domain_sync": [_P, _BOXP, _FP, _PP, _SZ, _U32],
+    "sphx_domain_reorder": [_P, _P, _SZ, C.c_int],
     "sphx_find_neighbors": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int],
     "sphx_find_neighbors_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.c_int],
     "sphx_copy_neighbors": [_P, _P, _U32, _U32],
@@ -47,6 +50,10 @@ _SIGS = {
     "sphx_momentum_energy_ve": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.POINTER(C.c_double)],
     "sphx_workspace_set_profiling": [_P, C.c_int],
     "sphx_kernel_ms": [_P, C.c_char_p, C.POINTER(C.c_double)],
+    "sphx_rho_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
+    "sphx_compute_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
+    "sphx_compute_positions": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double],
+    "sphx_update_smoothing_length": [_P, _FP, _PP, _U32, _U32],
     "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_iad_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_momentum_energy_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
@@ -182,6 +189,25 @@ class Workspace:
     def vjp(self, name, box, fields, params, first, last, cot, grad):
         """Adjoint entry point `name`: cotangents `cot`, gradients added into `grad` (abi.Adjoint)."""
         self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last, C.byref(cot), C.byref(grad))
+
+    def domain_reorder(self, tensor, n):
+        """Apply the last domain sync's permutation to one more per-particle tensor."""
+        self.call("sphx_domain_reorder", C.c_void_p(tensor.data_ptr()), n, tensor.element_size())
+
+    def rho_timestep(self, fields, params, first, last, ts):
+        """ts.minDtRho = Krho / |max divv| (sphx_rho_timestep)."""
+        self.call("sphx_rho_timestep", C.byref(fields), C.byref(params), first, last, C.byref(ts))
+
+    def compute_timestep(self, fields, params, first, last, ts):
+        """Advance the time-step state ts (abi.Timestep) as computeTimestep does."""
+        self.call("sphx_compute_timestep", C.byref(fields), C.byref(params), first, last, C.byref(ts))
+
+    def compute_positions(self, box, fields, history, params, first, last, dt, dt_m1):
+        self.call("sphx_compute_positions", C.byref(box), C.byref(fields), C.byref(history), C.byref(params), first,
+                  last, dt, dt_m1)
+
+    def update_smoothing_length(self, fields, params, first, last):
+        self.call("sphx_update_smoothing_length", C.byref(fields), C.byref(params), first, last)
 
     def momentum_energy_ve(self, box, fields, params, first, last, av_clean=False):
         dt = C.c_double()

@@ -66,6 +66,21 @@ class ParticleSet:
     def fields(self, exclude=()):
         return abi.fields_from({k: v for k, v in self.t.items() if k not in exclude})
 
+    def history(self):
+        """abi.History of the set's previous-step tensors (x_m1, y_m1, z_m1, du_m1, mui; absent ones NULL)."""
+        h = abi.History()
+        for k in abi.HISTORY_FIELDS:
+            if k in self.t:
+                setattr(h, k, self.t[k].data_ptr())
+        return h
+
+    def reorder_history(self, ws):
+        """Apply the last domain sync's order to the previous-step tensors, as Domain::sync reorders
+        every conserved field."""
+        for k in abi.HISTORY_FIELDS:
+            if k in self.t:
+                ws.domain_reorder(self.t[k], self.n)
+
     def host(self, name):
         a = self.t[name][: self.n].cpu().numpy()
         if name == "keys":
@@ -92,6 +107,7 @@ def std_step(ws, ps, box, params, bucket=64):
     n = ps.n
     f = ps.fields()
     ws.domain_sync(box, f, params, n, bucket)
+    ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, params, n, rho=True)
     ws.eos("sphx_eos_std", f, params, 0, n)
     ws.pair("sphx_iad_std", box, f, params, 0, n)
@@ -99,15 +115,49 @@ def std_step(ws, ps, box, params, bucket=64):
     return converged, dt
 
 
-def ve_step(ws, ps, box, params, bucket=64, av_clean=False):
-    """One VE pass; returns (converged, minDtCourant)."""
+def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None):
+    """One VE pass; returns (converged, minDtCourant). With a time-step state ts (abi.Timestep) it
+    also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171)."""
     n = ps.n
     f = ps.fields()
     ws.domain_sync(box, f, params, n, bucket)
+    ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, params, n, rho=False)
     ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
     ws.eos("sphx_eos_ve", f, params, 0, n)
     ws.pair("sphx_iad_divv_curlv", box, f, params, 0, n)
+    if ts is not None:
+        ws.rho_timestep(f, params, 0, n, ts)
     ws.pair("sphx_av_switches", box, f, params, 0, n)
     dt = ws.momentum_energy_ve(box, f, params, 0, n, av_clean)
     return converged, dt
+
+
+def integrate(ws, ps, box, params, ts):
+    """The integrate phase of HydroProp / HydroVeProp (main/src/propagator/std_hydro.hpp:170-181,
+    ve_hydro.hpp:206-217): computeTimestep, computePositions with the step and the previous step
+    passed as float like the reference's GPU path, updateSmoothingLength. ts (abi.Timestep) is
+    advanced in place; ts.minDtCourant must hold the last momentum-energy pass's value."""
+    f = ps.fields()
+    ws.compute_timestep(f, params, 0, ps.n, ts)
+    dt, dt_m1 = float(np.float32(ts.minDt)), float(np.float32(ts.minDt_m1))
+    ws.compute_positions(box, f, ps.history(), params, 0, ps.n, dt, dt_m1)
+    ws.update_smoothing_length(f, params, 0, ps.n)
+
+
+def run_steps(ws, ps, box, params, ts, steps, form="std", av_clean=False):
+    """`steps` full time steps (forces, then integrate) of the std or VE propagator on one rank.
+    params.minDt follows ts.minDt (the AV switches read it). Returns the Courant step of each."""
+    out = []
+    for _ in range(steps):
+        params.minDt = ts.minDt
+        if form == "std":
+            conv, dtc = std_step(ws, ps, box, params)
+        else:
+            conv, dtc = ve_step(ws, ps, box, params, av_clean=av_clean, ts=ts)
+        if not conv:
+            raise RuntimeError("coupled nc/h-updated failed to converge")
+        ts.minDtCourant = dtc
+        integrate(ws, ps, box, params, ts)
+        out.append(dtc)
+    return out
