@@ -12,6 +12,7 @@
  */
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -80,6 +81,7 @@ struct DeviceFields
     HydroType *h{}, *m{}, *vx{}, *vy{}, *vz{}, *rho{}, *p{}, *c{}, *prho{}, *tdpdTrho{};
     HydroType *c11{}, *c12{}, *c13{}, *c22{}, *c23{}, *c33{}, *xm{}, *kx{}, *gradh{}, *divv{}, *curlv{}, *alpha{};
     HydroType *ax{}, *ay{}, *az{}, *dV11{}, *dV12{}, *dV13{}, *dV22{}, *dV23{}, *dV33{};
+    HydroType *x_m1{}, *y_m1{}, *z_m1{}, *du_m1{}, *mui{}; // XM1Type = HydroType, mui optional
     uint64_t* keys{};
     uint32_t* nc{};
 };
@@ -99,7 +101,14 @@ struct DeviceDataset
     double   minDt{1e-12}, minDtCourant{INFINITY};
     double   alphamin{0.05}, alphamax{1.0}, decay_constant{0.2};
     double   Atmin{0.1}, Atmax{0.2}, ramp{1.0 / (0.2 - 0.1)};
+    // time stepping (particles_data.hpp:96-141)
+    double                  ttot{0.0}, minDt_m1{1e-12}, minDtRho{INFINITY};
+    double                  g{0.0}, etaAcc{0.2}, eps{0.005}, Krho{0.06};
+    constexpr static double maxDtIncrease = 1.1;
 };
+
+//! block time-step rungs of the reference's integrator (sph/include/sph/timestep.h:19)
+constexpr int maxNumRungs = 4;
 
 namespace detail
 {
@@ -138,9 +147,29 @@ sphx_fields_t fields(Dataset& d)
     return f;
 }
 
+template<class Dataset>
+sphx_history_t history(Dataset& d)
+{
+    auto&          s = d.devData;
+    sphx_history_t h{};
+    h.x_m1 = s.x_m1, h.y_m1 = s.y_m1, h.z_m1 = s.z_m1, h.du_m1 = s.du_m1, h.mui = s.mui;
+    return h;
+}
+
+template<class Dataset>
+sphx_timestep_t timestep(const Dataset& d)
+{
+    sphx_timestep_t t{};
+    t.minDt = d.minDt, t.minDt_m1 = d.minDt_m1, t.ttot = d.ttot, t.minDtCourant = d.minDtCourant;
+    t.minDtRho = d.minDtRho, t.maxDtIncrease = d.maxDtIncrease, t.g = d.g, t.etaAcc = d.etaAcc, t.eps = d.eps;
+    t.Krho = d.Krho;
+    return t;
+}
+
 } // namespace detail
 
-//! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234): keys, sort, permute, tree
+//! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234): keys, sort, permute
+//! every field, including the integrator's previous-step fields, then the tree
 template<class Dataset>
 void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucketSize = 64)
 {
@@ -148,6 +177,16 @@ void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucket
     auto f = detail::fields(d);
     auto p = detail::params(d);
     detail::check(d.workspace, sphx_domain_sync(d.workspace, &b, &f, &p, numParticles, bucketSize));
+    auto& s = d.devData;
+    for (void* extra : {static_cast<void*>(s.x_m1), static_cast<void*>(s.y_m1), static_cast<void*>(s.z_m1),
+                        static_cast<void*>(s.du_m1), static_cast<void*>(s.mui)})
+    {
+        if (extra)
+        {
+            detail::check(d.workspace, sphx_domain_reorder(d.workspace, extra, numParticles,
+                                                           int(sizeof(typename Dataset::HydroT))));
+        }
+    }
 }
 
 //! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56)
@@ -263,6 +302,54 @@ void computeMomentumEnergy(const GroupView& grp, float* groupDt, Dataset& d, con
     detail::check(d.workspace, sphx_momentum_energy_ve(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody,
                                                        avClean ? 1 : 0, &dt));
     d.minDtCourant = dt;
+}
+
+//! sph::rhoTimestep (sph/include/sph/ts_global.hpp:70-93): Krho / |max divv| over [first, last)
+template<class Dataset>
+double rhoTimestep(size_t first, size_t last, Dataset& d)
+{
+    if (d.devData.divv == nullptr) { throw std::runtime_error("Divv needs to be available in rhoTimestep\n"); }
+    auto f  = detail::fields(d);
+    auto p  = detail::params(d);
+    auto ts = detail::timestep(d);
+    detail::check(d.workspace, sphx_rho_timestep(d.workspace, &f, &p, uint32_t(first), uint32_t(last), &ts));
+    return ts.minDtRho;
+}
+
+//! sph::computeTimestep (ts_global.hpp:95-113) on one rank: sets d.minDt, d.minDt_m1, d.ttot
+template<class Dataset>
+void computeTimestep(size_t first, size_t last, Dataset& d)
+{
+    auto f  = detail::fields(d);
+    auto p  = detail::params(d);
+    auto ts = detail::timestep(d);
+    detail::check(d.workspace, sphx_compute_timestep(d.workspace, &f, &p, uint32_t(first), uint32_t(last), &ts));
+    d.minDt = ts.minDt, d.minDt_m1 = ts.minDt_m1, d.ttot = ts.ttot;
+}
+
+//! sph::computePositions (sph/include/sph/positions.hpp:177-200), its GPU branch: positions,
+//! velocities, x_m1..z_m1, and temp or u with du_m1. Block time steps (rung != nullptr) are not
+//! supported: every particle advances by dt_forward from the previous step dt_m1[0].
+template<class Dataset>
+void computePositions(const GroupView& grp, Dataset& d, const Box& box, float dt_forward,
+                      std::array<float, maxNumRungs> dt_m1, const uint8_t* rung = nullptr)
+{
+    if (rung != nullptr) { throw std::runtime_error("block time-step rungs are not supported"); }
+    auto b = box.c();
+    auto f = detail::fields(d);
+    auto h = detail::history(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_compute_positions(d.workspace, &b, &f, &h, &p, grp.firstBody, grp.lastBody,
+                                                      double(dt_forward), double(dt_m1[0])));
+}
+
+//! sph::updateSmoothingLength (sph/include/sph/update_h.hpp:26-35)
+template<class Dataset>
+void updateSmoothingLength(const GroupView& grp, Dataset& d)
+{
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_update_smoothing_length(d.workspace, &f, &p, grp.firstBody, grp.lastBody));
 }
 
 } // namespace sphexa_amd

@@ -3,11 +3,15 @@
 // main/src/propagator/ve_hydro.hpp:130-204), reading the initial conditions from a binary file
 // and writing the SFC-ordered results for comparison with the CPU oracle.
 //
-//   dataset_api <in.bin> <out.bin> std|ve [cpu|gpu]
+//   dataset_api <in.bin> <out.bin> std|ve [cpu|gpu] [steps]
 //
 // "cpu" (the default) calls findNeighborsSfc and then computeDensity / computeXMass, as the
 // reference's CPU build does; "gpu" calls the fused findNeighborsDensity / findNeighborsXMass,
 // which is how the reference's GPU build runs the search (inside computeDensity / computeXMass).
+// With `steps` the driver runs that many full std steps, forces then the integrate phase of
+// HydroProp (main/src/propagator/std_hydro.hpp:170-181: computeTimestep, computePositions,
+// updateSmoothingLength) from minDt = minDt_m1 = 1e-4, and writes instead
+// out.bin: n doubles each of x y z vx vy vz h u, then minDt, minDt_m1, ttot.
 //
 // in.bin : uint64 n, uint32 periodic, double lo, double hi, then n doubles each of
 //          x y z h m u vx vy vz alpha (u is the temperature for the VE pass)
@@ -15,7 +19,9 @@
 //          VE: xm kx gradh prho c c11..c33 divv curlv alpha ax ay az du), n uint32 nc, double minDt
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,9 +42,11 @@ using namespace sphexa_amd;
 
 int main(int argc, char** argv)
 {
-    if (argc != 4 && argc != 5) { return 1; }
+    if (argc < 4 || argc > 6) { return 1; }
     std::string form  = argv[3];
-    bool        fused = argc == 5 && std::string(argv[4]) == "gpu";
+    bool        fused = argc >= 5 && std::string(argv[4]) == "gpu";
+    int         steps = argc == 6 ? std::atoi(argv[5]) : 0;
+    if (steps > 0 && form != "std") { return 1; }
     FILE*       in   = std::fopen(argv[1], "rb");
     if (!in) { return 1; }
     uint64_t n;
@@ -55,7 +63,8 @@ int main(int argc, char** argv)
     DeviceDataset<double> d;
     if (sphx_workspace_create(&d.workspace, 0, nullptr) != SPHX_OK) { return 2; }
     if (sphx_workspace_set_kernel(d.workspace, 6.0, &d.K) != SPHX_OK) { return 2; }
-    d.minDt = 1e-3;
+    d.minDt = steps > 0 ? 1e-4 : 1e-3;
+    d.minDt_m1 = d.minDt;
 
     std::vector<double*> owned;
     auto alloc = [&](double*& p) {
@@ -72,6 +81,8 @@ int main(int argc, char** argv)
         for (double** p : {&s.temp, &s.prho, &s.xm, &s.kx, &s.gradh, &s.divv, &s.curlv, &s.alpha}) alloc(*p);
     else
         for (double** p : {&s.u, &s.rho, &s.p}) alloc(*p);
+    if (steps > 0)
+        for (double** p : {&s.x_m1, &s.y_m1, &s.z_m1, &s.du_m1}) alloc(*p);
     HIPCHECK(hipMalloc(&s.keys, n * 8));
     HIPCHECK(hipMalloc(&s.nc, n * 4));
     double* icDst[10] = {s.x, s.y, s.z, s.h, s.m, ve ? s.temp : s.u, s.vx, s.vy, s.vz, ve ? s.alpha : nullptr};
@@ -82,6 +93,8 @@ int main(int argc, char** argv)
     GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
     try
     {
+      for (int step = 0; step < std::max(steps, 1); ++step)
+      {
         syncDomain(n, d, box);
         if (!fused) { findNeighborsSfc(0, n, d, box); }
         if (ve)
@@ -102,6 +115,13 @@ int main(int argc, char** argv)
             computeIAD(grp, d, box);
             computeMomentumEnergySTD(grp, d, box);
         }
+        if (steps > 0)
+        {
+            computeTimestep(0, n, d);
+            computePositions(grp, d, box, float(d.minDt), {float(d.minDt_m1)});
+            updateSmoothingLength(grp, d);
+        }
+      }
     }
     catch (const std::exception& e)
     {
@@ -109,6 +129,22 @@ int main(int argc, char** argv)
         return 3;
     }
 
+    if (steps > 0)
+    {
+        FILE*               out = std::fopen(argv[2], "wb");
+        std::vector<double> buf(n);
+        for (double* p : {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.h, s.u})
+        {
+            HIPCHECK(hipMemcpy(buf.data(), p, n * 8, hipMemcpyDeviceToHost));
+            std::fwrite(buf.data(), 8, n, out);
+        }
+        for (double v : {d.minDt, d.minDt_m1, d.ttot}) std::fwrite(&v, 8, 1, out);
+        std::fclose(out);
+        for (double* p : owned) (void)hipFree(p);
+        (void)hipFree(s.keys), (void)hipFree(s.nc);
+        sphx_workspace_destroy(d.workspace);
+        return 0;
+    }
     std::vector<double*> outs = ve ? std::vector<double*>{s.x, s.xm, s.kx, s.gradh, s.prho, s.c, s.c11, s.c12,
                                                           s.c13, s.c22, s.c23, s.c33, s.divv, s.curlv, s.alpha,
                                                           s.ax, s.ay, s.az, s.du}

@@ -65,3 +65,40 @@ def test_cpp_interface_pass(tmp_path, orc, form, calls):
         err = np.max(np.abs(vals[i] - ref)) / max(np.max(np.abs(ref)), 1e-300)
         assert err < 1e-10, (name, err)
     assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
+
+
+def test_cpp_interface_steps(tmp_path, orc):
+    """Three full std steps through the C++ interface (syncDomain ... computeMomentumEnergySTD,
+    then computeTimestep, computePositions, updateSmoothingLength) against the oracle's three
+    steps: positions, velocities, h, u and the time-step state."""
+    from test_gpu_integrate import _oracle_steps
+
+    arr, spec = ic.sedov_lattice(16, seed=17)
+    n = arr["x"].size
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([n], np.uint64).tobytes())
+        fh.write(np.array([1], np.uint32).tobytes())
+        fh.write(np.array([spec["lo"], spec["hi"]], np.float64).tobytes())
+        for col in (arr["x"], arr["y"], arr["z"], arr["h"], arr["m"], arr["u"], arr["vx"], arr["vy"], arr["vz"],
+                    np.zeros(n)):
+            fh.write(np.ascontiguousarray(col, np.float64).tobytes())
+    r = subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "std", "cpu", "3"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.frombuffer(open(tmp_path / "out.bin", "rb").read(), np.float64)
+    names = ("x", "y", "z", "vx", "vy", "vz", "h", "u")
+    got = {k: raw[i * n:(i + 1) * n] for i, k in enumerate(names)}
+    minDt, minDt_m1, ttot = raw[len(names) * n:]
+
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
+        arr[k] = np.zeros(n)
+    ts = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
+    a, _ = _oracle_steps(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0)), ts, 3, np.float64)
+    assert abs(minDt - ts.minDt) <= 1e-12 * ts.minDt and abs(minDt_m1 - ts.minDt_m1) <= 1e-12 * ts.minDt_m1
+    assert abs(ttot - ts.ttot) <= 1e-12 * ts.ttot
+    og, oo = np.argsort(got["x"], kind="stable"), np.argsort(a["x"], kind="stable")
+    for k in names:
+        ref = np.asarray(a[k], np.float64)[oo]
+        err = np.max(np.abs(got[k][og] - ref)) / max(np.max(np.abs(ref)), 1e-300)
+        assert err < 1e-10, (k, err)
