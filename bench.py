@@ -90,6 +90,15 @@ def make_ic(side, seed=42):
     return arr, spec
 
 
+# previous-step fields of the integrator, permuted by the domain sync with the others
+HISTORY = ("x_m1", "y_m1", "z_m1", "du_m1")
+
+# HBM bytes per particle of the position / energy update (temperature form, no fixed boundary):
+# read x, y, z (FP64), x_m1.., ax.. (T), temp, du (FP64), du_m1 (T); write x, y, z, x_m1..,
+# vx.. , temp, du_m1
+POSITIONS_BYTES = {"mixed": 24 + 12 + 12 + 8 + 8 + 4 + 24 + 12 + 12 + 8 + 4,
+                   "fp64": 24 + 24 + 24 + 8 + 8 + 8 + 24 + 24 + 24 + 8 + 8}
+
 # main kernels timed with HIP events on the workspace stream (sphx_kernel_ms)
 KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
 
@@ -124,12 +133,12 @@ def run_gpu(arr, spec, mixed, steps, warmup):
 
     from sphexa_amd import abi
     from sphexa_amd.lib import Workspace
-    from sphexa_amd.particles import VE_FIELDS, ParticleSet, search_first_sum
+    from sphexa_amd.particles import VE_FIELDS, ParticleSet, integrate, search_first_sum
 
     ws = Workspace(0)
     box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
     params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-4)
-    ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+    ps = ParticleSet.from_host(arr, VE_FIELDS + HISTORY, mixed=mixed)
     n = ps.n
     f = ps.fields()
     stream = torch.cuda.current_stream()
@@ -141,6 +150,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     def one_step(ev):
         ev[0].record(stream)
         ws.domain_sync(box, f, params, n, 64)
+        ps.reorder_history(ws)
         ev[1].record(stream)
         search_first_sum(ws, box, f, params, n, rho=False)
         ev[2].record(stream)
@@ -156,6 +166,8 @@ def run_gpu(arr, spec, mixed, steps, warmup):
         ev[7].record(stream)
         return dt
 
+    dt_courant = 0.0
+
     def read_kernels():
         # the search and momentum-energy entry points end with a stream synchronisation, so every
         # event of the step has completed and these reads do not stall
@@ -170,7 +182,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     t0 = time.perf_counter()
     for _ in range(steps):
         ev = mk()
-        one_step(ev)
+        dt_courant = one_step(ev)
         read_kernels()
         events.append(ev)
     torch.cuda.synchronize()
@@ -183,6 +195,24 @@ def run_gpu(arr, spec, mixed, steps, warmup):
                fields={k: ps.host(k).astype(np.float64) for k in COMPARED},
                nc=ps.host("nc").copy())
     out["fields"]["rho"] = out["fields"]["kx"] * ps.host("m").astype(np.float64) / out["fields"]["xm"]
+
+    # the integrate phase (HydroVeProp::integrate: rho and global time step, positions and energy,
+    # smoothing lengths), timed apart from the hydro pass that the headline value measures
+    ts = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
+    int_ms, kms = [], {"positions": [], "update_h": []}
+    for it in range(warmup + steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts.minDtCourant = dt_courant
+        e0.record(stream)
+        ws.rho_timestep(f, params, 0, n, ts)
+        integrate(ws, ps, box, params, ts)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if it >= warmup:
+            int_ms.append(e0.elapsed_time(e1))
+            for k in kms:
+                kms[k].append(ws.kernel_ms(k))
+    out["integrate"] = dict(ms=float(np.mean(int_ms)), kernel_ms={k: float(np.mean(v)) for k, v in kms.items()})
     del ps, f
     ws.close()
     torch.cuda.empty_cache()
@@ -277,6 +307,15 @@ def main():
                          "nc_equal_fraction": float(np.mean(res["mixed"]["nc"] == res["fp64"]["nc"]))}
                         if len(res) == 2 else None),
         "neighbor_pairs": head["pairs"],
+        "integrate": {f: dict(ms_per_step=round(r["integrate"]["ms"], 4),
+                              kernel_ms={k: round(v, 4) for k, v in r["integrate"]["kernel_ms"].items()},
+                              positions_bytes_per_particle=POSITIONS_BYTES[f],
+                              positions_gbs=round(POSITIONS_BYTES[f] * r["n"]
+                                                  / (r["integrate"]["kernel_ms"]["positions"] * 1e-3) / 1e9, 1),
+                              positions_hbm_frac=round(POSITIONS_BYTES[f] * r["n"]
+                                                       / (r["integrate"]["kernel_ms"]["positions"] * 1e-3) / 1e9
+                                                       / PEAK_HBM_GBS, 4))
+                      for f, r in res.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -205,7 +205,8 @@ const char* sphx_last_error(const sphx_workspace_t* ws);
  * search and every pair-loop entry point record HIP events on the workspace stream around their
  * main kernel (not around the record packing or host reads). sphx_kernel_ms waits for and returns
  * the duration of the last such launch of `kernel`: "search", "density_std", "iad_std",
- * "momentum_std", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve".
+ * "momentum_std", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve",
+ * "positions", "update_h".
  * Returns SPHX_ERR_STATE when that kernel has not run since profiling was switched on. */
 int sphx_workspace_set_profiling(sphx_workspace_t* ws, int on);
 int sphx_kernel_ms(sphx_workspace_t* ws, const char* kernel, double* ms);

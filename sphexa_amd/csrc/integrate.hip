@@ -147,22 +147,34 @@ __host__ __device__ __forceinline__ double fromOrderedKey(unsigned long long k)
 
 /* Maximum over [first, last) of |a|^2 = (ax^2 + ay^2) + az^2 in T (the reference's GPU reduction,
  * domain/include/cstone/primitives/primitives_gpu.cu:179-196), or of divv (MinMaxGpu, :162-171):
- * wave maximum, then one atomic maximum per wave on the ordered key. */
+ * a grid-stride loop per thread, wave then block maxima through LDS, and one atomic maximum per
+ * block on the ordered key (a few hundred atomics on the one address, not one per wave). */
+constexpr unsigned kMaxBlocks = 1024;
+
 template<class T, bool Norm>
 __global__ void __launch_bounds__(kIntThreads) maxKernel(uint32_t first, uint32_t last, const T* __restrict__ x,
                                                        const T* __restrict__ y, const T* __restrict__ z,
                                                        unsigned long long* out)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
-    double         v = -INFINITY;
-    if (i < last)
+    __shared__ double waveMaxima[kIntThreads / kWave];
+    double            v = -INFINITY;
+    for (uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x; i < last; i += gridDim.x * blockDim.x)
     {
-        if constexpr (Norm) { v = double(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]); }
-        else { v = double(x[i]); }
+        double e;
+        if constexpr (Norm) { e = double(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]); }
+        else { e = double(x[i]); }
+        v = e > v ? e : v;
     }
     v = waveMax(v);
-    if (laneId() == 0) { atomicMax(out, orderedKey(v)); }
+    if (laneId() == 0) { waveMaxima[threadIdx.x / kWave] = v; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (unsigned w = 1; w < kIntThreads / kWave; ++w)
+            v = waveMaxima[w] > v ? waveMaxima[w] : v;
+        atomicMax(out, orderedKey(v));
+    }
 }
 
 template<class T>
@@ -172,7 +184,7 @@ double deviceMax(sphx_workspace* ws, uint32_t first, uint32_t last, const void* 
     ws->counters.reserve(16);
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 4;
     fill64(ws, reinterpret_cast<uint64_t*>(slot), 0, 1); // below the key of every double
-    const unsigned blocks = ceilDiv(last - first, kIntThreads);
+    const unsigned blocks = std::min(ceilDiv(last - first, kIntThreads), kMaxBlocks);
     if (norm)
         maxKernel<T, true><<<blocks, kIntThreads, 0, ws->stream>>>(first, last, static_cast<const T*>(x),
                                                                    static_cast<const T*>(y),
@@ -194,6 +206,7 @@ void launchPositions(sphx_workspace* ws, PosArgs& a, const sphx_fields_t* f, con
     // constCv as the reference's host side forms it (positions.hpp:183): idealGasCv(muiConst, gamma)
     // in the mass type (the HydroType here), -1 when a per-particle mui is given
     a.constCv = hist->mui ? -1.0 : double(idealGasCv(T(p->muiConst), p->gamma));
+    KernelScope timer(ws, "positions");
     positionsKernel<T><<<ceilDiv(a.last - a.first, kIntThreads), kIntThreads, 0, ws->stream>>>(a);
     SPHX_CHECK_LAUNCH();
 }
@@ -262,6 +275,7 @@ void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sph
     SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
     if (first == last) { return; }
     const unsigned blocks = ceilDiv(last - first, kIntThreads);
+    KernelScope    timer(ws, "update_h");
     if (p->hydro_type == SPHX_F32)
         updateHKernel<float><<<blocks, kIntThreads, 0, ws->stream>>>(first, last, p->ng0, f->nc, (float*)f->h);
     else
