@@ -350,13 +350,15 @@ int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sp
 
 /* Press second-order position update and Adams-Bashforth energy update over [first, last):
  * x, y, z (put back into a periodic box), x_m1..z_m1, vx, vy, vz, and temp (when fields.temp is
- * set, through cv of mui or muiConst) or u, with du_m1 = du. dt and dt_m1 are the step and the
- * previous step (the reference passes them as float). Replaces computePositions / the GPU
- * computePositionsKernel (sph/include/sph/positions.hpp:177-200, positions_gpu.cu:116-168).
+ * set) or u, with du_m1 = du. dt and dt_m1 are the step and the previous step (the reference
+ * passes them as float). The temperature update uses cv = idealGasCv(mui_i, gamma) when hist->mui
+ * is set, else constCv, or idealGasCv(prm->muiConst, gamma) formed as the reference's host does
+ * when constCv < 0. Replaces computePositions / computePositionsGpu and its kernel
+ * (sph/include/sph/positions.hpp:177-200, sph_gpu.hpp:78-82, positions_gpu.cu:116-168).
  * fields: x, y, z, vx, vy, vz, ax, ay, az, du, temp or u, h (fixed boundaries only). */
 int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                            const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first, uint32_t last,
-                           double dt, double dt_m1);
+                           double dt, double dt_m1, double constCv);
 
 /* h = updateH(ng0, nc, h) over [first, last). Replaces updateSmoothingLength
  * (sph/include/sph/update_h.hpp:26-35). fields: h, nc. */

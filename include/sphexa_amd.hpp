@@ -340,7 +340,7 @@ void computePositions(const GroupView& grp, Dataset& d, const Box& box, float dt
     auto h = detail::history(d);
     auto p = detail::params(d);
     detail::check(d.workspace, sphx_compute_positions(d.workspace, &b, &f, &h, &p, grp.firstBody, grp.lastBody,
-                                                      double(dt_forward), double(dt_m1[0])));
+                                                      double(dt_forward), double(dt_m1[0]), -1.0));
 }
 
 //! sph::updateSmoothingLength (sph/include/sph/update_h.hpp:26-35)

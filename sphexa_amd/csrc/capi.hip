@@ -429,11 +429,11 @@ int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sp
 
 int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                            const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first, uint32_t last,
-                           double dt, double dt_m1)
+                           double dt, double dt_m1, double constCv)
 {
     return guarded(ws, [&] {
         SPHX_REQUIRE(box && f && hist && prm, SPHX_ERR_ARG, "null argument");
-        computePositions(ws, box, f, hist, prm, first, last, dt, dt_m1);
+        computePositions(ws, box, f, hist, prm, first, last, dt, dt_m1, constCv);
         return SPHX_OK;
     });
 }

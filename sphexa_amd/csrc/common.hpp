@@ -415,7 +415,7 @@ void rhoTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t
 void computeTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                      uint32_t last, sphx_timestep_t* ts);
 void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
-                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1);
+                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv);
 void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                            uint32_t last);
 void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,

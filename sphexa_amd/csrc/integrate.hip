@@ -201,11 +201,12 @@ double deviceMax(sphx_workspace* ws, uint32_t first, uint32_t last, const void* 
 
 template<class T>
 void launchPositions(sphx_workspace* ws, PosArgs& a, const sphx_fields_t* f, const sphx_history_t* hist,
-                     const sphx_params_t* p)
+                     const sphx_params_t* p, double constCv)
 {
-    // constCv as the reference's host side forms it (positions.hpp:183): idealGasCv(muiConst, gamma)
-    // in the mass type (the HydroType here), -1 when a per-particle mui is given
-    a.constCv = hist->mui ? -1.0 : double(idealGasCv(T(p->muiConst), p->gamma));
+    // constCv: -1 selects the per-particle idealGasCv(mui, gamma); without mui the caller's value,
+    // or the reference host side's idealGasCv(muiConst, gamma) in the mass type (positions.hpp:183)
+    if (hist->mui) { a.constCv = -1.0; }
+    else { a.constCv = constCv >= 0 ? constCv : double(idealGasCv(T(p->muiConst), p->gamma)); }
     KernelScope timer(ws, "positions");
     positionsKernel<T><<<ceilDiv(a.last - a.first, kIntThreads), kIntThreads, 0, ws->stream>>>(a);
     SPHX_CHECK_LAUNCH();
@@ -242,7 +243,7 @@ void computeTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_para
 }
 
 void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
-                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1)
+                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv)
 {
     SPHX_REQUIRE(f->x && f->y && f->z && f->vx && f->vy && f->vz && f->ax && f->ay && f->az, SPHX_ERR_ARG,
                  "positions need x, y, z, vx, vy, vz, ax, ay, az");
@@ -264,8 +265,8 @@ void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fiel
     a.x = f->x, a.y = f->y, a.z = f->z, a.temp = f->temp, a.u = f->temp ? nullptr : f->u, a.du = f->du;
     a.vx = f->vx, a.vy = f->vy, a.vz = f->vz, a.ax = f->ax, a.ay = f->ay, a.az = f->az, a.h = f->h;
     a.x_m1 = hist->x_m1, a.y_m1 = hist->y_m1, a.z_m1 = hist->z_m1, a.du_m1 = hist->du_m1, a.mui = hist->mui;
-    if (p->hydro_type == SPHX_F32) { launchPositions<float>(ws, a, f, hist, p); }
-    else { launchPositions<double>(ws, a, f, hist, p); }
+    if (p->hydro_type == SPHX_F32) { launchPositions<float>(ws, a, f, hist, p, constCv); }
+    else { launchPositions<double>(ws, a, f, hist, p, constCv); }
 }
 
 void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,

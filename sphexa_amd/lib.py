@@ -52,7 +52,7 @@ _SIGS = {
     "sphx_kernel_ms": [_P, C.c_char_p, C.POINTER(C.c_double)],
     "sphx_rho_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
     "sphx_compute_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
-    "sphx_compute_positions": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double],
+    "sphx_compute_positions": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double, C.c_double],
     "sphx_update_smoothing_length": [_P, _FP, _PP, _U32, _U32],
     "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_iad_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
@@ -202,9 +202,10 @@ class Workspace:
         """Advance the time-step state ts (abi.Timestep) as computeTimestep does."""
         self.call("sphx_compute_timestep", C.byref(fields), C.byref(params), first, last, C.byref(ts))
 
-    def compute_positions(self, box, fields, history, params, first, last, dt, dt_m1):
+    def compute_positions(self, box, fields, history, params, first, last, dt, dt_m1, const_cv=-1.0):
+        """Press positions and Adams-Bashforth energy; const_cv < 0: cv from params.muiConst."""
         self.call("sphx_compute_positions", C.byref(box), C.byref(fields), C.byref(history), C.byref(params), first,
-                  last, dt, dt_m1)
+                  last, dt, dt_m1, const_cv)
 
     def update_smoothing_length(self, fields, params, first, last):
         self.call("sphx_update_smoothing_length", C.byref(fields), C.byref(params), first, last)

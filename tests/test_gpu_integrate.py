@@ -192,3 +192,30 @@ def test_three_std_steps_match_oracle(ws, orc, mixed):
         err = float(np.max(np.abs(g - o))) / s
         print(f"  {k:3s} max |gpu - oracle| / scale = {err:.2e}", flush=True)
         assert err <= tol, (k, err)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_positions_explicit_const_cv(ws, mixed):
+    """constCv passed in (the reference's computePositionsGpu argument) equal to the value formed
+    from muiConst gives the same update bit for bit."""
+    import torch
+
+    rng = np.random.default_rng(14)
+    n = 5000
+    hydro = np.float32 if mixed else np.float64
+    d = _positions_case(rng, n, hydro, abi.PERIODIC, "temp")
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC)
+    params = abi.Params.defaults(1.0, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
+    # idealGasCv(muiConst, gamma) with R and the result in the mass type (eos.hpp:18-23)
+    cv = float(hydro(hydro(8.317e7) / hydro(10.0) / (5.0 / 3.0 - 1.0)))
+    out = []
+    for const_cv in (-1.0, cv):
+        t = to_device(d, mixed)
+        hs = abi.History()
+        for k in abi.HISTORY_FIELDS:
+            if k in t:
+                setattr(hs, k, t[k].data_ptr())
+        ws.compute_positions(box, abi.fields_from(t), hs, params, 0, n, 0.01, 0.012, const_cv)
+        torch.cuda.synchronize()
+        out.append(t["temp"].cpu().numpy())
+    assert np.array_equal(out[0], out[1])
