@@ -365,6 +365,24 @@ int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sp
 int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
                                  uint32_t first, uint32_t last);
 
+/* ---- observables -------------------------------------------------------------------------------
+ * Totals over [first, last) of one rank: eKin = 1/2 sum m |v|^2, eInt = sum m u, or
+ * cv sum m temp when fields.temp is set (cv as given, or idealGasCv(muiConst, gamma) formed as the
+ * reference's host does when cv < 0), linmom = sum m v,
+ * angmom = sum m (x cross v), ncsum = sum nc (0 when fields.nc is NULL), all in FP64. Replaces
+ * computeConservedQuantities / conservedQuantitiesGpu (main/src/observables/
+ * conserved_quantities.hpp:50-177, conserved_gpu.cu:50-95) before its MPI reduction. The sums run
+ * in a fixed order (per-block partials, then one block), so repeated calls agree bit for bit. */
+typedef struct sphx_conserved
+{
+    double eKin, eInt;
+    double linmom[3], angmom[3];
+    double ncsum;
+} sphx_conserved_t;
+
+int sphx_conserved_quantities(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                              uint32_t first, uint32_t last, double cv, sphx_conserved_t* out);
+
 /* ---- adjoints (no counterpart in the reference) ---------------------------------------------
  * Vector-Jacobian products of the std density, IAD and momentum-energy passes with respect to
  * positions, smoothing lengths and a per-particle AV coefficient, for gradient-based calibration

@@ -352,4 +352,18 @@ void updateSmoothingLength(const GroupView& grp, Dataset& d)
     detail::check(d.workspace, sphx_update_smoothing_length(d.workspace, &f, &p, grp.firstBody, grp.lastBody));
 }
 
+//! sphexa::computeConservedQuantities (main/src/observables/conserved_quantities.hpp:117-177) on
+//! one rank: {eKin, eInt, egrav, linmom[3], angmom[3], sum nc}, the reference's quantities array
+template<class Dataset>
+std::array<double, 10> computeConservedQuantities(size_t startIndex, size_t endIndex, Dataset& d, double egrav = 0)
+{
+    auto             f = detail::fields(d);
+    auto             p = detail::params(d);
+    sphx_conserved_t c{};
+    detail::check(d.workspace,
+                  sphx_conserved_quantities(d.workspace, &f, &p, uint32_t(startIndex), uint32_t(endIndex), -1.0, &c));
+    return {c.eKin, c.eInt, egrav, c.linmom[0], c.linmom[1], c.linmom[2], c.angmom[0], c.angmom[1], c.angmom[2],
+            c.ncsum};
+}
+
 } // namespace sphexa_amd

@@ -60,6 +60,7 @@ _SIGS = {
     "update_smoothing_length": (None, [_FP, _PP, _U32, _U32]),
     "compute_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
     "rho_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
+    "conserved_quantities": (None, [_FP, _PP, _U32, _U32, _P]),
     "density_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "iad_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "momentum_energy_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),

@@ -616,4 +616,22 @@ void orc_rho_timestep(const sphx_fields_t* f, const sphx_params_t* p, uint32_t f
     ts->minDtRho = ts->Krho / std::abs(m);
 }
 
+//! conserved quantities over [first, last): out = eKin, eInt, linmom[3], angmom[3], ncsum
+void orc_conserved_quantities(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                              double* out)
+{
+    if (p->hydro_type == SPHX_F32)
+    {
+        double cv = double(idealGasCv(float(p->muiConst), p->gamma));
+        conservedQuantities(f->x, f->y, f->z, (const float*)f->vx, (const float*)f->vy, (const float*)f->vz,
+                            (const float*)f->m, f->u, f->temp, cv, f->nc, first, last, out);
+    }
+    else
+    {
+        double cv = idealGasCv(p->muiConst, p->gamma);
+        conservedQuantities(f->x, f->y, f->z, (const double*)f->vx, (const double*)f->vy, (const double*)f->vz,
+                            (const double*)f->m, f->u, f->temp, cv, f->nc, first, last, out);
+    }
+}
+
 } // extern "C"

@@ -1517,4 +1517,36 @@ inline void computeTimestep(double maxAccSq, double& minDt, double& minDt_m1, do
     minDt    = dt;
 }
 
+/* ------------------------------------------------------------------------------------------------
+ * Conserved quantities (main/src/observables/conserved_quantities.hpp:50-107), with the GPU path's
+ * shared cv for temperature (conserved_gpu.cu:71-95): eKin = 1/2 sum m |v|^2, eInt, linmom, angmom.
+ * ----------------------------------------------------------------------------------------------*/
+template<class T>
+inline void conservedQuantities(const double* x, const double* y, const double* z, const T* vx, const T* vy,
+                                const T* vz, const T* m, const double* u, const double* temp, double cv,
+                                const uint32_t* nc, size_t first, size_t last, double out[9])
+{
+    double eKin = 0, eInt = 0, lin[3] = {0, 0, 0}, ang[3] = {0, 0, 0}, ncsum = 0;
+    for (size_t i = first; i < last; ++i)
+    {
+        double X[3] = {x[i], y[i], z[i]};
+        double V[3] = {double(vx[i]), double(vy[i]), double(vz[i])};
+        double mi   = double(m[i]);
+        eKin += mi * (V[0] * V[0] + (V[1] * V[1] + V[2] * V[2]));
+        for (int d = 0; d < 3; ++d)
+            lin[d] += mi * V[d];
+        ang[0] += mi * (X[1] * V[2] - X[2] * V[1]);
+        ang[1] += mi * (X[2] * V[0] - X[0] * V[2]);
+        ang[2] += mi * (X[0] * V[1] - X[1] * V[0]);
+        if (temp) { eInt += mi * temp[i]; }
+        else if (u) { eInt += mi * u[i]; }
+        if (nc) { ncsum += double(nc[i]); }
+    }
+    out[0] = 0.5 * eKin;
+    out[1] = temp ? cv * eInt : eInt;
+    for (int d = 0; d < 3; ++d)
+        out[2 + d] = lin[d], out[5 + d] = ang[d];
+    out[8] = ncsum;
+}
+
 } // namespace orc

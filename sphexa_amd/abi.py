@@ -132,3 +132,13 @@ def fields_from(arrays):
         addr = a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
         setattr(f, name, addr)
     return f
+
+
+class Conserved(C.Structure):
+    """sphx_conserved_t: totals of sphx_conserved_quantities."""
+    _fields_ = [("eKin", C.c_double), ("eInt", C.c_double), ("linmom", C.c_double * 3), ("angmom", C.c_double * 3),
+                ("ncsum", C.c_double)]
+
+    def as_dict(self):
+        return {"eKin": self.eKin, "eInt": self.eInt, "linmom": list(self.linmom), "angmom": list(self.angmom),
+                "ncsum": self.ncsum}

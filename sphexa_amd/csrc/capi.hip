@@ -448,6 +448,16 @@ int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, c
     });
 }
 
+int sphx_conserved_quantities(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                              uint32_t first, uint32_t last, double cv, sphx_conserved_t* out)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm && out, SPHX_ERR_ARG, "null argument");
+        conservedQuantities(ws, f, prm, first, last, cv, out);
+        return SPHX_OK;
+    });
+}
+
 int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                              const sphx_params_t* prm, uint32_t first, uint32_t last, double* minDtCourant)
 {

@@ -418,6 +418,8 @@ void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fiel
                       const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv);
 void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                            uint32_t last);
+void conservedQuantities(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                         uint32_t last, double cv, sphx_conserved_t* out);
 void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
                   uint32_t first, uint32_t last);
 void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,

@@ -54,6 +54,7 @@ _SIGS = {
     "sphx_compute_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
     "sphx_compute_positions": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double, C.c_double],
     "sphx_update_smoothing_length": [_P, _FP, _PP, _U32, _U32],
+    "sphx_conserved_quantities": [_P, _FP, _PP, _U32, _U32, C.c_double, C.POINTER(abi.Conserved)],
     "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_iad_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
     "sphx_momentum_energy_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
@@ -206,6 +207,13 @@ class Workspace:
         """Press positions and Adams-Bashforth energy; const_cv < 0: cv from params.muiConst."""
         self.call("sphx_compute_positions", C.byref(box), C.byref(fields), C.byref(history), C.byref(params), first,
                   last, dt, dt_m1, const_cv)
+
+    def conserved_quantities(self, fields, params, first, last, cv=-1.0):
+        """Energies and momenta over [first, last) (sphx_conserved_quantities) as a dict; cv < 0: from
+        params.muiConst."""
+        out = abi.Conserved()
+        self.call("sphx_conserved_quantities", C.byref(fields), C.byref(params), first, last, cv, C.byref(out))
+        return out.as_dict()
 
     def update_smoothing_length(self, fields, params, first, last):
         self.call("sphx_update_smoothing_length", C.byref(fields), C.byref(params), first, last)

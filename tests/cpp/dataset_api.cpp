@@ -11,7 +11,8 @@
 // With `steps` the driver runs that many full std steps, forces then the integrate phase of
 // HydroProp (main/src/propagator/std_hydro.hpp:170-181: computeTimestep, computePositions,
 // updateSmoothingLength) from minDt = minDt_m1 = 1e-4, and writes instead
-// out.bin: n doubles each of x y z vx vy vz h u, then minDt, minDt_m1, ttot.
+// out.bin: n doubles each of x y z vx vy vz h u, then minDt, minDt_m1, ttot, and the 10 conserved
+// quantities of computeConservedQuantities.
 //
 // in.bin : uint64 n, uint32 periodic, double lo, double hi, then n doubles each of
 //          x y z h m u vx vy vz alpha (u is the temperature for the VE pass)
@@ -139,6 +140,8 @@ int main(int argc, char** argv)
             std::fwrite(buf.data(), 8, n, out);
         }
         for (double v : {d.minDt, d.minDt_m1, d.ttot}) std::fwrite(&v, 8, 1, out);
+        auto q = computeConservedQuantities(0, n, d);
+        std::fwrite(q.data(), 8, q.size(), out);
         std::fclose(out);
         for (double* p : owned) (void)hipFree(p);
         (void)hipFree(s.keys), (void)hipFree(s.nc);

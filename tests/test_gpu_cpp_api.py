@@ -1,5 +1,6 @@
 """The C++ host interface (include/sphexa_amd.hpp) driven like the reference propagators
 (tests/cpp/dataset_api.cpp) reproduces the CPU oracle's pass."""
+import ctypes as C
 import os
 import subprocess
 
@@ -88,7 +89,8 @@ def test_cpp_interface_steps(tmp_path, orc):
     raw = np.frombuffer(open(tmp_path / "out.bin", "rb").read(), np.float64)
     names = ("x", "y", "z", "vx", "vy", "vz", "h", "u")
     got = {k: raw[i * n:(i + 1) * n] for i, k in enumerate(names)}
-    minDt, minDt_m1, ttot = raw[len(names) * n:]
+    minDt, minDt_m1, ttot = raw[len(names) * n:len(names) * n + 3]
+    q = raw[len(names) * n + 3:]
 
     box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
     for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
@@ -97,6 +99,11 @@ def test_cpp_interface_steps(tmp_path, orc):
     a, _ = _oracle_steps(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0)), ts, 3, np.float64)
     assert abs(minDt - ts.minDt) <= 1e-12 * ts.minDt and abs(minDt_m1 - ts.minDt_m1) <= 1e-12 * ts.minDt_m1
     assert abs(ttot - ts.ttot) <= 1e-12 * ts.ttot
+    # the conserved quantities of the final state against the oracle's on its own final state
+    oq = np.zeros(9)
+    orc.conserved_quantities(C.byref(abi.fields_from(a)), C.byref(abi.Params.defaults(1.0)), 0, n, oq.ctypes.data)
+    np.testing.assert_allclose([q[0], q[1]], oq[:2], rtol=1e-10)
+    assert q[2] == 0.0 and q[9] == oq[8]
     og, oo = np.argsort(got["x"], kind="stable"), np.argsort(a["x"], kind="stable")
     for k in names:
         ref = np.asarray(a[k], np.float64)[oo]
