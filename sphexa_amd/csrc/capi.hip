@@ -288,9 +288,7 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         DevBuf<uint32_t>& order = ws->order;
         sortKeys(ws, keys, order.get(), n);
 
-        // permute every allocated field through one staging buffer
-        ws->fieldTmp.reserve(n * 8);
-        void*                              tmp = ws->fieldTmp.get();
+        // permute every allocated field, batched per element size (the permutation read once per batch)
         std::vector<std::pair<void*, int>> fields;
         for (double* r : {f->x, f->y, f->z, f->u, f->temp, f->du})
             if (r) fields.push_back({r, 8});
@@ -302,11 +300,7 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         for (void* h : hyd)
             if (h) fields.push_back({h, hb});
         if (f->nc) fields.push_back({f->nc, 4});
-        for (auto& [ptr, eb] : fields)
-        {
-            gather(ws, order.get(), ptr, tmp, n, eb);
-            SPHX_HIP(hipMemcpyAsync(ptr, tmp, n * size_t(eb), hipMemcpyDeviceToDevice, ws->stream));
-        }
+        permuteFields(ws, order.get(), fields, n);
         buildTree(ws, keys, n, bucketSize, b);
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         own.release();

@@ -381,6 +381,9 @@ void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const d
                  size_t n);
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
 void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
+//! permute every field in place by `order` (4- or 8-byte elements), batched per element size
+void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<std::pair<void*, int>>& fields,
+                   size_t n);
 void buildTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);
 
 //! what findNeighbors computes besides the lists: nothing, or the fused XMass sum as xm or as rho

@@ -1,4 +1,5 @@
 // Hilbert keys, stable key sort and field permutation (the one-rank Domain::sync front end).
+#include <vector>
 #include <cstring> // rocprim uses memset without including it
 #include <rocprim/rocprim.hpp>
 
@@ -120,6 +121,76 @@ void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out
     else
         throw Error{SPHX_ERR_ARG, "gather: element size must be 4 or 8 bytes"};
     SPHX_CHECK_LAUNCH();
+}
+
+//! up to kGatherBatch fields of one element size, permuted or copied in one launch
+constexpr int kGatherBatch = 16;
+
+template<class E>
+struct FieldBatch
+{
+    const E* in[kGatherBatch];
+    E*       out[kGatherBatch];
+    int      count;
+};
+
+//! out_f[i] = in_f[order[i]] for every field f of the batch: the permutation is read once
+template<class E>
+__global__ void __launch_bounds__(256) gatherBatchKernel(const uint32_t* __restrict__ order, FieldBatch<E> b, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    const uint32_t o = order[i];
+    for (int f = 0; f < b.count; ++f)
+        b.out[f][i] = b.in[f][o];
+}
+
+template<class E>
+__global__ void __launch_bounds__(256) copyBatchKernel(FieldBatch<E> b, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) { return; }
+    for (int f = 0; f < b.count; ++f)
+        b.out[f][i] = b.in[f][i];
+}
+
+template<class E>
+void permuteBatches(sphx_workspace* ws, const uint32_t* order, const std::vector<void*>& fields, size_t n)
+{
+    for (size_t f0 = 0; f0 < fields.size(); f0 += kGatherBatch)
+    {
+        const int     count = int(std::min(fields.size() - f0, size_t(kGatherBatch)));
+        FieldBatch<E> g{}, c{};
+        g.count = c.count = count;
+        E* tmp           = reinterpret_cast<E*>(ws->fieldTmp.get());
+        for (int f = 0; f < count; ++f)
+        {
+            g.in[f]  = static_cast<const E*>(fields[f0 + f]);
+            g.out[f] = tmp + size_t(f) * n;
+            c.in[f]  = tmp + size_t(f) * n;
+            c.out[f] = static_cast<E*>(fields[f0 + f]);
+        }
+        gatherBatchKernel<E><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(order, g, n);
+        SPHX_CHECK_LAUNCH();
+        copyBatchKernel<E><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(c, n);
+        SPHX_CHECK_LAUNCH();
+    }
+}
+
+void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<std::pair<void*, int>>& fields,
+                   size_t n)
+{
+    if (n == 0) { return; }
+    std::vector<void*> f8, f4;
+    for (auto& [ptr, eb] : fields)
+    {
+        if (eb == 8) { f8.push_back(ptr); }
+        else if (eb == 4) { f4.push_back(ptr); }
+        else { throw Error{SPHX_ERR_ARG, "permute: element size must be 4 or 8 bytes"}; }
+    }
+    ws->fieldTmp.reserve(n * 8 * kGatherBatch);
+    permuteBatches<uint64_t>(ws, order, f8, n);
+    permuteBatches<uint32_t>(ws, order, f4, n);
 }
 
 } // namespace sphx
