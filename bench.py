@@ -55,7 +55,7 @@ MEASURED_TRAFFIC_MOM_VE = {(200, "mixed"): 5.648e9, (200, "fp64"): 7.290e9}
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[1])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--side", type=int, default=200, help="lattice side (N = side^3)")
     ap.add_argument("--cpu-side", type=int, default=100, help="lattice side of the CPU-baseline sample")
@@ -99,8 +99,11 @@ HISTORY = ("x_m1", "y_m1", "z_m1", "du_m1")
 POSITIONS_BYTES = {"mixed": 24 + 12 + 12 + 8 + 8 + 4 + 24 + 12 + 12 + 8 + 4,
                    "fp64": 24 + 24 + 24 + 8 + 8 + 8 + 24 + 24 + 24 + 8 + 8}
 
-# main kernels timed with HIP events on the workspace stream (sphx_kernel_ms)
-KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
+# main kernels timed with HIP events on the workspace stream (sphx_kernel_ms), and the stages of
+# the domain sync
+KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve", "sfc_keys", "sort",
+           "permute", "tree")
+PAIR_KERNELS = ("xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve")
 
 # outputs of the VE pass compared between the mixed and FP64 forms (rho = kx m / xm added);
 # positive quantities pointwise, signed ones relative to the field's largest magnitude
@@ -141,6 +144,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     ps = ParticleSet.from_host(arr, VE_FIELDS + HISTORY, mixed=mixed)
     n = ps.n
     f = ps.fields()
+    sf = ps.sync_fields()  # the sync permutes the conserved fields only, as Domain::sync does
     stream = torch.cuda.current_stream()
 
     ws.set_profiling(True)
@@ -149,7 +153,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
 
     def one_step(ev):
         ev[0].record(stream)
-        ws.domain_sync(box, f, params, n, 64)
+        ws.domain_sync(box, sf, params, n, 64)
         ps.reorder_history(ws)
         ev[1].record(stream)
         search_first_sum(ws, box, f, params, n, rho=False)
@@ -188,8 +192,10 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     phase_ms = {p: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in events])) for i, p in enumerate(PHASES)}
+    step_ms_median = float(np.median([e[0].elapsed_time(e[-1]) for e in events]))
     pairs, maxnc = ws.neighbor_stats()
-    out = dict(n=n, wall_s=wall, ms_per_step=1e3 * wall / steps, phase_ms=phase_ms, pairs=int(pairs),
+    out = dict(n=n, wall_s=wall, ms_per_step=1e3 * wall / steps, step_ms_median=step_ms_median, phase_ms=phase_ms,
+               pairs=int(pairs),
                max_nc=int(maxnc), rate=n * steps / wall,
                kernel_ms={k: float(np.mean(v)) for k, v in kernel_ms.items()},
                fields={k: ps.host(k).astype(np.float64) for k in COMPARED},
@@ -300,6 +306,11 @@ def main():
                    "bucket_size": 64, "ngmax": 150},
         "fp64_value": round(res["fp64"]["rate"], 1) if "fp64" in res else None,
         "mixed_over_fp64": round(res["mixed"]["rate"] / res["fp64"]["rate"], 3) if len(res) == 2 else None,
+        # the device time of one pass from its first to its last event, median over the timed passes
+        "step_ms_median": {f: round(r["step_ms_median"], 4) for f, r in res.items()},
+        # particle-updates/s of the pair loops alone (XMass to momentum-energy kernel times)
+        "pair_loop_rate": {f: round(r["n"] / (1e-3 * sum(r["kernel_ms"][k] for k in PAIR_KERNELS)), 1)
+                           for f, r in res.items()},
         "phase_ms": {f: {k: round(v, 4) for k, v in r["phase_ms"].items()} for f, r in res.items()},
         "kernel_ms": {f: {k: round(v, 4) for k, v in r["kernel_ms"].items()} for f, r in res.items()},
         "mixed_error": ({"reference": "fp64 form of this pass", "fields": field_errors(res["mixed"]["fields"],

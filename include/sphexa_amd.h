@@ -206,7 +206,8 @@ const char* sphx_last_error(const sphx_workspace_t* ws);
  * main kernel (not around the record packing or host reads). sphx_kernel_ms waits for and returns
  * the duration of the last such launch of `kernel`: "search", "density_std", "iad_std",
  * "momentum_std", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve",
- * "positions", "update_h".
+ * "positions", "update_h"; and the stages of sphx_domain_sync: "sfc_keys", "sort", "permute",
+ * "tree" (the last with its host synchronisations).
  * Returns SPHX_ERR_STATE when that kernel has not run since profiling was switched on. */
 int sphx_workspace_set_profiling(sphx_workspace_t* ws, int on);
 int sphx_kernel_ms(sphx_workspace_t* ws, const char* kernel, double* ms);
@@ -238,16 +239,21 @@ int sphx_build_tree(sphx_workspace_t* ws, const uint64_t* sortedKeys, size_t n, 
 /* Device view of the tree built by the last sphx_build_tree / sphx_domain_sync. */
 int sphx_tree_view(const sphx_workspace_t* ws, sphx_tree_view_t* out);
 
-/* keys -> stable sort -> permute every non-NULL field of `f` (conserved and dependent) -> tree.
- * One-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234). */
+/* keys -> stable sort -> permute every non-NULL field of `f` -> tree. One-rank Domain::sync
+ * (domain/include/cstone/domain/domain.hpp:182-234). As there, only the conserved fields need to be
+ * passed (std: x y z h m u vx vy vz, HydroProp::ConservedFields, main/src/propagator/std_hydro.hpp:73;
+ * VE: x y z h m temp vx vy vz alpha, ve_hydro.hpp:72): a pass recomputes every dependent field
+ * before it reads it, so leaving those NULL here saves their permutation. keys, if given, receives
+ * the sorted SFC keys. */
 int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t* f, const sphx_params_t* prm,
                      size_t n, uint32_t bucketSize);
 
-/* Reorder one more per-particle array (elemBytes = 4 or 8) by the permutation of the last
- * sphx_domain_sync over the same n. For conserved fields outside sphx_fields_t (x_m1, y_m1, z_m1,
- * du_m1, mui, ...), which Domain::sync reorders together with the others
- * (domain/include/cstone/domain/domain.hpp:182-234). */
-int sphx_domain_reorder(sphx_workspace_t* ws, void* field, size_t n, int elemBytes);
+/* Reorder `count` more per-particle arrays (fields[k] of elemBytes[k] = 4 or 8 bytes each) by the
+ * permutation of the last sphx_domain_sync over the same n, in batches that read the permutation
+ * once. For conserved fields outside sphx_fields_t (x_m1, y_m1, z_m1, du_m1, mui, ...), which
+ * Domain::sync reorders together with the others (domain/include/cstone/domain/domain.hpp:182-234). */
+int sphx_domain_reorder(sphx_workspace_t* ws, void* const* fields, const int* elemBytes, size_t count,
+                        size_t n);
 
 /* ---- neighbour search ---------------------------------------------------------------------- */
 

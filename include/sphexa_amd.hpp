@@ -168,25 +168,29 @@ sphx_timestep_t timestep(const Dataset& d)
 
 } // namespace detail
 
-//! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234): keys, sort, permute
-//! every field, including the integrator's previous-step fields, then the tree
+//! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234) as the propagators call
+//! it (main/src/propagator/std_hydro.hpp:108-122, ve_hydro.hpp:114-128): keys, sort, permute the
+//! conserved fields (x y z h m u temp vx vy vz alpha and the integrator's previous-step fields),
+//! then the tree. The dependent fields keep their order; every pass rewrites them before reading.
 template<class Dataset>
 void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucketSize = 64)
 {
-    auto b = box.c();
-    auto f = detail::fields(d);
+    auto          b = box.c();
+    auto&         s = d.devData;
+    sphx_fields_t f{};
+    f.x = s.x, f.y = s.y, f.z = s.z, f.h = s.h, f.m = s.m, f.u = s.u, f.temp = s.temp;
+    f.vx = s.vx, f.vy = s.vy, f.vz = s.vz, f.alpha = s.alpha, f.keys = s.keys;
     auto p = detail::params(d);
     detail::check(d.workspace, sphx_domain_sync(d.workspace, &b, &f, &p, numParticles, bucketSize));
-    auto& s = d.devData;
-    for (void* extra : {static_cast<void*>(s.x_m1), static_cast<void*>(s.y_m1), static_cast<void*>(s.z_m1),
-                        static_cast<void*>(s.du_m1), static_cast<void*>(s.mui)})
+    void* extra[5];
+    int   bytes[5];
+    size_t count = 0;
+    for (void* e : {static_cast<void*>(s.x_m1), static_cast<void*>(s.y_m1), static_cast<void*>(s.z_m1),
+                    static_cast<void*>(s.du_m1), static_cast<void*>(s.mui)})
     {
-        if (extra)
-        {
-            detail::check(d.workspace, sphx_domain_reorder(d.workspace, extra, numParticles,
-                                                           int(sizeof(typename Dataset::HydroT))));
-        }
+        if (e) { extra[count] = e, bytes[count++] = int(sizeof(typename Dataset::HydroT)); }
     }
+    detail::check(d.workspace, sphx_domain_reorder(d.workspace, extra, bytes, count, numParticles));
 }
 
 //! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56)

@@ -282,11 +282,17 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
             own.reserve(n);
             keys = own.get();
         }
-        fill64(ws, keys, 0, n);
-        computeKeys(ws, b, f->x, f->y, f->z, keys, n);
+        {
+            KernelScope timer(ws, "sfc_keys");
+            fill64(ws, keys, 0, n);
+            computeKeys(ws, b, f->x, f->y, f->z, keys, n);
+        }
         ws->order.reserve(n);
         DevBuf<uint32_t>& order = ws->order;
-        sortKeys(ws, keys, order.get(), n);
+        {
+            KernelScope timer(ws, "sort");
+            sortKeys(ws, keys, order.get(), n);
+        }
 
         // permute every allocated field, batched per element size (the permutation read once per batch)
         std::vector<std::pair<void*, int>> fields;
@@ -300,8 +306,14 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         for (void* h : hyd)
             if (h) fields.push_back({h, hb});
         if (f->nc) fields.push_back({f->nc, 4});
-        permuteFields(ws, order.get(), fields, n);
-        buildTree(ws, keys, n, bucketSize, b);
+        {
+            KernelScope timer(ws, "permute");
+            permuteFields(ws, order.get(), fields, n);
+        }
+        {
+            KernelScope timer(ws, "tree");
+            buildTree(ws, keys, n, bucketSize, b);
+        }
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         own.release();
         ws->syncN = n;
@@ -309,17 +321,19 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
     });
 }
 
-int sphx_domain_reorder(sphx_workspace_t* ws, void* field, size_t n, int elemBytes)
+int sphx_domain_reorder(sphx_workspace_t* ws, void* const* fields, const int* elemBytes, size_t count, size_t n)
 {
     return guarded(ws, [&] {
-        SPHX_REQUIRE(field || n == 0, SPHX_ERR_ARG, "null argument");
-        SPHX_REQUIRE(elemBytes == 4 || elemBytes == 8, SPHX_ERR_ARG, "element size must be 4 or 8 bytes");
+        SPHX_REQUIRE(count == 0 || (fields && elemBytes), SPHX_ERR_ARG, "null argument");
         SPHX_REQUIRE(n == ws->syncN, SPHX_ERR_STATE, "no sphx_domain_sync over the same particle count");
-        if (n == 0) { return SPHX_OK; }
-        ws->fieldTmp.reserve(n * 8);
-        gather(ws, ws->order.get(), field, ws->fieldTmp.get(), n, elemBytes);
-        SPHX_HIP(hipMemcpyAsync(field, ws->fieldTmp.get(), n * size_t(elemBytes), hipMemcpyDeviceToDevice,
-                                ws->stream));
+        std::vector<std::pair<void*, int>> batch;
+        for (size_t k = 0; k < count; ++k)
+        {
+            SPHX_REQUIRE(elemBytes[k] == 4 || elemBytes[k] == 8, SPHX_ERR_ARG, "element size must be 4 or 8 bytes");
+            SPHX_REQUIRE(fields[k] || n == 0, SPHX_ERR_ARG, "null field");
+            batch.push_back({fields[k], elemBytes[k]});
+        }
+        permuteFields(ws, ws->order.get(), batch, n);
         return SPHX_OK;
     });
 }

@@ -33,7 +33,7 @@ _SIGS = {
     "sphx_build_tree": [_P, _P, _SZ, _U32, _BOXP],
     "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
     "sphx_domain_sync": [_P, _BOXP, _FP, _PP, _SZ, _U32],
-    "sphx_domain_reorder": [_P, _P, _SZ, C.c_int],
+    "sphx_domain_reorder": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_int), _SZ, _SZ],
     "sphx_find_neighbors": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int],
     "sphx_find_neighbors_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.c_int],
     "sphx_copy_neighbors": [_P, _P, _U32, _U32],
@@ -191,9 +191,14 @@ class Workspace:
         """Adjoint entry point `name`: cotangents `cot`, gradients added into `grad` (abi.Adjoint)."""
         self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last, C.byref(cot), C.byref(grad))
 
-    def domain_reorder(self, tensor, n):
-        """Apply the last domain sync's permutation to one more per-particle tensor."""
-        self.call("sphx_domain_reorder", C.c_void_p(tensor.data_ptr()), n, tensor.element_size())
+    def domain_reorder(self, tensors, n):
+        """Apply the last domain sync's permutation to more per-particle tensors (one or a list)."""
+        if not isinstance(tensors, (list, tuple)):
+            tensors = [tensors]
+        k = len(tensors)
+        ptrs = (C.c_void_p * k)(*[t.data_ptr() for t in tensors])
+        sizes = (C.c_int * k)(*[t.element_size() for t in tensors])
+        self.call("sphx_domain_reorder", ptrs, sizes, k, n)
 
     def rho_timestep(self, fields, params, first, last, ts):
         """ts.minDtRho = Krho / |max divv| (sphx_rho_timestep)."""

@@ -19,6 +19,10 @@ STD_FIELDS = ("x", "y", "z", "u", "du", "h", "m", "vx", "vy", "vz", "rho", "p", 
 VE_FIELDS = ("x", "y", "z", "temp", "du", "h", "m", "vx", "vy", "vz", "prho", "c", "c11", "c12", "c13", "c22",
              "c23", "c33", "xm", "kx", "gradh", "divv", "curlv", "alpha", "ax", "ay", "az", "keys", "nc")
 GRADV_FIELDS = ("dV11", "dV12", "dV13", "dV22", "dV23", "dV33")
+# the fields a domain sync permutes: HydroProp / HydroVeProp::ConservedFields with x, y, z, h, m
+# (main/src/propagator/std_hydro.hpp:73, ve_hydro.hpp:72); every other field is recomputed by the
+# pass before it is read. keys receives the sorted SFC keys.
+CONSERVED_FIELDS = ("x", "y", "z", "h", "m", "u", "temp", "vx", "vy", "vz", "alpha", "keys")
 
 IDEAL_GAS_R = 8.317e7
 
@@ -74,12 +78,16 @@ class ParticleSet:
                 setattr(h, k, self.t[k].data_ptr())
         return h
 
+    def sync_fields(self):
+        """abi.Fields of the conserved tensors only, for sphx_domain_sync."""
+        return abi.fields_from({k: v for k, v in self.t.items() if k in CONSERVED_FIELDS})
+
     def reorder_history(self, ws):
         """Apply the last domain sync's order to the previous-step tensors, as Domain::sync reorders
-        every conserved field."""
-        for k in abi.HISTORY_FIELDS:
-            if k in self.t:
-                ws.domain_reorder(self.t[k], self.n)
+        every conserved field (one batched call)."""
+        ts = [self.t[k] for k in abi.HISTORY_FIELDS if k in self.t]
+        if ts:
+            ws.domain_reorder(ts, self.n)
 
     def host(self, name):
         a = self.t[name][: self.n].cpu().numpy()
@@ -106,7 +114,7 @@ def std_step(ws, ps, box, params, bucket=64):
     """One std-SPH pass; returns (converged, minDtCourant)."""
     n = ps.n
     f = ps.fields()
-    ws.domain_sync(box, f, params, n, bucket)
+    ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
     ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, params, n, rho=True)
     ws.eos("sphx_eos_std", f, params, 0, n)
@@ -120,7 +128,7 @@ def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None):
     also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171)."""
     n = ps.n
     f = ps.fields()
-    ws.domain_sync(box, f, params, n, bucket)
+    ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
     ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, params, n, rho=False)
     ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)

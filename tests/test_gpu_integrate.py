@@ -124,10 +124,15 @@ def test_domain_reorder_applies_the_sync_permutation(ws):
     ps = ParticleSet.from_host(arr, STD_FIELDS + ("x_m1",))
     n = ps.n
     ps.t["x_m1"][:n] = ps.t["x"][:n].clone()  # the history carries a copy of x ...
-    ws.domain_sync(box, ps.fields(), params, n, 64)
+    # ... and more arrays of both element sizes, reordered in one batched call
+    extra = [ps.t["y"][:n].clone(), ps.t["z"][:n].float(), ps.t["h"][:n].float(), ps.t["m"][:n].clone()]
+    ws.domain_sync(box, ps.sync_fields(), params, n, 64)
     ps.reorder_history(ws)
+    ws.domain_reorder(extra, n)
     torch.cuda.synchronize()
     assert torch.equal(ps.t["x_m1"][:n], ps.t["x"][:n])  # ... and stays aligned with it
+    assert torch.equal(extra[0], ps.t["y"][:n]) and torch.equal(extra[3], ps.t["m"][:n])
+    assert torch.equal(extra[1], ps.t["z"][:n].float()) and torch.equal(extra[2], ps.t["h"][:n].float())
     with pytest.raises(Exception):
         ws.domain_reorder(ps.t["x_m1"], n - 1)
 
