@@ -239,6 +239,15 @@ int sphx_build_tree(sphx_workspace_t* ws, const uint64_t* sortedKeys, size_t n, 
 /* Device view of the tree built by the last sphx_build_tree / sphx_domain_sync. */
 int sphx_tree_view(const sphx_workspace_t* ws, sphx_tree_view_t* out);
 
+/* The box update at the start of Domain::sync (GlobalAssignment::assign,
+ * domain/include/cstone/domain/assignment.hpp:80-82, with makeGlobalBox, sfc/box_mpi.hpp:69-100, and
+ * limitBoxShrinking, sfc/box.hpp:398-412): per non-periodic dimension, the min and max of the n
+ * coordinates; with firstCall = 0 each side then moves inwards by at most 0.05 of the previous edge
+ * length. Periodic dimensions keep their limits. Updates box in place (reads its limits as the
+ * previous box). */
+int sphx_global_box(sphx_workspace_t* ws, sphx_box_t* box, const double* x, const double* y, const double* z,
+                    size_t n, int firstCall);
+
 /* keys -> stable sort -> permute every non-NULL field of `f` -> tree. One-rank Domain::sync
  * (domain/include/cstone/domain/domain.hpp:182-234). As there, only the conserved fields need to be
  * passed (std: x y z h m u vx vy vz, HydroProp::ConservedFields, main/src/propagator/std_hydro.hpp:73;

@@ -52,6 +52,13 @@ struct Box
     {
     }
 
+    explicit Box(const sphx_box_t& b)
+        : lo{b.lo[0], b.lo[1], b.lo[2]}
+        , hi{b.hi[0], b.hi[1], b.hi[2]}
+        , boundary{BoundaryType(b.boundary[0]), BoundaryType(b.boundary[1]), BoundaryType(b.boundary[2])}
+    {
+    }
+
     sphx_box_t c() const
     {
         sphx_box_t b;
@@ -192,6 +199,46 @@ void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucket
     }
     detail::check(d.workspace, sphx_domain_reorder(d.workspace, extra, bytes, count, numParticles));
 }
+
+/*! One-rank cstone::Domain (domain/include/cstone/domain/domain.hpp): owns the box and the
+ *  first-call state. sync() refits the box as GlobalAssignment::assign does (assignment.hpp:80-82:
+ *  the fitting box on the first call, then limitBoxShrinking; periodic dimensions fixed), then runs
+ *  syncDomain. All particles are local: startIndex() = 0, endIndex() = nParticles(). */
+class Domain
+{
+public:
+    Domain(int /*rank*/, int nRanks, unsigned bucketSize, unsigned /*bucketSizeFocus*/, float /*theta*/,
+           const Box& box)
+        : box_(box)
+        , bucketSize_(bucketSize)
+    {
+        if (nRanks != 1) { throw std::runtime_error("this domain holds one rank's particles only"); }
+    }
+
+    template<class Dataset>
+    void sync(size_t numParticles, Dataset& d)
+    {
+        auto& s = d.devData;
+        auto  b = box_.c();
+        detail::check(d.workspace, sphx_global_box(d.workspace, &b, s.x, s.y, s.z, numParticles, firstCall_));
+        box_ = Box(b);
+        syncDomain(numParticles, d, box_, bucketSize_);
+        numParticles_ = numParticles;
+        firstCall_    = false;
+    }
+
+    const Box& box() const { return box_; }
+    size_t     startIndex() const { return 0; }
+    size_t     endIndex() const { return numParticles_; }
+    size_t     nParticles() const { return numParticles_; }
+    size_t     nParticlesWithHalos() const { return numParticles_; }
+
+private:
+    Box      box_;
+    unsigned bucketSize_;
+    size_t   numParticles_ = 0;
+    bool     firstCall_    = true;
+};
 
 //! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56)
 template<class Dataset>

@@ -61,6 +61,8 @@ _SIGS = {
     "compute_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
     "rho_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
     "conserved_quantities": (None, [_FP, _PP, _U32, _U32, _P]),
+    "global_box": (None, [_BOXP, _P, _P, _P, _SZ, C.c_int]),
+    "limit_box_shrinking": (None, [_P, _P, _P]),
     "density_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "iad_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),
     "momentum_energy_std_vjp": (None, [_BOXP, _FP, _PP, _U32, _U32, _P, _P, _P, _AP, _AP]),

@@ -616,6 +616,22 @@ void orc_rho_timestep(const sphx_fields_t* f, const sphx_params_t* p, uint32_t f
     ts->minDtRho = ts->Krho / std::abs(m);
 }
 
+//! box update of one Domain::sync (globalBox): limits in box->lo / box->hi, updated in place
+void orc_global_box(sphx_box_t* box, const double* x, const double* y, const double* z, size_t n, int firstCall)
+{
+    double lim[6] = {box->lo[0], box->hi[0], box->lo[1], box->hi[1], box->lo[2], box->hi[2]};
+    int    bc[3]  = {box->boundary[0], box->boundary[1], box->boundary[2]};
+    globalBox(x, y, z, n, bc, firstCall != 0, lim);
+    for (int d = 0; d < 3; ++d)
+        box->lo[d] = lim[2 * d], box->hi[d] = lim[2 * d + 1];
+}
+
+//! limitBoxShrinking alone (limits6 = xmin xmax ymin ymax zmin zmax)
+void orc_limit_box_shrinking(const double* fit6, const double* prev6, double* out6)
+{
+    limitBoxShrinking(fit6, prev6, 0.05, out6);
+}
+
 //! conserved quantities over [first, last): out = eKin, eInt, linmom[3], angmom[3], ncsum
 void orc_conserved_quantities(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
                               double* out)

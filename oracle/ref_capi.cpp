@@ -422,6 +422,17 @@ int ref_fbc_check_f32(double coord, float h, double top, double bottom, int fbc)
     return sph::fbcCheck(coord, h, top, bottom, fbc != 0);
 }
 
+//! cstone::limitBoxShrinking (domain/include/cstone/sfc/box.hpp:398-412), default shrink limit
+void ref_limit_box_shrinking(const double* fit6, const double* prev6, double* out6)
+{
+    cstone::Box<double> fit(fit6[0], fit6[1], fit6[2], fit6[3], fit6[4], fit6[5]);
+    cstone::Box<double> prev(prev6[0], prev6[1], prev6[2], prev6[3], prev6[4], prev6[5]);
+    auto                out = cstone::limitBoxShrinking(fit, prev);
+    double              r[6] = {out.xmin(), out.xmax(), out.ymin(), out.ymax(), out.zmin(), out.zmax()};
+    for (int k = 0; k < 6; ++k)
+        out6[k] = r[k];
+}
+
 //! updateSmoothingLengthCpu (sph/include/sph/update_h.hpp:12-24)
 void ref_update_smoothing_length(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
 {

@@ -1422,6 +1422,62 @@ inline void putInBox(double X[3], const Box& box)
     }
 }
 
+/*! makeGlobalBox (sfc/box_mpi.hpp:69-100) on one rank. limits6 = xmin xmax ymin ymax zmin zmax.
+ * A periodic dimension keeps the previous limits; any other takes the min and max of the
+ * coordinates, accumulated with std::min / std::max from +INFINITY / -INFINITY (MinMax,
+ * box_mpi.hpp:36-52). */
+inline void fittingBox(const double* x, const double* y, const double* z, size_t n, const int bc[3],
+                       const double prev[6], double out[6])
+{
+    const double* c[3] = {x, y, z};
+    for (int d = 0; d < 3; ++d)
+    {
+        if (bc[d] == 1)
+        {
+            out[2 * d] = prev[2 * d], out[2 * d + 1] = prev[2 * d + 1];
+            continue;
+        }
+        double lo = INFINITY, hi = -INFINITY;
+        for (size_t i = 0; i < n; ++i)
+        {
+            lo = std::min(lo, c[d][i]);
+            hi = std::max(hi, c[d][i]);
+        }
+        out[2 * d] = lo, out[2 * d + 1] = hi;
+    }
+}
+
+/*! limitBoxShrinking (sfc/box.hpp:398-412): each side moves inwards by at most shrinkLimit times
+ * the previous edge length (the length as the Box constructor forms it, xmax - xmin). */
+inline void limitBoxShrinking(const double fit[6], const double prev[6], double shrinkLimit, double out[6])
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        double len   = prev[2 * d + 1] - prev[2 * d];
+        double step  = shrinkLimit * len;
+        double lower = prev[2 * d] + step;
+        double upper = prev[2 * d + 1] - step;
+        out[2 * d]     = std::min(fit[2 * d], lower);
+        out[2 * d + 1] = std::max(fit[2 * d + 1], upper);
+    }
+}
+
+/*! The box update of GlobalAssignment::assign (domain/assignment.hpp:80-82): the fitting box on the
+ * first call, afterwards the fitting box with limited shrinking against the previous one. */
+inline void globalBox(const double* x, const double* y, const double* z, size_t n, const int bc[3], bool firstCall,
+                      double limits6[6])
+{
+    double fit[6];
+    fittingBox(x, y, z, n, bc, limits6, fit);
+    if (firstCall) { std::copy(fit, fit + 6, limits6); }
+    else
+    {
+        double prev[6];
+        std::copy(limits6, limits6 + 6, prev);
+        limitBoxShrinking(fit, prev, 0.05, limits6);
+    }
+}
+
 //! positionUpdate, positions.hpp:63-85 (Press 2nd order): (Xn, An, dXn) -> (Xn+1, Vn+1, dXn+1)
 inline void positionUpdate(double dt, double dt_m1, const double Xn[3], const double An[3], const double dXn[3],
                            const Box& box, double Xnp1[3], double Vnp1[3], double dXnp1[3])

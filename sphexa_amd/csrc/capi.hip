@@ -321,6 +321,36 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
     });
 }
 
+int sphx_global_box(sphx_workspace_t* ws, sphx_box_t* box, const double* x, const double* y, const double* z,
+                    size_t n, int firstCall)
+{
+#pragma clang fp contract(off)
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box, SPHX_ERR_ARG, "null argument");
+        SPHX_REQUIRE(n == 0 || (x && y && z), SPHX_ERR_ARG, "x, y, z are required");
+        if (box->boundary[0] == 1 && box->boundary[1] == 1 && box->boundary[2] == 1) { return SPHX_OK; }
+        double ext[6];
+        coordinateExtent(ws, x, y, z, n, ext);
+        for (int d = 0; d < 3; ++d)
+        {
+            if (box->boundary[d] == 1) { continue; } // periodic: the limits stay
+            double lo = ext[2 * d], hi = ext[2 * d + 1];
+            if (!firstCall)
+            {
+                // limitBoxShrinking with shrink limit 0.05 of the previous edge length
+                const double step  = 0.05 * (box->hi[d] - box->lo[d]);
+                const double lower = box->lo[d] + step;
+                const double upper = box->hi[d] - step;
+                lo                 = std::min(lo, lower);
+                hi                 = std::max(hi, upper);
+            }
+            box->lo[d] = lo;
+            box->hi[d] = hi;
+        }
+        return SPHX_OK;
+    });
+}
+
 int sphx_domain_reorder(sphx_workspace_t* ws, void* const* fields, const int* elemBytes, size_t count, size_t n)
 {
     return guarded(ws, [&] {

@@ -382,6 +382,8 @@ void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const d
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
 void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
 //! permute every field in place by `order` (4- or 8-byte elements), batched per element size
+//! min x, max x, min y, max y, min z, max z of n coordinates (+-INFINITY for n = 0), on the host
+void coordinateExtent(sphx_workspace* ws, const double* x, const double* y, const double* z, size_t n, double ext[6]);
 void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<std::pair<void*, int>>& fields,
                    size_t n);
 void buildTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);

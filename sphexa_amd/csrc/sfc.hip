@@ -193,4 +193,80 @@ void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<
     permuteBatches<uint32_t>(ws, order, f4, n);
 }
 
+namespace
+{
+constexpr unsigned kExtentThreads = 256;
+constexpr unsigned kExtentBlocks  = 512;
+
+/*! per block: min x, max x, min y, max y, min z, max z over a grid-stride range, each as
+ *  std::min / std::max from +-INFINITY would accumulate it (min and max are exact, so the order of
+ *  the reduction does not change the result) */
+__global__ void __launch_bounds__(kExtentThreads) extentKernel(const double* __restrict__ x,
+                                                              const double* __restrict__ y,
+                                                              const double* __restrict__ z, size_t n,
+                                                              double* partial)
+{
+    double        lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const double* c[3]  = {x, y, z};
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            const double v = c[d][i];
+            lo[d]          = v < lo[d] ? v : lo[d];
+            hi[d]          = v > hi[d] ? v : hi[d];
+        }
+    }
+    __shared__ double part[kExtentThreads / kWave][6];
+    for (int d = 0; d < 3; ++d)
+    {
+        lo[d] = waveMin(lo[d]);
+        hi[d] = waveMax(hi[d]);
+    }
+    if (laneId() == 0)
+    {
+        for (int d = 0; d < 3; ++d)
+            part[threadIdx.x / kWave][2 * d] = lo[d], part[threadIdx.x / kWave][2 * d + 1] = hi[d];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6)
+    {
+        const bool isMax = threadIdx.x % 2;
+        double     r     = part[0][threadIdx.x];
+        for (unsigned w = 1; w < kExtentThreads / kWave; ++w)
+        {
+            const double v = part[w][threadIdx.x];
+            r              = isMax ? (v > r ? v : r) : (v < r ? v : r);
+        }
+        partial[size_t(blockIdx.x) * 6 + threadIdx.x] = r;
+    }
+}
+
+//! one block of 64: the partials of `blocks` blocks into out[6]
+__global__ void extentFinalKernel(const double* __restrict__ partial, unsigned blocks, double* out)
+{
+    if (threadIdx.x >= 6) { return; }
+    const bool isMax = threadIdx.x % 2;
+    double     r     = isMax ? -INFINITY : INFINITY;
+    for (unsigned b = 0; b < blocks; ++b)
+    {
+        const double v = partial[size_t(b) * 6 + threadIdx.x];
+        r              = isMax ? (v > r ? v : r) : (v < r ? v : r);
+    }
+    out[threadIdx.x] = r;
+}
+} // namespace
+
+void coordinateExtent(sphx_workspace* ws, const double* x, const double* y, const double* z, size_t n, double ext[6])
+{
+    const unsigned blocks = std::max(1u, std::min(kExtentBlocks, ceilDiv(n, kExtentThreads)));
+    double*        buf    = static_cast<double*>(scratchBytes(ws, (size_t(blocks) + 1) * 6 * sizeof(double)));
+    extentKernel<<<blocks, kExtentThreads, 0, ws->stream>>>(x, y, z, n, buf + 6);
+    SPHX_CHECK_LAUNCH();
+    extentFinalKernel<<<1, 64, 0, ws->stream>>>(buf + 6, blocks, buf);
+    SPHX_CHECK_LAUNCH();
+    SPHX_HIP(hipMemcpyAsync(ext, buf, 6 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    SPHX_HIP(hipStreamSynchronize(ws->stream));
+}
+
 } // namespace sphx

@@ -46,28 +46,28 @@ __global__ void classifyKernel(const uint32_t* lo, const uint32_t* hi, uint32_t*
     flag[i] = (hi[i] - lo[i] > bucket) && (level < kMaxLevel);
 }
 
-//! each internal node of the level writes its 8 children behind the level
+//! each internal node of the level writes its 8 children behind the level; 8 consecutive threads
+//! per node, thread c finding the first particle of child c (the searches run side by side
+//! instead of as one chain of dependent loads)
 __global__ void expandKernel(const Key* __restrict__ keys, Key* nodeKey, uint32_t* lo, uint32_t* hi, uint8_t* nodeLevel,
                              const uint32_t* flag, const uint32_t* inclRank, uint32_t internalBefore,
                              uint32_t begin, uint32_t end, unsigned level)
 {
-    uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= end || !flag[i]) { return; }
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t i = begin + t / 8;
+    const int    c = int(t % 8);
+    if (i >= end || !flag[i]) { return; } // uniform over the node's 8 threads
     uint32_t r     = inclRank[i] - 1 - internalBefore; // rank among this level's internal nodes
-    uint32_t c0    = end + 8 * r;
+    uint32_t slot  = end + 8 * r + uint32_t(c);
     Key      k     = nodeKey[i];
     Key      range = nodeRange(level + 1);
     uint32_t a = lo[i], b = hi[i];
-    uint32_t prev = a;
-    for (int c = 0; c < 8; ++c)
-    {
-        uint32_t next     = (c == 7) ? b : lowerBound(keys, prev, b, k + Key(c + 1) * range);
-        nodeKey[c0 + c]   = k + Key(c) * range;
-        lo[c0 + c]        = prev;
-        hi[c0 + c]        = next;
-        nodeLevel[c0 + c] = uint8_t(level + 1);
-        prev              = next;
-    }
+    uint32_t first = (c == 0) ? a : lowerBound(keys, a, b, k + Key(c) * range);
+    uint32_t next  = __shfl_down(first, 1, 8); // first particle of child c + 1
+    nodeKey[slot]   = k + Key(c) * range;
+    lo[slot]        = first;
+    hi[slot]        = (c == 7) ? b : next;
+    nodeLevel[slot] = uint8_t(level + 1);
 }
 
 //! prefixes, child links, parents, centres; leaves are collected for the key sort
@@ -205,9 +205,9 @@ void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned 
         growCopy(ws, s.nodeKey, end, need), growCopy(ws, s.lo, end, need), growCopy(ws, s.hi, end, need);
         growCopy(ws, s.flag, end, need), growCopy(ws, s.rank, end, need), growCopy(ws, s.level, end, need);
 
-        expandKernel<<<ceilDiv(cnt, 256), 256, 0, st>>>(keys, s.nodeKey.get(), s.lo.get(), s.hi.get(), s.level.get(),
-                                                        s.flag.get(), s.rank.get(), internalBefore, begin, end,
-                                                        level);
+        expandKernel<<<ceilDiv(size_t(cnt) * 8, 256), 256, 0, st>>>(keys, s.nodeKey.get(), s.lo.get(), s.hi.get(),
+                                                                   s.level.get(), s.flag.get(), s.rank.get(),
+                                                                   internalBefore, begin, end, level);
         SPHX_CHECK_LAUNCH();
         internalBefore = totalInternal;
         begin          = end;
@@ -250,11 +250,14 @@ void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned 
         s.leafNode.get(), s.leafRank.get());
     SPHX_CHECK_LAUNCH();
 
-    size_t sortBytes = 0;
+    // leaf keys lie below 2^63 and are multiples of nodeRange(deepest level): only the bits
+    // between decide their order
+    const unsigned lowBit    = 3u * (kMaxLevel - std::min(level, kMaxLevel));
+    size_t         sortBytes = 0;
     SPHX_HIP(rocprim::radix_sort_pairs(nullptr, sortBytes, s.leafKey.get(), s.leafKeySorted.get(), s.leafNode.get(),
-                                       s.leafNodeSorted.get(), L, 0, 64, st));
+                                       s.leafNodeSorted.get(), L, lowBit, 63, st));
     SPHX_HIP(rocprim::radix_sort_pairs(scratchBytes(ws, sortBytes), sortBytes, s.leafKey.get(), s.leafKeySorted.get(),
-                                       s.leafNode.get(), s.leafNodeSorted.get(), L, 0, 64, st));
+                                       s.leafNode.get(), s.leafNodeSorted.get(), L, lowBit, 63, st));
     leafFinalize<<<ceilDiv(L + 1, 256), 256, 0, st>>>(s.leafKeySorted.get(), s.leafNodeSorted.get(), s.lo.get(),
                                                       s.hi.get(), t.leaves.get(), t.counts.get(), t.layout.get(),
                                                       t.internalToLeaf.get(), L, n);

@@ -33,6 +33,7 @@ _SIGS = {
     "sphx_build_tree": [_P, _P, _SZ, _U32, _BOXP],
     "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
     "sphx_domain_sync": [_P, _BOXP, _FP, _PP, _SZ, _U32],
+    "sphx_global_box": [_P, _BOXP, _P, _P, _P, _SZ, C.c_int],
     "sphx_domain_reorder": [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_int), _SZ, _SZ],
     "sphx_find_neighbors": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int],
     "sphx_find_neighbors_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.c_int],
@@ -190,6 +191,13 @@ class Workspace:
     def vjp(self, name, box, fields, params, first, last, cot, grad):
         """Adjoint entry point `name`: cotangents `cot`, gradients added into `grad` (abi.Adjoint)."""
         self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last, C.byref(cot), C.byref(grad))
+
+    def global_box(self, box, x, y, z, n, first_call):
+        """Update box (abi.Box, in place) as Domain::sync does: refit the non-periodic dimensions to
+        the particles, with limited shrinking unless first_call."""
+        self.call("sphx_global_box", C.byref(box), C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                  C.c_void_p(z.data_ptr()), n, int(first_call))
+        return box
 
     def domain_reorder(self, tensors, n):
         """Apply the last domain sync's permutation to more per-particle tensors (one or a list)."""

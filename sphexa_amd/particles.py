@@ -153,11 +153,15 @@ def integrate(ws, ps, box, params, ts):
     ws.update_smoothing_length(f, params, 0, ps.n)
 
 
-def run_steps(ws, ps, box, params, ts, steps, form="std", av_clean=False):
+def run_steps(ws, ps, box, params, ts, steps, form="std", av_clean=False, refit_box=False, first_sync=True):
     """`steps` full time steps (forces, then integrate) of the std or VE propagator on one rank.
-    params.minDt follows ts.minDt (the AV switches read it). Returns the Courant step of each."""
+    params.minDt follows ts.minDt (the AV switches read it). With refit_box, each step's sync first
+    updates `box` in place as Domain::sync does (sphx_global_box: the fitting box on the first sync,
+    then limited shrinking). Returns the Courant step of each."""
     out = []
-    for _ in range(steps):
+    for step in range(steps):
+        if refit_box:
+            ws.global_box(box, ps.t["x"], ps.t["y"], ps.t["z"], ps.n, first_sync and step == 0)
         params.minDt = ts.minDt
         if form == "std":
             conv, dtc = std_step(ws, ps, box, params)

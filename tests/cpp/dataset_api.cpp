@@ -90,13 +90,17 @@ int main(int argc, char** argv)
     for (int k = 0; k < 10; ++k)
         if (icDst[k]) HIPCHECK(hipMemcpy(icDst[k], ic[k].data(), n * 8, hipMemcpyHostToDevice));
 
-    Box       box(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
+    Box       box0(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
+    Domain    domain(0, 1, 64, 64, 0.5f, box0);
     GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
     try
     {
       for (int step = 0; step < std::max(steps, 1); ++step)
       {
-        syncDomain(n, d, box);
+        // the pass alone keeps the given box (the CPU oracle's); steps refit it as Domain::sync does
+        if (steps > 0) { domain.sync(n, d); }
+        else { syncDomain(n, d, box0); }
+        const Box& box = steps > 0 ? domain.box() : box0;
         if (!fused) { findNeighborsSfc(0, n, d, box); }
         if (ve)
         {

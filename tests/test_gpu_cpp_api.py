@@ -68,17 +68,19 @@ def test_cpp_interface_pass(tmp_path, orc, form, calls):
     assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
 
 
-def test_cpp_interface_steps(tmp_path, orc):
-    """Three full std steps through the C++ interface (syncDomain ... computeMomentumEnergySTD,
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_cpp_interface_steps(tmp_path, orc, periodic):
+    """Three full std steps through the C++ interface (Domain::sync ... computeMomentumEnergySTD,
     then computeTimestep, computePositions, updateSmoothingLength) against the oracle's three
-    steps: positions, velocities, h, u and the time-step state."""
+    steps: positions, velocities, h, u and the time-step state. In the open box the domain refits
+    the box at every sync, and so does the oracle."""
     from test_gpu_integrate import _oracle_steps
 
     arr, spec = ic.sedov_lattice(16, seed=17)
     n = arr["x"].size
     with open(tmp_path / "in.bin", "wb") as fh:
         fh.write(np.array([n], np.uint64).tobytes())
-        fh.write(np.array([1], np.uint32).tobytes())
+        fh.write(np.array([periodic], np.uint32).tobytes())
         fh.write(np.array([spec["lo"], spec["hi"]], np.float64).tobytes())
         for col in (arr["x"], arr["y"], arr["z"], arr["h"], arr["m"], arr["u"], arr["vx"], arr["vy"], arr["vz"],
                     np.zeros(n)):
@@ -92,11 +94,12 @@ def test_cpp_interface_steps(tmp_path, orc):
     minDt, minDt_m1, ttot = raw[len(names) * n:len(names) * n + 3]
     q = raw[len(names) * n + 3:]
 
-    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC if periodic else abi.OPEN)
     for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
         arr[k] = np.zeros(n)
     ts = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
-    a, _ = _oracle_steps(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0)), ts, 3, np.float64)
+    a, _ = _oracle_steps(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0)), ts, 3, np.float64,
+                         refit_box=not periodic)
     assert abs(minDt - ts.minDt) <= 1e-12 * ts.minDt and abs(minDt_m1 - ts.minDt_m1) <= 1e-12 * ts.minDt_m1
     assert abs(ttot - ts.ttot) <= 1e-12 * ts.ttot
     # the conserved quantities of the final state against the oracle's on its own final state

@@ -5,10 +5,10 @@
   exp (device and host exp may differ in the last bit: 2 ulp allowed there).
 * sphx_update_smoothing_length: within 4 ulp (device pow against the C library's).
 * sphx_compute_timestep / sphx_rho_timestep: equal to the oracle's (device maxima, host scalars).
-* sphx_domain_reorder applies the sync's permutation.
+* sphx_domain_reorder applies the sync's permutation; sphx_global_box equals the oracle's box update.
 * Three full steps of the std propagator (forces, time step, positions, energy, h) on a Sedov-like
-  lattice, FP64 and mixed, against the oracle's three steps: positions, velocities, u and h after
-  the last step, and the step sizes.
+  lattice, FP64 and mixed, periodic, and FP64 in an open box refitted at every sync, against the
+  oracle's three steps: positions, velocities, u and h after the last step, and the step sizes.
 """
 import ctypes as C
 
@@ -115,6 +115,32 @@ def test_timestep_matches_oracle(ws, orc, mixed):
     assert res[0] == res[1]
 
 
+@pytest.mark.parametrize("bc", [(abi.OPEN,) * 3, (abi.OPEN, abi.PERIODIC, abi.FIXED), (abi.PERIODIC,) * 3])
+def test_global_box_matches_oracle(ws, orc, bc):
+    """sphx_global_box: the fitting box on the first call, limited shrinking afterwards, periodic
+    dimensions untouched; bit for bit against the oracle's globalBox (min and max are exact)."""
+    import torch
+
+    rng = np.random.default_rng(15)
+    n = 100003
+    pts = [rng.normal(0, 1, n), rng.uniform(-3, 2, n), rng.normal(1, 0.2, n)]
+    box_g, box_o = abi.Box.make(-4, 4, list(bc)), abi.Box.make(-4, 4, list(bc))
+    for call, scale in enumerate((1.0, 0.5, 1.5)):
+        p = [np.ascontiguousarray(c * scale) for c in pts]
+        t = [torch.from_numpy(c).cuda() for c in p]
+        ws.global_box(box_g, *t, n, call == 0)
+        orc.global_box(C.byref(box_o), p[0].ctypes.data, p[1].ctypes.data, p[2].ctypes.data, n, int(call == 0))
+        for d in range(3):
+            assert (box_g.lo[d], box_g.hi[d]) == (box_o.lo[d], box_o.hi[d]), (call, d)
+            if bc[d] == abi.PERIODIC:
+                assert (box_g.lo[d], box_g.hi[d]) == (-4.0, 4.0)
+    e = torch.zeros(1, dtype=torch.float64, device="cuda")
+    box_g = abi.Box.make(-1, 1, list(bc))
+    ws.global_box(box_g, e, e, e, 0, False)  # no particles: the shrink limit on every refitted side
+    for d in range(3):
+        assert (box_g.lo[d], box_g.hi[d]) == ((-1.0, 1.0) if bc[d] == abi.PERIODIC else (-0.9, 0.9))
+
+
 def test_domain_reorder_applies_the_sync_permutation(ws):
     import torch
 
@@ -137,7 +163,7 @@ def test_domain_reorder_applies_the_sync_permutation(ws):
         ws.domain_reorder(ps.t["x_m1"], n - 1)
 
 
-def _oracle_steps(orc, arr, box, params, ts, steps, hydro):
+def _oracle_steps(orc, arr, box, params, ts, steps, hydro, refit_box=False):
     from oracle import pipeline
 
     from test_integrate_oracle import run_positions as orc_positions
@@ -145,7 +171,10 @@ def _oracle_steps(orc, arr, box, params, ts, steps, hydro):
     a = dict(arr)
     n = len(a["x"])
     dts = []
-    for _ in range(steps):
+    for step in range(steps):
+        if refit_box:
+            x, y, z = (np.ascontiguousarray(a[k], np.float64) for k in "xyz")
+            orc.global_box(C.byref(box), x.ctypes.data, y.ctypes.data, z.ctypes.data, n, int(step == 0))
         params.minDt = ts.minDt
         o = pipeline.run(orc, a, box, params, form="std", hydro=hydro)
         a = {k: v for k, v in o["a"].items() if v is not None}
@@ -158,23 +187,32 @@ def _oracle_steps(orc, arr, box, params, ts, steps, hydro):
     return a, dts
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_three_std_steps_match_oracle(ws, orc, mixed):
+@pytest.mark.parametrize("mixed,boundary", [(False, abi.PERIODIC), (True, abi.PERIODIC), (False, abi.OPEN)])
+def test_three_std_steps_match_oracle(ws, orc, mixed, boundary):
+    """With open boundaries every sync also refits the box (sphx_global_box against the oracle's
+    globalBox), so the keys, the order and the tree follow the reference's moving box."""
     arr, spec = ic.sedov_lattice(16, seed=17)
     n = arr["x"].size
     for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
         arr[k] = np.zeros(n)
-    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    refit = boundary != abi.PERIODIC
+    box_o = abi.Box.make(spec["lo"], spec["hi"], boundary)
     steps = 3
 
     params64 = abi.Params.defaults(ws.K)
     ts_o = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
-    a, dts_o = _oracle_steps(orc, arr, box, params64, ts_o, steps, np.float64)
+    a, dts_o = _oracle_steps(orc, arr, box_o, params64, ts_o, steps, np.float64, refit_box=refit)
 
+    box = abi.Box.make(spec["lo"], spec["hi"], boundary)
     params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
     ts = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
     ps = ParticleSet.from_host(arr, STD_FIELDS + ("x_m1", "y_m1", "z_m1", "du_m1"), mixed=mixed)
-    dts = run_steps(ws, ps, box, params, ts, steps, form="std")
+    dts = run_steps(ws, ps, box, params, ts, steps, form="std", refit_box=refit)
+    if refit:
+        # the last step's box, from the same positions on both sides
+        assert [box.lo[d] for d in range(3)] == [box_o.lo[d] for d in range(3)]
+        assert [box.hi[d] for d in range(3)] == [box_o.hi[d] for d in range(3)]
+        assert all(spec["lo"] < box.lo[d] < box.hi[d] < spec["hi"] for d in range(3))  # fitted inside the lattice box
     print(f"{'mixed' if mixed else 'fp64'} Courant steps {dts} oracle {dts_o}; ttot {ts.ttot:.6e} / {ts_o.ttot:.6e}",
           flush=True)
     tol_dt = 1e-12 if not mixed else 1e-5
