@@ -312,6 +312,12 @@ __device__ __forceinline__ void values(const DirectWD& f, float& w, float& dw)
 //! x^k for a small integer k >= 0 (wave-uniform loop), or 2^(k log2 x) when k is not an integer
 __device__ __forceinline__ double sincPower(double x, float k, int ki)
 {
+    if (ki == 5)
+    {
+        // the default sinc index 6 (sinc^5 here): the products of the loop below, unrolled
+        const double x2 = x * x;
+        return x * (x2 * x2);
+    }
     if (ki < 0) { return double(__builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(float(x)))); }
     double r = 1.0, b = x;
     for (int e = ki; e > 0; e >>= 1)
