@@ -5,7 +5,8 @@
 // search radius 2*h_max (exact periodic box-box distance, so no cell holding a neighbour of any
 // target is skipped), collecting candidate leaves in LDS; internal nodes live on an LDS stack
 // popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Candidate leaves that no
-// target's sphere reaches are dropped (one leaf per lane, boxes broadcast, one ballot per leaf).
+// target's sphere reaches are dropped (one leaf per lane stages its box, moved to the periodic
+// image the group sees, in LDS; the wave reads each back as a broadcast, one ballot per leaf).
 // The particles of the surviving leaves within reach of the group box are compacted into full
 // tiles of 64 in a double-buffered LDS area (coalesced loads of the next window overlap the
 // sweep), and the wave tests each tile reading every candidate with one broadcast LDS access,
@@ -14,7 +15,7 @@
 // dx = x_j - x_i, periodic images folded by one box length, unfused FP64 arithmetic, keep if
 // d2 < Th(4)*h_i*h_i and j != i. The kept set and count are therefore bit-identical to the CPU
 // search; the first ngmax hits are stored in the group layout of common.hpp (listOffset): a lane
-// collects its entries in a register quad and stores 16 bytes at a time, and the pair loops read
+// shifts its entries through a register quad and stores 16 bytes at a time, and the pair loops read
 // four slots of all 64 lanes with one coalesced load.
 #include "pairs.hpp"
 
@@ -87,22 +88,26 @@ struct Target
 struct ListWriter
 {
     uint32_t* base; // the lane's slot 0
-    uint32_t  q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+    uint32_t  q0 = 0, q1 = 0, q2 = 0, q3 = 0; // the last four entries, oldest first
 
+    //! entry k = j: shifted in behind the previous three, stored when it completes a quad
     __device__ __forceinline__ void put(unsigned k, uint32_t j)
     {
-        const unsigned s = k & 3u;
-        q0 = s == 0 ? j : q0, q1 = s == 1 ? j : q1, q2 = s == 2 ? j : q2, q3 = s == 3 ? j : q3;
-        if (s == 3) { store(k); }
+        q0 = q1, q1 = q2, q2 = q3, q3 = j;
+        if ((k & 3u) == 3u) { store(k, q0, q1, q2, q3); }
     }
-    __device__ __forceinline__ void store(unsigned k)
+    __device__ __forceinline__ void store(unsigned k, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
     {
-        *reinterpret_cast<uint4*>(base + laneSlot(k & ~3u)) = make_uint4(q0, q1, q2, q3);
+        *reinterpret_cast<uint4*>(base + laneSlot(k & ~3u)) = make_uint4(a, b, c, d);
     }
-    //! flush after `stored` entries
+    //! flush after `stored` entries: the r = stored % 4 last ones (q3 the newest) open the quad;
+    //! its remaining slots (never read) repeat the newest entry
     __device__ __forceinline__ void finish(unsigned stored)
     {
-        if (stored & 3u) { store(stored - 1); }
+        const unsigned r = stored & 3u;
+        if (r == 1) { store(stored - 1, q3, q3, q3, q3); }
+        else if (r == 2) { store(stored - 1, q2, q3, q3, q3); }
+        else if (r == 3) { store(stored - 1, q1, q2, q3, q3); }
     }
 };
 
@@ -334,18 +339,47 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
     uint64_t keepMask = 0;
     {
         const double p[3] = {t.x, t.y, t.z};
-        for (int q = 0; q < nLeaf; ++q)
+        if (!ballot(code == kPerPairImage))
         {
-            double d2 = 0.0;
-            for (int d = 0; d < 3; ++d)
+            // every leaf has one image for the whole group: its box, centre moved to that image,
+            // goes to the idle tile buffers and is read back as an LDS broadcast per leaf
+            if (int(lane) < nLeaf)
             {
-                double v = readlaneD(c[d], q) - p[d];
-                if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
-                v = fabs(v) - readlaneD(sz[d], q);
-                v = v > 0.0 ? v : 0.0;
-                d2 += v * v;
+                lds.tx[0][lane] = c[0] - imageShift(b, code, 0), lds.tx[1][lane] = sz[0];
+                lds.ty[0][lane] = c[1] - imageShift(b, code, 1), lds.ty[1][lane] = sz[1];
+                lds.tz[0][lane] = c[2] - imageShift(b, code, 2), lds.tz[1][lane] = sz[2];
             }
-            keepMask |= ballot(t.active && d2 < t.reach2) ? (uint64_t(1) << q) : uint64_t(0);
+            __builtin_amdgcn_wave_barrier();
+            for (int q = 0; q < nLeaf; ++q)
+            {
+                const double cq[3] = {lds.tx[0][q], lds.ty[0][q], lds.tz[0][q]};
+                const double sq[3] = {lds.tx[1][q], lds.ty[1][q], lds.tz[1][q]};
+                double       d2    = 0.0;
+                for (int d = 0; d < 3; ++d)
+                {
+                    double v = fabs(cq[d] - p[d]) - sq[d];
+                    v        = v > 0.0 ? v : 0.0;
+                    d2 += v * v;
+                }
+                keepMask |= ballot(t.active && d2 < t.reach2) ? (uint64_t(1) << q) : uint64_t(0);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        else
+        {
+            for (int q = 0; q < nLeaf; ++q)
+            {
+                double d2 = 0.0;
+                for (int d = 0; d < 3; ++d)
+                {
+                    double v = readlaneD(c[d], q) - p[d];
+                    if (b.pbc[d]) { v = v - b.len[d] * rint(v * b.ilen[d]); }
+                    v = fabs(v) - readlaneD(sz[d], q);
+                    v = v > 0.0 ? v : 0.0;
+                    d2 += v * v;
+                }
+                keepMask |= ballot(t.active && d2 < t.reach2) ? (uint64_t(1) << q) : uint64_t(0);
+            }
         }
     }
     const bool     kept   = (keepMask >> lane) & 1u;
