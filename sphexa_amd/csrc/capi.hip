@@ -152,7 +152,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
         if (kv.second.stop) { (void)hipEventDestroy(kv.second.stop); }
     }
     ws->tree.releaseAll();
-    ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release(), ws->idxTmp.release();
+    ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release();
     ws->fieldTmp.release(), ws->order.release(), ws->records.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
     delete ws;
     return SPHX_OK;
@@ -208,7 +208,7 @@ int sphx_compute_sfc_keys(sphx_workspace_t* ws, const sphx_box_t* box, const dou
 {
     return guarded(ws, [&] {
         SPHX_REQUIRE(box && (n == 0 || (x && y && z && keys)), SPHX_ERR_ARG, "null argument");
-        computeKeys(ws, makeDevBox(box), x, y, z, keys, n);
+        computeKeys(ws, makeDevBox(box), x, y, z, keys, n, true);
         return SPHX_OK;
     });
 }
@@ -284,8 +284,7 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         }
         {
             KernelScope timer(ws, "sfc_keys");
-            fill64(ws, keys, 0, n);
-            computeKeys(ws, b, f->x, f->y, f->z, keys, n);
+            computeKeys(ws, b, f->x, f->y, f->z, keys, n, false); // the sync keys every particle
         }
         ws->order.reserve(n);
         DevBuf<uint32_t>& order = ws->order;

@@ -178,7 +178,6 @@ struct sphx_workspace
     // scratch
     sphx::DevBuf<char>     scratch;      // sort / scan temporaries
     sphx::DevBuf<uint64_t> keyTmp;
-    sphx::DevBuf<uint32_t> idxTmp;
     sphx::DevBuf<char>     fieldTmp;     // permutation staging
     sphx::DevBuf<uint32_t> order;        // sort permutation of the domain sync
     size_t                 syncN = 0;    // particles of that sync (0: none yet)
@@ -377,8 +376,9 @@ inline DevBox makeDevBox(const sphx_box_t* b)
  * host launchers implemented in the .hip translation units
  * -------------------------------------------------------------------------------------------*/
 void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n);
+//! Hilbert keys; keepRemoved: keys equal to removeKey (2^63) on entry stay
 void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
-                 size_t n);
+                 size_t n, bool keepRemoved);
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
 void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
 //! permute every field in place by `order` (4- or 8-byte elements), batched per element size

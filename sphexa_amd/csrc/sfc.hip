@@ -46,6 +46,8 @@ __device__ __forceinline__ Key hilbertKeyOf(double x, double y, double z, const 
     return key;
 }
 
+//! keys of n particles; with KeepRemoved, keys already equal to removeKey = 2^63 stay
+template<bool KeepRemoved>
 __global__ void __launch_bounds__(256) keysKernel(DevBox box, const double* __restrict__ x,
                                                   const double* __restrict__ y, const double* __restrict__ z,
                                                   Key* __restrict__ keys, size_t n)
@@ -53,14 +55,15 @@ __global__ void __launch_bounds__(256) keysKernel(DevBox box, const double* __re
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) { return; }
     constexpr Key removeKey = Key(1) << 63;
-    if (keys[i] != removeKey) { keys[i] = hilbertKeyOf(x[i], y[i], z[i], box); }
+    if (!KeepRemoved || keys[i] != removeKey) { keys[i] = hilbertKeyOf(x[i], y[i], z[i], box); }
 }
 
 void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
-                 size_t n)
+                 size_t n, bool keepRemoved)
 {
     if (n == 0) { return; }
-    keysKernel<<<ceilDiv(n, 256), 256, 0, ws->stream>>>(box, x, y, z, keys, n);
+    if (keepRemoved) { keysKernel<true><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(box, x, y, z, keys, n); }
+    else { keysKernel<false><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(box, x, y, z, keys, n); }
     SPHX_CHECK_LAUNCH();
 }
 
@@ -78,28 +81,19 @@ void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n)
     SPHX_CHECK_LAUNCH();
 }
 
-__global__ void iotaKernel(uint32_t* v, size_t n)
-{
-    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) { v[i] = uint32_t(i); }
-}
-
 //! Stable LSD radix sort of (key, original index) over all 64 key bits (removeKey = 2^63 sorts
 //! last), i.e. the order of std::stable_sort on the keys.
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n)
 {
     if (n == 0) { return; }
     ws->keyTmp.reserve(n);
-    ws->idxTmp.reserve(n);
-    iotaKernel<<<ceilDiv(n, 256), 256, 0, ws->stream>>>(ws->idxTmp.get(), n);
-    SPHX_CHECK_LAUNCH();
+    // the original indices come from a counting iterator: no index array is written or read
+    const rocprim::counting_iterator<uint32_t> iota(0);
 
     size_t tmpBytes = 0;
-    SPHX_HIP(rocprim::radix_sort_pairs(nullptr, tmpBytes, keys, ws->keyTmp.get(), ws->idxTmp.get(), order, n, 0, 64,
-                                       ws->stream));
+    SPHX_HIP(rocprim::radix_sort_pairs(nullptr, tmpBytes, keys, ws->keyTmp.get(), iota, order, n, 0, 64, ws->stream));
     void* tmp = scratchBytes(ws, tmpBytes);
-    SPHX_HIP(rocprim::radix_sort_pairs(tmp, tmpBytes, keys, ws->keyTmp.get(), ws->idxTmp.get(), order, n, 0, 64,
-                                       ws->stream));
+    SPHX_HIP(rocprim::radix_sort_pairs(tmp, tmpBytes, keys, ws->keyTmp.get(), iota, order, n, 0, 64, ws->stream));
     SPHX_HIP(hipMemcpyAsync(keys, ws->keyTmp.get(), n * sizeof(Key), hipMemcpyDeviceToDevice, ws->stream));
 }
 
