@@ -182,7 +182,9 @@ void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<
         else if (eb == 4) { f4.push_back(ptr); }
         else { throw Error{SPHX_ERR_ARG, "permute: element size must be 4 or 8 bytes"}; }
     }
-    ws->fieldTmp.reserve(n * 8 * kGatherBatch);
+    // scratch for the widest batch: up to kGatherBatch fields of one element size
+    const size_t w8 = std::min(f8.size(), size_t(kGatherBatch)), w4 = std::min(f4.size(), size_t(kGatherBatch));
+    ws->fieldTmp.reserve(n * std::max(8 * w8, 4 * w4)); // bytes
     permuteBatches<uint64_t>(ws, order, f8, n);
     permuteBatches<uint32_t>(ws, order, f4, n);
 }
