@@ -1,4 +1,5 @@
 // Hilbert keys, stable key sort and field permutation (the one-rank Domain::sync front end).
+#include <algorithm>
 #include <vector>
 #include <cstring> // rocprim uses memset without including it
 #include <rocprim/rocprim.hpp>
@@ -175,12 +176,24 @@ void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<
                    size_t n)
 {
     if (n == 0) { return; }
+    // an array named twice is permuted once; arrays that overlap otherwise have no defined result
+    std::vector<std::pair<void*, int>> sorted(fields);
+    std::sort(sorted.begin(), sorted.end());
     std::vector<void*> f8, f4;
-    for (auto& [ptr, eb] : fields)
+    for (size_t k = 0; k < sorted.size(); ++k)
     {
-        if (eb == 8) { f8.push_back(ptr); }
-        else if (eb == 4) { f4.push_back(ptr); }
-        else { throw Error{SPHX_ERR_ARG, "permute: element size must be 4 or 8 bytes"}; }
+        auto [ptr, eb] = sorted[k];
+        if (eb != 8 && eb != 4) { throw Error{SPHX_ERR_ARG, "permute: element size must be 4 or 8 bytes"}; }
+        if (k > 0)
+        {
+            auto [prev, peb] = sorted[k - 1];
+            if (ptr == prev && eb == peb) { continue; }
+            if (static_cast<char*>(prev) + n * size_t(peb) > static_cast<char*>(ptr))
+            {
+                throw Error{SPHX_ERR_ARG, "permute: two of the arrays overlap"};
+            }
+        }
+        (eb == 8 ? f8 : f4).push_back(ptr);
     }
     // scratch for the widest batch: up to kGatherBatch fields of one element size
     const size_t w8 = std::min(f8.size(), size_t(kGatherBatch)), w4 = std::min(f4.size(), size_t(kGatherBatch));

@@ -161,6 +161,16 @@ def test_domain_reorder_applies_the_sync_permutation(ws):
     assert torch.equal(extra[1], ps.t["z"][:n].float()) and torch.equal(extra[2], ps.t["h"][:n].float())
     with pytest.raises(Exception):
         ws.domain_reorder(ps.t["x_m1"], n - 1)
+    # an array named twice is permuted once; overlapping arrays are refused before any launch
+    twice = ps.t["y"][:n].clone()
+    once = ps.t["y"][:n].clone()
+    ws.domain_reorder([twice, twice], n)
+    ws.domain_reorder([once], n)
+    torch.cuda.synchronize()
+    assert torch.equal(twice, once)
+    big = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    with pytest.raises(Exception):
+        ws.domain_reorder([big[:n], big[n // 2: n // 2 + n]], n)
 
 
 def _oracle_steps(orc, arr, box, params, ts, steps, hydro, refit_box=False):
