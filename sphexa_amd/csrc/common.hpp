@@ -138,6 +138,12 @@ struct Tree
     DevBuf<TreeNodeIndex> childOffsets, parents, internalToLeaf, levelRange;
     DevBuf<uint32_t>     counts, layout;
     DevBuf<double>       centers, sizes;
+    // the neighbour search's visiting order (singleTraversal, traversal/traversal.hpp:52-93):
+    // leaf children in octant order, then internal children in reverse octant order, depth first.
+    // rankLo: position of a node's first leaf in that order; childPush: an internal node's
+    // children from last to first visited, 3 bits per octant; subtreeLeaves: scratch
+    DevBuf<int32_t>      rankLo, subtreeLeaves;
+    DevBuf<uint32_t>     childPush;
     bool                 valid = false;
     TreeBuildScratch     build;
 
@@ -145,7 +151,8 @@ struct Tree
     {
         prefixes.release(), leaves.release(), childOffsets.release(), parents.release();
         internalToLeaf.release(), levelRange.release(), counts.release(), layout.release();
-        centers.release(), sizes.release(), build.releaseAll();
+        centers.release(), sizes.release(), rankLo.release(), subtreeLeaves.release(), childPush.release();
+        build.releaseAll();
         valid = false;
     }
 };

@@ -4,7 +4,10 @@
 // Per group the wave walks the octree once with the group's bounding box grown by the largest
 // search radius 2*h_max (exact periodic box-box distance, so no cell holding a neighbour of any
 // target is skipped), collecting candidate leaves in LDS; internal nodes live on an LDS stack
-// popped 64 at a time, one node per lane, compacted with ballot/mbcnt. Candidate leaves that no
+// popped 64 at a time, one node per lane, compacted with ballot/mbcnt. The stack is kept in the
+// visiting order of the reference's per-target traversal (leaf children in octant order, then
+// internal children in reverse octant order, depth first), and found leaves are swept in that
+// order, so every list holds its neighbours in the reference's order. Candidate leaves that no
 // target's sphere reaches are dropped (one leaf per lane stages its box, moved to the periodic
 // image the group sees, in LDS; the wave reads each back as a broadcast, one ballot per leaf).
 // The particles of the surviving leaves within reach of the group box are compacted into full
@@ -13,8 +16,9 @@
 // every lane against its own target with the reference predicate
 // (domain/include/cstone/findneighbors.hpp:33-60,93,116-140): d2 = dx*dx + dy*dy + dz*dz with
 // dx = x_j - x_i, periodic images folded by one box length, unfused FP64 arithmetic, keep if
-// d2 < Th(4)*h_i*h_i and j != i. The kept set and count are therefore bit-identical to the CPU
-// search; the first ngmax hits are stored in the group layout of common.hpp (listOffset): a lane
+// d2 < Th(4)*h_i*h_i and j != i. The lists, counts and, past ngmax, the kept first ngmax hits
+// are therefore bit-identical to the CPU search; they are stored in the group layout of
+// common.hpp (listOffset): a lane
 // shifts its entries through a register quad and stores 16 bytes at a time, and the pair loops read
 // four slots of all 64 lanes with one coalesced load.
 #include "pairs.hpp"
@@ -26,8 +30,9 @@ namespace
 {
 
 constexpr int kWavesPerBlock = 4;
-constexpr int kStackCap      = 1024; // LDS node stack per wave (bound argued at searchGroup)
-constexpr int kLeafCap       = 64;   // LDS candidate-leaf buffer per wave (one leaf per lane)
+constexpr int kStackCap      = 952;  // LDS node stack per wave (bound argued at searchGroupInto)
+constexpr int kLeafCap       = 64;   // leaves per sweep of candidate leaves (one per lane)
+constexpr int kBufCap        = 128;  // LDS buffer of found leaves awaiting their turn in visiting order
 constexpr int kPerPairImage  = 64;   // image code of a leaf swept with the per-pair fold
 constexpr int kNoShiftImage  = 13;   // image code of shift (0, 0, 0)
 
@@ -38,6 +43,8 @@ struct SearchArgs
     const double*        z;
     const TreeNodeIndex* childOffsets;
     const TreeNodeIndex* internalToLeaf;
+    const int32_t*       rankLo;    // visiting order of the reference traversal (first leaf of a node)
+    const uint32_t*      childPush; // an internal node's children from last to first visited
     const uint32_t*      layout;
     const double*        centers;
     const double*        sizes;
@@ -62,8 +69,9 @@ struct SearchArgs
 //! double-buffered candidate tile
 struct WaveLds
 {
-    int32_t  stack[kStackCap];
-    int32_t  leaves[kLeafCap];      // candidate leaf nodes, traversal order
+    int32_t  stack[kStackCap];      // pending nodes, the first in visiting order on top
+    int32_t  bufNode[kBufCap];      // found leaves ...
+    int32_t  bufRank[kBufCap];      // ... and their visiting-order ranks
     uint32_t leafLo[kLeafCap];      // kept leaves: first particle
     uint32_t leafPre[kLeafCap + 1]; // kept leaves: exclusive prefix of particle counts, then the total
     int32_t  leafImg[kLeafCap];     // kept leaves: periodic image code (imageShift)
@@ -71,6 +79,9 @@ struct WaveLds
     uint32_t tj[2][kWave]; // particle index of each staged candidate
     double   tm[2][kWave]; // its mass (fused XMass only)
 };
+
+// four 4-wave blocks share a CU's 160 KB of LDS
+static_assert(kWavesPerBlock * sizeof(WaveLds) <= 160 * 1024 / 4, "search LDS must leave room for four blocks per CU");
 
 //! per-target state of a lane
 struct Target
@@ -312,7 +323,8 @@ __device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t
  *    window's loads are in flight. The candidate order is therefore the traversal order, as in
  *    the reference. */
 template<class Sink>
-__device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, int nLeaf, const double gc[3],
+__device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, const int32_t* leaves, int nLeaf,
+                            const double gc[3],
                             const double ge[3], double reach, unsigned& count, ListWriter& out, Sink& sink)
 {
 #pragma clang fp contract(off)
@@ -325,7 +337,7 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
     int      code = kNoShiftImage;
     if (int(lane) < nLeaf)
     {
-        const int32_t node = lds.leaves[lane];
+        const int32_t node = leaves[lane];
         for (int d = 0; d < 3; ++d)
         {
             c[d]  = a.centers[3 * size_t(node) + d];
@@ -494,13 +506,61 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
     if (acc > 0) { sweepBuffer(a, t, lds, cur, acc, tileImg, count, out, sink); }
 }
 
+//! Sort the first n entries of the found-leaf buffer by rank (bitonic network over the next
+//! power of two, the padding ranked last; ranks of distinct leaves are distinct)
+__device__ __forceinline__ void sortFoundLeaves(WaveLds& lds, int n)
+{
+    const unsigned lane = laneId();
+    int            size = 2;
+    while (size < n) { size <<= 1; }
+    for (int q = n + int(lane); q < size; q += kWave)
+    {
+        lds.bufRank[q] = INT32_MAX;
+        lds.bufNode[q] = -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 2; k <= size; k <<= 1)
+    {
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int p = int(lane); p < (size >> 1); p += kWave)
+            {
+                const int  i  = 2 * j * (p / j) + (p % j);
+                const int  l  = i + j;
+                const bool up = (i & k) == 0;
+                const int  ri = lds.bufRank[i], rl = lds.bufRank[l];
+                if ((ri > rl) == up)
+                {
+                    const int ni = lds.bufNode[i], nl = lds.bufNode[l];
+                    lds.bufRank[i] = rl, lds.bufRank[l] = ri;
+                    lds.bufNode[i] = nl, lds.bufNode[l] = ni;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
 /* Collect all leaves whose (periodic) distance to the active targets' bounding box is below the
- * largest active search radius and sweep them. Returns each lane's neighbour count.
+ * largest active search radius and sweep them in the reference's visiting order
+ * (singleTraversal, traversal/traversal.hpp:52-93: at each node its leaf children in octant
+ * order, then its internal children in reverse octant order, depth first), which is the order
+ * of each target's list. Returns each lane's neighbour count.
  *
- * Stack bound: nodes are popped 64 at a time only while top + 7*64 <= kStackCap - 160, else one
- * at a time. A one-at-a-time pop replaces a node by at most 8 children and the next pop takes
- * the last child, i.e. it proceeds depth first, which adds at most 7 entries per tree level
- * below the popped node (<= 7*20 = 140 < 160) on top of the stack it started from. */
+ * Every node carries the rank of its first leaf in that order (rankLo), and the leaves of a
+ * subtree have consecutive ranks. The stack holds pending nodes sorted by rank, the first on top:
+ * a pop takes the up to 64 first pending nodes (lane 0 the first), whose subtrees all precede the
+ * rest of the stack, and pushes the children of the overlapping internal ones back on top in the
+ * same order (childPush). Found leaves wait in an LDS buffer with their ranks; once no pending
+ * node can hold an earlier leaf (rank below the top node's), they are final and are swept in rank
+ * order, 64 at a time.
+ *
+ * Bounds: nodes are popped 64 at a time only while top + 7*64 <= kStackCap - 160 and the buffer
+ * can take 64 more leaves with 8 to spare, else one at a time. A single pop takes the first
+ * pending node and replaces it by at most 8 children, the next pop taking the first of them,
+ * i.e. it proceeds depth first and adds at most 7 entries per tree level below the popped node
+ * (<= 7*20 = 140 < 160) on top of the stack it started from. A single pop adds at most one leaf,
+ * the first in order of all, and the buffer is swept whenever it reaches its spare slots. */
 template<class Sink>
 __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out,
                                     Sink& sink)
@@ -523,15 +583,41 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
 
     unsigned count = 0;
     int      top   = 1;
-    int      nLeaf = 0;
+    int      nBuf  = 0;
     if (lane == 0) { lds.stack[0] = 0; }
     __builtin_amdgcn_wave_barrier();
 
+    // sweep the buffered leaves ranked below `limit`, in rank order, and keep the others
+    auto sweepFound = [&](int32_t limit) {
+        sortFoundLeaves(lds, nBuf);
+        int final = 0;
+        for (int q = 0; q < nBuf; q += kWave)
+        {
+            const int i = q + int(lane);
+            final += __popcll(ballot(i < nBuf && lds.bufRank[i] < limit));
+        }
+        for (int done = 0; done < final; done += kLeafCap)
+        {
+            flushLeaves(a, t, lds, lds.bufNode + done, min(kLeafCap, final - done), gc, ge, reach, count, out, sink);
+        }
+        // move the rest to the front
+        const int rest = nBuf - final;
+        int32_t   n0 = 0, r0 = 0, n1 = 0, r1 = 0;
+        if (int(lane) < rest) { n0 = lds.bufNode[final + lane], r0 = lds.bufRank[final + lane]; }
+        if (int(lane) + kWave < rest) { n1 = lds.bufNode[final + kWave + lane], r1 = lds.bufRank[final + kWave + lane]; }
+        __builtin_amdgcn_wave_barrier();
+        if (int(lane) < rest) { lds.bufNode[lane] = n0, lds.bufRank[lane] = r0; }
+        if (int(lane) + kWave < rest) { lds.bufNode[kWave + lane] = n1, lds.bufRank[kWave + lane] = r1; }
+        __builtin_amdgcn_wave_barrier();
+        nBuf = rest;
+    };
+
     while (top > 0)
     {
-        int k = (top + 7 * kWave <= kStackCap - 160) ? min(kWave, top) : 1;
-        int node = -1;
-        if (int(lane) < k) { node = lds.stack[top - k + int(lane)]; }
+        const bool batch = top + 7 * kWave <= kStackCap - 160 && nBuf + kWave <= kBufCap - 8;
+        const int  k     = batch ? min(kWave, top) : 1;
+        int        node  = -1;
+        if (int(lane) < k) { node = lds.stack[top - 1 - int(lane)]; }
         top -= k;
         __builtin_amdgcn_wave_barrier();
 
@@ -552,26 +638,41 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
             child  = a.childOffsets[node];
             isLeaf = child == 0;
         }
-        uint64_t mLeaf = ballot(ov && isLeaf);
-        uint64_t mInt  = ballot(ov && !isLeaf);
-        int      nl    = __popcll(mLeaf);
-        if (nLeaf + nl > kLeafCap)
+        const uint64_t mLeaf = ballot(ov && isLeaf);
+        const uint64_t mInt  = ballot(ov && !isLeaf);
+        if (ov && isLeaf)
         {
-            flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out, sink);
-            nLeaf = 0;
+            const int pos  = nBuf + int(laneRank(mLeaf));
+            lds.bufNode[pos] = node;
+            lds.bufRank[pos] = a.rankLo[node];
         }
-        if (ov && isLeaf) { lds.leaves[nLeaf + laneRank(mLeaf)] = node; }
         if (ov && !isLeaf)
         {
-            int pos = top + 8 * int(laneRank(mInt));
+            // the children of later lanes go deeper: lane 0's, the first in order, end on top
+            const int      later = __popcll(lane + 1 < kWave ? mInt >> (lane + 1) : uint64_t(0));
+            const int      pos   = top + 8 * later;
+            const uint32_t push  = a.childPush[node];
             for (int c = 0; c < 8; ++c)
-                lds.stack[pos + c] = child + c;
+                lds.stack[pos + c] = child + int((push >> (3 * c)) & 7u);
         }
-        nLeaf += nl;
+        nBuf += __popcll(mLeaf);
         top += 8 * __popcll(mInt);
         __builtin_amdgcn_wave_barrier();
+
+        // leaves ranked below the first pending node are final
+        if (nBuf >= kLeafCap || nBuf > kBufCap - 8 - kWave)
+        {
+            const int32_t limit = top > 0 ? a.rankLo[lds.stack[top - 1]] : INT32_MAX;
+            int           final = 0;
+            for (int q = 0; q < nBuf; q += kWave)
+            {
+                const int i = q + int(lane);
+                final += __popcll(ballot(i < nBuf && lds.bufRank[i] < limit));
+            }
+            if (final >= kLeafCap || (nBuf > kBufCap - 8 && final > 0)) { sweepFound(limit); }
+        }
     }
-    flushLeaves(a, t, lds, nLeaf, gc, ge, reach, count, out, sink);
+    sweepFound(INT32_MAX);
     return count;
 }
 
@@ -747,6 +848,8 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.x = f->x, a.y = f->y, a.z = f->z;
     a.childOffsets   = t.childOffsets.get();
     a.internalToLeaf = t.internalToLeaf.get();
+    a.rankLo         = t.rankLo.get();
+    a.childPush      = t.childPush.get();
     a.layout         = t.layout.get();
     a.centers        = t.centers.get();
     a.sizes          = t.sizes.get();

@@ -149,6 +149,58 @@ __global__ void leafFinalize(const Key* sortedLeafKey, const uint32_t* sortedLea
     internalToLeaf[node] = TreeNodeIndex(s);
 }
 
+//! leaves in the subtree of each node of one level (the deeper levels are done)
+__global__ void subtreeLeavesKernel(const TreeNodeIndex* childOffsets, int32_t* leaves, int32_t first, int32_t last)
+{
+    const int32_t i = first + int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= last) { return; }
+    const TreeNodeIndex c = childOffsets[i];
+    int32_t             s = 1;
+    if (c != 0)
+    {
+        s = 0;
+        for (int o = 0; o < 8; ++o)
+            s += leaves[c + o];
+    }
+    leaves[i] = s;
+}
+
+/*! Visiting order of the search for the children of each internal node of one level (the
+ *  shallower levels are done): singleTraversal (traversal/traversal.hpp:52-93) handles a node's
+ *  leaf children in octant order as it meets them, pushes its internal children in octant order
+ *  and pops the last pushed first. rankLo of a child = rankLo of the parent plus the leaves visited
+ *  before it; childPush lists the children from last to first visited. */
+__global__ void visitOrderKernel(const TreeNodeIndex* childOffsets, const int32_t* leaves, int32_t* rankLo,
+                                 uint32_t* childPush, int32_t first, int32_t last)
+{
+    const int32_t i = first + int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= last) { return; }
+    const TreeNodeIndex c = childOffsets[i];
+    if (c == 0) { return; }
+    int32_t  base = rankLo[i];
+    bool     isLeaf[8];
+    for (int o = 0; o < 8; ++o)
+        isLeaf[o] = childOffsets[c + o] == 0;
+    uint32_t push = 0;
+    int      slot = 7; // childPush slot 7 = the first visited
+    for (int o = 0; o < 8; ++o)
+    {
+        if (!isLeaf[o]) { continue; }
+        rankLo[c + o] = base++;
+        push |= uint32_t(o) << (3 * slot--);
+    }
+    for (int o = 7; o >= 0; --o)
+    {
+        if (isLeaf[o]) { continue; }
+        rankLo[c + o] = base;
+        base += leaves[c + o];
+        push |= uint32_t(o) << (3 * slot--);
+    }
+    childPush[i] = push;
+}
+
+__global__ void zeroRankKernel(int32_t* rankLo) { rankLo[0] = 0; }
+
 template<class T>
 void growCopy(sphx_workspace* ws, DevBuf<T>& buf, size_t used, size_t need)
 {
@@ -268,6 +320,31 @@ void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned 
         levelRange[l] = levelStart[l];
     levelRange[kMaxLevel + 1] = int32_t(numNodes);
     SPHX_HIP(hipMemcpyAsync(t.levelRange.get(), levelRange.data(), (kMaxLevel + 2) * 4, hipMemcpyHostToDevice, st));
+
+    // the search's visiting order: subtree leaf counts bottom-up, then ranks top-down
+    t.rankLo.reserve(numNodes), t.subtreeLeaves.reserve(numNodes), t.childPush.reserve(numNodes);
+    for (int l = int(level); l >= 0; --l)
+    {
+        const int32_t a = levelRange[l], b = levelRange[l + 1];
+        if (b > a)
+        {
+            subtreeLeavesKernel<<<ceilDiv(size_t(b - a), 256), 256, 0, st>>>(t.childOffsets.get(),
+                                                                          t.subtreeLeaves.get(), a, b);
+            SPHX_CHECK_LAUNCH();
+        }
+    }
+    zeroRankKernel<<<1, 1, 0, st>>>(t.rankLo.get());
+    SPHX_CHECK_LAUNCH();
+    for (int l = 0; l < int(level); ++l)
+    {
+        const int32_t a = levelRange[l], b = levelRange[l + 1];
+        if (b > a)
+        {
+            visitOrderKernel<<<ceilDiv(size_t(b - a), 256), 256, 0, st>>>(
+                t.childOffsets.get(), t.subtreeLeaves.get(), t.rankLo.get(), t.childPush.get(), a, b);
+            SPHX_CHECK_LAUNCH();
+        }
+    }
     SPHX_HIP(hipStreamSynchronize(st));
 
     t.numLeafNodes     = int32_t(L);

@@ -144,6 +144,77 @@ def test_neighbors_bit_exact(ws, orc, case, iterate, mixed):
     assert s == int(np.minimum(o["a"]["nc"].astype(np.int64) - 1, params.ngmax).sum())
 
 
+def assert_same_lists(nb_g, nb_o, nc, ngmax):
+    """Stored entries equal in order: the traversal order of the reference's DFS, and with
+    nc - 1 > ngmax the first ngmax of that order."""
+    for i in range(nb_g.shape[0]):
+        k = min(int(nc[i]) - 1, ngmax)
+        np.testing.assert_array_equal(nb_g[i, :k], nb_o[i, :k], err_msg=f"target {i}")
+
+
+@pytest.mark.parametrize("case", ["sedov16", "evrard6k"])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_neighbor_order_and_truncation(ws, orc, case, mixed):
+    """Lists in traversal order, bit for bit; with ngmax = 40 (no h iteration) nearly every target
+    overflows and keeps the first 40 neighbours of the traversal
+    (findNeighbors, domain/include/cstone/findneighbors.hpp:116-140), and the counts still hold
+    the full neighbour number."""
+    from oracle import pipeline
+
+    if case == "sedov16":
+        arr, spec = ic.sedov_lattice(16, seed=13)
+    else:
+        arr, spec = ic.evrard_sphere(6000, seed=9)
+    box = make_box(spec)
+    hyd = np.float32 if mixed else np.float64
+    for ngmax in (150, 40):
+        # ng0 must stay below ngmax (findNeighborsSfc's check); without the h iteration it is unused
+        params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, ngmax=ngmax,
+                                     ng0=min(100, ngmax - 10))
+        o = pipeline.run(orc, arr, box, params, form="search", iterate_h=False, hydro=hyd)
+        ps = ParticleSet.from_host(arr, STD_FIELDS, mixed=mixed)
+        f = ps.fields()
+        ws.domain_sync(box, f, params, ps.n)
+        ws.find_neighbors(box, f, params, 0, ps.n, False)
+        nc_g = ps.host("nc")
+        np.testing.assert_array_equal(nc_g, o["a"]["nc"])
+        if ngmax == 40:
+            assert np.mean(nc_g - 1 > ngmax) > 0.5
+        assert_same_lists(gpu_lists(ws, ps, ngmax), o["nbrs"], nc_g, ngmax)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 63, 64, 65, 130])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_neighbors_few_particles(ws, orc, n, periodic):
+    """Tiny sets: one partial group, exactly one group, one group plus one target; a lone
+    particle has no neighbours (nc = 1). Random points with h from the local spacing."""
+    from oracle import pipeline
+
+    rng = np.random.default_rng(100 + n)
+    arr = {k: rng.uniform(-0.5, 0.5, n) for k in "xyz"}
+    arr["h"] = np.full(n, 0.5 * (1.0 / n) ** (1 / 3) + 0.05)
+    arr["m"] = np.full(n, 1.0 / n)
+    for k in ("u", "vx", "vy", "vz"):
+        arr[k] = np.zeros(n)
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC if periodic else abi.OPEN)
+    params = abi.Params.defaults(ws.K)
+    for iterate in (False, True):
+        o = pipeline.run(orc, arr, box, params, form="search", iterate_h=iterate)
+        ps = ParticleSet.from_host(arr, STD_FIELDS)
+        f = ps.fields()
+        ws.domain_sync(box, f, params, ps.n)
+        ws.find_neighbors(box, f, params, 0, ps.n, iterate)
+        nc_g = ps.host("nc")
+        np.testing.assert_array_equal(nc_g, o["a"]["nc"])
+        if iterate:
+            # a lone particle never reaches ng0/4 neighbours and takes all 10 h updates; each may
+            # differ by an ulp between the device pow and the C library's
+            np.testing.assert_allclose(ps.host("h"), o["a"]["h"], rtol=32 * np.finfo(np.float64).eps, atol=0)
+        assert_same_lists(gpu_lists(ws, ps, params.ngmax), o["nbrs"], nc_g, params.ngmax)
+        if n == 1:
+            assert nc_g[0] == 1
+
+
 def _std_oracle(orc, arr, box, params):
     from oracle import pipeline
 
