@@ -233,9 +233,11 @@ def test_std_fp64_parity(ws, orc):
     conv, dt = std_step(ws, ps, box, params)
     assert conv
     np.testing.assert_array_equal(ps.host("nc"), o["a"]["nc"])
+    # the lists (and so the sums) follow the reference's order; what remains is the rounding of
+    # differently grouped products in a few terms: within a few ulp of each field's largest value
     for k in ("rho", "p", "c", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az", "du"):
         err = rel_to_max(ps.host(k), o["a"][k])
-        assert err < 1e-10, (k, err)
+        assert err < 1e-13, (k, err)
     assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
 
 
@@ -259,9 +261,10 @@ def test_ve_fp64_parity(ws, orc, av_clean):
     assert conv
     outs = ("xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "c33", "divv", "curlv", "alpha",
             "ax", "ay", "az", "du")
+    # within a few ulp of each field's largest value (divv, a difference of larger terms, 5e-14)
     for k in outs + (GRADV_FIELDS if av_clean else ()):
         err = rel_to_max(ps.host(k), o["a"][k])
-        assert err < 1e-10, (k, err)
+        assert err < 1e-12, (k, err)
     assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
 
 
