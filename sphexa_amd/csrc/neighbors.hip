@@ -511,7 +511,12 @@ __device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, 
 __device__ __forceinline__ void sortFoundLeaves(WaveLds& lds, int n)
 {
     const unsigned lane = laneId();
-    int            size = 2;
+    // leaves found in one pop arrive in order: often there is nothing to sort
+    bool inversion = false;
+    for (int q = int(lane); q + 1 < n; q += kWave)
+        inversion = inversion || lds.bufRank[q] > lds.bufRank[q + 1];
+    if (!ballot(inversion)) { return; }
+    int size = 2;
     while (size < n) { size <<= 1; }
     for (int q = n + int(lane); q < size; q += kWave)
     {
