@@ -626,10 +626,16 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
         top -= k;
         __builtin_amdgcn_wave_barrier();
 
-        bool ov = false, isLeaf = false;
-        int  child = 0;
+        bool     ov = false, isLeaf = false;
+        int      child = 0;
+        int32_t  rank  = 0;
+        uint32_t push  = 0;
         if (node >= 0)
         {
+            // the node's rank and child order load with its box (childPush is read for leaves too,
+            // unused there)
+            rank = a.rankLo[node];
+            push = a.childPush[node];
             double d2 = 0.0;
             for (int d = 0; d < 3; ++d)
             {
@@ -649,14 +655,13 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
         {
             const int pos  = nBuf + int(laneRank(mLeaf));
             lds.bufNode[pos] = node;
-            lds.bufRank[pos] = a.rankLo[node];
+            lds.bufRank[pos] = rank;
         }
         if (ov && !isLeaf)
         {
             // the children of later lanes go deeper: lane 0's, the first in order, end on top
-            const int      later = __popcll(lane + 1 < kWave ? mInt >> (lane + 1) : uint64_t(0));
-            const int      pos   = top + 8 * later;
-            const uint32_t push  = a.childPush[node];
+            const int later = __popcll(lane + 1 < kWave ? mInt >> (lane + 1) : uint64_t(0));
+            const int pos   = top + 8 * later;
             for (int c = 0; c < 8; ++c)
                 lds.stack[pos + c] = child + int((push >> (3 * c)) & 7u);
         }
@@ -667,7 +672,11 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
         // leaves ranked below the first pending node are final
         if (nBuf >= kLeafCap || nBuf > kBufCap - 8 - kWave)
         {
-            const int32_t limit = top > 0 ? a.rankLo[lds.stack[top - 1]] : INT32_MAX;
+            // the first pending node: the first visited child of the first pushed node, whose rank
+            // is its parent's, or else the node left below the popped ones
+            int32_t limit = INT32_MAX;
+            if (mInt) { limit = __builtin_amdgcn_readlane(rank, __builtin_ctzll(mInt)); }
+            else if (top > 0) { limit = a.rankLo[lds.stack[top - 1]]; }
             int           final = 0;
             for (int q = 0; q < nBuf; q += kWave)
             {
