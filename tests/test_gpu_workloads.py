@@ -1,7 +1,7 @@
 """The benchmark configurations as parity cases at their full sizes.
 
 * Sedov-like std pass, 30^3 = 27,000 and 100^3 = 1,000,000 particles (FP64 and mixed) against the
-  FP64 oracle: neighbour counts and sets bit-exact (FP64 form), FP64 fields within 1e-10 of each
+  FP64 oracle: neighbour counts and lists bit-exact and in order (FP64 form), FP64 fields within 1e-10 of each
   field's largest magnitude, and the mixed form's max and RMS relative error reported per field
   with the density (pointwise) <= 1e-5 and the accelerations (of the largest magnitude) <= 1e-4.
 * Evrard-like collapsing sphere, 4,000,000 particles in the open box [-1,1]^3 with h spanning
@@ -90,10 +90,11 @@ def test_sedov_std_against_oracle(ws, orc, side):
     ws.copy_neighbors(out, 0, ps.n)
     nb_g = out.cpu().numpy().view(np.uint32).reshape(ps.n, params64.ngmax)
     nb_o = np.asarray(o["nbrs"]).reshape(ps.n, params64.ngmax)
-    cg, skipped = canonical_lists(nb_g, nc, params64.ngmax)
-    co, _ = canonical_lists(nb_o, nc, params64.ngmax)
-    assert np.array_equal(cg, co)
-    print(f"{side}^3: neighbour sets identical for {ps.n - skipped} targets ({skipped} over ngmax)", flush=True)
+    # every stored entry, in order (the reference's visiting order, and its first ngmax past ngmax)
+    k = np.minimum(nc.astype(np.int64) - 1, params64.ngmax)
+    used = np.arange(params64.ngmax)[None, :] < k[:, None]
+    assert np.array_equal(np.where(used, nb_g, 0), np.where(used, nb_o, 0))
+    print(f"{side}^3: neighbour lists identical, in order, for all {ps.n} targets", flush=True)
     e64 = errors({k: ps.host(k) for k in STD_OUT}, o["a"], STD_OUT)
     report(f"{side}^3 fp64", e64)
     assert max(v[0] for v in e64.values()) < 1e-10
@@ -155,3 +156,65 @@ def test_evrard_4m_mixed_against_fp64(ws, form):
         return
     for k in ("ax", "ay", "az"):
         assert e[k][0] <= 1e-4, k
+
+
+def test_turbulence_8m_properties(ws):
+    """The headline configuration (turbulence box, 200^3 = 8M particles, VE, both forms) checked
+    through properties that need no CPU reference at this size: keys sorted; the tree's leaf
+    counts sum to N, none above the bucket size, and the layout is their prefix sum; every stored
+    neighbour is another particle within the search radius (minimum-image distance in FP64 against
+    4 h^2, to the rounding of 4 h^2 in h's type); each
+    list holds min(nc - 1, ngmax) entries, their total equals the search's pair count; the h/nc
+    iteration converged with nc in [ng0/4, ngmax + 1]."""
+    import torch
+
+    from test_gpu_parity import d2h
+
+    arr, spec = ic.turbulence_box(200, seed=1)
+    arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    L = spec["hi"] - spec["lo"]
+    for mixed in (True, False):
+        params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-4)
+        ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+        n = ps.n
+        conv, dt = ve_step(ws, ps, box, params)
+        assert conv and np.isfinite(dt) and dt > 0
+        keys = ps.t["keys"][:n]
+        assert bool((keys[1:] >= keys[:-1]).all())
+        tv = ws.tree_view()
+        counts = d2h(tv.counts, tv.numLeafNodes, np.uint32).astype(np.int64)
+        layout = d2h(tv.layout, tv.numLeafNodes + 1, np.uint32).astype(np.int64)
+        assert counts.sum() == n and counts.max() <= 64
+        assert np.array_equal(layout[1:] - layout[:-1], counts) and layout[0] == 0
+        nc = ps.t["nc"][:n].to(torch.int64)
+        assert int(nc.min()) >= params.ng0 // 4 and int(nc.max()) <= params.ngmax + 1
+        lists = torch.empty(n * params.ngmax, dtype=torch.int32, device="cuda")
+        ws.copy_neighbors(lists, 0, n)
+        lists = lists.view(n, params.ngmax)
+        pairs, _ = ws.neighbor_stats()
+        stored = (nc - 1).clamp(max=params.ngmax)
+        assert int(stored.sum()) == pairs
+        h = ps.t["h"][:n].to(torch.float64)
+        x, y, z = (ps.t[c][:n] for c in "xyz")
+        chunk = 1 << 20
+        for f in range(0, n, chunk):
+            l = min(f + chunk, n)
+            j = lists[f:l].to(torch.int64)
+            used = torch.arange(params.ngmax, device="cuda")[None, :] < stored[f:l, None]
+            jj = torch.where(used, j, torch.arange(f, l, device="cuda")[:, None])
+            assert int(jj.min()) >= 0 and int(jj.max()) < n
+            assert not bool(((jj == torch.arange(f, l, device="cuda")[:, None]) & used).any())
+            d2 = torch.zeros_like(jj, dtype=torch.float64)
+            for c in (x, y, z):
+                d = c[jj] - c[f:l, None]
+                d = d - L * torch.round(d / L)
+                d2 += d * d
+            # the search compares with 4 h^2 formed in h's type (FP32 in the mixed form)
+            r2 = (4.0 * h[f:l, None] * h[f:l, None]).expand_as(d2)
+            ratio = float((d2[used] / r2[used]).max())
+            assert ratio < 1 + 1e-6, ratio
+        print(f"{'mixed' if mixed else 'fp64'} 8M: {pairs} pairs, nc in [{int(nc.min())}, {int(nc.max())}], "
+              f"{tv.numLeafNodes} leaves", flush=True)
+        del ps, lists
+        torch.cuda.empty_cache()
