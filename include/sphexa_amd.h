@@ -128,6 +128,17 @@ typedef struct sphx_params
  * std adjoint differentiates against. */
 #define SPHX_FLAG_STD_ALPHA 1
 
+/* Record chaining of the VE pass. Set on the calls of sphx_iad_divv_curlv, sphx_av_switches and
+ * sphx_momentum_energy_ve over the full range [0, n), it promises that the caller changes no
+ * particle field between these consecutive calls on one workspace. sphx_iad_divv_curlv then writes
+ * the neighbour records of sphx_av_switches in its epilogue, and sphx_av_switches those of
+ * sphx_momentum_energy_ve, so the next call skips its packing kernel. The next call uses them only
+ * if it comes directly after the producing one (sphx_rho_timestep, sphx_compute_timestep and
+ * sphx_conserved_quantities, which read fields only, may sit between), on the same fields, box,
+ * form and particle count; otherwise it packs as without the flag. Results are bitwise the same
+ * with or without the flag. */
+#define SPHX_FLAG_CHAIN 2
+
 /* Previous-step fields of the time integrator (particles_data.hpp:221,233,238). x_m1, y_m1, z_m1
  * (X_n - X_n-1) and du_m1 are XM1Type, which is the HydroType here (double in the plain form,
  * float in the mixed form, as SphTypes); mui is HydroType, NULL for the constant muiConst. */

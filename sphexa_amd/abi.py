@@ -78,6 +78,15 @@ class Params(C.Structure):
 
 
 SPHX_FLAG_STD_ALPHA = 1
+SPHX_FLAG_CHAIN = 2
+
+
+def chained(params):
+    """A copy of params with SPHX_FLAG_CHAIN: the VE IAD and AV passes write the next pass's
+    neighbour records in their epilogues (bitwise the same results, two pack kernels fewer)."""
+    p = Params.from_buffer_copy(params)
+    p.flags |= SPHX_FLAG_CHAIN
+    return p
 
 FIXED = 2  # cstone::BoundaryType::fixed (box.hpp:80-85)
 

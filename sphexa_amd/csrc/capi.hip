@@ -13,10 +13,12 @@ namespace sphx
 namespace
 {
 
+//! runs an entry point's body with error translation; `writes` = it may change particle fields
 template<class F>
-int guarded(sphx_workspace* ws, F&& body)
+int guarded(sphx_workspace* ws, F&& body, bool writes = true)
 {
     if (ws == nullptr) { return SPHX_ERR_ARG; }
+    if (writes) { ++ws->callSerial; }
     try
     {
         SPHX_HIP(hipSetDevice(ws->device));
@@ -451,7 +453,7 @@ int sphx_rho_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_p
         SPHX_REQUIRE(f && prm && ts, SPHX_ERR_ARG, "null argument");
         rhoTimestep(ws, f, prm, first, last, ts);
         return SPHX_OK;
-    });
+    }, false);
 }
 
 int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
@@ -461,7 +463,7 @@ int sphx_compute_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sp
         SPHX_REQUIRE(f && prm && ts, SPHX_ERR_ARG, "null argument");
         computeTimestep(ws, f, prm, first, last, ts);
         return SPHX_OK;
-    });
+    }, false);
 }
 
 int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
@@ -492,7 +494,7 @@ int sphx_conserved_quantities(sphx_workspace_t* ws, const sphx_fields_t* f, cons
         SPHX_REQUIRE(f && prm && out, SPHX_ERR_ARG, "null argument");
         conservedQuantities(ws, f, prm, first, last, cv, out);
         return SPHX_OK;
-    });
+    }, false);
 }
 
 int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,

@@ -189,6 +189,24 @@ struct sphx_workspace
     sphx::DevBuf<uint32_t> order;        // sort permutation of the domain sync
     size_t                 syncN = 0;    // particles of that sync (0: none yet)
     sphx::DevBuf<sphx::Chunk16> records; // packed neighbour records of the current pair loop
+    sphx::DevBuf<sphx::Chunk16> records2; // records a pair loop's epilogue writes for the next loop
+
+    // record chaining (SPHX_FLAG_CHAIN): every entry point that may write particle fields or the
+    // neighbour lists increments callSerial; a pair loop that wrote the next loop's records in its
+    // epilogue leaves this stamp, and the next call uses those records only if it is the very
+    // next such call, on the same particles, fields, box and form
+    uint64_t callSerial = 0;
+    struct RecordChain
+    {
+        int           kind   = 0; // record type written (0: none)
+        uint64_t      serial = 0; // callSerial of the producing call
+        uint32_t      n      = 0;
+        int           form   = -1;
+        int           buf    = 0; // record buffer holding them
+        sphx_fields_t fields{};
+        double        lo[3]{}, hi[3]{};
+        int           pbc[3]{};
+    } chain;
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions

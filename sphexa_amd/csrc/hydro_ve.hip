@@ -97,6 +97,41 @@ __global__ void __launch_bounds__(256) eosVeKernel(const double* __restrict__ u,
     if (pOut) { pOut[i] = T(pi); }
 }
 
+/* Record builders shared by the pack kernels and the epilogues that write the next pass's records
+ * (SPHX_FLAG_CHAIN), so that both give the same bits. */
+template<class T>
+__device__ __forceinline__ RecAv<T> avRecord(Coord<T> x, Coord<T> y, Coord<T> z, T xm, T kx, T c, T divv, T vx, T vy,
+                                             T vz)
+{
+    RecAv<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.vol  = fastDiv(xm, kx);
+    r.c    = c;
+    r.divv = divv;
+    r.vx = vx, r.vy = vy, r.vz = vz;
+    return r;
+}
+
+//! C = {c11, c12, c13, c22, c23, c33}
+template<class T>
+__device__ __forceinline__ RecMomVe<T> momVeRecord(Coord<T> x, Coord<T> y, Coord<T> z, T vx, T vy, T vz, T h, T m,
+                                                   T c, T xm, T kx, T alpha, T prho, const T C[6])
+{
+    RecMomVe<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.vx = vx, r.vy = vy, r.vz = vz;
+    r.hInv  = T(1) / h;
+    r.m     = m;
+    r.c     = c;
+    r.xm    = xm;
+    r.rho   = kx * m / xm;
+    r.vol   = fastDiv(m, r.rho);
+    r.alpha = alpha;
+    r.prho  = prho;
+    r.c11 = C[0], r.c12 = C[1], r.c13 = C[2], r.c22 = C[3], r.c23 = C[4], r.c33 = C[5];
+    return r;
+}
+
 struct IadVeFields
 {
     const void *h, *xm, *kx, *vx, *vy, *vz;
@@ -123,7 +158,8 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
  * so the 9 moments M are accumulated alongside tau and combined with C_i after the inversion. */
 template<class T>
 __global__ void SPHX_LIGHT_LOOP(T)
-    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, SincKernel<T> kern)
+    iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, SincKernel<T> kern,
+                       RecAv<T>* __restrict__ next, const T* __restrict__ cNext)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -185,8 +221,11 @@ __global__ void SPHX_LIGHT_LOOP(T)
             g[aa][b] = -(C[b][0] * M[aa][0] + C[b][1] * M[aa][1] + C[b][2] * M[aa][2]);
 
     T hiInv3   = hiInv * hiInv * hiInv;
-    T norm_kxi = T(a.K * double(hiInv3) / double(ld<T>(f.kx, i)));
-    st<T>(f.divv, i, norm_kxi * (g[0][0] + g[1][1] + g[2][2]));
+    T kxi      = ld<T>(f.kx, i);
+    T norm_kxi = T(a.K * double(hiInv3) / double(kxi));
+    T divv_i   = norm_kxi * (g[0][0] + g[1][1] + g[2][2]);
+    st<T>(f.divv, i, divv_i);
+    if (next) { next[i] = avRecord<T>(ri.x, ri.y, ri.z, ri.xm, kxi, cNext[i], divv_i, ri.vx, ri.vy, ri.vz); }
     if (f.curlv)
     {
         T cx = g[2][1] - g[1][2], cy = g[0][2] - g[2][0], cz = g[1][0] - g[0][1];
@@ -207,6 +246,7 @@ struct AvFields
 {
     const void *h, *c, *xm, *kx, *vx, *vy, *vz, *divv, *c11, *c12, *c13, *c22, *c23, *c33;
     void*       alpha;
+    const void *m, *prho; // read only to write the momentum-energy records (next != NULL)
 };
 
 template<class T>
@@ -214,13 +254,9 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
-    RecAv<T> r;
-    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
-    r.vol  = fastDiv(ld<T>(f.xm, i), ld<T>(f.kx, i));
-    r.c    = ld<T>(f.c, i);
-    r.divv = ld<T>(f.divv, i);
-    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
-    rec[i] = r;
+    rec[i] = avRecord<T>(coordOf<T>(a.box, 0, a.x[i]), coordOf<T>(a.box, 1, a.y[i]), coordOf<T>(a.box, 2, a.z[i]),
+                         ld<T>(f.xm, i), ld<T>(f.kx, i), ld<T>(f.c, i), ld<T>(f.divv, i), ld<T>(f.vx, i),
+                         ld<T>(f.vy, i), ld<T>(f.vz, i));
 }
 
 /* AVswitchesJLoop (hydro_ve/av_switches_kern.hpp:44-137). grad(div v) is linear in C_i:
@@ -228,7 +264,7 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
 template<class T>
 __global__ void SPHX_LIGHT_LOOP(T)
     avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, SincKernel<T> kern, double dt,
-                     T alphamin, T alphamax, T decay)
+                     T alphamin, T alphamax, T decay, RecMomVe<T>* __restrict__ next)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -290,6 +326,12 @@ __global__ void SPHX_LIGHT_LOOP(T)
         alpha_i    = T(double(alpha_i) + double(alphadot) * dt);
     }
     st<T>(f.alpha, i, alpha_i);
+    if (next)
+    {
+        const T C[6] = {c11i, c12i, c13i, c22i, c23i, c33i};
+        next[i] = momVeRecord<T>(ri.x, ri.y, ri.z, ri.vx, ri.vy, ri.vz, hi, ld<T>(f.m, i), ri.c, ld<T>(f.xm, i),
+                                 ld<T>(f.kx, i), alpha_i, ld<T>(f.prho, i), C);
+    }
 }
 
 struct MomVeFields
@@ -305,20 +347,10 @@ __global__ void __launch_bounds__(kPackThreads) packMomVe(PairArgs a, MomVeField
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
-    RecMomVe<T> r;
-    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
-    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
-    r.hInv  = T(1) / ld<T>(f.h, i);
-    r.m     = ld<T>(f.m, i);
-    r.c     = ld<T>(f.c, i);
-    r.xm    = ld<T>(f.xm, i);
-    r.rho   = ld<T>(f.kx, i) * r.m / r.xm;
-    r.vol   = fastDiv(r.m, r.rho);
-    r.alpha = ld<T>(f.alpha, i);
-    r.prho  = ld<T>(f.prho, i);
-    r.c11 = ld<T>(f.c11, i), r.c12 = ld<T>(f.c12, i), r.c13 = ld<T>(f.c13, i);
-    r.c22 = ld<T>(f.c22, i), r.c23 = ld<T>(f.c23, i), r.c33 = ld<T>(f.c33, i);
-    rec[i] = r;
+    const T C[6] = {ld<T>(f.c11, i), ld<T>(f.c12, i), ld<T>(f.c13, i), ld<T>(f.c22, i), ld<T>(f.c23, i), ld<T>(f.c33, i)};
+    rec[i] = momVeRecord<T>(coordOf<T>(a.box, 0, a.x[i]), coordOf<T>(a.box, 1, a.y[i]), coordOf<T>(a.box, 2, a.z[i]),
+                            ld<T>(f.vx, i), ld<T>(f.vy, i), ld<T>(f.vz, i), ld<T>(f.h, i), ld<T>(f.m, i),
+                            ld<T>(f.c, i), ld<T>(f.xm, i), ld<T>(f.kx, i), ld<T>(f.alpha, i), ld<T>(f.prho, i), C);
 }
 
 //! avRvCorrection (hydro_ve/momentum_energy_kern.hpp:44-63) with the reference's symv layout
@@ -518,43 +550,106 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
     SPHX_CHECK_LAUNCH();
 }
 
+//! record kinds a pair loop's epilogue can leave for the next call (SPHX_FLAG_CHAIN)
+enum ChainKind
+{
+    kChainAv    = 1,
+    kChainMomVe = 2
+};
+
+//! the call takes part in record chaining: flag set and the full particle range
+bool chainable(const PairArgs& a, const sphx_params_t* p)
+{
+    return (p->flags & SPHX_FLAG_CHAIN) && a.L.first == 0 && a.L.last == a.numParticles;
+}
+
+//! the record buffer holding the records of `kind` that the previous call wrote for this one, or -1
+int takeChain(sphx_workspace* ws, int kind, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
+{
+    auto& c  = ws->chain;
+    bool  ok = chainable(a, p) && c.kind == kind && c.serial + 1 == ws->callSerial && c.n == a.numParticles &&
+              c.form == p->hydro_type && std::memcmp(&c.fields, f, sizeof(*f)) == 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        ok = ok && c.lo[k] == a.box.lo[k] && c.hi[k] == a.box.hi[k] && c.pbc[k] == a.box.pbc[k];
+    }
+    c.kind = 0;
+    return ok ? c.buf : -1;
+}
+
+void leaveChain(sphx_workspace* ws, int kind, int buf, const PairArgs& a, const sphx_fields_t* f,
+                const sphx_params_t* p)
+{
+    auto& c  = ws->chain;
+    c.kind   = kind;
+    c.serial = ws->callSerial;
+    c.n      = a.numParticles;
+    c.form   = p->hydro_type;
+    c.buf    = buf;
+    c.fields = *f;
+    for (int k = 0; k < 3; ++k)
+    {
+        c.lo[k] = a.box.lo[k], c.hi[k] = a.box.hi[k], c.pbc[k] = a.box.pbc[k];
+    }
+}
+
 template<class T>
-void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf)
+void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf, const sphx_fields_t* f,
+                 const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
-    RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles);
+    RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles, 0);
     packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
     SPHX_CHECK_LAUNCH();
-    KernelScope timer(ws, "iad_divv_curlv");
-    iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, sincKernel<T>(ws));
-    SPHX_CHECK_LAUNCH();
+    RecAv<T>* next = chainable(a, p) && f->c ? recordBuffer<RecAv<T>>(ws, a.numParticles, 1) : nullptr;
+    {
+        KernelScope timer(ws, "iad_divv_curlv");
+        iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, sincKernel<T>(ws), next,
+                                                                            (const T*)f->c);
+        SPHX_CHECK_LAUNCH();
+    }
+    if (next) { leaveChain(ws, kChainAv, 1, a, f, p); }
 }
 
 template<class T>
-void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const sphx_params_t* p)
+void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const sphx_fields_t* f,
+              const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainAv, a, f, p);
     if (blocks == 0) { return; }
-    RecAv<T>* rec = recordBuffer<RecAv<T>>(ws, a.numParticles);
-    packAv<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, af, rec);
-    SPHX_CHECK_LAUNCH();
-    KernelScope timer(ws, "av_switches");
-    avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, af, rec, sincKernel<T>(ws), p->minDt,
-                                                                        T(p->alphamin), T(p->alphamax),
-                                                                        T(p->decay_constant));
-    SPHX_CHECK_LAUNCH();
+    RecAv<T>* rec = recordBuffer<RecAv<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        in = 0;
+        packAv<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, af, rec);
+        SPHX_CHECK_LAUNCH();
+    }
+    RecMomVe<T>* next =
+        chainable(a, p) && f->m && f->prho ? recordBuffer<RecMomVe<T>>(ws, a.numParticles, 1 - in) : nullptr;
+    {
+        KernelScope timer(ws, "av_switches");
+        avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            a, af, rec, sincKernel<T>(ws), p->minDt, T(p->alphamin), T(p->alphamax), T(p->decay_constant), next);
+        SPHX_CHECK_LAUNCH();
+    }
+    if (next) { leaveChain(ws, kChainMomVe, 1 - in, a, f, p); }
 }
 
 template<class T>
-void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, const sphx_params_t* p, bool avClean,
-                 unsigned long long* dt)
+void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, const sphx_fields_t* f,
+                 const sphx_params_t* p, bool avClean, unsigned long long* dt)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainMomVe, a, f, p);
     if (blocks == 0) { return; }
-    RecMomVe<T>* rec = recordBuffer<RecMomVe<T>>(ws, a.numParticles);
-    packMomVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
-    SPHX_CHECK_LAUNCH();
+    RecMomVe<T>* rec = recordBuffer<RecMomVe<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        packMomVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
+        SPHX_CHECK_LAUNCH();
+    }
     KernelScope timer(ws, "momentum_ve");
     if (avClean)
         momentumVeKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
@@ -611,8 +706,8 @@ void iadDivvCurlv(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
                  "dV11..dV33 must be all set or all NULL");
     IadVeFields vf{f->h,   f->xm,   f->kx,    f->vx,   f->vy,   f->vz,   f->c11,  f->c12,  f->c13,  f->c22,
                    f->c23, f->c33, f->divv, f->curlv, f->dV11, f->dV12, f->dV13, f->dV22, f->dV23, f->dV33};
-    if (p->hydro_type == SPHX_F32) launchIadVe<float>(ws, a, vf);
-    else launchIadVe<double>(ws, a, vf);
+    if (p->hydro_type == SPHX_F32) launchIadVe<float>(ws, a, vf, f, p);
+    else launchIadVe<double>(ws, a, vf, f, p);
 }
 
 void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
@@ -622,10 +717,10 @@ void avSwitches(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, c
     SPHX_REQUIRE(f->h && f->c && f->xm && f->kx && f->vx && f->vy && f->vz && f->divv && f->c11 && f->c12 && f->c13 &&
                      f->c22 && f->c23 && f->c33 && f->alpha,
                  SPHX_ERR_ARG, "AVswitches needs h, c, xm, kx, v, divv, c11..c33 and alpha");
-    AvFields af{f->h, f->c, f->xm, f->kx, f->vx, f->vy, f->vz, f->divv, f->c11, f->c12, f->c13, f->c22, f->c23, f->c33,
-                f->alpha};
-    if (p->hydro_type == SPHX_F32) launchAv<float>(ws, a, af, p);
-    else launchAv<double>(ws, a, af, p);
+    AvFields af{f->h,   f->c,   f->xm,  f->kx,  f->vx,  f->vy,  f->vz,    f->divv, f->c11,
+                f->c12, f->c13, f->c22, f->c23, f->c33, f->alpha, f->m,  f->prho};
+    if (p->hydro_type == SPHX_F32) launchAv<float>(ws, a, af, f, p);
+    else launchAv<double>(ws, a, af, f, p);
 }
 
 double momentumEnergyVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
@@ -641,8 +736,8 @@ double momentumEnergyVe(sphx_workspace* ws, const DevBox& box, const sphx_fields
                    f->c13, f->c22,  f->c23,  f->c33,  f->kx,   f->xm,   f->alpha,    f->dV11, f->dV12, f->dV13,
                    f->dV22, f->dV23, f->dV33, f->ax,   f->ay,   f->az,   f->du};
     unsigned long long* dt = resetDt(ws);
-    if (p->hydro_type == SPHX_F32) launchMomVe<float>(ws, a, mf, p, avClean, dt);
-    else launchMomVe<double>(ws, a, mf, p, avClean, dt);
+    if (p->hydro_type == SPHX_F32) launchMomVe<float>(ws, a, mf, f, p, avClean, dt);
+    else launchMomVe<double>(ws, a, mf, f, p, avClean, dt);
     return readDt(ws, dt);
 }
 

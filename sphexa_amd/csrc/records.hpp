@@ -78,12 +78,13 @@ static_assert(sizeof(RecGradh<float>) == 32 && sizeof(RecIadVe<float>) == 32 && 
               "record layout");
 static_assert(sizeof(RecMomStd<float>) == 80 && sizeof(RecMomVe<float>) == 80, "record layout");
 
-//! workspace buffer for `n` records of type R
+//! workspace buffer `buf` (0 or 1) for `n` records of type R
 template<class R>
-R* recordBuffer(sphx_workspace* ws, size_t n)
+R* recordBuffer(sphx_workspace* ws, size_t n, int buf = 0)
 {
-    ws->records.reserve((n * sizeof(R) + 15) / 16);
-    return reinterpret_cast<R*>(ws->records.get());
+    auto& b = buf ? ws->records2 : ws->records;
+    b.reserve((n * sizeof(R) + 15) / 16);
+    return reinterpret_cast<R*>(b.get());
 }
 
 } // namespace sphx

@@ -123,9 +123,10 @@ def std_step(ws, ps, box, params, bucket=64):
     return converged, dt
 
 
-def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None):
+def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None, chain=True):
     """One VE pass; returns (converged, minDtCourant). With a time-step state ts (abi.Timestep) it
-    also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171)."""
+    also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171).
+    With chain, the IAD, AV and momentum passes run with SPHX_FLAG_CHAIN (bitwise the same)."""
     n = ps.n
     f = ps.fields()
     ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
@@ -133,11 +134,12 @@ def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None):
     converged = search_first_sum(ws, box, f, params, n, rho=False)
     ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
     ws.eos("sphx_eos_ve", f, params, 0, n)
-    ws.pair("sphx_iad_divv_curlv", box, f, params, 0, n)
+    cp = abi.chained(params) if chain else params
+    ws.pair("sphx_iad_divv_curlv", box, f, cp, 0, n)
     if ts is not None:
         ws.rho_timestep(f, params, 0, n, ts)
-    ws.pair("sphx_av_switches", box, f, params, 0, n)
-    dt = ws.momentum_energy_ve(box, f, params, 0, n, av_clean)
+    ws.pair("sphx_av_switches", box, f, cp, 0, n)
+    dt = ws.momentum_energy_ve(box, f, cp, 0, n, av_clean)
     return converged, dt
 
 

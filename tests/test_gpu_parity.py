@@ -303,6 +303,63 @@ def test_ve_mixed_accuracy(ws, orc):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("av_clean", [False, True])
+def test_ve_record_chain_bitwise(ws, mixed, av_clean):
+    """SPHX_FLAG_CHAIN: the IAD and AV epilogues write the next pass's records; every output is
+    bitwise the one of the pass that packs them."""
+    arr, spec = _ve_ic(16, 7)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+    names = VE_FIELDS + (GRADV_FIELDS if av_clean else ())
+    outs = ("divv", "curlv", "alpha", "ax", "ay", "az", "du") + (GRADV_FIELDS if av_clean else ())
+    res = []
+    for chain in (False, True):
+        ps = ParticleSet.from_host(arr, names, mixed=mixed)
+        conv, dt = ve_step(ws, ps, box, params, av_clean=av_clean, chain=chain)
+        assert conv
+        res.append(({k: ps.host(k) for k in outs}, dt))
+    for k in outs:
+        np.testing.assert_array_equal(res[0][0][k], res[1][0][k], err_msg=k)
+    assert res[0][1] == res[1][1]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_ve_record_chain_taken_and_broken(ws, mixed):
+    """The AV pass uses the records the IAD pass left only when it is the next field-writing call.
+    To see which records it used, c is doubled behind the library's back after the IAD pass: the
+    chained records hold the old c (alpha as without the change), packed records the new one. A
+    read-only call in between keeps the chain; a writing call (an empty EOS range) breaks it."""
+    arr, spec = _ve_ic(14, 8)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+
+    def run(chain, scale_c, between=None):
+        ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+        n, f = ps.n, ps.fields()
+        ws.domain_sync(box, ps.sync_fields(), params, n, 64)
+        assert ws.find_neighbors(box, f, params, 0, n, True)
+        ws.pair("sphx_xmass", box, f, params, 0, n)
+        ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+        ws.eos("sphx_eos_ve", f, params, 0, n)
+        cp = abi.chained(params) if chain else params
+        ws.pair("sphx_iad_divv_curlv", box, f, cp, 0, n)
+        if scale_c:
+            ps.t["c"].mul_(2)
+        if between == "read":
+            ws.rho_timestep(f, params, 0, n, abi.Timestep.defaults(minDt=1e-3, minDt_m1=1e-3))
+        elif between == "write":
+            ws.eos("sphx_eos_ve", f, params, 0, 0)
+        ws.pair("sphx_av_switches", box, f, cp, 0, n)
+        return ps.host("alpha")
+
+    plain, doubled = run(False, False), run(False, True)
+    assert not np.array_equal(plain, doubled)
+    np.testing.assert_array_equal(run(True, True), plain)
+    np.testing.assert_array_equal(run(True, True, "read"), plain)
+    np.testing.assert_array_equal(run(True, True, "write"), doubled)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("form", ["std", "ve"])
 def test_fused_search_xmass_matches_separate_passes(ws, mixed, form):
     """sphx_find_neighbors_xmass == sphx_find_neighbors + sphx_density_std / sphx_xmass: same lists,
