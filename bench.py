@@ -145,7 +145,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     n = ps.n
     f = ps.fields()
     sf = ps.sync_fields()  # the sync permutes the conserved fields only, as Domain::sync does
-    cparams = abi.chained(params)  # IAD and AV write the next pass's records (bitwise the same)
+    cparams = abi.chained(params)  # XMass, IAD and AV write the next pass's records (bitwise the same)
     stream = torch.cuda.current_stream()
 
     ws.set_profiling(True)
@@ -157,9 +157,9 @@ def run_gpu(arr, spec, mixed, steps, warmup):
         ws.domain_sync(box, sf, params, n, 64)
         ps.reorder_history(ws)
         ev[1].record(stream)
-        search_first_sum(ws, box, f, params, n, rho=False)
+        search_first_sum(ws, box, f, cparams, n, rho=False)
         ev[2].record(stream)
-        ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+        ws.pair("sphx_ve_def_gradh", box, f, cparams, 0, n)
         ev[3].record(stream)
         ws.eos("sphx_eos_ve", f, params, 0, n)
         ev[4].record(stream)

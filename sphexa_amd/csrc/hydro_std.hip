@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
 template<class T, bool toRho>
 __global__ void SPHX_LIGHT_LOOP(T)
     xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
-                T* __restrict__ out)
+                T* __restrict__ out, RecGradh<T>* __restrict__ next)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -102,6 +102,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     double rhoZero = double(rho0.value()) * a.K * double(h3Inv);
     T      xmi     = T(double(ri.m) / rhoZero);
     out[i]         = toRho ? T(ri.m / xmi) : xmi;
+    if (!toRho && next) { next[i] = gradhRecord<T>(ri.x, ri.y, ri.z, ri.m, xmi); }
 }
 
 //! idealGasEOS_u (sph/include/sph/eos.hpp:35-44) or idealGasEOS with cv from mui (:47-51)
@@ -256,23 +257,29 @@ __global__ void SPHX_MOM_LOOP(T)
 }
 
 template<class T>
-void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, bool toRho, void* out)
+void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p, bool toRho,
+                   void* out)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
-    RecMass<T>* rec = recordBuffer<RecMass<T>>(ws, a.numParticles);
+    RecMass<T>* rec = recordBuffer<RecMass<T>>(ws, a.numParticles, 0);
     packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.box, a.x, a.y, a.z,
                                                                                          (const T*)f->m, rec,
                                                                                          a.numParticles);
     SPHX_CHECK_LAUNCH();
-    KernelScope timer(ws, toRho ? "density_std" : "xmass");
-    if (toRho)
-        xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
-                                                                            (T*)out);
-    else
-        xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
-                                                                             (T*)out);
-    SPHX_CHECK_LAUNCH();
+    // XMass leaves the VeDefGradh records (position, m, xm) when chaining
+    RecGradh<T>* next = !toRho && chainable(a, p) ? recordBuffer<RecGradh<T>>(ws, a.numParticles, 1) : nullptr;
+    {
+        KernelScope timer(ws, toRho ? "density_std" : "xmass");
+        if (toRho)
+            xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
+                                                                                sincKernel<T>(ws), (T*)out, nullptr);
+        else
+            xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
+                                                                                 sincKernel<T>(ws), (T*)out, next);
+        SPHX_CHECK_LAUNCH();
+    }
+    if (next) { leaveChain(ws, kChainGradh, 1, a, f, p); }
 }
 
 template<class T>
@@ -316,8 +323,8 @@ void densityStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, c
 {
     PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->rho, SPHX_ERR_ARG, "density needs h, m and rho");
-    if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, true, f->rho);
-    else launchDensity<double>(ws, a, f, true, f->rho);
+    if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, p, true, f->rho);
+    else launchDensity<double>(ws, a, f, p, true, f->rho);
 }
 
 void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
@@ -325,8 +332,8 @@ void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, cons
 {
     PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->xm, SPHX_ERR_ARG, "xmass needs h, m and xm");
-    if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, false, f->xm);
-    else launchDensity<double>(ws, a, f, false, f->xm);
+    if (p->hydro_type == SPHX_F32) launchDensity<float>(ws, a, f, p, false, f->xm);
+    else launchDensity<double>(ws, a, f, p, false, f->xm);
 }
 
 void eosStd(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)

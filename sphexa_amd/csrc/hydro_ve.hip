@@ -2,7 +2,6 @@
 // XMass (shared with std density, hydro_std.hip), VeDefGradh, EOS, IAD + div/curl v,
 // AV switches, momentum-energy with optional AV cleaning. Neighbour data is read from packed
 // per-particle records (records.hpp).
-#include <cstring>
 
 #include "records.hpp"
 
@@ -20,10 +19,8 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
-    RecGradh<T> r;
-    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
-    r.m = m[i], r.xm = xm[i];
-    rec[i] = r;
+    rec[i] = gradhRecord<T>(coordOf<T>(a.box, 0, a.x[i]), coordOf<T>(a.box, 1, a.y[i]), coordOf<T>(a.box, 2, a.z[i]),
+                            m[i], xm[i]);
 }
 
 /* veDefGradhJLoop (hydro_ve/ve_def_gradh_kern.hpp:44-90): kx and the grad-h term */
@@ -536,61 +533,22 @@ double readDt(sphx_workspace* ws, unsigned long long* dt)
 }
 
 template<class T>
-void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
+void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainGradh, a, f, p);
     if (blocks == 0) { return; }
-    RecGradh<T>* rec = recordBuffer<RecGradh<T>>(ws, a.numParticles);
-    packGradh<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, (const T*)f->m,
-                                                                                          (const T*)f->xm, rec);
-    SPHX_CHECK_LAUNCH();
+    RecGradh<T>* rec = recordBuffer<RecGradh<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        packGradh<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, (const T*)f->m,
+                                                                                              (const T*)f->xm, rec);
+        SPHX_CHECK_LAUNCH();
+    }
     KernelScope timer(ws, "ve_def_gradh");
     veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
                                                                         (T*)f->kx, (T*)f->gradh);
     SPHX_CHECK_LAUNCH();
-}
-
-//! record kinds a pair loop's epilogue can leave for the next call (SPHX_FLAG_CHAIN)
-enum ChainKind
-{
-    kChainAv    = 1,
-    kChainMomVe = 2
-};
-
-//! the call takes part in record chaining: flag set and the full particle range
-bool chainable(const PairArgs& a, const sphx_params_t* p)
-{
-    return (p->flags & SPHX_FLAG_CHAIN) && a.L.first == 0 && a.L.last == a.numParticles;
-}
-
-//! the record buffer holding the records of `kind` that the previous call wrote for this one, or -1
-int takeChain(sphx_workspace* ws, int kind, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
-{
-    auto& c  = ws->chain;
-    bool  ok = chainable(a, p) && c.kind == kind && c.serial + 1 == ws->callSerial && c.n == a.numParticles &&
-              c.form == p->hydro_type && std::memcmp(&c.fields, f, sizeof(*f)) == 0;
-    for (int k = 0; k < 3; ++k)
-    {
-        ok = ok && c.lo[k] == a.box.lo[k] && c.hi[k] == a.box.hi[k] && c.pbc[k] == a.box.pbc[k];
-    }
-    c.kind = 0;
-    return ok ? c.buf : -1;
-}
-
-void leaveChain(sphx_workspace* ws, int kind, int buf, const PairArgs& a, const sphx_fields_t* f,
-                const sphx_params_t* p)
-{
-    auto& c  = ws->chain;
-    c.kind   = kind;
-    c.serial = ws->callSerial;
-    c.n      = a.numParticles;
-    c.form   = p->hydro_type;
-    c.buf    = buf;
-    c.fields = *f;
-    for (int k = 0; k < 3; ++k)
-    {
-        c.lo[k] = a.box.lo[k], c.hi[k] = a.box.hi[k], c.pbc[k] = a.box.pbc[k];
-    }
 }
 
 template<class T>
@@ -667,8 +625,8 @@ void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, c
 {
     PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->xm && f->kx && f->gradh, SPHX_ERR_ARG, "VeDefGradh needs h, m, xm, kx, gradh");
-    if (p->hydro_type == SPHX_F32) launchGradh<float>(ws, a, f);
-    else launchGradh<double>(ws, a, f);
+    if (p->hydro_type == SPHX_F32) launchGradh<float>(ws, a, f, p);
+    else launchGradh<double>(ws, a, f, p);
 }
 
 void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)

@@ -11,6 +11,8 @@
 // pair loop would evaluate, so results are unchanged.
 #pragma once
 
+#include <cstring>
+
 #include "pairs.hpp"
 
 namespace sphx
@@ -78,6 +80,16 @@ static_assert(sizeof(RecGradh<float>) == 32 && sizeof(RecIadVe<float>) == 32 && 
               "record layout");
 static_assert(sizeof(RecMomStd<float>) == 80 && sizeof(RecMomVe<float>) == 80, "record layout");
 
+//! the VeDefGradh record, from the pack kernel or the XMass epilogue (the same bits)
+template<class T>
+__device__ __forceinline__ RecGradh<T> gradhRecord(Coord<T> x, Coord<T> y, Coord<T> z, T m, T xm)
+{
+    RecGradh<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.m = m, r.xm = xm;
+    return r;
+}
+
 //! workspace buffer `buf` (0 or 1) for `n` records of type R
 template<class R>
 R* recordBuffer(sphx_workspace* ws, size_t n, int buf = 0)
@@ -85,6 +97,50 @@ R* recordBuffer(sphx_workspace* ws, size_t n, int buf = 0)
     auto& b = buf ? ws->records2 : ws->records;
     b.reserve((n * sizeof(R) + 15) / 16);
     return reinterpret_cast<R*>(b.get());
+}
+
+//! record kinds a pair loop's epilogue can leave for the next call (SPHX_FLAG_CHAIN)
+enum ChainKind
+{
+    kChainAv    = 1, // sphx_iad_divv_curlv -> sphx_av_switches
+    kChainMomVe = 2, // sphx_av_switches -> sphx_momentum_energy_ve
+    kChainGradh = 3  // sphx_xmass -> sphx_ve_def_gradh
+};
+
+//! the call takes part in record chaining: flag set and the full particle range
+inline bool chainable(const PairArgs& a, const sphx_params_t* p)
+{
+    return (p->flags & SPHX_FLAG_CHAIN) && a.L.first == 0 && a.L.last == a.numParticles;
+}
+
+//! the record buffer holding the records of `kind` that the previous call wrote for this one, or -1
+inline int takeChain(sphx_workspace* ws, int kind, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
+{
+    auto& c  = ws->chain;
+    bool  ok = chainable(a, p) && c.kind == kind && c.serial + 1 == ws->callSerial && c.n == a.numParticles &&
+              c.form == p->hydro_type && std::memcmp(&c.fields, f, sizeof(*f)) == 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        ok = ok && c.lo[k] == a.box.lo[k] && c.hi[k] == a.box.hi[k] && c.pbc[k] == a.box.pbc[k];
+    }
+    c.kind = 0;
+    return ok ? c.buf : -1;
+}
+
+inline void leaveChain(sphx_workspace* ws, int kind, int buf, const PairArgs& a, const sphx_fields_t* f,
+                const sphx_params_t* p)
+{
+    auto& c  = ws->chain;
+    c.kind   = kind;
+    c.serial = ws->callSerial;
+    c.n      = a.numParticles;
+    c.form   = p->hydro_type;
+    c.buf    = buf;
+    c.fields = *f;
+    for (int k = 0; k < 3; ++k)
+    {
+        c.lo[k] = a.box.lo[k], c.hi[k] = a.box.hi[k], c.pbc[k] = a.box.pbc[k];
+    }
 }
 
 } // namespace sphx

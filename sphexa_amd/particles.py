@@ -126,15 +126,15 @@ def std_step(ws, ps, box, params, bucket=64):
 def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None, chain=True):
     """One VE pass; returns (converged, minDtCourant). With a time-step state ts (abi.Timestep) it
     also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171).
-    With chain, the IAD, AV and momentum passes run with SPHX_FLAG_CHAIN (bitwise the same)."""
+    With chain, the pair passes run with SPHX_FLAG_CHAIN (bitwise the same)."""
     n = ps.n
     f = ps.fields()
+    cp = abi.chained(params) if chain else params
     ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
     ps.reorder_history(ws)
-    converged = search_first_sum(ws, box, f, params, n, rho=False)
-    ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+    converged = search_first_sum(ws, box, f, cp, n, rho=False)
+    ws.pair("sphx_ve_def_gradh", box, f, cp, 0, n)
     ws.eos("sphx_eos_ve", f, params, 0, n)
-    cp = abi.chained(params) if chain else params
     ws.pair("sphx_iad_divv_curlv", box, f, cp, 0, n)
     if ts is not None:
         ws.rho_timestep(f, params, 0, n, ts)

@@ -305,13 +305,13 @@ def test_ve_mixed_accuracy(ws, orc):
 @pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("av_clean", [False, True])
 def test_ve_record_chain_bitwise(ws, mixed, av_clean):
-    """SPHX_FLAG_CHAIN: the IAD and AV epilogues write the next pass's records; every output is
-    bitwise the one of the pass that packs them."""
+    """SPHX_FLAG_CHAIN: the XMass, IAD and AV epilogues write the next pass's records; every
+    output is bitwise the one of the pass that packs them."""
     arr, spec = _ve_ic(16, 7)
     box = make_box(spec)
     params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
     names = VE_FIELDS + (GRADV_FIELDS if av_clean else ())
-    outs = ("divv", "curlv", "alpha", "ax", "ay", "az", "du") + (GRADV_FIELDS if av_clean else ())
+    outs = ("xm", "kx", "gradh", "divv", "curlv", "alpha", "ax", "ay", "az", "du") + (GRADV_FIELDS if av_clean else ())
     res = []
     for chain in (False, True):
         ps = ParticleSet.from_host(arr, names, mixed=mixed)
@@ -357,6 +357,32 @@ def test_ve_record_chain_taken_and_broken(ws, mixed):
     np.testing.assert_array_equal(run(True, True), plain)
     np.testing.assert_array_equal(run(True, True, "read"), plain)
     np.testing.assert_array_equal(run(True, True, "write"), doubled)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_xmass_record_chain_taken(ws, mixed):
+    """sphx_ve_def_gradh uses the records sphx_xmass left: with every other m doubled behind the
+    library's back in between (grad-h does not change under a uniform mass scale), grad-h is the one
+    of the unchanged masses; without the flag it is not."""
+    arr, spec = _ve_ic(14, 6)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+
+    def run(chain, scale_m):
+        ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+        n, f = ps.n, ps.fields()
+        ws.domain_sync(box, ps.sync_fields(), params, n, 64)
+        assert ws.find_neighbors(box, f, params, 0, n, True)
+        cp = abi.chained(params) if chain else params
+        ws.pair("sphx_xmass", box, f, cp, 0, n)
+        if scale_m:
+            ps.t["m"][::2].mul_(2)
+        ws.pair("sphx_ve_def_gradh", box, f, cp, 0, n)
+        return ps.host("gradh")
+
+    plain = run(False, False)
+    assert not np.array_equal(plain, run(False, True))
+    np.testing.assert_array_equal(run(True, True), plain)
 
 
 @pytest.mark.parametrize("mixed", [False, True])
