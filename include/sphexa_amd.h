@@ -128,12 +128,12 @@ typedef struct sphx_params
  * std adjoint differentiates against. */
 #define SPHX_FLAG_STD_ALPHA 1
 
-/* Record chaining of the VE pass. Set on the calls of sphx_xmass, sphx_ve_def_gradh,
- * sphx_iad_divv_curlv, sphx_av_switches and sphx_momentum_energy_ve over the full range [0, n), it
- * promises that the caller changes no particle field between consecutive calls on one workspace.
- * sphx_xmass then writes the neighbour records of sphx_ve_def_gradh in its epilogue,
- * sphx_iad_divv_curlv those of sphx_av_switches and sphx_av_switches those of
- * sphx_momentum_energy_ve, so the next call skips its packing kernel. The next call uses them only
+/* Record chaining. Set on the pair-loop calls of a pass over the full range [0, n), it promises
+ * that the caller changes no particle field between consecutive calls on one workspace. Then
+ * sphx_xmass writes the neighbour records of sphx_ve_def_gradh in its epilogue,
+ * sphx_iad_divv_curlv those of sphx_av_switches, sphx_av_switches those of
+ * sphx_momentum_energy_ve and sphx_iad_std those of sphx_momentum_energy_std, so the next call
+ * skips its packing kernel. The next call uses them only
  * if it comes directly after the producing one (sphx_rho_timestep, sphx_compute_timestep and
  * sphx_conserved_quantities, which read fields only, may sit between), on the same fields, box,
  * form and particle count; otherwise it packs as without the flag. Results are bitwise the same

@@ -82,8 +82,8 @@ SPHX_FLAG_CHAIN = 2
 
 
 def chained(params):
-    """A copy of params with SPHX_FLAG_CHAIN: the VE XMass, IAD and AV passes write the next pass's
-    neighbour records in their epilogues (bitwise the same results, three pack kernels fewer)."""
+    """A copy of params with SPHX_FLAG_CHAIN: XMass, the IAD passes and AV switches write the next
+    pass's neighbour records in their epilogues (bitwise the same results, fewer pack kernels)."""
     p = Params.from_buffer_copy(params)
     p.flags |= SPHX_FLAG_CHAIN
     return p

@@ -48,23 +48,33 @@ struct MomStdFields
     const void* alpha; // per-particle AV coefficient (SPHX_FLAG_STD_ALPHA), else the reference's 1
 };
 
+/* The std momentum-energy record, built by the pack kernel or by the IAD epilogue under
+ * SPHX_FLAG_CHAIN with this one function, so both give the same bits. C = {c11, .., c33}. */
+template<class T>
+__device__ __forceinline__ RecMomStd<T> momStdRecord(Coord<T> x, Coord<T> y, Coord<T> z, T vx, T vy, T vz, T h, T m,
+                                                     T rho, T p, T c, const T C[6])
+{
+    RecMomStd<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.vx = vx, r.vy = vy, r.vz = vz;
+    r.hInv = T(1) / h;
+    r.m    = m;
+    r.vol  = fastDiv(m, rho);
+    r.pro  = fastDiv(p, rho); // p_j / (rho_j gradh_j), gradh_j = 1
+    r.c    = c;
+    r.c11 = C[0], r.c12 = C[1], r.c13 = C[2], r.c22 = C[3], r.c23 = C[4], r.c33 = C[5];
+    return r;
+}
+
 template<class T>
 __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFields f, RecMomStd<T>* __restrict__ rec)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
-    RecMomStd<T> r;
-    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
-    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
-    T rho  = ld<T>(f.rho, i);
-    r.hInv = T(1) / ld<T>(f.h, i);
-    r.m    = ld<T>(f.m, i);
-    r.vol  = fastDiv(r.m, rho);
-    r.pro  = fastDiv(ld<T>(f.p, i), rho); // p_j / (rho_j gradh_j), gradh_j = 1
-    r.c    = ld<T>(f.c, i);
-    r.c11 = ld<T>(f.c11, i), r.c12 = ld<T>(f.c12, i), r.c13 = ld<T>(f.c13, i);
-    r.c22 = ld<T>(f.c22, i), r.c23 = ld<T>(f.c23, i), r.c33 = ld<T>(f.c33, i);
-    rec[i] = r;
+    const T C[6] = {ld<T>(f.c11, i), ld<T>(f.c12, i), ld<T>(f.c13, i), ld<T>(f.c22, i), ld<T>(f.c23, i), ld<T>(f.c33, i)};
+    rec[i] = momStdRecord<T>(coordOf<T>(a.box, 0, a.x[i]), coordOf<T>(a.box, 1, a.y[i]), coordOf<T>(a.box, 2, a.z[i]),
+                             ld<T>(f.vx, i), ld<T>(f.vy, i), ld<T>(f.vz, i), ld<T>(f.h, i), ld<T>(f.m, i),
+                             ld<T>(f.rho, i), ld<T>(f.p, i), ld<T>(f.c, i), C);
 }
 
 /* xmassJLoop (sph/include/sph/hydro_ve/xmass_kern.hpp:51-79):
@@ -123,7 +133,7 @@ __global__ void __launch_bounds__(256) eosStdKernel(const double* __restrict__ u
 template<class T>
 __global__ void SPHX_LIGHT_LOOP(T)
     iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
-                 T* c11, T* c12, T* c13, T* c22, T* c23, T* c33)
+                 T* c11, T* c12, T* c13, T* c22, T* c23, T* c33, MomStdFields nf, RecMomStd<T>* __restrict__ next)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -163,8 +173,14 @@ __global__ void SPHX_LIGHT_LOOP(T)
     });
     double cij[6];
     iadInvert(t11, t12, t13, t22, t23, t33, double(hi), a.K, cij);
-    c11[i] = T(cij[0]), c12[i] = T(cij[1]), c13[i] = T(cij[2]);
-    c22[i] = T(cij[3]), c23[i] = T(cij[4]), c33[i] = T(cij[5]);
+    const T C[6] = {T(cij[0]), T(cij[1]), T(cij[2]), T(cij[3]), T(cij[4]), T(cij[5])};
+    c11[i] = C[0], c12[i] = C[1], c13[i] = C[2];
+    c22[i] = C[3], c23[i] = C[4], c33[i] = C[5];
+    if (next)
+    {
+        next[i] = momStdRecord<T>(ri.x, ri.y, ri.z, ld<T>(nf.vx, i), ld<T>(nf.vy, i), ld<T>(nf.vz, i), hi,
+                                  ld<T>(nf.m, i), ld<T>(nf.rho, i), ld<T>(nf.p, i), ld<T>(nf.c, i), C);
+    }
 }
 
 /* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant.
@@ -283,30 +299,42 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
 }
 
 template<class T>
-void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f)
+void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
     if (blocks == 0) { return; }
-    RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles);
+    RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles, 0);
     packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
         a.box, a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
     SPHX_CHECK_LAUNCH();
-    KernelScope timer(ws, "iad_std");
-    iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
-                                                                    (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
-                                                                    (T*)f->c23, (T*)f->c33);
-    SPHX_CHECK_LAUNCH();
+    // IAD leaves the momentum-energy records when chaining (it needs v, p and c besides its own fields)
+    MomStdFields  nf{f->vx, f->vy, f->vz, f->h, f->m, f->rho, f->p, f->c};
+    RecMomStd<T>* next = chainable(a, p) && f->vx && f->vy && f->vz && f->p && f->c
+                             ? recordBuffer<RecMomStd<T>>(ws, a.numParticles, 1)
+                             : nullptr;
+    {
+        KernelScope timer(ws, "iad_std");
+        iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22, (T*)f->c23,
+            (T*)f->c33, nf, next);
+        SPHX_CHECK_LAUNCH();
+    }
+    if (next) { leaveChain(ws, kChainMomStd, 1, a, f, p); }
 }
 
 template<class T>
-void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields& mf, float Kcour,
-                       unsigned long long* dt, bool perAlpha)
+void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields& mf, const sphx_fields_t* f,
+                       const sphx_params_t* p, float Kcour, unsigned long long* dt, bool perAlpha)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainMomStd, a, f, p);
     if (blocks == 0) { return; }
-    RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles);
-    packMomStd<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
-    SPHX_CHECK_LAUNCH();
+    RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        packMomStd<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, mf, rec);
+        SPHX_CHECK_LAUNCH();
+    }
     KernelScope timer(ws, "momentum_std");
     if (perAlpha)
         momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws), Kcour, dt);
@@ -363,8 +391,8 @@ void iadStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const
     PairArgs a = makePairArgs(ws, box, f, p, first, last);
     SPHX_REQUIRE(f->h && f->m && f->rho && f->c11 && f->c12 && f->c13 && f->c22 && f->c23 && f->c33, SPHX_ERR_ARG,
                  "IAD needs h, m, rho and c11..c33");
-    if (p->hydro_type == SPHX_F32) launchIadStd<float>(ws, a, f);
-    else launchIadStd<double>(ws, a, f);
+    if (p->hydro_type == SPHX_F32) launchIadStd<float>(ws, a, f, p);
+    else launchIadStd<double>(ws, a, f, p);
 }
 
 double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
@@ -381,8 +409,8 @@ double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_field
     SPHX_REQUIRE(!perAlpha || f->alpha, SPHX_ERR_ARG, "SPHX_FLAG_STD_ALPHA set but fields.alpha is NULL");
     MomStdFields mf{f->vx,  f->vy,  f->vz,  f->h,  f->m,  f->rho, f->p,  f->c,  f->c11, f->c12,
                     f->c13, f->c22, f->c23, f->c33, f->ax, f->ay,  f->az, f->du, f->alpha};
-    if (p->hydro_type == SPHX_F32) launchMomentumStd<float>(ws, a, mf, float(p->Kcour), dt, perAlpha);
-    else launchMomentumStd<double>(ws, a, mf, float(p->Kcour), dt, perAlpha);
+    if (p->hydro_type == SPHX_F32) launchMomentumStd<float>(ws, a, mf, f, p, float(p->Kcour), dt, perAlpha);
+    else launchMomentumStd<double>(ws, a, mf, f, p, float(p->Kcour), dt, perAlpha);
     unsigned long long bits = 0;
     SPHX_HIP(hipMemcpyAsync(&bits, dt, 8, hipMemcpyDeviceToHost, ws->stream));
     SPHX_HIP(hipStreamSynchronize(ws->stream));

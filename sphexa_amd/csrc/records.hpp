@@ -104,7 +104,8 @@ enum ChainKind
 {
     kChainAv    = 1, // sphx_iad_divv_curlv -> sphx_av_switches
     kChainMomVe = 2, // sphx_av_switches -> sphx_momentum_energy_ve
-    kChainGradh = 3  // sphx_xmass -> sphx_ve_def_gradh
+    kChainGradh = 3, // sphx_xmass -> sphx_ve_def_gradh
+    kChainMomStd = 4 // sphx_iad_std -> sphx_momentum_energy_std
 };
 
 //! the call takes part in record chaining: flag set and the full particle range

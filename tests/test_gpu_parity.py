@@ -360,6 +360,47 @@ def test_ve_record_chain_taken_and_broken(ws, mixed):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_std_record_chain_bitwise_and_taken(ws, mixed):
+    """std pass: the IAD epilogue writes the momentum-energy records; outputs are bitwise those of
+    the packed pass. With every other c doubled behind the library's back after IAD, the chained
+    momentum-energy still sees the old c (the records came from IAD), the packed one the new c."""
+    arr, spec = ic.sedov_lattice(14, seed=11)
+    rng = np.random.default_rng(11)
+    for k in ("vx", "vy", "vz"):  # approaching pairs, so that the viscosity (and c) enters ax
+        arr[k] = rng.normal(0.0, 0.1, arr["x"].size)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
+    outs = ("rho", "c11", "c33", "ax", "ay", "az", "du")
+    res = []
+    for chain in (False, True):
+        ps = ParticleSet.from_host(arr, STD_FIELDS, mixed=mixed)
+        conv, dt = std_step(ws, ps, box, params, chain=chain)
+        assert conv
+        res.append(({k: ps.host(k) for k in outs}, dt))
+    for k in outs:
+        np.testing.assert_array_equal(res[0][0][k], res[1][0][k], err_msg=k)
+    assert res[0][1] == res[1][1]
+
+    def run(chain, scale_c):
+        ps = ParticleSet.from_host(arr, STD_FIELDS, mixed=mixed)
+        n, f = ps.n, ps.fields()
+        ws.domain_sync(box, ps.sync_fields(), params, n, 64)
+        assert ws.find_neighbors(box, f, params, 0, n, True)
+        ws.pair("sphx_density_std", box, f, params, 0, n)
+        ws.eos("sphx_eos_std", f, params, 0, n)
+        cp = abi.chained(params) if chain else params
+        ws.pair("sphx_iad_std", box, f, cp, 0, n)
+        if scale_c:
+            ps.t["c"][::2].mul_(2)
+        dt = ws.momentum_energy_std(box, f, cp, 0, n)
+        return np.append(ps.host("ax").astype(np.float64), dt)
+
+    plain = run(False, False)
+    assert not np.array_equal(plain, run(False, True))
+    np.testing.assert_array_equal(run(True, True), plain)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_xmass_record_chain_taken(ws, mixed):
     """sphx_ve_def_gradh uses the records sphx_xmass left: with every other m doubled behind the
     library's back in between (grad-h does not change under a uniform mass scale), grad-h is the one
