@@ -18,6 +18,8 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 #include "sphexa_amd.h"
 
@@ -112,6 +114,9 @@ struct DeviceDataset
     double                  ttot{0.0}, minDt_m1{1e-12}, minDtRho{INFINITY};
     double                  g{0.0}, etaAcc{0.2}, eps{0.005}, Krho{0.06};
     constexpr static double maxDtIncrease = 1.1;
+    // SPHX_FLAG_* bits passed with every call (no counterpart in ParticlesData); a Dataset
+    // without this member passes 0, the reference behaviour
+    int32_t sphxFlags{0};
 };
 
 //! block time-step rungs of the reference's integrator (sph/include/sph/timestep.h:19)
@@ -126,6 +131,15 @@ inline void check(sphx_workspace_t* ws, int code)
     throw std::runtime_error(std::string("sphexa_amd: ") + sphx_last_error(ws));
 }
 
+template<class D, class = void>
+struct HasSphxFlags : std::false_type
+{
+};
+template<class D>
+struct HasSphxFlags<D, std::void_t<decltype(std::declval<const D&>().sphxFlags)>> : std::true_type
+{
+};
+
 template<class Dataset>
 sphx_params_t params(const Dataset& d)
 {
@@ -136,6 +150,7 @@ sphx_params_t params(const Dataset& d)
     p.Atmin = d.Atmin, p.Atmax = d.Atmax, p.ramp = d.ramp;
     p.ng0 = d.ng0, p.ngmax = d.ngmax;
     p.hydro_type = sizeof(H) == 4 ? SPHX_F32 : SPHX_F64;
+    if constexpr (HasSphxFlags<Dataset>::value) { p.flags = d.sphxFlags; }
     return p;
 }
 
