@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--cpu-side", type=int, default=100, help="lattice side of the CPU-baseline sample")
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-std", action="store_true", help="skip the std-pass side measurement")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (counter calibration under rocprofv3)")
@@ -226,6 +227,49 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     return out
 
 
+# main kernels of the std pass (sphx_kernel_ms names)
+STD_KERNELS = ("search", "density_std", "iad_std", "momentum_std", "sfc_keys", "sort", "permute", "tree")
+
+
+def run_std(arr, spec, mixed, steps, warmup):
+    """Side measurement of the std formulation on the same particles (u-based EOS): domain sync,
+    search with the h iteration, density, EOS, IAD, momentum-energy, with record chaining. Not
+    the headline value: the headline configuration is the VE pass."""
+    import torch
+
+    from sphexa_amd import abi
+    from sphexa_amd.lib import Workspace
+    from sphexa_amd.particles import STD_FIELDS, ParticleSet, ideal_gas_cv, std_step
+
+    a = dict(arr)
+    a["u"] = a.pop("temp") * ideal_gas_cv(10.0, 5.0 / 3.0)
+    ws = Workspace(0)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-4)
+    ps = ParticleSet.from_host(a, STD_FIELDS, mixed=mixed)
+    stream = torch.cuda.current_stream()
+    ws.set_profiling(True)
+    for _ in range(warmup):
+        std_step(ws, ps, box, params)
+    torch.cuda.synchronize()
+    ms, kms = [], {k: [] for k in STD_KERNELS}
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        std_step(ws, ps, box, params)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+        for k in kms:
+            kms[k].append(ws.kernel_ms(k))
+    t = float(np.mean(ms))
+    out = dict(ms_per_step=round(t, 4), rate=round(ps.n / (t * 1e-3), 1),
+               kernel_ms={k: round(float(np.mean(v)), 4) for k, v in kms.items()})
+    del ps, ws
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(side, passes):
     from oracle import oracle as O
     from oracle import pipeline
@@ -272,6 +316,14 @@ def main():
         res[form] = run_gpu(arr, spec, form == "mixed", args.steps, args.warmup)
         print(f"# {form}: {res[form]['ms_per_step']:.3f} ms/step, phases (ms) "
               + ", ".join(f"{k}={v:.3f}" for k, v in res[form]["phase_ms"].items()), file=sys.stderr, flush=True)
+
+    std = {}
+    if not args.no_std:
+        for form in forms:
+            std[form] = run_std(arr, spec, form == "mixed", args.steps, args.warmup)
+            print(f"# std {form}: {std[form]['ms_per_step']:.3f} ms/step, kernels (ms) "
+                  + ", ".join(f"{k}={v:.3f}" for k, v in std[form]["kernel_ms"].items()), file=sys.stderr,
+                  flush=True)
 
     head = res.get("mixed") or res[forms[0]]
     mixed = "mixed" in res
@@ -328,6 +380,9 @@ def main():
                                                        / (r["integrate"]["kernel_ms"]["positions"] * 1e-3) / 1e9
                                                        / PEAK_HBM_GBS, 4))
                       for f, r in res.items()},
+        # the std formulation on the same 8M particles (u-based EOS), per form: ms per pass,
+        # particle-updates/s and kernel times; a side measurement, not the headline
+        "std_pass": std or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
