@@ -442,6 +442,8 @@ int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_
     return guarded(ws, [&] {
         SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
         eosVe(ws, f, prm, first, last);
+        ws->eosSerial = ws->callSerial;
+        ws->eosFields = *f;
         return SPHX_OK;
     });
 }

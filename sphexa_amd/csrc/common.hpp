@@ -207,6 +207,10 @@ struct sphx_workspace
         double        lo[3]{}, hi[3]{};
         int           pbc[3]{};
     } chain;
+    // the last sphx_eos_ve call (its callSerial and fields): it writes p, c, rho and p/rho only,
+    // so the IAD records that VeDefGradh leaves survive it
+    uint64_t      eosSerial = 0;
+    sphx_fields_t eosFields{};
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions

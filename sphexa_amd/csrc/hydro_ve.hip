@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
 template<class T>
 __global__ void SPHX_LIGHT_LOOP(T)
     veDefGradhKernel(PairArgs a, const RecGradh<T>* __restrict__ rec, const T* __restrict__ h,
-                     SincKernel<T> kern, T* __restrict__ kx, T* __restrict__ gradh)
+                     SincKernel<T> kern, T* __restrict__ kx, T* __restrict__ gradh, RecIadVe<T>* __restrict__ next,
+                     const T* __restrict__ vx, const T* __restrict__ vy, const T* __restrict__ vz)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -70,8 +71,10 @@ __global__ void SPHX_LIGHT_LOOP(T)
     whom          = whom * double(mi) / double(xmi) + (kxi - a.K * double(xmi) * h3Inv) * wrho0;
     double rhoi   = kxi * double(mi) / double(xmi);
     double dhdrho = -double(hi) / (rhoi * 3.0);
-    kx[i]         = T(kxi);
+    const T kxT   = T(kxi);
+    kx[i]         = kxT;
     gradh[i]      = T(1.0 - dhdrho * whom);
+    if (next) { next[i] = iadVeRecord<T>(ri.x, ri.y, ri.z, xmi, kxT, vx[i], vy[i], vz[i]); }
 }
 
 /* VE ideal-gas EOS (hydro_ve/eos.hpp:53-95) */
@@ -140,12 +143,8 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.numParticles) { return; }
-    RecIadVe<T> r;
-    r.x = coordOf<T>(a.box, 0, a.x[i]), r.y = coordOf<T>(a.box, 1, a.y[i]), r.z = coordOf<T>(a.box, 2, a.z[i]);
-    r.xm  = ld<T>(f.xm, i);
-    r.vol = fastDiv(r.xm, ld<T>(f.kx, i));
-    r.vx = ld<T>(f.vx, i), r.vy = ld<T>(f.vy, i), r.vz = ld<T>(f.vz, i);
-    rec[i] = r;
+    rec[i] = iadVeRecord<T>(coordOf<T>(a.box, 0, a.x[i]), coordOf<T>(a.box, 1, a.y[i]), coordOf<T>(a.box, 2, a.z[i]),
+                            ld<T>(f.xm, i), ld<T>(f.kx, i), ld<T>(f.vx, i), ld<T>(f.vy, i), ld<T>(f.vz, i));
 }
 
 /* IADJLoop (hydro_ve/iad_kern.hpp:44-109) + divV_curlVJLoop (hydro_ve/divv_curlv_kern.hpp:44-123)
@@ -545,10 +544,19 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, 
                                                                                               (const T*)f->xm, rec);
         SPHX_CHECK_LAUNCH();
     }
-    KernelScope timer(ws, "ve_def_gradh");
-    veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h, sincKernel<T>(ws),
-                                                                        (T*)f->kx, (T*)f->gradh);
-    SPHX_CHECK_LAUNCH();
+    // VeDefGradh leaves the IAD records (position, xm, xm/kx, v) when chaining
+    const int    out  = in < 0 ? 1 : 1 - in;
+    RecIadVe<T>* next = chainable(a, p) && f->vx && f->vy && f->vz
+                            ? recordBuffer<RecIadVe<T>>(ws, a.numParticles, out)
+                            : nullptr;
+    {
+        KernelScope timer(ws, "ve_def_gradh");
+        veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->kx, (T*)f->gradh, next, (const T*)f->vx,
+            (const T*)f->vy, (const T*)f->vz);
+        SPHX_CHECK_LAUNCH();
+    }
+    if (next) { leaveChain(ws, kChainIadVe, out, a, f, p); }
 }
 
 template<class T>
@@ -556,18 +564,23 @@ void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf, c
                  const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainIadVe, a, f, p);
     if (blocks == 0) { return; }
-    RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles, 0);
-    packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
-    SPHX_CHECK_LAUNCH();
-    RecAv<T>* next = chainable(a, p) && f->c ? recordBuffer<RecAv<T>>(ws, a.numParticles, 1) : nullptr;
+    RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        in = 0;
+        packIadVe<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a, vf, rec);
+        SPHX_CHECK_LAUNCH();
+    }
+    RecAv<T>* next = chainable(a, p) && f->c ? recordBuffer<RecAv<T>>(ws, a.numParticles, 1 - in) : nullptr;
     {
         KernelScope timer(ws, "iad_divv_curlv");
         iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, sincKernel<T>(ws), next,
                                                                             (const T*)f->c);
         SPHX_CHECK_LAUNCH();
     }
-    if (next) { leaveChain(ws, kChainAv, 1, a, f, p); }
+    if (next) { leaveChain(ws, kChainAv, 1 - in, a, f, p); }
 }
 
 template<class T>

@@ -90,6 +90,18 @@ __device__ __forceinline__ RecGradh<T> gradhRecord(Coord<T> x, Coord<T> y, Coord
     return r;
 }
 
+//! the VE IAD record, from the pack kernel or the VeDefGradh epilogue (the same bits)
+template<class T>
+__device__ __forceinline__ RecIadVe<T> iadVeRecord(Coord<T> x, Coord<T> y, Coord<T> z, T xm, T kx, T vx, T vy, T vz)
+{
+    RecIadVe<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.xm  = xm;
+    r.vol = fastDiv(xm, kx);
+    r.vx = vx, r.vy = vy, r.vz = vz;
+    return r;
+}
+
 //! workspace buffer `buf` (0 or 1) for `n` records of type R
 template<class R>
 R* recordBuffer(sphx_workspace* ws, size_t n, int buf = 0)
@@ -105,7 +117,8 @@ enum ChainKind
     kChainAv    = 1, // sphx_iad_divv_curlv -> sphx_av_switches
     kChainMomVe = 2, // sphx_av_switches -> sphx_momentum_energy_ve
     kChainGradh = 3, // sphx_xmass -> sphx_ve_def_gradh
-    kChainMomStd = 4 // sphx_iad_std -> sphx_momentum_energy_std
+    kChainMomStd = 4, // sphx_iad_std -> sphx_momentum_energy_std
+    kChainIadVe  = 5  // sphx_ve_def_gradh -> (sphx_eos_ve) -> sphx_iad_divv_curlv
 };
 
 //! the call takes part in record chaining: flag set and the full particle range
@@ -117,8 +130,13 @@ inline bool chainable(const PairArgs& a, const sphx_params_t* p)
 //! the record buffer holding the records of `kind` that the previous call wrote for this one, or -1
 inline int takeChain(sphx_workspace* ws, int kind, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
-    auto& c  = ws->chain;
-    bool  ok = chainable(a, p) && c.kind == kind && c.serial + 1 == ws->callSerial && c.n == a.numParticles &&
+    auto& c = ws->chain;
+    // the producing call directly before this one, or (IAD records only) with one sphx_eos_ve
+    // on the same fields between: the EOS writes none of the fields those records hold
+    bool next    = c.serial + 1 == ws->callSerial;
+    bool overEos = kind == kChainIadVe && c.serial + 2 == ws->callSerial && ws->eosSerial == c.serial + 1 &&
+                   std::memcmp(&ws->eosFields, f, sizeof(*f)) == 0;
+    bool ok = chainable(a, p) && c.kind == kind && (next || overEos) && c.n == a.numParticles &&
               c.form == p->hydro_type && std::memcmp(&c.fields, f, sizeof(*f)) == 0;
     for (int k = 0; k < 3; ++k)
     {

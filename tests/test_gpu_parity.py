@@ -360,6 +360,36 @@ def test_ve_record_chain_taken_and_broken(ws, mixed):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_gradh_record_chain_across_eos(ws, mixed):
+    """sphx_iad_divv_curlv uses the records sphx_ve_def_gradh left across one sphx_eos_ve (which
+    writes none of their fields), seen by doubling every other vx behind the library's back after
+    the EOS: div v stays the one of the unchanged velocities. A second EOS call breaks the chain."""
+    arr, spec = _ve_ic(14, 12)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+
+    def run(chain, scale_v, eos_calls=1):
+        ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+        n, f = ps.n, ps.fields()
+        ws.domain_sync(box, ps.sync_fields(), params, n, 64)
+        assert ws.find_neighbors(box, f, params, 0, n, True)
+        cp = abi.chained(params) if chain else params
+        ws.pair("sphx_xmass", box, f, cp, 0, n)
+        ws.pair("sphx_ve_def_gradh", box, f, cp, 0, n)
+        for _ in range(eos_calls):
+            ws.eos("sphx_eos_ve", f, params, 0, n)
+        if scale_v:
+            ps.t["vx"][::2].mul_(2)
+        ws.pair("sphx_iad_divv_curlv", box, f, cp, 0, n)
+        return ps.host("divv")
+
+    plain, scaled = run(False, False), run(False, True)
+    assert not np.array_equal(plain, scaled)
+    np.testing.assert_array_equal(run(True, True), plain)
+    np.testing.assert_array_equal(run(True, True, eos_calls=2), scaled)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_std_record_chain_bitwise_and_taken(ws, mixed):
     """std pass: the IAD epilogue writes the momentum-energy records; outputs are bitwise those of
     the packed pass. With every other c doubled behind the library's back after IAD, the chained
