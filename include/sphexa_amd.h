@@ -132,12 +132,14 @@ typedef struct sphx_params
  * that the caller changes no particle field between consecutive calls on one workspace. Then
  * sphx_xmass writes the neighbour records of sphx_ve_def_gradh in its epilogue,
  * sphx_ve_def_gradh those of sphx_iad_divv_curlv, sphx_iad_divv_curlv those of
- * sphx_av_switches, sphx_av_switches those of sphx_momentum_energy_ve and sphx_iad_std those of
- * sphx_momentum_energy_std, so the next call skips its packing kernel. The next call uses them
+ * sphx_av_switches, sphx_av_switches those of sphx_momentum_energy_ve, sphx_density_std those of
+ * sphx_iad_std and sphx_iad_std those of sphx_momentum_energy_std, so the next call skips its
+ * packing kernel. The next call uses them
  * only
  * if it comes directly after the producing one (sphx_rho_timestep, sphx_compute_timestep and
- * sphx_conserved_quantities, which read fields only, may sit between, and one sphx_eos_ve on the
- * same fields between sphx_ve_def_gradh and sphx_iad_divv_curlv), on the same fields, box,
+ * sphx_conserved_quantities, which read fields only, may sit between, and one sphx_eos_ve or
+ * sphx_eos_std on the same fields between sphx_ve_def_gradh and sphx_iad_divv_curlv or between
+ * sphx_density_std and sphx_iad_std), on the same fields, box,
  * form and particle count; otherwise it packs as without the flag. Results are bitwise the same
  * with or without the flag. */
 #define SPHX_FLAG_CHAIN 2

@@ -432,6 +432,9 @@ int sphx_eos_std(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params
     return guarded(ws, [&] {
         SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
         eosStd(ws, f, prm, first, last);
+        ws->eosSerial = ws->callSerial;
+        ws->eosFields = *f;
+        ws->eosKind   = 1;
         return SPHX_OK;
     });
 }
@@ -444,6 +447,7 @@ int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_
         eosVe(ws, f, prm, first, last);
         ws->eosSerial = ws->callSerial;
         ws->eosFields = *f;
+        ws->eosKind   = 2;
         return SPHX_OK;
     });
 }

@@ -207,10 +207,12 @@ struct sphx_workspace
         double        lo[3]{}, hi[3]{};
         int           pbc[3]{};
     } chain;
-    // the last sphx_eos_ve call (its callSerial and fields): it writes p, c, rho and p/rho only,
-    // so the IAD records that VeDefGradh leaves survive it
+    // the last EOS call (its callSerial, fields and kind: 1 std, 2 VE). The std EOS writes p and
+    // c, the VE EOS p, c, rho and p/rho: the IAD records that density / VeDefGradh leave survive
+    // an EOS of their own formulation
     uint64_t      eosSerial = 0;
     sphx_fields_t eosFields{};
+    int           eosKind = 0;
     sphx::DevBuf<uint32_t> counters;     // small device counters / flags
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions

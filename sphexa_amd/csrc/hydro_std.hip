@@ -34,10 +34,7 @@ __global__ void __launch_bounds__(kPackThreads) packVol(DevBox b, const double* 
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) { return; }
-    RecVol<T> r;
-    r.x = coordOf<T>(b, 0, x[i]), r.y = coordOf<T>(b, 1, y[i]), r.z = coordOf<T>(b, 2, z[i]);
-    r.vol  = fastDiv(m[i], rho[i]);
-    rec[i] = r;
+    rec[i] = volRecord<T>(coordOf<T>(b, 0, x[i]), coordOf<T>(b, 1, y[i]), coordOf<T>(b, 2, z[i]), m[i], rho[i]);
 }
 
 struct MomStdFields
@@ -83,7 +80,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
 template<class T, bool toRho>
 __global__ void SPHX_LIGHT_LOOP(T)
     xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
-                T* __restrict__ out, RecGradh<T>* __restrict__ next)
+                T* __restrict__ out, RecGradh<T>* __restrict__ next, RecVol<T>* __restrict__ nextVol)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -111,8 +108,10 @@ __global__ void SPHX_LIGHT_LOOP(T)
     });
     double rhoZero = double(rho0.value()) * a.K * double(h3Inv);
     T      xmi     = T(double(ri.m) / rhoZero);
-    out[i]         = toRho ? T(ri.m / xmi) : xmi;
+    const T outi   = toRho ? T(ri.m / xmi) : xmi;
+    out[i]         = outi;
     if (!toRho && next) { next[i] = gradhRecord<T>(ri.x, ri.y, ri.z, ri.m, xmi); }
+    if (toRho && nextVol) { nextVol[i] = volRecord<T>(ri.x, ri.y, ri.z, ri.m, outi); }
 }
 
 //! idealGasEOS_u (sph/include/sph/eos.hpp:35-44) or idealGasEOS with cv from mui (:47-51)
@@ -283,34 +282,41 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
                                                                                          (const T*)f->m, rec,
                                                                                          a.numParticles);
     SPHX_CHECK_LAUNCH();
-    // XMass leaves the VeDefGradh records (position, m, xm) when chaining
+    // XMass leaves the VeDefGradh records (position, m, xm), density the IAD ones (position, m/rho)
     RecGradh<T>* next = !toRho && chainable(a, p) ? recordBuffer<RecGradh<T>>(ws, a.numParticles, 1) : nullptr;
+    RecVol<T>*   nextVol = toRho && chainable(a, p) ? recordBuffer<RecVol<T>>(ws, a.numParticles, 1) : nullptr;
     {
         KernelScope timer(ws, toRho ? "density_std" : "xmass");
         if (toRho)
             xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
-                                                                                sincKernel<T>(ws), (T*)out, nullptr);
+                                                                                sincKernel<T>(ws), (T*)out, nullptr, nextVol);
         else
             xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
-                                                                                 sincKernel<T>(ws), (T*)out, next);
+                                                                                 sincKernel<T>(ws), (T*)out, next, nullptr);
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainGradh, 1, a, f, p); }
+    if (nextVol) { leaveChain(ws, kChainVol, 1, a, f, p); }
 }
 
 template<class T>
 void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
     unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    int      in     = takeChain(ws, kChainVol, a, f, p);
     if (blocks == 0) { return; }
-    RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles, 0);
-    packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
-        a.box, a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
-    SPHX_CHECK_LAUNCH();
+    RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles, in < 0 ? 0 : in);
+    if (in < 0)
+    {
+        in = 0;
+        packVol<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(
+            a.box, a.x, a.y, a.z, (const T*)f->m, (const T*)f->rho, rec, a.numParticles);
+        SPHX_CHECK_LAUNCH();
+    }
     // IAD leaves the momentum-energy records when chaining (it needs v, p and c besides its own fields)
     MomStdFields  nf{f->vx, f->vy, f->vz, f->h, f->m, f->rho, f->p, f->c};
     RecMomStd<T>* next = chainable(a, p) && f->vx && f->vy && f->vz && f->p && f->c
-                             ? recordBuffer<RecMomStd<T>>(ws, a.numParticles, 1)
+                             ? recordBuffer<RecMomStd<T>>(ws, a.numParticles, 1 - in)
                              : nullptr;
     {
         KernelScope timer(ws, "iad_std");
@@ -319,7 +325,7 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f,
             (T*)f->c33, nf, next);
         SPHX_CHECK_LAUNCH();
     }
-    if (next) { leaveChain(ws, kChainMomStd, 1, a, f, p); }
+    if (next) { leaveChain(ws, kChainMomStd, 1 - in, a, f, p); }
 }
 
 template<class T>

@@ -90,6 +90,16 @@ __device__ __forceinline__ RecGradh<T> gradhRecord(Coord<T> x, Coord<T> y, Coord
     return r;
 }
 
+//! the std IAD record, from the pack kernel or the density epilogue (the same bits)
+template<class T>
+__device__ __forceinline__ RecVol<T> volRecord(Coord<T> x, Coord<T> y, Coord<T> z, T m, T rho)
+{
+    RecVol<T> r;
+    r.x = x, r.y = y, r.z = z;
+    r.vol = fastDiv(m, rho);
+    return r;
+}
+
 //! the VE IAD record, from the pack kernel or the VeDefGradh epilogue (the same bits)
 template<class T>
 __device__ __forceinline__ RecIadVe<T> iadVeRecord(Coord<T> x, Coord<T> y, Coord<T> z, T xm, T kx, T vx, T vy, T vz)
@@ -118,7 +128,8 @@ enum ChainKind
     kChainMomVe = 2, // sphx_av_switches -> sphx_momentum_energy_ve
     kChainGradh = 3, // sphx_xmass -> sphx_ve_def_gradh
     kChainMomStd = 4, // sphx_iad_std -> sphx_momentum_energy_std
-    kChainIadVe  = 5  // sphx_ve_def_gradh -> (sphx_eos_ve) -> sphx_iad_divv_curlv
+    kChainIadVe  = 5, // sphx_ve_def_gradh -> (sphx_eos_ve) -> sphx_iad_divv_curlv
+    kChainVol    = 6  // sphx_density_std -> (sphx_eos_std) -> sphx_iad_std
 };
 
 //! the call takes part in record chaining: flag set and the full particle range
@@ -131,10 +142,11 @@ inline bool chainable(const PairArgs& a, const sphx_params_t* p)
 inline int takeChain(sphx_workspace* ws, int kind, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
     auto& c = ws->chain;
-    // the producing call directly before this one, or (IAD records only) with one sphx_eos_ve
-    // on the same fields between: the EOS writes none of the fields those records hold
+    // the producing call directly before this one, or (IAD records only) with one EOS of the same
+    // formulation on the same fields between: that EOS writes none of the fields the records hold
     bool next    = c.serial + 1 == ws->callSerial;
-    bool overEos = kind == kChainIadVe && c.serial + 2 == ws->callSerial && ws->eosSerial == c.serial + 1 &&
+    bool eosKind = (kind == kChainIadVe && ws->eosKind == 2) || (kind == kChainVol && ws->eosKind == 1);
+    bool overEos = eosKind && c.serial + 2 == ws->callSerial && ws->eosSerial == c.serial + 1 &&
                    std::memcmp(&ws->eosFields, f, sizeof(*f)) == 0;
     bool ok = chainable(a, p) && c.kind == kind && (next || overEos) && c.n == a.numParticles &&
               c.form == p->hydro_type && std::memcmp(&c.fields, f, sizeof(*f)) == 0;

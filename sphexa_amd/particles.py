@@ -111,14 +111,14 @@ def search_first_sum(ws, box, f, params, n, rho):
 
 
 def std_step(ws, ps, box, params, bucket=64, chain=True):
-    """One std-SPH pass; returns (converged, minDtCourant). With chain, IAD and momentum-energy run
-    with SPHX_FLAG_CHAIN (bitwise the same)."""
+    """One std-SPH pass; returns (converged, minDtCourant). With chain, the pair passes run with
+    SPHX_FLAG_CHAIN (bitwise the same)."""
     n = ps.n
     f = ps.fields()
     cp = abi.chained(params) if chain else params
     ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
     ps.reorder_history(ws)
-    converged = search_first_sum(ws, box, f, params, n, rho=True)
+    converged = search_first_sum(ws, box, f, cp, n, rho=True)
     ws.eos("sphx_eos_std", f, params, 0, n)
     ws.pair("sphx_iad_std", box, f, cp, 0, n)
     dt = ws.momentum_energy_std(box, f, cp, 0, n)

@@ -390,6 +390,35 @@ def test_gradh_record_chain_across_eos(ws, mixed):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
+def test_density_record_chain_across_eos(ws, mixed):
+    """sphx_iad_std uses the records (m/rho) sphx_density_std left across one sphx_eos_std: with every
+    other m doubled behind the library's back after the EOS, c_ij stays the one of the unchanged
+    masses. A second std EOS between breaks the chain."""
+    arr, spec = ic.sedov_lattice(14, seed=13)
+    box = make_box(spec)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
+
+    def run(chain, scale_m, eos_calls=1):
+        ps = ParticleSet.from_host(arr, STD_FIELDS, mixed=mixed)
+        n, f = ps.n, ps.fields()
+        ws.domain_sync(box, ps.sync_fields(), params, n, 64)
+        assert ws.find_neighbors(box, f, params, 0, n, True)
+        cp = abi.chained(params) if chain else params
+        ws.pair("sphx_density_std", box, f, cp, 0, n)
+        for _ in range(eos_calls):
+            ws.eos("sphx_eos_std", f, params, 0, n)
+        if scale_m:
+            ps.t["m"][::2].mul_(2)
+        ws.pair("sphx_iad_std", box, f, cp, 0, n)
+        return ps.host("c11")
+
+    plain, scaled = run(False, False), run(False, True)
+    assert not np.array_equal(plain, scaled)
+    np.testing.assert_array_equal(run(True, True), plain)
+    np.testing.assert_array_equal(run(True, True, eos_calls=2), scaled)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
 def test_std_record_chain_bitwise_and_taken(ws, mixed):
     """std pass: the IAD epilogue writes the momentum-energy records; outputs are bitwise those of
     the packed pass. With every other c doubled behind the library's back after IAD, the chained
