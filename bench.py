@@ -34,22 +34,32 @@ PEAK_FP64_VECTOR_TFLOPS = 78.6
 PEAK_FP32_VECTOR_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic flops per neighbour pair of the VE momentum-energy loop
-# (sph/include/sph/hydro_ve/momentum_energy_kern.hpp:121-212, avClean = false), counting
-# + - * as 1, division and square root as 1, each table lookup as 7 (index, two loads'
-# difference, slope, offset, fma): separation 3, fold/r2/dist 6, v_ij 3, 1/h_j 1, v1 v2 2,
-# h_j^-3 2, W_i W_j 2+14, IAD projections 2*(3*5+1) = 32, rho_j 2, rv 5, w_ij 1, viscosity 9,
-# signal 4, Atwood 4 + branch 2, a_mom 2, AV terms 2+2+3*4 = 16, AV energy 5, energy 7,
-# momentum 3*6 = 18, accumulations 4 -> 142.
-FLOPS_PER_PAIR_MOM_VE = 142
-# per-target part of the same kernel (loads aside): rho_i, 1/h_i, h_i^-3, eta_crit, epilogue
-FLOPS_PER_TARGET_MOM_VE = 40
+# Algorithmic flops per unit of each kernel of the VE pass, counting + - * as 1, division and
+# square root as 1, each table lookup as 7 (index, two loads' difference, slope, offset, fma),
+# whichever way the kernel obtains W:
+#  search        per candidate distance test: 3 differences, 3 squares, 2 adds (FP64 in both forms,
+#                the reference's exact predicate), plus the h iteration's re-searches
+#  xmass         per pair 19 (xmassJLoop, sph/include/sph/hydro_ve/xmass_kern.hpp:51-79)
+#  ve_def_gradh  per pair 33 (veDefGradhJLoop, hydro_ve/ve_def_gradh_kern.hpp:44-90)
+#  iad_divv_curlv per pair 37 + 59 (IADJLoop, hydro_ve/iad_kern.hpp:44-109; divV_curlVJLoop,
+#                hydro_ve/divv_curlv_kern.hpp:44-123)
+#  av_switches   per pair 58 (AVswitchesJLoop, hydro_ve/av_switches_kern.hpp:44-137)
+#  momentum_ve   per pair 142 (momentumAndEnergyJLoop, hydro_ve/momentum_energy_kern.hpp:121-212,
+#                avClean = false: separation 3, fold/r2/dist 6, v_ij 3, 1/h_j 1, v1 v2 2, h_j^-3 2,
+#                W_i W_j 2+14, IAD projections 2*(3*5+1) = 32, rho_j 2, rv 5, w_ij 1, viscosity 9,
+#                signal 4, Atwood 4 + branch 2, a_mom 2, AV terms 2+2+3*4 = 16, AV energy 5,
+#                energy 7, momentum 3*6 = 18, accumulations 4), plus 40 per target (rho_i, 1/h_i,
+#                h_i^-3, eta_crit, epilogue)
+FLOPS_PER_TEST_SEARCH = 8
+FLOPS_PER_PAIR = {"xmass": 19, "ve_def_gradh": 33, "iad_divv_curlv": 37 + 59, "av_switches": 58, "momentum_ve": 142}
+FLOPS_PER_TARGET = {"momentum_ve": 40}
+FLOPS_PER_PAIR_MOM_VE = FLOPS_PER_PAIR["momentum_ve"]
 
-# HBM bytes per launch of the momentum-energy kernel on this workload (200^3), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE counter runs scaled by a plain copy kernel of known size (read x 2.0,
-# write x 1.0); see profiles/ve_turbulence_8M_counters.md. Counters cannot be read from inside the
-# timed run, so the measured figure is carried here for the default workload only.
-MEASURED_TRAFFIC_MOM_VE = {(200, "mixed"): 4.992e9, (200, "fp64"): 6.928e9}
+# HBM bytes per launch on the default workload (200^3), from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# counter runs scaled by a plain copy kernel of known size (read x 2.0, write x 1.0); see the
+# counter summaries under profiles/. Counters cannot be read from inside the timed run, so the
+# measured figures are carried here for the default workload only.
+MEASURED_TRAFFIC = {("momentum_ve", 200, "mixed"): 4.992e9, ("momentum_ve", 200, "fp64"): 6.928e9}
 
 
 def parse():
@@ -58,10 +68,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--side", type=int, default=200, help="lattice side (N = side^3)")
-    ap.add_argument("--cpu-side", type=int, default=100, help="lattice side of the CPU-baseline sample")
-    ap.add_argument("--cpu-passes", type=int, default=3)
+    ap.add_argument("--cpu-passes", type=int, default=2, help="passes of the CPU baseline (median)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-std", action="store_true", help="skip the std-pass side measurement")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the side workloads (Evrard sphere, adjoint pass)")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (counter calibration under rocprofv3)")
@@ -197,7 +208,7 @@ def run_gpu(arr, spec, mixed, steps, warmup):
     step_ms_median = float(np.median([e[0].elapsed_time(e[-1]) for e in events]))
     pairs, maxnc = ws.neighbor_stats()
     out = dict(n=n, wall_s=wall, ms_per_step=1e3 * wall / steps, step_ms_median=step_ms_median, phase_ms=phase_ms,
-               pairs=int(pairs),
+               pairs=int(pairs), search_tests=int(ws.search_tests()),
                max_nc=int(maxnc), rate=n * steps / wall,
                kernel_ms={k: float(np.mean(v)) for k, v in kernel_ms.items()},
                fields={k: ps.host(k).astype(np.float64) for k in COMPARED},
@@ -270,27 +281,245 @@ def run_std(arr, spec, mixed, steps, warmup):
     return out
 
 
-def cpu_baseline(side, passes):
+EVRARD_KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve", "sfc_keys",
+                  "sort", "permute", "tree")
+
+
+def run_evrard(n, mixed, steps, warmup):
+    """Side measurement on the load-balance workload: the Evrard-like sphere (rho ~ 1/r, h spanning
+    ~12x, open box [-1,1]^3), VE pass with record chaining; ms per pass, particle-updates/s and
+    kernel times, after a first pass that converges h from the initial estimate."""
+    import torch
+
+    from sphexa_amd import abi, ic
+    from sphexa_amd.lib import Workspace
+    from sphexa_amd.particles import VE_FIELDS, ParticleSet, ideal_gas_cv, ve_step
+
+    arr, spec = ic.evrard_sphere(n, seed=11)
+    arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
+    ws = Workspace(0)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.OPEN)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-4)
+    ps = ParticleSet.from_host(arr, VE_FIELDS, mixed=mixed)
+    stream = torch.cuda.current_stream()
+    ws.set_profiling(True)
+    for _ in range(1 + warmup):
+        ve_step(ws, ps, box, params)
+    torch.cuda.synchronize()
+    ms, kms = [], {k: [] for k in EVRARD_KERNELS}
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ve_step(ws, ps, box, params)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+        for k in kms:
+            kms[k].append(ws.kernel_ms(k))
+    pairs, maxnc = ws.neighbor_stats()
+    nc = ps.host("nc")
+    t = float(np.mean(ms))
+    out = dict(particles=ps.n, ms_per_step=round(t, 4), rate=round(ps.n / (t * 1e-3), 1),
+               nc_mean=round(float(nc.mean()), 2), nc_max=int(maxnc), pairs=int(pairs),
+               search_tests=int(ws.search_tests()),
+               kernel_ms={k: round(float(np.mean(v)), 4) for k, v in kms.items()})
+    del ps, ws
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_adjoint(side, mixed, steps, warmup):
+    """The differentiation workload: the Sedov-like geometry at side^3 particles, std pass with a
+    per-particle AV coefficient (forward: density, EOS, IAD, momentum-energy over the lists of one
+    search) and its reverse pass through the three adjoint entry points for
+    L = sum (rho - rho_t)^2 + 1e-3 sum |a|^2 (rho_t = 1 + 0.05 sin(seeded phase)), as dL/dx, dL/dh
+    and dL/dalpha. The search and domain sync are not repeated per step: the lists are the
+    forward's checkpoint."""
+    import torch
+
+    from sphexa_amd import abi, ic
+    from sphexa_amd.lib import Workspace
+    from sphexa_amd.particles import STD_FIELDS, ParticleSet
+
+    arr, spec = ic.sedov_lattice(side, seed=42)
+    ws = Workspace(0)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    hydro = abi.SPHX_F32 if mixed else abi.SPHX_F64
+    params = abi.Params.defaults(ws.K, hydro_type=hydro)
+    ps = ParticleSet.from_host(arr, STD_FIELDS + ("alpha",), mixed=mixed)
+    n = ps.n
+    ws.domain_sync(box, ps.sync_fields(), params, n)
+    f = ps.fields()
+    ws.find_neighbors(box, f, params, 0, n, True)
+    params.flags |= abi.SPHX_FLAG_STD_ALPHA
+    dt = ps.hydro_dtype
+    rng = np.random.default_rng(5)
+    phase = torch.from_numpy(rng.uniform(0, 2 * np.pi, 3)).cuda()
+    x = ps.t["x"][:n]
+    rho_t = 1.0 + 0.05 * torch.sin(2 * np.pi * x + phase[0])
+    w = 1e-3
+    z = lambda: torch.zeros(n, dtype=dt, device="cuda")
+    g = {k: z() for k in ("x", "y", "z", "h", "rho", "p", "alpha", "c11", "c12", "c13", "c22", "c23", "c33")}
+    cot = {k: z() for k in ("rho", "c11", "c12", "c13", "c22", "c23", "c33", "ax", "ay", "az")}
+    gm = abi.Adjoint()
+    for k, t in g.items():
+        setattr(gm, k, t.data_ptr())
+    cm = abi.Adjoint()
+    for k in ("ax", "ay", "az"):
+        setattr(cm, k, cot[k].data_ptr())
+    gi = abi.Adjoint()
+    for k in ("x", "y", "z", "h", "rho"):
+        setattr(gi, k, g[k].data_ptr())
+    ci = abi.Adjoint()
+    for k in ("c11", "c12", "c13", "c22", "c23", "c33"):
+        setattr(ci, k, g[k].data_ptr())  # the momentum adjoint's c_ij gradients are the IAD cotangents
+    gd = abi.Adjoint()
+    for k in ("x", "y", "z", "h"):
+        setattr(gd, k, g[k].data_ptr())
+    cd = abi.Adjoint()
+    cd.rho = cot["rho"].data_ptr()
+    u = ps.t["u"][:n]
+    stream = torch.cuda.current_stream()
+
+    def step():
+        for t in g.values():
+            t.zero_()
+        ws.pair("sphx_density_std", box, f, params, 0, n)
+        ws.eos("sphx_eos_std", f, params, 0, n)
+        ws.pair("sphx_iad_std", box, f, params, 0, n)
+        ws.momentum_energy_std(box, f, params, 0, n)
+        for k in ("ax", "ay", "az"):
+            cot[k].copy_(2.0 * w * ps.t[k][:n])
+        ws.vjp("sphx_momentum_energy_std_vjp", box, f, params, 0, n, cm, gm)
+        # rho's cotangent: the loss term, and p = rho u (gamma - 1) of the EOS
+        g["rho"].add_((2.0 * (ps.t["rho"][:n].double() - rho_t) + g["p"].double() * u * (params.gamma - 1.0))
+                      .to(dt))
+        ws.vjp("sphx_iad_std_vjp", box, f, params, 0, n, ci, gi)
+        cot["rho"].copy_(g["rho"])
+        ws.vjp("sphx_density_std_vjp", box, f, params, 0, n, cd, gd)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = float(np.mean(ms))
+    finite = all(bool(torch.isfinite(g[k]).all()) for k in ("x", "y", "z", "h", "alpha"))
+    out = dict(particles=n, ms_per_step=round(t, 4), rate=round(n / (t * 1e-3), 1), gradients_finite=finite)
+    del ps, ws, g, cot
+    torch.cuda.empty_cache()
+    return out
+
+
+def kernel_rooflines(r, mixed):
+    """Roofline entry of every kernel of the VE pass: algorithmic flops per launch (per unit above
+    times the units of the launch) over the kernel's mean duration from HIP events on its stream,
+    against the FP64 vector peak for FP64 work (the search's distance tests in both forms) and the
+    FP32 peaks for the FP32 pair loops: 157.3 TFLOP/s with packed FP32 issue, 78.6 TFLOP/s without
+    (this build compiles with packed FP32 off)."""
+    out = {}
+    side = round(r["n"] ** (1.0 / 3.0))
+    form = "mixed" if mixed else "fp64"
+    for k in ("search",) + PAIR_KERNELS:
+        ms = r["kernel_ms"][k]
+        if k == "search":
+            flops = FLOPS_PER_TEST_SEARCH * r["search_tests"]
+            fp64 = True
+            units = dict(candidate_tests=r["search_tests"], tests_per_stored_pair=round(r["search_tests"] / r["pairs"], 2))
+        else:
+            flops = FLOPS_PER_PAIR[k] * r["pairs"] + FLOPS_PER_TARGET.get(k, 0) * r["n"]
+            fp64 = not mixed
+            units = dict(pairs=r["pairs"])
+        ach = flops / (ms * 1e-3) / 1e12
+        e = dict(bound="valu", achieved=round(ach, 3), unit="TFLOP/s", avg_launch_ms=round(ms, 4),
+                 flops_per_launch=int(flops), precision="fp64" if fp64 else "fp32", **units)
+        if fp64:
+            e.update(peak=PEAK_FP64_VECTOR_TFLOPS, frac=round(ach / PEAK_FP64_VECTOR_TFLOPS, 4))
+        else:
+            e.update(peak=PEAK_FP32_VECTOR_TFLOPS, frac=round(ach / PEAK_FP32_VECTOR_TFLOPS, 4),
+                     peak_nonpacked=PEAK_FP64_VECTOR_TFLOPS, frac_nonpacked=round(ach / PEAK_FP64_VECTOR_TFLOPS, 4))
+        tr = MEASURED_TRAFFIC.get((k, side, form))
+        e["traffic"] = tr
+        if tr:
+            e["traffic_gbs"] = round(tr / (ms * 1e-3) / 1e9, 1)
+        out[k] = e
+    return out
+
+
+def cpu_info():
+    """CPU model, physical cores (distinct physical id / core id pairs of /proc/cpuinfo), logical
+    CPUs, CPUs this process may run on, and whether the CPU runs x86-64-v4 (AVX-512) code."""
+    model, cores, flags = None, set(), set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif k == "flags" and not flags:
+                    flags = set(v.split())
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return dict(model=model, physical_cores=len(cores) or None, logical_cpus=os.cpu_count(), affinity=affinity,
+                x86_64_v4=v4)
+
+
+def cpu_baseline(arr, spec, passes):
+    """The reference's own CPU path (its headers compiled with its Release flags, -O3 -DNDEBUG and
+    OpenMP, -march fixed to the x86-64 level this host runs) timed on the same particles as the GPU
+    pass; the bit-exact reference build or, without the reference checkout at build time, the oracle
+    when that is all there is."""
     from oracle import oracle as O
     from oracle import pipeline
     from sphexa_amd import abi
 
-    kind = "reference" if O.available("ref") else "port"
-    lib = O.load("ref" if kind == "reference" else "orc")
-    arr, spec = make_ic(side, seed=7)
+    info = cpu_info()
+    candidates = [("ref_rel_v4" if info["x86_64_v4"] else "ref_rel_v3", "reference", "-O3 -DNDEBUG -fopenmp "
+                   + ("-march=x86-64-v4" if info["x86_64_v4"] else "-march=x86-64-v3")),
+                  ("ref", "reference", "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"),
+                  ("orc", "port", "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off")]
+    lib_kind, kind, flags = next(c for c in candidates if O.available(c[0]))
+    lib = O.load(lib_kind)
     box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
     params = abi.Params.defaults(lib.kernel_K(6.0), minDt=1e-4)
-    times = []
+    times, phases = [], []
     for _ in range(passes):
         t0 = time.perf_counter()
-        pipeline.run(lib, arr, box, params, form="ve")
+        r = pipeline.run(lib, arr, box, params, form="ve")
         times.append(time.perf_counter() - t0)
-    n = side ** 3
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=n / float(np.median(times)), unit="particle-updates/s", cores=threads, kind=kind,
+        phases.append(r["times"])
+        del r
+    n = len(arr["x"])
+    threads = int(os.environ.get("OMP_NUM_THREADS", info["affinity"]))
+    return dict(value=round(n / float(np.median(times)), 1), unit="particle-updates/s", cores=threads, kind=kind,
                 sample=f"VE FP64 pass (keys, sort, tree, findNeighbors+h-iteration, XMass, VeDefGradh, EOS, "
-                       f"IAD+divv/curlv, AVswitches, momentum-energy) of the same turbulence workload at "
-                       f"{side}^3 = {n} particles, median of {passes} passes, OpenMP threads = {threads}")
+                       f"IAD+divv/curlv, AVswitches, momentum-energy) of the same turbulence workload and particles "
+                       f"as the GPU value, {n} particles, median of {passes} passes",
+                threads=threads, build=flags, library=os.path.basename(O.LIBS[lib_kind]), cpu=info,
+                pass_s=[round(t, 3) for t in times],
+                phase_s={k: round(float(np.median([p[k] for p in phases])), 3) for k in phases[0]})
 
 
 def main():
@@ -325,22 +554,22 @@ def main():
                   + ", ".join(f"{k}={v:.3f}" for k, v in std[form]["kernel_ms"].items()), file=sys.stderr,
                   flush=True)
 
+    side_runs = {}
+    if not args.no_side:
+        side_runs["evrard_4M_ve"] = {f: run_evrard(4_000_000, f == "mixed", args.steps, args.warmup) for f in forms}
+        side_runs["adjoint_2M_std"] = {f: run_adjoint(126, f == "mixed", args.steps, args.warmup) for f in forms}
+        for name, r in side_runs.items():
+            print(f"# {name}: " + ", ".join(f"{f} {v['ms_per_step']:.3f} ms" for f, v in r.items()), file=sys.stderr,
+                  flush=True)
+
     head = res.get("mixed") or res[forms[0]]
     mixed = "mixed" in res
-    mom_ms = head["kernel_ms"]["momentum_ve"]
-    flops = FLOPS_PER_PAIR_MOM_VE * head["pairs"] + FLOPS_PER_TARGET_MOM_VE * head["n"]
-    achieved = flops / (mom_ms * 1e-3) / 1e12
-    peak = PEAK_FP32_VECTOR_TFLOPS if mixed else PEAK_FP64_VECTOR_TFLOPS
-    traffic = MEASURED_TRAFFIC_MOM_VE.get((args.side, "mixed" if mixed else "fp64"))
-    # compute-bound roofline: MI355X's FP32 / FP64 matrix peaks (157.3 / 78.6 TFLOP/s) equal its
-    # non-packed vector peaks, and this gather-reduce runs on the vector ALUs ("pipe")
-    roofline = dict(bound="mfma", pipe="valu", kernel="momentumVeKernel", achieved=round(achieved, 3), peak=peak,
-                    unit="TFLOP/s", frac=round(achieved / peak, 4), traffic=traffic,
-                    traffic_gbs=round(traffic / (mom_ms * 1e-3) / 1e9, 1) if traffic else None,
-                    hbm_peak_gbs=PEAK_HBM_GBS,
-                    flops_per_launch=int(flops), avg_launch_ms=round(mom_ms, 4))
+    rl = {f: kernel_rooflines(r, f == "mixed") for f, r in res.items()}
+    head_rl = rl["mixed" if mixed else forms[0]]
+    dominant = max(head_rl, key=lambda k: head_rl[k]["avg_launch_ms"])
+    roofline = dict(kernel=dominant, **head_rl[dominant])
 
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_side, args.cpu_passes)
+    cpu = None if args.no_cpu else cpu_baseline(arr, spec, args.cpu_passes)
     line = {
         "metric": METRIC,
         "value": round(head["rate"], 1),
@@ -383,7 +612,12 @@ def main():
         # the std formulation on the same 8M particles (u-based EOS), per form: ms per pass,
         # particle-updates/s and kernel times; a side measurement, not the headline
         "std_pass": std or None,
+        # the kernel with the largest share of the headline pass, then every kernel of the pass per form
         "roofline": roofline,
+        "roofline_kernels": rl,
+        # the load-balance workload (Evrard-like sphere, 4M, VE) and the differentiation workload
+        # (2M, std forward + adjoint), per form
+        "side_workloads": side_runs or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

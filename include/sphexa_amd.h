@@ -121,7 +121,18 @@ typedef struct sphx_params
     uint32_t ng0, ngmax;
     int32_t  hydro_type;     /* sphx_dtype of the HydroType fields */
     int32_t  flags;          /* SPHX_FLAG_* bits, 0 = reference behaviour */
+    int32_t  eos;            /* eosChoice (sph/include/sph/eos.hpp:10-15): SPHX_EOS_IDEAL_GAS (0), SPHX_EOS_ISOTHERMAL,
+                                SPHX_EOS_POLYTROPIC */
+    int32_t  reserved;
+    double   soundSpeedConst;  /* isothermal sound speed (particles_data.hpp:127, default 1) */
+    double   polytropic_const; /* polytropic constant K (particles_data.hpp:123, default 1) */
+    double   polytropic_index; /* polytropic exponent (particles_data.hpp:121, default 5/3) */
 } sphx_params_t;
+
+/* sphx_params_t.eos, the reference's EosType (sph/include/sph/eos.hpp:10-15) */
+#define SPHX_EOS_IDEAL_GAS 0
+#define SPHX_EOS_ISOTHERMAL 1
+#define SPHX_EOS_POLYTROPIC 2
 
 /* std momentum-energy with a per-particle AV coefficient from fields.alpha instead of the
  * reference's constant alpha = 1 (hydro_std/momentum_energy_kern.hpp:98); the coefficient the
@@ -304,13 +315,19 @@ int sphx_find_neighbors(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_
 int sphx_find_neighbors_xmass(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                               const sphx_params_t* prm, uint32_t first, uint32_t last, int iterateH, int out);
 
-/* Copy the stored neighbour list of targets [first,last) into `out` (device or host memory,
- * row-major [last-first][ngmax], entries beyond min(nc-1, ngmax) unspecified). Test/inspection
+/* Copy the stored neighbour list of targets [first,last), a sub-range of the last search's range,
+ * into `out` (device or host memory, row-major [last-first][ngmax], row k for target first + k,
+ * entries beyond min(nc-1, ngmax) unspecified). Test/inspection
  * helper matching the CPU Dataset's d.neighbors layout (particles_data.hpp:250-251). */
 int sphx_copy_neighbors(sphx_workspace_t* ws, uint32_t* out, uint32_t first, uint32_t last);
 
 /* Pair-test statistics of the last search: total stored pairs and maximum count. */
 int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t* maxCount);
+
+/* Candidate distance tests the last search performed (each candidate swept against all 64 targets of
+ * its group, h iterations included), for work accounting; no counterpart in the reference, whose
+ * neighbour driver counts the same quantity as sumP2P (domain/test/performance/neighbor_driver.cu:146). */
+int sphx_search_tests(const sphx_workspace_t* ws, uint64_t* tests);
 
 /* ---- std SPH (sph/include/sph/hydro_std) ---------------------------------------------------- */
 
@@ -318,8 +335,11 @@ int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t
 int sphx_density_std(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                      const sphx_params_t* prm, uint32_t first, uint32_t last);
 
-/* Ideal-gas EOS from u (or temp when u is NULL): p, c. Replaces computeEOS_HydroStd with
- * eosChoice = idealGas (hydro_std/eos.hpp:55-153). */
+/* std EOS selected by prm->eos, as computeEOS_HydroStd selects by eosChoice (hydro_std/eos.hpp:146-152).
+ * Ideal gas (computeIdealGasEOS_HydroStd, :55-82): p, c from u (or temp when u is NULL). Isothermal
+ * (:82-97): p = rho c0^2 with c0 = soundSpeedConst, c = c0 as the reference's GPU kernel stores it
+ * (hydro_std/eos_gpu.cu:74-82), temp = 0 when given. Polytropic (:99-114): p = K rho^n,
+ * c = sqrt(n p / rho), temp = 0 when given. */
 int sphx_eos_std(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
                  uint32_t last);
 
@@ -342,8 +362,12 @@ int sphx_xmass(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t*
 int sphx_ve_def_gradh(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
                       const sphx_params_t* prm, uint32_t first, uint32_t last);
 
-/* VE ideal-gas EOS: prho, c (and rho, p when non-NULL) from temp (u when temp is NULL).
- * Replaces computeEOS with eosChoice = idealGas (hydro_ve/eos.hpp:53-197). */
+/* VE EOS selected by prm->eos, as computeEOS selects by eosChoice (hydro_ve/eos.hpp:191-197); each writes
+ * prho = p / (kx m^2 gradh) and c, and rho = kx m / xm and p when those are non-NULL. Ideal gas
+ * (computeIdealGasEOS_Impl, :53-95): from temp (u when temp is NULL). Isothermal (:97-123): p = rho c0^2,
+ * c = c0, temp = 0 when given. Polytropic (:125-151): p = K rho^n, c = sqrt(n p / rho), temp = 0 when
+ * given. Replaces computeIdealGasEOS, computeIsothermalEOS and computePolytropicEOS
+ * (sph/include/sph/sph_gpu.hpp:48-62). */
 int sphx_eos_ve(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm, uint32_t first,
                 uint32_t last);
 

@@ -110,6 +110,11 @@ struct DeviceDataset
     double   minDt{1e-12}, minDtCourant{INFINITY};
     double   alphamin{0.05}, alphamax{1.0}, decay_constant{0.2};
     double   Atmin{0.1}, Atmax{0.2}, ramp{1.0 / (0.2 - 0.1)};
+    // equation of state (particles_data.hpp:117-127): eosChoice as sph::EosType (0 ideal gas,
+    // 1 isothermal, 2 polytropic)
+    int       eosChoice{SPHX_EOS_IDEAL_GAS};
+    double    polytropic_index{5. / 3.}, polytropic_const{1.};
+    HydroType soundSpeedConst{1.0};
     // time stepping (particles_data.hpp:96-141)
     double                  ttot{0.0}, minDt_m1{1e-12}, minDtRho{INFINITY};
     double                  g{0.0}, etaAcc{0.2}, eps{0.005}, Krho{0.06};
@@ -140,6 +145,15 @@ struct HasSphxFlags<D, std::void_t<decltype(std::declval<const D&>().sphxFlags)>
 {
 };
 
+template<class D, class = void>
+struct HasEosChoice : std::false_type
+{
+};
+template<class D>
+struct HasEosChoice<D, std::void_t<decltype(std::declval<const D&>().eosChoice)>> : std::true_type
+{
+};
+
 template<class Dataset>
 sphx_params_t params(const Dataset& d)
 {
@@ -151,6 +165,15 @@ sphx_params_t params(const Dataset& d)
     p.ng0 = d.ng0, p.ngmax = d.ngmax;
     p.hydro_type = sizeof(H) == 4 ? SPHX_F32 : SPHX_F64;
     if constexpr (HasSphxFlags<Dataset>::value) { p.flags = d.sphxFlags; }
+    // a Dataset without an EOS choice runs the ideal gas, the reference's default
+    p.eos = SPHX_EOS_IDEAL_GAS, p.soundSpeedConst = 1.0, p.polytropic_const = 1.0, p.polytropic_index = 5. / 3.;
+    if constexpr (HasEosChoice<Dataset>::value)
+    {
+        p.eos              = int32_t(d.eosChoice);
+        p.soundSpeedConst  = double(d.soundSpeedConst);
+        p.polytropic_const = double(d.polytropic_const);
+        p.polytropic_index = double(d.polytropic_index);
+    }
     return p;
 }
 
@@ -324,7 +347,7 @@ SPHEXA_AMD_GROUP_PASS(computeAVswitches, sphx_av_switches)
 
 #undef SPHEXA_AMD_GROUP_PASS
 
-//! sph::computeEOS_HydroStd, idealGas (hydro_std/eos.hpp:147-153)
+//! sph::computeEOS_HydroStd (hydro_std/eos.hpp:146-152): ideal gas, isothermal or polytropic by d.eosChoice
 template<class Dataset>
 void computeEOS_HydroStd(size_t startIndex, size_t endIndex, Dataset& d)
 {
@@ -333,7 +356,7 @@ void computeEOS_HydroStd(size_t startIndex, size_t endIndex, Dataset& d)
     detail::check(d.workspace, sphx_eos_std(d.workspace, &f, &p, uint32_t(startIndex), uint32_t(endIndex)));
 }
 
-//! sph::computeEOS, idealGas (hydro_ve/eos.hpp:192-197)
+//! sph::computeEOS (hydro_ve/eos.hpp:191-197): ideal gas, isothermal or polytropic by d.eosChoice
 template<class Dataset>
 void computeEOS(size_t startIndex, size_t endIndex, Dataset& d)
 {

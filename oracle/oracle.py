@@ -13,7 +13,11 @@ import numpy as np
 from sphexa_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBS = {"orc": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libsphx_ref.so")}
+LIBS = {"orc": os.path.join(HERE, "liboracle.so"), "ref": os.path.join(HERE, "_ref", "libsphx_ref.so"),
+        # the reference built with its Release flags (timed CPU baseline only), per x86-64 level
+        "ref_rel_v3": os.path.join(HERE, "_ref", "libsphx_ref_rel_v3.so"),
+        "ref_rel_v4": os.path.join(HERE, "_ref", "libsphx_ref_rel_v4.so")}
+PREFIX = {"orc": "orc", "ref": "ref", "ref_rel_v3": "ref", "ref_rel_v4": "ref"}
 
 _P = C.c_void_p
 _SZ = C.c_size_t
@@ -76,7 +80,7 @@ def available(kind="orc"):
 
 
 def load(kind="orc"):
-    """Return a namespace of the oracle's C functions (kind 'orc' or 'ref')."""
+    """Return a namespace of the oracle's C functions (kind 'orc', 'ref', 'ref_rel_v3' or 'ref_rel_v4')."""
     if kind in _cache:
         return _cache[kind]
     path = LIBS[kind]
@@ -85,9 +89,9 @@ def load(kind="orc"):
     lib = C.CDLL(path)
     ns = {}
     for name, (res, args) in _SIGS.items():
-        if not hasattr(lib, f"{kind}_{name}"):
+        if not hasattr(lib, f"{PREFIX[kind]}_{name}"):
             continue
-        fn = getattr(lib, f"{kind}_{name}")
+        fn = getattr(lib, f"{PREFIX[kind]}_{name}")
         fn.restype = res
         fn.argtypes = args
         ns[name] = fn

@@ -73,10 +73,41 @@ void xmass(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, Local
     }
 }
 
-//! computeIdealGasEOS_HydroStd_Impl, hydro_std/eos.hpp:55-82
+/* computeEOS_HydroStd, hydro_std/eos.hpp:146-152: the isothermal and polytropic branches
+ * (computeIsothermalEOS_HydroStd_Impl :82-97, computePolytropicEOS_HydroStd_Impl :99-114). The isothermal
+ * branch also stores c = c0, as the reference's GPU kernel does (hydro_std/eos_gpu.cu:73-82); its CPU
+ * loop leaves c unchanged. */
+template<class T>
+void eosStdVariant(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
+{
+    const T* rho = F<T>(f->rho);
+    T*       p   = F<T>(f->p);
+    T*       c   = F<T>(f->c);
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        if (prm->eos == SPHX_EOS_ISOTHERMAL)
+        {
+            const T c0 = T(prm->soundSpeedConst);
+            p[i]       = isothermalP(c0, rho[i]);
+            c[i]       = c0;
+        }
+        else
+        {
+            double pi, ci;
+            polytropicPC(prm->polytropic_const, prm->polytropic_index, rho[i], pi, ci);
+            p[i] = pi;
+            c[i] = ci;
+        }
+        if (f->temp) { f->temp[i] = 0; }
+    }
+}
+
+//! computeIdealGasEOS_HydroStd_Impl, hydro_std/eos.hpp:55-80
 template<class T>
 void eosStd(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
 {
+    if (prm->eos != SPHX_EOS_IDEAL_GAS) { return eosStdVariant<T>(f, prm, first, last); }
     T* rho = F<T>(f->rho);
     T* p   = F<T>(f->p);
     T* c   = F<T>(f->c);
@@ -211,10 +242,49 @@ void veDefGradh(const Box& box, const sphx_fields_t* f, const sphx_params_t* p, 
     }
 }
 
+/* computeIsothermalEOS_Impl and computePolytropicEOS_Impl, hydro_ve/eos.hpp:97-123 and :126-151, as
+ * computeEOS selects them (:191-197) */
+template<class T>
+void eosVeVariant(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
+{
+    const T* m     = F<T>(f->m);
+    const T* kx    = F<T>(f->kx);
+    const T* xm    = F<T>(f->xm);
+    const T* gradh = F<T>(f->gradh);
+    T*       prho  = F<T>(f->prho);
+    T*       c     = F<T>(f->c);
+    T*       rhoO  = F<T>(f->rho);
+    T*       pO    = F<T>(f->p);
+#pragma omp parallel for schedule(static)
+    for (LocalIndex i = first; i < last; ++i)
+    {
+        auto rho = kx[i] * m[i] / xm[i];
+        if (prm->eos == SPHX_EOS_ISOTHERMAL)
+        {
+            const T c0 = T(prm->soundSpeedConst);
+            auto    pi = isothermalP(c0, rho);
+            prho[i]    = pi / (kx[i] * m[i] * m[i] * gradh[i]);
+            c[i]       = c0;
+            if (pO) { pO[i] = pi; }
+        }
+        else
+        {
+            double pi, ci;
+            polytropicPC(prm->polytropic_const, prm->polytropic_index, rho, pi, ci);
+            prho[i] = pi / (kx[i] * m[i] * m[i] * gradh[i]);
+            c[i]    = ci;
+            if (pO) { pO[i] = pi; }
+        }
+        if (rhoO) { rhoO[i] = rho; }
+        if (f->temp) { f->temp[i] = 0; }
+    }
+}
+
 //! computeIdealGasEOS_Impl, hydro_ve/eos.hpp:53-95 (temp-based when temp != NULL, else u-based)
 template<class T>
 void eosVe(const sphx_fields_t* f, const sphx_params_t* prm, LocalIndex first, LocalIndex last)
 {
+    if (prm->eos != SPHX_EOS_IDEAL_GAS) { return eosVeVariant<T>(f, prm, first, last); }
     const T* m     = F<T>(f->m);
     const T* kx    = F<T>(f->kx);
     const T* xm    = F<T>(f->xm);

@@ -127,6 +127,9 @@ Data<T> makeData(const sphx_fields_t* f, const sphx_params_t* p, size_t n, const
     d.ng0 = p->ng0, d.ngmax = p->ngmax, d.K = p->K, d.Kcour = p->Kcour, d.gamma = p->gamma, d.minDt = p->minDt;
     d.muiConst = T(p->muiConst), d.alphamin = T(p->alphamin), d.alphamax = T(p->alphamax);
     d.decay_constant = T(p->decay_constant), d.Atmin = T(p->Atmin), d.Atmax = T(p->Atmax), d.ramp = T(p->ramp);
+    d.eosChoice        = sph::EosType(p->eos);
+    d.soundSpeedConst  = T(p->soundSpeedConst);
+    d.polytropic_const = p->polytropic_const, d.polytropic_index = p->polytropic_index;
     return d;
 }
 
@@ -363,8 +366,15 @@ void ref_eos_ve(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, 
     sphx_fields_t g = *f;
     if (g.temp != nullptr) { g.u = nullptr; }
     f = &g;
-    if (p->hydro_type == SPHX_F32) { REF_DATA(float); sph::computeIdealGasEOS_Impl(first, last, d); }
-    else { REF_DATA(double); sph::computeIdealGasEOS_Impl(first, last, d); }
+    // computeEOS's selection (hydro_ve/eos.hpp:191-197) among the CPU loops
+    auto run = [&](auto& d)
+    {
+        if (p->eos == SPHX_EOS_ISOTHERMAL) { sph::computeIsothermalEOS_Impl(first, last, d); }
+        else if (p->eos == SPHX_EOS_POLYTROPIC) { sph::computePolytropicEOS_Impl(first, last, d); }
+        else { sph::computeIdealGasEOS_Impl(first, last, d); }
+    };
+    if (p->hydro_type == SPHX_F32) { REF_DATA(float); run(d); }
+    else { REF_DATA(double); run(d); }
 }
 
 void ref_iad_divv_curlv(const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,

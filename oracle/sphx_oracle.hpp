@@ -1396,6 +1396,23 @@ inline void idealGasFromU(T1 u, T2 rho, T3 gamma, std::common_type_t<T1, T2, T3>
     c        = std::sqrt(gamma * tmp);
 }
 
+//! isothermalEOS, eos.hpp:58-64: p = rho c c in the common type
+template<class T1, class T2>
+inline std::common_type_t<T1, T2> isothermalP(T1 c, T2 rho)
+{
+    using Tc = std::common_type_t<T1, T2>;
+    Tc p     = rho * c * c;
+    return p;
+}
+
+//! polytropicEOS, eos.hpp:77-86: p = K rho^n, c = sqrt(n p / rho) in the common type
+template<class T1, class T2, class T3>
+inline void polytropicPC(T1 K, T2 n, T3 rho, std::common_type_t<T1, T2, T3>& p, std::common_type_t<T1, T2, T3>& c)
+{
+    p = K * std::pow(rho, n);
+    c = std::sqrt(n * p / rho);
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Time integration (sph/include/sph/positions.hpp, positions_gpu.cu, update_h.hpp, ts_global.hpp).
  * Follows the reference's GPU kernel and its instantiated types (positions_gpu.cu:116-193):

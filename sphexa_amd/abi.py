@@ -63,20 +63,24 @@ class Params(C.Structure):
                 ("minDt", C.c_double), ("alphamin", C.c_double), ("alphamax", C.c_double),
                 ("decay_constant", C.c_double), ("Atmin", C.c_double), ("Atmax", C.c_double),
                 ("ramp", C.c_double), ("ng0", C.c_uint32), ("ngmax", C.c_uint32),
-                ("hydro_type", C.c_int32), ("flags", C.c_int32)]
+                ("hydro_type", C.c_int32), ("flags", C.c_int32), ("eos", C.c_int32), ("reserved", C.c_int32),
+                ("soundSpeedConst", C.c_double), ("polytropic_const", C.c_double), ("polytropic_index", C.c_double)]
 
     @classmethod
     def defaults(cls, K, hydro_type=SPHX_F64, **kw):
         """Reference defaults: ng0 100, ngmax 150, Kcour 0.2, gamma 5/3, muiConst 10,
-        alphamin 0.05, alphamax 1, decay 0.2, Atmin 0.1, Atmax 0.2, ramp 1/(Atmax-Atmin)."""
+        alphamin 0.05, alphamax 1, decay 0.2, Atmin 0.1, Atmax 0.2, ramp 1/(Atmax-Atmin); ideal-gas EOS,
+        soundSpeedConst 1, polytropic_const 1, polytropic_index 5/3 (particles_data.hpp:91-146)."""
         p = cls(K=K, Kcour=0.2, gamma=5.0 / 3.0, muiConst=10.0, minDt=1e-12, alphamin=0.05, alphamax=1.0,
                 decay_constant=0.2, Atmin=0.1, Atmax=0.2, ramp=1.0 / (0.2 - 0.1), ng0=100, ngmax=150,
-                hydro_type=hydro_type, flags=0)
+                hydro_type=hydro_type, flags=0, eos=SPHX_EOS_IDEAL_GAS, soundSpeedConst=1.0, polytropic_const=1.0,
+                polytropic_index=5.0 / 3.0)
         for k, v in kw.items():
             setattr(p, k, v)
         return p
 
 
+SPHX_EOS_IDEAL_GAS, SPHX_EOS_ISOTHERMAL, SPHX_EOS_POLYTROPIC = 0, 1, 2
 SPHX_FLAG_STD_ALPHA = 1
 SPHX_FLAG_CHAIN = 2
 

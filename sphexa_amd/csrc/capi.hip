@@ -407,6 +407,14 @@ int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t
     return SPHX_OK;
 }
 
+int sphx_search_tests(const sphx_workspace_t* ws, uint64_t* tests)
+{
+    if (ws == nullptr || tests == nullptr) { return SPHX_ERR_ARG; }
+    if (!ws->nbValid) { return SPHX_ERR_STATE; }
+    *tests = ws->statTests;
+    return SPHX_OK;
+}
+
 #define SPHX_PAIR_ENTRY(name, fn)                                                                                      \
     int name(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_params_t* prm,            \
              uint32_t first, uint32_t last)                                                                            \

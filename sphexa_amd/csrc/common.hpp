@@ -179,8 +179,10 @@ struct sphx_workspace
     uint32_t               nbFirst = 0, nbLast = 0, nbNgmax = 0;
     const uint32_t*        nbNc = nullptr; // nc array the lists were built with
     bool                   nbValid = false;
+    bool                   nbWide  = false; // some target's 2h reaches half a box length (FP32 loops: exact separations)
     uint64_t               statSumPairs = 0;
     uint32_t               statMaxCount = 0;
+    uint64_t               statTests    = 0; // candidate distance tests of the last search
 
     // scratch
     sphx::DevBuf<char>     scratch;      // sort / scan temporaries
@@ -382,7 +384,8 @@ struct DevBox
     int    anyPbc;
     // 32-bit fixed-point coordinates of the FP32 pair loops: u = rint((x - lo) 2^32 / len) mod 2^32
     double fixScale[3]; // 2^32 / len
-    float  fixStep[3];  // len / 2^32
+    double fixStepD[3]; // len / 2^32
+    float  fixStep[3];  // len / 2^32, rounded to float
 };
 
 inline DevBox makeDevBox(const sphx_box_t* b)
@@ -398,7 +401,8 @@ inline DevBox makeDevBox(const sphx_box_t* b)
         d.pbc[k]  = b->boundary[k] == 1;
         d.anyPbc |= d.pbc[k];
         d.fixScale[k] = 4294967296.0 / d.len[k];
-        d.fixStep[k]  = float(d.len[k] / 4294967296.0);
+        d.fixStepD[k] = d.len[k] / 4294967296.0;
+        d.fixStep[k]  = float(d.fixStepD[k]);
     }
     return d;
 }

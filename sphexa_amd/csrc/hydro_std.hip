@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kPackThreads) packMomStd(PairArgs a, MomStdFie
 /* xmassJLoop (sph/include/sph/hydro_ve/xmass_kern.hpp:51-79):
  *   xm_i = m_i / (K / h_i^3 * (m_i + sum_j m_j W(r_ij / h_i)))
  * toRho: computeDensity (hydro_std/density.hpp:42-53) stores rho = m / xm instead. */
-template<class T, bool toRho>
+template<class T, bool toRho, bool Wide>
 __global__ void SPHX_LIGHT_LOOP(T)
     xmassKernel(PairArgs a, const RecMass<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
                 T* __restrict__ out, RecGradh<T>* __restrict__ next, RecVol<T>* __restrict__ nextVol)
@@ -91,7 +91,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T                h3Inv = hInv * hInv * hInv;
     double           r2h   = 2.0 * double(hi);
     Acc<T>           rho0(ri.m);
-    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                U = kBatchLight<T>;
         NeighborBatch<U, RecMass<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -128,8 +128,40 @@ __global__ void __launch_bounds__(256) eosStdKernel(const double* __restrict__ u
     c[i]       = T(sqrt(gamma * tmp));
 }
 
-/* IADJLoopSTD (hydro_std/iad_kern.hpp:11-76): tau_ab = sum_j m_j/rho_j W_ij r_a r_b, inverted */
+/* isothermalEOS (sph/include/sph/eos.hpp:59-64) as the reference's GPU kernel applies it
+ * (hydro_std/eos_gpu.cu:73-82): p = rho c0 c0 (in FP64, once rounded to the HydroType in the FP32
+ * form), c = c0, temp = 0 */
 template<class T>
+__global__ void __launch_bounds__(256) eosStdIsothermalKernel(const T* __restrict__ rho, T* __restrict__ p,
+                                                              T* __restrict__ c, double* __restrict__ temp, T c0,
+                                                              uint32_t first, uint32_t last)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) { return; }
+    const double cc = double(c0);
+    p[i]            = T(double(rho[i]) * cc * cc);
+    c[i]            = c0;
+    if (temp) { temp[i] = 0.0; }
+}
+
+/* polytropicEOS (sph/include/sph/eos.hpp:77-86) per particle (hydro_std/eos.hpp:99-114):
+ * p = K rho^n, c = sqrt(n p / rho) in FP64, temp = 0 */
+template<class T>
+__global__ void __launch_bounds__(256) eosStdPolytropicKernel(const T* __restrict__ rho, T* __restrict__ p,
+                                                              T* __restrict__ c, double* __restrict__ temp, double K,
+                                                              double n, uint32_t first, uint32_t last)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) { return; }
+    const double r  = double(rho[i]);
+    const double pi = K * pow(r, n);
+    p[i]            = T(pi);
+    c[i]            = T(sqrt(n * pi / r));
+    if (temp) { temp[i] = 0.0; }
+}
+
+/* IADJLoopSTD (hydro_std/iad_kern.hpp:11-76): tau_ab = sum_j m_j/rho_j W_ij r_a r_b, inverted */
+template<class T, bool Wide>
 __global__ void SPHX_LIGHT_LOOP(T)
     iadStdKernel(PairArgs a, const RecVol<T>* __restrict__ rec, const T* __restrict__ h, SincKernel<T> kern,
                  T* c11, T* c12, T* c13, T* c22, T* c23, T* c33, MomStdFields nf, RecMomStd<T>* __restrict__ next)
@@ -142,7 +174,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T               hiInv = T(1) / hi;
     double          r2h   = 2.0 * double(hi);
     T               t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
-    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
         constexpr int               U = kBatchLight<T>;
         NeighborBatch<U, RecVol<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -184,7 +216,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
 
 /* momentumAndEnergyJLoop, std (hydro_std/momentum_energy_kern.hpp:11-134) + tsKCourant.
  * PerAlpha: AV coefficients alpha_i, alpha_j from f.alpha instead of the constant 1. */
-template<class T, bool PerAlpha>
+template<class T, bool PerAlpha, bool Wide>
 __global__ void SPHX_MOM_LOOP(T)
     momentumStdKernel(PairArgs a, MomStdFields f, const RecMomStd<T>* __restrict__ rec, SincKernel<T> kern,
                       float Kcour, unsigned long long* minDt)
@@ -206,7 +238,7 @@ __global__ void SPHX_MOM_LOOP(T)
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy;
-        withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+        withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
             constexpr int                  U = kBatchMom<T>;
             NeighborBatch<U, RecMomStd<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)
@@ -287,12 +319,15 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
     RecVol<T>*   nextVol = toRho && chainable(a, p) ? recordBuffer<RecVol<T>>(ws, a.numParticles, 1) : nullptr;
     {
         KernelScope timer(ws, toRho ? "density_std" : "xmass");
-        if (toRho)
-            xmassKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
-                                                                                sincKernel<T>(ws), (T*)out, nullptr, nextVol);
-        else
-            xmassKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, rec, (const T*)f->h,
-                                                                                 sincKernel<T>(ws), (T*)out, next, nullptr);
+        launchWide<T>(ws->nbWide, [&](auto wide) {
+            constexpr bool W = decltype(wide)::value;
+            if (toRho)
+                xmassKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                    a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)out, nullptr, nextVol);
+            else
+                xmassKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                    a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)out, next, nullptr);
+        });
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainGradh, 1, a, f, p); }
@@ -320,9 +355,11 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f,
                              : nullptr;
     {
         KernelScope timer(ws, "iad_std");
-        iadStdKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22, (T*)f->c23,
-            (T*)f->c33, nf, next);
+        launchWide<T>(ws->nbWide, [&](auto wide) {
+            iadStdKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
+                (T*)f->c23, (T*)f->c33, nf, next);
+        });
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainMomStd, 1 - in, a, f, p); }
@@ -342,11 +379,15 @@ void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields
         SPHX_CHECK_LAUNCH();
     }
     KernelScope timer(ws, "momentum_std");
-    if (perAlpha)
-        momentumStdKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws), Kcour, dt);
-    else
-        momentumStdKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws), Kcour,
-                                                                                    dt);
+    launchWide<T>(ws->nbWide, [&](auto wide) {
+        constexpr bool W = decltype(wide)::value;
+        if (perAlpha)
+            momentumStdKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws),
+                                                                                        Kcour, dt);
+        else
+            momentumStdKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec,
+                                                                                         sincKernel<T>(ws), Kcour, dt);
+    });
     SPHX_CHECK_LAUNCH();
 }
 
@@ -370,10 +411,34 @@ void xmassVe(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, cons
     else launchDensity<double>(ws, a, f, p, false, f->xm);
 }
 
+template<class T>
+void launchEosStdVariant(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                         uint32_t last)
+{
+    const unsigned blocks = ceilDiv(last - first, 256);
+    if (p->eos == SPHX_EOS_ISOTHERMAL)
+        eosStdIsothermalKernel<T><<<blocks, 256, 0, ws->stream>>>((const T*)f->rho, (T*)f->p, (T*)f->c, f->temp,
+                                                                  T(p->soundSpeedConst), first, last);
+    else
+        eosStdPolytropicKernel<T><<<blocks, 256, 0, ws->stream>>>((const T*)f->rho, (T*)f->p, (T*)f->c, f->temp,
+                                                                  p->polytropic_const, p->polytropic_index, first,
+                                                                  last);
+    SPHX_CHECK_LAUNCH();
+}
+
 void eosStd(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
 {
-    SPHX_REQUIRE(f->rho && f->p && f->c && (f->u || f->temp), SPHX_ERR_ARG, "EOS needs rho, p, c and u or temp");
+    SPHX_REQUIRE(p->eos >= SPHX_EOS_IDEAL_GAS && p->eos <= SPHX_EOS_POLYTROPIC, SPHX_ERR_ARG, "unknown EOS choice");
     SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
+    if (p->eos != SPHX_EOS_IDEAL_GAS)
+    {
+        SPHX_REQUIRE(f->rho && f->p && f->c, SPHX_ERR_ARG, "EOS needs rho, p and c");
+        if (last == first) { return; }
+        if (p->hydro_type == SPHX_F32) launchEosStdVariant<float>(ws, f, p, first, last);
+        else launchEosStdVariant<double>(ws, f, p, first, last);
+        return;
+    }
+    SPHX_REQUIRE(f->rho && f->p && f->c && (f->u || f->temp), SPHX_ERR_ARG, "EOS needs rho, p, c and u or temp");
     if (last == first) { return; }
     // idealGasCv (sph/include/sph/eos.hpp:19-23), R = 8.317e7
     double cv = 8.317e7 / p->muiConst / (p->gamma - 1.0);

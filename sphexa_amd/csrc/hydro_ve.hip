@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(kPackThreads) packGradh(PairArgs a, const T* _
 }
 
 /* veDefGradhJLoop (hydro_ve/ve_def_gradh_kern.hpp:44-90): kx and the grad-h term */
-template<class T>
+template<class T, bool Wide>
 __global__ void SPHX_LIGHT_LOOP(T)
     veDefGradhKernel(PairArgs a, const RecGradh<T>* __restrict__ rec, const T* __restrict__ h,
                      SincKernel<T> kern, T* __restrict__ kx, T* __restrict__ gradh, RecIadVe<T>* __restrict__ next,
@@ -38,7 +38,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     T                 hInv = T(1) / hi;
     double            r2h  = 2.0 * double(hi);
     Acc<T>            kxs(xmi), wom(-T(3) * xmi), wr0(-T(3) * mi);
-    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecGradh<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -97,6 +97,51 @@ __global__ void __launch_bounds__(256) eosVeKernel(const double* __restrict__ u,
     if (pOut) { pOut[i] = T(pi); }
 }
 
+/* VE isothermal EOS (hydro_ve/eos.hpp:97-123, hydro_ve/eos_gpu.cu:89-103): rho = kx m / xm in the
+ * HydroType, p = rho c0^2, prho = p / (kx m^2 gradh) (in FP64, rounded once in the FP32 form), c = c0,
+ * rho and p stored when given, temp = 0 */
+template<class T>
+__global__ void __launch_bounds__(256) eosVeIsothermalKernel(const T* __restrict__ m, const T* __restrict__ kx,
+                                                             const T* __restrict__ xm, const T* __restrict__ gradh,
+                                                             T* __restrict__ prho, T* __restrict__ c, T* rhoOut,
+                                                             T* pOut, double* temp, T c0, uint32_t first,
+                                                             uint32_t last)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) { return; }
+    const T      rho = kx[i] * m[i] / xm[i];
+    const double cc  = double(c0);
+    const double pi  = double(rho) * cc * cc;
+    prho[i]          = T(pi / double(kx[i] * m[i] * m[i] * gradh[i]));
+    c[i]             = c0;
+    if (rhoOut) { rhoOut[i] = rho; }
+    if (pOut) { pOut[i] = T(pi); }
+    if (temp) { temp[i] = 0.0; }
+}
+
+/* VE polytropic EOS (computePolytropicEOS_Impl, hydro_ve/eos.hpp:125-151): rho = kx m / xm in the
+ * HydroType, p = K rho^n and c = sqrt(n p / rho) in FP64 (polytropicEOS, sph/include/sph/eos.hpp:77-86),
+ * prho = p / (kx m^2 gradh), rho and p stored when given, temp = 0. c is stored as the reference's CPU
+ * path does; its GPU kernel leaves c unchanged (hydro_ve/eos_gpu.cu:134). */
+template<class T>
+__global__ void __launch_bounds__(256) eosVePolytropicKernel(const T* __restrict__ m, const T* __restrict__ kx,
+                                                             const T* __restrict__ xm, const T* __restrict__ gradh,
+                                                             T* __restrict__ prho, T* __restrict__ c, T* rhoOut,
+                                                             T* pOut, double* temp, double K, double n,
+                                                             uint32_t first, uint32_t last)
+{
+    uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= last) { return; }
+    const T      rho = kx[i] * m[i] / xm[i];
+    const double r   = double(rho);
+    const double pi  = K * pow(r, n);
+    prho[i]          = T(pi / double(kx[i] * m[i] * m[i] * gradh[i]));
+    c[i]             = T(sqrt(n * pi / r));
+    if (rhoOut) { rhoOut[i] = rho; }
+    if (pOut) { pOut[i] = T(pi); }
+    if (temp) { temp[i] = 0.0; }
+}
+
 /* Record builders shared by the pack kernels and the epilogues that write the next pass's records
  * (SPHX_FLAG_CHAIN), so that both give the same bits. */
 template<class T>
@@ -152,7 +197,7 @@ __global__ void __launch_bounds__(kPackThreads) packIadVe(PairArgs a, IadVeField
  *   dV_a[b] = sum_j (v_ja - v_ia) xm_j (-(C_i r_ij)_b W_ij) = -sum_c C_i[b][c] M[a][c],
  *   M[a][c] = sum_j (v_ja - v_ia) xm_j W_ij r_ij,c,
  * so the 9 moments M are accumulated alongside tau and combined with C_i after the inversion. */
-template<class T>
+template<class T, bool Wide>
 __global__ void SPHX_LIGHT_LOOP(T)
     iadDivvCurlvKernel(PairArgs a, IadVeFields f, const RecIadVe<T>* __restrict__ rec, SincKernel<T> kern,
                        RecAv<T>* __restrict__ next, const T* __restrict__ cNext)
@@ -166,7 +211,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     double            r2h   = 2.0 * double(hi);
     T                 t11 = 0, t12 = 0, t13 = 0, t22 = 0, t23 = 0, t33 = 0;
     T                 M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
         constexpr int                 U = kBatchLight<T>;
         NeighborBatch<U, RecIadVe<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -257,7 +302,7 @@ __global__ void __launch_bounds__(kPackThreads) packAv(PairArgs a, AvFields f, R
 
 /* AVswitchesJLoop (hydro_ve/av_switches_kern.hpp:44-137). grad(div v) is linear in C_i:
  * gd_b = -K/h_i^3 sum_c C_i[b][c] sum_j f_j W_ij r_ij,c with f_j = xm_j/kx_j (divv_i - divv_j). */
-template<class T>
+template<class T, bool Wide>
 __global__ void SPHX_LIGHT_LOOP(T)
     avSwitchesKernel(PairArgs a, AvFields f, const RecAv<T>* __restrict__ rec, SincKernel<T> kern, double dt,
                      T alphamin, T alphamax, T decay, RecMomVe<T>* __restrict__ next)
@@ -271,7 +316,7 @@ __global__ void SPHX_LIGHT_LOOP(T)
     double         r2h   = 2.0 * double(hi);
     T              vsig  = T(1.e-40) * ri.c;
     T              S[3]  = {0, 0, 0};
-    withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+    withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
         constexpr int              U = kBatchLight<T>;
         NeighborBatch<U, RecAv<T>> nb(t, rec);
         for (unsigned k = 0; k < t.n; k += U)
@@ -391,7 +436,7 @@ __device__ __forceinline__ T ramExp2(T v)
 }
 
 /* momentumAndEnergyJLoop<avClean> (hydro_ve/momentum_energy_kern.hpp:66-222) + tsKCourant */
-template<class T, bool avClean>
+template<class T, bool avClean, bool Wide>
 __global__ void SPHX_MOM_LOOP(T)
     momentumVeKernel(PairArgs a, MomVeFields f, const RecMomVe<T>* __restrict__ rec, SincKernel<T> kern, T Atmin,
                      T Atmax, T ramp, float Kcour, unsigned long long* minDt)
@@ -417,7 +462,7 @@ __global__ void SPHX_MOM_LOOP(T)
         T      maxvs = 0;
         T      momx = 0, momy = 0, momz = 0;
         Acc<T>       energy, avEnergy;
-        withFold(recordNeedsFold<T>(a.box, ri, r2h), [&](auto fold) {
+        withFold(recordNeedsFold<T, Wide>(a.box, ri, r2h), [&](auto fold) {
             constexpr int                 U = kBatchMom<T>;
             NeighborBatch<U, RecMomVe<T>> nb(t, rec);
             for (unsigned k = 0; k < t.n; k += U)
@@ -551,9 +596,11 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, 
                             : nullptr;
     {
         KernelScope timer(ws, "ve_def_gradh");
-        veDefGradhKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->kx, (T*)f->gradh, next, (const T*)f->vx,
-            (const T*)f->vy, (const T*)f->vz);
+        launchWide<T>(ws->nbWide, [&](auto wide) {
+            veDefGradhKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->kx, (T*)f->gradh, next, (const T*)f->vx,
+                (const T*)f->vy, (const T*)f->vz);
+        });
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainIadVe, out, a, f, p); }
@@ -576,8 +623,10 @@ void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf, c
     RecAv<T>* next = chainable(a, p) && f->c ? recordBuffer<RecAv<T>>(ws, a.numParticles, 1 - in) : nullptr;
     {
         KernelScope timer(ws, "iad_divv_curlv");
-        iadDivvCurlvKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, vf, rec, sincKernel<T>(ws), next,
-                                                                            (const T*)f->c);
+        launchWide<T>(ws->nbWide, [&](auto wide) {
+            iadDivvCurlvKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, vf, rec, sincKernel<T>(ws), next, (const T*)f->c);
+        });
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainAv, 1 - in, a, f, p); }
@@ -601,8 +650,10 @@ void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const s
         chainable(a, p) && f->m && f->prho ? recordBuffer<RecMomVe<T>>(ws, a.numParticles, 1 - in) : nullptr;
     {
         KernelScope timer(ws, "av_switches");
-        avSwitchesKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, af, rec, sincKernel<T>(ws), p->minDt, T(p->alphamin), T(p->alphamax), T(p->decay_constant), next);
+        launchWide<T>(ws->nbWide, [&](auto wide) {
+            avSwitchesKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, af, rec, sincKernel<T>(ws), p->minDt, T(p->alphamin), T(p->alphamax), T(p->decay_constant), next);
+        });
         SPHX_CHECK_LAUNCH();
     }
     if (next) { leaveChain(ws, kChainMomVe, 1 - in, a, f, p); }
@@ -622,12 +673,15 @@ void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, c
         SPHX_CHECK_LAUNCH();
     }
     KernelScope timer(ws, "momentum_ve");
-    if (avClean)
-        momentumVeKernel<T, true><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
-    else
-        momentumVeKernel<T, false><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
-            a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+    launchWide<T>(ws->nbWide, [&](auto wide) {
+        constexpr bool W = decltype(wide)::value;
+        if (avClean)
+            momentumVeKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+        else
+            momentumVeKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
+    });
     SPHX_CHECK_LAUNCH();
 }
 
@@ -642,11 +696,37 @@ void veDefGradh(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, c
     else launchGradh<double>(ws, a, f, p);
 }
 
+template<class T>
+void launchEosVeVariant(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
+                        uint32_t last)
+{
+    const unsigned blocks = ceilDiv(last - first, 256);
+    if (p->eos == SPHX_EOS_ISOTHERMAL)
+        eosVeIsothermalKernel<T><<<blocks, 256, 0, ws->stream>>>(
+            (const T*)f->m, (const T*)f->kx, (const T*)f->xm, (const T*)f->gradh, (T*)f->prho, (T*)f->c, (T*)f->rho,
+            (T*)f->p, f->temp, T(p->soundSpeedConst), first, last);
+    else
+        eosVePolytropicKernel<T><<<blocks, 256, 0, ws->stream>>>(
+            (const T*)f->m, (const T*)f->kx, (const T*)f->xm, (const T*)f->gradh, (T*)f->prho, (T*)f->c, (T*)f->rho,
+            (T*)f->p, f->temp, p->polytropic_const, p->polytropic_index, first, last);
+    SPHX_CHECK_LAUNCH();
+}
+
 void eosVe(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
 {
+    SPHX_REQUIRE(p->eos >= SPHX_EOS_IDEAL_GAS && p->eos <= SPHX_EOS_POLYTROPIC, SPHX_ERR_ARG, "unknown EOS choice");
+    SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
+    if (p->eos != SPHX_EOS_IDEAL_GAS)
+    {
+        SPHX_REQUIRE(f->m && f->kx && f->xm && f->gradh && f->prho && f->c, SPHX_ERR_ARG,
+                     "VE EOS needs m, kx, xm, gradh, prho and c");
+        if (last == first) { return; }
+        if (p->hydro_type == SPHX_F32) launchEosVeVariant<float>(ws, f, p, first, last);
+        else launchEosVeVariant<double>(ws, f, p, first, last);
+        return;
+    }
     SPHX_REQUIRE(f->m && f->kx && f->xm && f->gradh && f->prho && f->c && (f->temp || f->u), SPHX_ERR_ARG,
                  "VE EOS needs m, kx, xm, gradh, prho, c and temp or u");
-    SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
     if (last == first) { return; }
     double cv = 8.317e7 / p->muiConst / (p->gamma - 1.0);
     if (p->hydro_type == SPHX_F32)

@@ -54,7 +54,9 @@ struct SearchArgs
     int                  iterateH;
     uint32_t*            nbrs;
     uint32_t*            nc;
-    unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles
+    unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles, [3] wide support,
+                                // [4] candidate distance tests (swept candidates x 64 lanes)
+    double               minLen; // shortest box length
     // fused XMass: masses, kernel table (TabPair of h's type), normalisation, output (xm or rho)
     const void*          m;
     const void*          wh;    // FP64 form: the W table
@@ -100,6 +102,7 @@ struct ListWriter
 {
     uint32_t* base; // the lane's slot 0
     uint32_t  q0 = 0, q1 = 0, q2 = 0, q3 = 0; // the last four entries, oldest first
+    uint32_t  swept = 0; // candidates swept by the wave (each tested by all 64 lanes)
 
     //! entry k = j: shifted in behind the previous three, stored when it completes a quad
     __device__ __forceinline__ void put(unsigned k, uint32_t j)
@@ -297,6 +300,7 @@ __device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t
                                             int code, unsigned& count, ListWriter& out, Sink& sink)
 {
     __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
+    out.swept += n;
     const double* tx = lds.tx[buf];
     const double* ty = lds.ty[buf];
     const double* tz = lds.tz[buf];
@@ -693,11 +697,12 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
 //! search of the active lanes' targets into their lists (slot 0 at myList); returns the counts
 template<class Sink>
 __device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList,
-                                Sink& sink)
+                                Sink& sink, uint32_t& swept)
 {
     ListWriter out{myList};
     unsigned   count = searchGroupInto(a, t, radius, lds, out, sink);
     if (t.active) { out.finish(min(count, a.ngmax)); }
+    swept += out.swept;
     return count;
 }
 
@@ -773,7 +778,8 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
     resetSink();
 
     bool     valid = t.active;
-    unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList, sink);
+    uint32_t swept = 0;
+    unsigned count = searchGroup(a, t, double(2) * double(hi), lds, myList, sink, swept);
     unsigned ncSph = 1 + count;
 
     int      updates = 0;
@@ -792,7 +798,7 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
                 resetSink();
             }
             t.active       = repeat;
-            unsigned again = searchGroup(a, t, double(2) * double(hi), lds, myList, sink);
+            unsigned again = searchGroup(a, t, double(2) * double(hi), lds, myList, sink, swept);
             if (repeat) { ncSph = 1 + again; }
         }
     }
@@ -817,23 +823,29 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
     unsigned sum    = waveSum(stored);
     unsigned mx     = waveMax(valid ? ncSph : 0u);
     unsigned fails  = __popcll(ballot(valid && updates >= 10));
+    // support 2h reaching half a box length: the FP32 pair loops then take exact separations
+    unsigned wide = __popcll(ballot(valid && 4.0 * double(hi) * (1.0 + 1e-6) >= a.minLen));
     if (lane == 0)
     {
         atomicAdd(&a.stats[0], (unsigned long long)sum);
         atomicMax(&a.stats[1], (unsigned long long)mx);
         if (fails) { atomicAdd(&a.stats[2], (unsigned long long)fails); }
+        if (wide) { atomicAdd(&a.stats[3], (unsigned long long)wide); }
+        atomicAdd(&a.stats[4], (unsigned long long)swept * kWave);
     }
 }
 
+//! row i - first of `out` = stored list of target i; groups count from the searched range's first target
 __global__ void copyListsKernel(const uint32_t* __restrict__ nbrs, const uint32_t* __restrict__ nc,
-                                uint32_t* __restrict__ out, uint32_t first, uint32_t last, uint32_t ngmax)
+                                uint32_t* __restrict__ out, uint32_t searchFirst, uint32_t first, uint32_t last,
+                                uint32_t ngmax)
 {
     uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= last) { return; }
-    uint32_t r = i - first, g = r / kGroup, lane = r % kGroup;
+    uint32_t r = i - searchFirst, g = r / kGroup, lane = r % kGroup;
     uint32_t n = min(nc[i] - 1, ngmax);
     for (uint32_t k = 0; k < n; ++k)
-        out[size_t(r) * ngmax + k] = nbrs[listOffset(g, paddedNgmax(ngmax), k, lane)];
+        out[size_t(i - first) * ngmax + k] = nbrs[listOffset(g, paddedNgmax(ngmax), k, lane)];
 }
 
 } // namespace
@@ -852,7 +864,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     uint32_t    groups  = ceilDiv(numTgt, kGroup);
     ws->nbrs.reserve(std::max<size_t>(1, size_t(groups) * paddedNgmax(p->ngmax) * kGroup));
     ws->counters.reserve(8);
-    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 3);
+    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 5);
 
     double scale = 0;
     for (int d = 0; d < 3; ++d)
@@ -874,6 +886,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.nbrs     = ws->nbrs.get();
     a.nc       = f->nc;
     a.stats    = reinterpret_cast<unsigned long long*>(ws->counters.get());
+    a.minLen   = std::min(box.len[0], std::min(box.len[1], box.len[2]));
     a.m = nullptr, a.wh = nullptr, a.sincN = 0, a.sincInt = -1, a.K = 0, a.xmOut = nullptr, a.toRho = 0;
     const bool f32 = p->hydro_type == SPHX_F32;
     if (xmassOut != kSearchOnly)
@@ -907,7 +920,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
         }
         SPHX_CHECK_LAUNCH();
     }
-    unsigned long long stats[3];
+    unsigned long long stats[5];
     SPHX_HIP(hipMemcpyAsync(stats, ws->counters.get(), sizeof(stats), hipMemcpyDeviceToHost, ws->stream));
     SPHX_HIP(hipStreamSynchronize(ws->stream));
 
@@ -918,14 +931,16 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     ws->nbValid      = true;
     ws->statSumPairs = stats[0];
     ws->statMaxCount = uint32_t(stats[1]);
+    ws->nbWide       = stats[3] > 0;
+    ws->statTests    = stats[4];
     return stats[2] > 0 ? SPHX_ERR_NOT_CONVERGED : SPHX_OK;
 }
 
 void copyNeighbors(sphx_workspace* ws, uint32_t* out, uint32_t first, uint32_t last)
 {
     SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "no neighbour lists: call find_neighbors first");
-    SPHX_REQUIRE(first == ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
-                 "copy range must start at the first searched target and stay inside the searched range");
+    SPHX_REQUIRE(ws->nbFirst <= first && first <= last && last <= ws->nbLast, SPHX_ERR_ARG,
+                 "copy range must lie inside the searched range");
     SPHX_REQUIRE(out != nullptr, SPHX_ERR_ARG, "null output");
     if (last == first) { return; }
     size_t             len = size_t(last - first) * ws->nbNgmax;
@@ -939,8 +954,8 @@ void copyNeighbors(sphx_workspace* ws, uint32_t* out, uint32_t first, uint32_t l
         staging.reserve(len);
         dst = staging.get();
     }
-    copyListsKernel<<<ceilDiv(last - first, 256), 256, 0, ws->stream>>>(ws->nbrs.get(), ws->nbNc, dst, first, last,
-                                                                          ws->nbNgmax);
+    copyListsKernel<<<ceilDiv(last - first, 256), 256, 0, ws->stream>>>(ws->nbrs.get(), ws->nbNc, dst, ws->nbFirst, first,
+                                                                          last, ws->nbNgmax);
     SPHX_CHECK_LAUNCH();
     if (!onDevice)
     {

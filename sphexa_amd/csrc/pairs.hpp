@@ -461,11 +461,18 @@ struct Sep
 
 /*! Position of a particle in the neighbour records: FP64 in the plain form; in the FP32 form
  * 32-bit fixed point over the box, u = rint((x - lo) 2^32 / L) mod 2^32 per dimension (L the box
- * length). There x_i - x_j is the exact wrapping difference u_i - u_j read as a signed 32-bit
- * integer, the minimum image in a periodic dimension, which is what the reference's fold gives
- * for a listed pair, and harmless in an open one (listed pairs are far closer than L/2). It is
- * then converted to FP32 and scaled by L / 2^32 (about one ulp in all), from coordinates quantized
- * to L / 2^32 (2.3e-10 in a unit box). A record position is 12 bytes instead of 24. */
+ * length). A record position is 12 bytes instead of 24; coordinates are quantized to L / 2^32
+ * (2.3e-10 in a unit box). The separation is converted to FP32 and scaled by L / 2^32 (about one
+ * ulp in all). Which integer difference stands for x_i - x_j:
+ *  - while every target of the wave has 2h < L/2 in every dimension, the wrapping difference
+ *    u_i - u_j read as a signed 32-bit integer. A listed pair is closer than 2h in each
+ *    dimension, so this is the raw difference in an open dimension, and in a periodic one the
+ *    minimum image, which equals the reference's fold with cut-off 2h (sfc/box.hpp:218-238) for
+ *    any pair closer than 2h < L/2;
+ *  - otherwise (2h >= L/2: a box a few smoothing lengths wide, as for a few hundred particles or
+ *    fewer), the exact difference u_i - u_j in 64 bits, the raw difference of coordinates that lie
+ *    in [lo, hi] (the box update of the domain sync keeps open dimensions there), folded in a
+ *    periodic dimension with cut-off 2h as the reference does. */
 template<class T>
 using Coord = std::conditional_t<isMixed<T>, uint32_t, double>;
 
@@ -481,11 +488,30 @@ __device__ __forceinline__ Coord<T> coordOf(const DevBox& b, int d, double x)
     else { return x; }
 }
 
+/*! exact fixed-point separation of one dimension for boxes a few smoothing lengths wide: the 64-bit
+ * difference, then the legacy fold with cut-off r in a periodic dimension (see Coord above) */
+__device__ __forceinline__ float wideFixedSeparation(const DevBox& b, int q, double r, uint32_t ui, uint32_t uj)
+{
+    double d = double(int64_t(ui) - int64_t(uj)); // exact: |d| < 2^33
+    if (b.pbc[q])
+    {
+        double dx = d * b.fixStepD[q];
+        d         = dx > r ? d - 4294967296.0 : (dx < -r ? d + 4294967296.0 : d);
+    }
+    return float(d) * b.fixStep[q];
+}
+
 template<bool Fold, class T, class Ri, class Rj>
 __device__ __forceinline__ Sep<T> pairSeparation(const DevBox& b, double r2h, const Ri& ri, const Rj& rj)
 {
     Sep<T> s;
-    if constexpr (isMixed<T>)
+    if constexpr (isMixed<T> && Fold)
+    {
+        s.x = wideFixedSeparation(b, 0, r2h, ri.x, rj.x);
+        s.y = wideFixedSeparation(b, 1, r2h, ri.y, rj.y);
+        s.z = wideFixedSeparation(b, 2, r2h, ri.z, rj.z);
+    }
+    else if constexpr (isMixed<T>)
     {
         s.x = float(int32_t(ri.x - rj.x)) * b.fixStep[0];
         s.y = float(int32_t(ri.y - rj.y)) * b.fixStep[1];
@@ -496,12 +522,31 @@ __device__ __forceinline__ Sep<T> pairSeparation(const DevBox& b, double r2h, co
     return s;
 }
 
-//! whether the wave of target record ri must fold its FP64 separations (the FP32 form never does)
-template<class T, class R>
+/*! whether the wave of target record ri must fold its FP64 separations; in the FP32 form, Wide: the
+ * last search met a target whose support 2h reaches half a box length, where the wrapping 32-bit
+ * difference stops standing for the reference's separation (see Coord above), so the loop takes the
+ * exact 64-bit difference. Wide is a kernel template argument, chosen on the host (launchWide), so
+ * that the common loop carries none of the registers of the exact path. */
+template<class T, bool Wide, class R>
 __device__ __forceinline__ bool recordNeedsFold(const DevBox& b, const R& ri, double r2h)
 {
-    if constexpr (isMixed<T>) { return false; }
+    if constexpr (isMixed<T>) { return Wide; }
     else { return waveNeedsFold(b, ri.x, ri.y, ri.z, r2h); }
+}
+
+//! host: launch(std::bool_constant<Wide>) with Wide = `wide` in the FP32 form, false in the FP64 form
+template<class T, class F>
+inline void launchWide(bool wide, F&& launch)
+{
+    if constexpr (isMixed<T>)
+    {
+        if (wide)
+        {
+            launch(std::true_type{});
+            return;
+        }
+    }
+    launch(std::false_type{});
 }
 
 //! run `loop` with the fold compiled in or out, chosen once per wave

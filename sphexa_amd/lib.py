@@ -39,6 +39,7 @@ _SIGS = {
     "sphx_find_neighbors_xmass": [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, C.c_int],
     "sphx_copy_neighbors": [_P, _P, _U32, _U32],
     "sphx_neighbor_stats": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
+    "sphx_search_tests": [_P, C.POINTER(C.c_uint64)],
     "sphx_density_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_eos_std": [_P, _FP, _PP, _U32, _U32],
     "sphx_iad_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
@@ -166,6 +167,12 @@ class Workspace:
         s, m = C.c_uint64(), C.c_uint32()
         self.call("sphx_neighbor_stats", C.byref(s), C.byref(m))
         return s.value, m.value
+
+    def search_tests(self):
+        """candidate distance tests of the last search (sphx_search_tests)"""
+        t = C.c_uint64()
+        self.call("sphx_search_tests", C.byref(t))
+        return t.value
 
     def pair(self, name, box, fields, params, first, last):
         self.call(name, C.byref(box), C.byref(fields), C.byref(params), first, last)

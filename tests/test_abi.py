@@ -37,7 +37,7 @@ def test_struct_layouts_match_header():
     assert tuple(names) == abi.ALL_FIELDS
     assert C.sizeof(abi.Fields) == 8 * len(abi.ALL_FIELDS)
     assert C.sizeof(abi.Box) == 6 * 8 + 3 * 4 + 4  # padded to 8
-    assert C.sizeof(abi.Params) == 11 * 8 + 4 * 4
+    assert C.sizeof(abi.Params) == 11 * 8 + 6 * 4 + 3 * 8
     assert C.sizeof(abi.TreeView) == 4 * 4 + 10 * 8
     body = text[text.index("typedef struct sphx_adjoint"):text.index("} sphx_adjoint_t;")]
     names = tuple(re.findall(r"\*?(\w+)[,;]", body.split("{", 1)[1]))

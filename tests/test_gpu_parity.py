@@ -241,6 +241,66 @@ def test_std_fp64_parity(ws, orc):
     assert abs(dt - o["minDt"]) <= 1e-12 * o["minDt"]
 
 
+def _small_lattice(side, seed):
+    """side^3 particles on a lattice jittered by +-0.2 spacings in [-0.5,0.5]^3, with a smooth
+    compressive, shearing velocity field (so du and the viscous terms act)"""
+    rng = np.random.default_rng(seed)
+    x, y, z = ic.jittered_lattice(side, -0.5, 1.0, rng, 0.2)
+    n = x.size
+    arr = dict(x=x, y=y, z=z, m=np.full(n, 1.0 / n), u=1.0 + 0.5 * rng.uniform(size=n), alpha=np.full(n, 0.5))
+    arr["vx"] = 0.2 * np.sin(4.0 * y)
+    arr["vy"] = 0.1 * np.cos(3.0 * z)
+    arr["vz"] = -0.3 * z
+    return arr
+
+
+@pytest.mark.parametrize("side,h", [(3, 0.5), (4, 0.5), (5, 0.5), (4, 0.22), (5, 0.22)])
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("form", ["std", "ve"])
+def test_mixed_small_boxes(ws, orc, side, h, periodic, form):
+    """The mixed form on 27 to 125 particles against the FP64 oracle, with a fixed h (no h
+    iteration): h = 0.5 puts 2h at the box length, so separations beyond half a box length occur
+    among listed pairs, in an open box (raw differences) and in a periodic one (the reference's fold
+    with cut-off 2h); h = 0.22 keeps 2h below half a box length. Bars: density <= 1e-5 pointwise,
+    accelerations and du <= 1e-4 of their largest magnitude, nc identical."""
+    from oracle import pipeline
+
+    arr = _small_lattice(side, 300 + side)
+    n = arr["x"].size
+    arr["h"] = np.full(n, h)
+    if form == "ve":
+        arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC if periodic else abi.OPEN)
+    params64 = abi.Params.defaults(ws.K, minDt=1e-3)
+    o = pipeline.run(orc, arr, box, params64, form=form, iterate_h=False)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32, minDt=1e-3)
+    names = STD_FIELDS if form == "std" else VE_FIELDS
+    ps = ParticleSet.from_host(arr, names, mixed=True)
+    f = ps.fields()
+    ws.domain_sync(box, ps.sync_fields(), params, n)
+    ws.find_neighbors(box, f, params, 0, n, False)
+    np.testing.assert_array_equal(ps.host("nc"), o["a"]["nc"])
+    if form == "std":
+        ws.pair("sphx_density_std", box, f, params, 0, n)
+        ws.eos("sphx_eos_std", f, params, 0, n)
+        ws.pair("sphx_iad_std", box, f, params, 0, n)
+        ws.momentum_energy_std(box, f, params, 0, n)
+        rho_g, rho_o = ps.host("rho"), o["a"]["rho"]
+    else:
+        ws.pair("sphx_xmass", box, f, params, 0, n)
+        ws.pair("sphx_ve_def_gradh", box, f, params, 0, n)
+        ws.eos("sphx_eos_ve", f, params, 0, n)
+        ws.pair("sphx_iad_divv_curlv", box, f, params, 0, n)
+        ws.pair("sphx_av_switches", box, f, params, 0, n)
+        ws.momentum_energy_ve(box, f, params, 0, n)
+        rho_g = ps.host("kx").astype(np.float64) * ps.host("m") / ps.host("xm")
+        rho_o = o["a"]["kx"] * o["a"]["m"] / o["a"]["xm"]
+        assert rel_to_max(ps.host("gradh"), o["a"]["gradh"]) <= 1e-5
+    assert max_rel(rho_g, rho_o) <= 1e-5
+    for k in ("ax", "ay", "az", "du"):
+        assert rel_to_max(ps.host(k), o["a"][k]) <= 1e-4, k
+
+
 def _ve_ic(side, seed):
     arr, spec = ic.turbulence_box(side, seed=seed)
     arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
@@ -513,3 +573,50 @@ def test_fused_search_xmass_matches_separate_passes(ws, mixed, form):
     assert np.array_equal(nc0, nc1) and np.array_equal(h0, h1) and np.array_equal(l0, l1)
     tol = 1e-6 if mixed else 1e-13
     assert max_rel(v1, v0) <= tol
+
+
+@pytest.mark.parametrize("form", ["std", "ve"])
+@pytest.mark.parametrize("eos", [abi.SPHX_EOS_IDEAL_GAS, abi.SPHX_EOS_ISOTHERMAL, abi.SPHX_EOS_POLYTROPIC])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_eos_choices_against_oracle(ws, orc, form, eos, mixed):
+    """sphx_eos_std / sphx_eos_ve for every eosChoice against the oracle's FP64 EOS on the same inputs:
+    FP64 within 4 ulp (the device pow may differ from the C library's by an ulp), the FP32 form within
+    its one final rounding (3e-7 relative); temp is zeroed by the isothermal and polytropic branches."""
+    import ctypes as C
+
+    import torch
+
+    n = 100_000
+    rng = np.random.default_rng(40 + 3 * eos)
+    host = {"rho": rng.uniform(0.2, 5.0, n), "temp": rng.uniform(1.0, 2.0, n)}
+    if form == "ve":
+        host.update(m=rng.uniform(0.5, 1.5, n) * 1e-4, kx=rng.uniform(0.5, 2.0, n), xm=rng.uniform(0.5, 2.0, n) * 1e-4,
+                    gradh=rng.uniform(0.3, 1.2, n))
+    hydro = np.float32 if mixed else np.float64
+    for k in ("m", "kx", "xm", "gradh", "rho"):  # the FP32 form starts from the same rounded inputs
+        if k in host:
+            host[k] = host[k].astype(hydro).astype(np.float64)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, eos=eos,
+                                 soundSpeedConst=0.7, polytropic_const=2.5, polytropic_index=1.4)
+    params64 = abi.Params.from_buffer_copy(params)
+    params64.hydro_type = abi.SPHX_F64
+    outs = ("p", "c") if form == "std" else ("prho", "c", "rho", "p")
+    o = {k: v.copy() for k, v in host.items()}
+    for k in outs:  # outputs start at zero (for VE that includes rho)
+        o[k] = np.zeros(n)
+    fo = abi.fields_from(o)
+    (orc.eos_std if form == "std" else orc.eos_ve)(C.byref(fo), C.byref(params64), 0, n)
+
+    dt = torch.float32 if mixed else torch.float64
+    t = {k: torch.from_numpy(v).to(torch.float64 if k == "temp" else dt).cuda() for k, v in host.items()}
+    for k in outs:
+        t[k] = torch.zeros(n, dtype=dt, device="cuda")
+    ws.eos("sphx_eos_std" if form == "std" else "sphx_eos_ve", abi.fields_from(t), params, 0, n)
+    torch.cuda.synchronize()
+    tol = 3e-7 if mixed else 4 * np.finfo(np.float64).eps
+    for k in outs:
+        g = t[k].double().cpu().numpy()
+        err = float(np.max(np.abs(g - o[k]) / np.abs(o[k])))
+        assert err <= tol, (k, err)
+    zeroed = eos != abi.SPHX_EOS_IDEAL_GAS
+    assert bool((t["temp"] == 0).all()) == zeroed

@@ -428,6 +428,68 @@ def test_oracle_equals_reference_pass(orc, ref, form, hydro):
     assert o["minDt"] == r["minDt"]
 
 
+def _eos_inputs(form, hydro, n=4000, seed=21):
+    rng = np.random.default_rng(seed)
+    a = {"rho": rng.uniform(0.2, 5.0, n).astype(hydro), "p": np.zeros(n, hydro), "c": np.full(n, -1.0, hydro),
+         "temp": rng.uniform(1.0, 2.0, n)}
+    if form == "ve":
+        a.update(m=rng.uniform(0.5, 1.5, n).astype(hydro) * hydro(1e-4), kx=rng.uniform(0.5, 2.0, n).astype(hydro),
+                 xm=rng.uniform(0.5, 2.0, n).astype(hydro) * hydro(1e-4), gradh=rng.uniform(0.3, 1.2, n).astype(hydro),
+                 prho=np.zeros(n, hydro))
+        a["rho"] = np.zeros(n, hydro)
+    return a
+
+
+def _run_eos(lib, form, hydro, eos):
+    a = _eos_inputs(form, hydro)
+    ht = abi.SPHX_F32 if hydro == np.float32 else abi.SPHX_F64
+    p = abi.Params.defaults(1.0, hydro_type=ht, eos=eos, soundSpeedConst=0.7, polytropic_const=2.5,
+                            polytropic_index=1.4)
+    f = abi.fields_from(a)
+    (lib.eos_std if form == "std" else lib.eos_ve)(C.byref(f), C.byref(p), 0, a["temp"].size)
+    return a, p
+
+
+@pytest.mark.parametrize("form", ["std", "ve"])
+@pytest.mark.parametrize("eos", [abi.SPHX_EOS_ISOTHERMAL, abi.SPHX_EOS_POLYTROPIC])
+def test_eos_variants_closed_form(orc, form, eos):
+    """The isothermal (p = rho c0^2, c = c0) and polytropic (p = K rho^n, c = sqrt(n p / rho)) EOS of
+    sph/include/sph/eos.hpp:58-86 in FP64, with rho = kx m / xm and prho = p / (kx m^2 gradh) for VE;
+    temp is zeroed."""
+    a, p = _run_eos(orc, form, np.float64, eos)
+    rho = a["kx"] * a["m"] / a["xm"] if form == "ve" else a["rho"]
+    if eos == abi.SPHX_EOS_ISOTHERMAL:
+        pe, ce = rho * 0.7 * 0.7, np.full(rho.size, 0.7)
+    else:
+        pe = 2.5 * rho ** 1.4
+        ce = np.sqrt(1.4 * pe / rho)
+    tol = 4e-16 if eos == abi.SPHX_EOS_ISOTHERMAL else 1e-15  # pow, then a division: a few ulp
+    np.testing.assert_allclose(a["c"], ce, rtol=tol, atol=0)
+    if form == "ve":
+        np.testing.assert_allclose(a["prho"], pe / (a["kx"] * a["m"] * a["m"] * a["gradh"]), rtol=tol, atol=0)
+        np.testing.assert_allclose(a["rho"], rho, rtol=0, atol=0)
+    else:
+        np.testing.assert_allclose(a["p"], pe, rtol=tol, atol=0)
+    assert not a["temp"].any()
+
+
+@pytest.mark.parametrize("form", ["std", "ve"])
+@pytest.mark.parametrize("eos", [abi.SPHX_EOS_ISOTHERMAL, abi.SPHX_EOS_POLYTROPIC])
+@pytest.mark.parametrize("hydro", [np.float64, np.float32])
+def test_eos_variants_equal_reference(orc, ref, form, eos, hydro):
+    """Oracle == the reference's own CPU loops (computeEOS_HydroStd / computeEOS branches), bit for bit.
+    The reference's std isothermal CPU loop leaves c unchanged (hydro_std/eos.hpp:82-97); the oracle, like
+    the reference's GPU kernel (hydro_std/eos_gpu.cu:73-82), stores c = c0 there."""
+    o, p = _run_eos(orc, form, hydro, eos)
+    r, _ = _run_eos(ref, form, hydro, eos)
+    names = ("p", "c", "temp") if form == "std" else ("prho", "c", "rho", "temp")
+    for k in names:
+        if form == "std" and eos == abi.SPHX_EOS_ISOTHERMAL and k == "c":
+            assert np.all(r["c"] == hydro(-1.0)) and np.all(o["c"] == hydro(p.soundSpeedConst))
+            continue
+        assert np.array_equal(o[k], r[k]), k
+
+
 def test_oracle_tree_equals_reference(orc, ref):
     from oracle import oracle as O
 
