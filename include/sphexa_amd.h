@@ -210,6 +210,20 @@ typedef struct sphx_tree_view
     const double*   sizes;          /* [3*numNodes] half sizes */
 } sphx_tree_view_t;
 
+/* cstone::GroupView (domain/include/cstone/traversal/groups.hpp:28-34): target groups of at most 64
+ * particles, [groupStart[g], groupEnd[g]) for g < numGroups, inside [firstBody, lastBody); device
+ * pointers. groupDt, optional (NULL), is the per-group time step the momentum loops lower to the
+ * group's smallest Courant step (groupDt[g] = min(groupDt[g], ...), momentum_energy_gpu.cu:96-104). */
+typedef struct sphx_groups
+{
+    uint32_t        firstBody, lastBody;
+    uint32_t        numGroups;
+    uint32_t        reserved;
+    const uint32_t* groupStart;
+    const uint32_t* groupEnd;
+    float*          groupDt;
+} sphx_groups_t;
+
 typedef struct sphx_workspace sphx_workspace_t;
 
 /* ---- workspace ------------------------------------------------------------------------------- */
@@ -290,6 +304,29 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
  * Domain::sync reorders together with the others (domain/include/cstone/domain/domain.hpp:182-234). */
 int sphx_domain_reorder(sphx_workspace_t* ws, void* const* fields, const int* elemBytes, size_t count,
                         size_t n);
+
+/* ---- target groups ------------------------------------------------------------------------- */
+
+/* Spatial target groups of [first, last), as the reference's GPU path builds its GroupView
+ * (computeSpatialGroups, sph/include/sph/groups.hpp:42-55, with computeGroupSplits,
+ * domain/include/cstone/traversal/groups_gpu.cu:34-101): fixed groups of 64 consecutive particles, each
+ * split between consecutive particles farther apart (in unit-box coordinates) than tolFactor times
+ * the side of the smallest leaf holding one of the group's particles (tolFactor = 2 in the reference).
+ * Needs the tree of the last sphx_build_tree / sphx_domain_sync; fields: x, y, z. The groups live in
+ * the workspace until the next call; *out receives their view (groupDt NULL). */
+int sphx_compute_groups(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, uint32_t first,
+                        uint32_t last, float tolFactor, sphx_groups_t* out);
+
+/* Target groups for the calls that follow (the GroupView argument of the reference's drivers): the
+ * neighbour search, every pair loop, sphx_compute_positions and sphx_update_smoothing_length, when
+ * called with [first, last) = [groups->firstBody, groups->lastBody), run one wavefront per group
+ * instead of consecutive groups of 64 from first. The view is copied (its device arrays must stay
+ * valid while in use); NULL restores consecutive groups. Groups of the search must partition
+ * [firstBody, lastBody) and hold at most 64 particles each (the search returns SPHX_ERR_ARG for a
+ * larger one); the pair loops accept any subset of the searched targets (the block time-step
+ * propagators' active groups), whose neighbour lists stay indexed by particle. Record chaining is
+ * not applied while a view is set. */
+int sphx_set_groups(sphx_workspace_t* ws, const sphx_groups_t* groups);
 
 /* ---- neighbour search ---------------------------------------------------------------------- */
 

@@ -72,8 +72,9 @@ struct Box
     }
 };
 
-//! cstone::GroupView (domain/include/cstone/traversal/groups.hpp:28-34); the device path treats the
-//! targets [firstBody, lastBody) as consecutive 64-particle groups, so groupStart/End may be null
+//! cstone::GroupView (domain/include/cstone/traversal/groups.hpp:28-34), device arrays of at most 64
+//! particles per group; with null groupStart/groupEnd the targets [firstBody, lastBody) run as
+//! consecutive groups of 64
 struct GroupView
 {
     uint32_t        firstBody, lastBody;
@@ -177,6 +178,36 @@ sphx_params_t params(const Dataset& d)
     return p;
 }
 
+/*! sets the caller's target groups on the workspace for the lifetime of the guard (sphx_set_groups),
+ * with the per-group time steps the momentum loops lower; a view without group arrays leaves the
+ * consecutive groups of 64 */
+class GroupScope
+{
+public:
+    GroupScope(sphx_workspace_t* ws, const GroupView& grp, float* groupDt = nullptr)
+        : ws_(ws)
+    {
+        if (grp.groupStart == nullptr)
+        {
+            if (groupDt != nullptr) { throw std::runtime_error("per-group time steps need the group arrays"); }
+            return;
+        }
+        sphx_groups_t g{grp.firstBody, grp.lastBody, grp.numGroups, 0, grp.groupStart, grp.groupEnd, groupDt};
+        check(ws_, sphx_set_groups(ws_, &g));
+        set_ = true;
+    }
+    ~GroupScope()
+    {
+        if (set_) { sphx_set_groups(ws_, nullptr); }
+    }
+    GroupScope(const GroupScope&)            = delete;
+    GroupScope& operator=(const GroupScope&) = delete;
+
+private:
+    sphx_workspace_t* ws_;
+    bool              set_ = false;
+};
+
 template<class Dataset>
 sphx_fields_t fields(Dataset& d)
 {
@@ -278,6 +309,36 @@ private:
     bool     firstCall_    = true;
 };
 
+/*! sph::computeGroups for the GPU (sph/include/sph/groups.hpp:42-73): spatial target groups of
+ * [startIndex, endIndex) (computeGroupSplits, tolFactor 2) on the tree of the last sync; the view's
+ * arrays live in the workspace until the next call */
+template<class Dataset>
+GroupView computeGroups(size_t startIndex, size_t endIndex, Dataset& d, const Box& box)
+{
+    auto          b = box.c();
+    auto          f = detail::fields(d);
+    sphx_groups_t g{};
+    detail::check(d.workspace,
+                  sphx_compute_groups(d.workspace, &b, &f, uint32_t(startIndex), uint32_t(endIndex), 2.0f, &g));
+    return GroupView{g.firstBody, g.lastBody, g.numGroups, g.groupStart, g.groupEnd};
+}
+
+/*! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56); with a group view, the search runs
+ * one wavefront per group of the view (the reference GPU path searches per group inside computeXMass /
+ * computeDensity) */
+template<class Dataset>
+void findNeighborsSfc(const GroupView& grp, Dataset& d, const Box& box)
+{
+    if (d.ng0 > d.ngmax) { throw std::runtime_error("ng0 should be smaller than ngmax\n"); }
+    detail::GroupScope scope(d.workspace, grp);
+    auto               b    = box.c();
+    auto               f    = detail::fields(d);
+    auto               p    = detail::params(d);
+    int                code = sphx_find_neighbors(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody, 1);
+    if (code == SPHX_ERR_NOT_CONVERGED) { throw std::runtime_error("coupled nc/h-updated failed to converge"); }
+    detail::check(d.workspace, code);
+}
+
 //! sph::findNeighborsSfc (sph/include/sph/find_neighbors.hpp:48-56)
 template<class Dataset>
 void findNeighborsSfc(size_t startIndex, size_t endIndex, Dataset& d, const Box& box)
@@ -296,10 +357,11 @@ template<class Dataset>
 void searchXMass(const GroupView& grp, Dataset& d, const Box& box, int out)
 {
     if (d.ng0 > d.ngmax) { throw std::runtime_error("ng0 should be smaller than ngmax\n"); }
-    auto b    = box.c();
-    auto f    = detail::fields(d);
-    auto p    = detail::params(d);
-    int  code = sphx_find_neighbors_xmass(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody, 1, out);
+    GroupScope scope(d.workspace, grp);
+    auto       b    = box.c();
+    auto       f    = detail::fields(d);
+    auto       p    = detail::params(d);
+    int        code = sphx_find_neighbors_xmass(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody, 1, out);
     if (code == SPHX_ERR_NOT_CONVERGED) { throw std::runtime_error("coupled nc/h-updated failed to converge"); }
     detail::check(d.workspace, code);
 }
@@ -326,7 +388,8 @@ void findNeighborsDensity(const GroupView& grp, Dataset& d, const Box& box)
     template<class Dataset>                                                                                            \
     void name(const GroupView& grp, Dataset& d, const Box& box)                                                        \
     {                                                                                                                  \
-        auto b = box.c();                                                                                              \
+        detail::GroupScope scope(d.workspace, grp);                                                                    \
+        auto               b = box.c();                                                                                \
         auto f = detail::fields(d);                                                                                    \
         auto p = detail::params(d);                                                                                    \
         detail::check(d.workspace, cfn(d.workspace, &b, &f, &p, grp.firstBody, grp.lastBody));                         \
@@ -369,6 +432,7 @@ void computeEOS(size_t startIndex, size_t endIndex, Dataset& d)
 template<class Dataset>
 void computeMomentumEnergySTD(const GroupView& grp, Dataset& d, const Box& box)
 {
+    detail::GroupScope scope(d.workspace, grp);
     auto   b  = box.c();
     auto   f  = detail::fields(d);
     auto   p  = detail::params(d);
@@ -378,12 +442,13 @@ void computeMomentumEnergySTD(const GroupView& grp, Dataset& d, const Box& box)
     d.minDtCourant = dt;
 }
 
-//! sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:106-115); sets d.minDtCourant.
-//! Per-group time steps (groupDt) belong to the block-time-step propagators and are not produced.
+//! sph::computeMomentumEnergy<avClean> (hydro_ve/momentum_energy.hpp:106-115); sets d.minDtCourant and,
+//! when groupDt is given (the block time-step propagators), lowers groupDt[g] to group g's smallest
+//! Courant step
 template<bool avClean, class Dataset>
 void computeMomentumEnergy(const GroupView& grp, float* groupDt, Dataset& d, const Box& box)
 {
-    if (groupDt != nullptr) { throw std::runtime_error("per-group time steps are not supported"); }
+    detail::GroupScope scope(d.workspace, grp, groupDt);
     auto   b  = box.c();
     auto   f  = detail::fields(d);
     auto   p  = detail::params(d);
@@ -424,6 +489,7 @@ void computePositions(const GroupView& grp, Dataset& d, const Box& box, float dt
                       std::array<float, maxNumRungs> dt_m1, const uint8_t* rung = nullptr)
 {
     if (rung != nullptr) { throw std::runtime_error("block time-step rungs are not supported"); }
+    detail::GroupScope scope(d.workspace, grp);
     auto b = box.c();
     auto f = detail::fields(d);
     auto h = detail::history(d);
@@ -436,6 +502,7 @@ void computePositions(const GroupView& grp, Dataset& d, const Box& box, float dt
 template<class Dataset>
 void updateSmoothingLength(const GroupView& grp, Dataset& d)
 {
+    detail::GroupScope scope(d.workspace, grp);
     auto f = detail::fields(d);
     auto p = detail::params(d);
     detail::check(d.workspace, sphx_update_smoothing_length(d.workspace, &f, &p, grp.firstBody, grp.lastBody));

@@ -38,6 +38,7 @@ _SIGS = {
     "tree_sizes": (None, [_P, _P, _P, _P]),
     "tree_copy": (None, [_P] * 11),
     "tree_free": (None, [_P]),
+    "group_splits": (_SZ, [_P, _BOXP, _P, _P, _P, _U32, _U32, C.c_float, _P, _SZ]),
     "find_neighbors": (_SZ, [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, _P]),
     "kernel_K": (C.c_double, [C.c_double]),
     "distance_sq_pbc": (C.c_double, [_BOXP] + [C.c_double] * 6),
@@ -134,6 +135,16 @@ class Tree:
             self.lib.tree_free(self.handle)
         except Exception:
             pass
+
+
+def group_splits(lib, tree, box, x, y, z, first, last, tol_factor=2.0):
+    """Spatial target groups of [first, last) (computeGroupSplits, groupSize 64): the group starts
+    followed by `last`, as a uint32 array of numGroups + 1 entries."""
+    cap = (last - first) + 2
+    out = np.zeros(cap, np.uint32)
+    k = lib.group_splits(tree.handle, C.byref(box), ptr(x), ptr(y), ptr(z), first, last, tol_factor, ptr(out), cap)
+    assert k > 0
+    return out[:k]
 
 
 def tables(lib, sinc_index=6.0, dtype=np.float64):

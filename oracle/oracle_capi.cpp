@@ -419,6 +419,19 @@ void* orc_tree_build(const uint64_t* sortedKeys, size_t n, uint32_t bucket, cons
     return t;
 }
 
+/*! spatial groups of [first, last) on the tree `tp` (groupSplits): writes the group starts followed by
+ * `last` into out (capacity cap) and returns their count, numGroups + 1 (0 when cap is too small) */
+size_t orc_group_splits(const void* tp, const sphx_box_t* b, const double* x, const double* y, const double* z,
+                        uint32_t first, uint32_t last, float tolFactor, uint32_t* out, size_t cap)
+{
+    auto*                   t = static_cast<const NsTree*>(tp);
+    std::vector<LocalIndex> g;
+    groupSplits(first, last, x, y, z, t->leaves, t->layout, toBox(b), tolFactor, g);
+    if (g.size() > cap) { return 0; }
+    std::copy(g.begin(), g.end(), out);
+    return g.size();
+}
+
 void orc_tree_sizes(const void* tp, int32_t* numLeaf, int32_t* numInternal, int32_t* numNodes)
 {
     auto* t      = static_cast<const NsTree*>(tp);

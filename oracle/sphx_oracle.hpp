@@ -404,6 +404,50 @@ inline void buildNsTree(const Key* sortedKeys, size_t n, unsigned bucket, const 
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * Spatial target groups: computeGroupSplits with groupSize = nsGroupSize() = 64, one 64-wide
+ * wavefront per fixed group (N = 1), as the reference builds its GroupView on the GPU
+ * (sph/include/sph/groups.hpp:42-55, traversal/groups_gpu.cu:34-101, groups_gpu.cuh:37-76,149-216).
+ * ----------------------------------------------------------------------------------------------*/
+
+/*! Group starts of [first, last) followed by `last`. Each fixed group of 64 consecutive particles
+ * (lanes past last repeat particle last - 1) is split between consecutive particles whose
+ * distance in unit-box coordinates (x / Lx, ...) exceeds distCrit = tolFactor * cbrt(v), v the
+ * smallest unit-box volume of the leaves holding the group's particles. That volume is 8 (s/2)^3
+ * for a leaf of side s = 2^-level, so its cube root is exactly 2^-level, formed here directly;
+ * distCrit is float (h's type on the reference's GPU path) and squared in double. */
+inline void groupSplits(LocalIndex first, LocalIndex last, const double* x, const double* y, const double* z,
+                        const std::vector<Key>& leaves, const std::vector<LocalIndex>& layout, const Box& box,
+                        float tolFactor, std::vector<LocalIndex>& groups)
+{
+    groups.clear();
+    const TreeNodeIndex numLeaves = TreeNodeIndex(leaves.size()) - 1;
+    const double        il[3]     = {1.0 / box.len[0], 1.0 / box.len[1], 1.0 / box.len[2]};
+    for (LocalIndex g0 = first; g0 < last; g0 += 64)
+    {
+        auto body = [&](unsigned l) { return std::min(LocalIndex(g0 + l), LocalIndex(last - 1)); };
+        unsigned maxLevel = 0;
+        for (unsigned l = 0; l < 64; ++l)
+        {
+            LocalIndex    b    = body(l);
+            TreeNodeIndex leaf = TreeNodeIndex(std::upper_bound(layout.begin(), layout.begin() + numLeaves, b) -
+                                               layout.begin()) - 1;
+            maxLevel           = std::max(maxLevel, treeLevel(leaves[leaf + 1] - leaves[leaf]));
+        }
+        const float  distCrit = std::ldexp(1.0f, -int(maxLevel)) * tolFactor;
+        const double dc2      = double(distCrit) * double(distCrit);
+        groups.push_back(g0);
+        for (unsigned l = 0; l + 1 < 64; ++l)
+        {
+            LocalIndex b = body(l), c = body(l + 1);
+            double     d[3] = {x[c] * il[0] - x[b] * il[0], y[c] * il[1] - y[b] * il[1], z[c] * il[2] - z[b] * il[2]};
+            double     d2   = d[0] * d[0] + (d[1] * d[1] + d[2] * d[2]);
+            if (d2 > dc2) { groups.push_back(g0 + l + 1); }
+        }
+    }
+    groups.push_back(last);
+}
+
+/* ------------------------------------------------------------------------------------------------
  * Neighbour search (domain/include/cstone/findneighbors.hpp:33-170,
  * traversal/traversal.hpp:52-93, traversal/boxoverlap.hpp:184-215)
  * ----------------------------------------------------------------------------------------------*/

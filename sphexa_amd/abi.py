@@ -126,6 +126,22 @@ class Adjoint(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ADJOINT_FIELDS]
 
 
+class Groups(C.Structure):
+    """sphx_groups_t: cstone::GroupView (device arrays) plus the optional per-group time step."""
+    _fields_ = [("firstBody", C.c_uint32), ("lastBody", C.c_uint32), ("numGroups", C.c_uint32),
+                ("reserved", C.c_uint32), ("groupStart", C.c_void_p), ("groupEnd", C.c_void_p),
+                ("groupDt", C.c_void_p)]
+
+    @classmethod
+    def of(cls, starts, first, last, group_dt=None):
+        """a view of a device tensor `starts` holding the group starts followed by `last`"""
+        g = cls(firstBody=first, lastBody=last, numGroups=starts.numel() - 1)
+        g.groupStart = starts.data_ptr()
+        g.groupEnd = starts.data_ptr() + starts.element_size()
+        g.groupDt = group_dt.data_ptr() if group_dt is not None else None
+        return g
+
+
 class TreeView(C.Structure):
     _fields_ = [("numLeafNodes", C.c_int32), ("numInternalNodes", C.c_int32), ("numNodes", C.c_int32),
                 ("reserved", C.c_int32), ("prefixes", C.c_void_p), ("childOffsets", C.c_void_p),

@@ -103,14 +103,30 @@ sphx_params_t params(const Data& d)
     return p;
 }
 
-//! the pair loops take the view's particle range; lists and records are per particle
+//! the caller's GroupView (and per-group time steps) on the workspace while a call runs
+struct Groups
+{
+    Groups(sphx_workspace_t* ws, const GroupView& g, float* groupDt = nullptr)
+        : ws_(ws)
+    {
+        sphx_groups_t v{g.firstBody, g.lastBody, g.numGroups, 0, g.groupStart, g.groupEnd, groupDt};
+        check(ws_, sphx_set_groups(ws_, &v));
+    }
+    ~Groups() { sphx_set_groups(ws_, nullptr); }
+    Groups(const Groups&)            = delete;
+    Groups& operator=(const Groups&) = delete;
+    sphx_workspace_t* ws_;
+};
+
+//! a pair loop over the view's groups; lists and records are per particle
 template<class F>
 void pass(const GroupView& g, Data& d, const Box& b, F cfn)
 {
-    auto ws = workspace(d);
-    auto bx = box(b);
-    auto f  = fields(d);
-    auto p  = params(d);
+    auto   ws = workspace(d);
+    Groups scope(ws, g);
+    auto   bx = box(b);
+    auto   f  = fields(d);
+    auto   p  = params(d);
     check(ws, cfn(ws, &bx, &f, &p, g.firstBody, g.lastBody));
 }
 
@@ -128,6 +144,7 @@ void searchThenSum(const GroupView& g, Data& d, const Box& b, Sum sum)
     auto       p  = params(d);
     const auto n  = d.devData.keys.size();
     check(ws, sphx_build_tree(ws, rawPtr(d.devData.keys), n, 64, &bx));
+    Groups scope(ws, g); // the spatial groups of computeGroups, one wavefront each
     check(ws, sphx_find_neighbors(ws, &bx, &f, &p, g.firstBody, g.lastBody, /*iterateH=*/1));
     check(ws, sum(ws, &bx, &f, &p, g.firstBody, g.lastBody));
 }
@@ -154,6 +171,7 @@ template<class Dataset>
 void computeMomentumEnergyStdGpu(const GroupView& g, Dataset& d, const cstone::Box<typename Dataset::RealType>& b)
 {
     auto   ws = workspace(d);
+    Groups scope(ws, g);
     auto   bx = box(b);
     auto   f  = fields(d);
     auto   p  = params(d);
@@ -202,8 +220,8 @@ template<bool avClean, class Dataset>
 void computeMomentumEnergy(const GroupView& g, float* groupDt, Dataset& d,
                            const cstone::Box<typename Dataset::RealType>& b)
 {
-    if (groupDt) { unsupported("per-group time steps (groupDt)"); }
     auto   ws = workspace(d);
+    Groups scope(ws, g, groupDt); // the block time-step propagators pass groupDt
     auto   bx = box(b);
     auto   f  = fields(d);
     auto   p  = params(d);
@@ -323,6 +341,7 @@ void computePositionsGpu(const GroupView& grp, float dt, util::array<float, Time
                          Tc constCv, const cstone::Box<Tc>& b)
 {
     if (rung) { unsupported("block time-step rungs"); }
+    Groups        scope(theWorkspace(), grp);
     sphx_fields_t f{};
     f.x = x, f.y = y, f.z = z, f.vx = vx, f.vy = vy, f.vz = vz, f.ax = ax, f.ay = ay, f.az = az;
     f.temp = temp, f.u = u, f.du = du, f.h = h;
@@ -360,6 +379,7 @@ template void driftPositionsGpu(const GroupView&, float, float, util::array<floa
 template<class Th>
 void updateSmoothingLengthGpu(const GroupView& grp, unsigned ng0, const unsigned* nc, Th* h)
 {
+    Groups        scope(theWorkspace(), grp);
     sphx_fields_t f{};
     f.h = h, f.nc = const_cast<unsigned*>(nc);
     sphx_params_t p{};

@@ -316,7 +316,7 @@ void densityStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f
     requireAll({f->h, f->m, in->rho, out->x, out->y, out->z, out->h},
                "density adjoint needs fields h, m, cotangent rho and gradients x, y, z, h");
     DensityVjpArgs d{f->h, f->m, in->rho, out->x, out->y, out->z, out->h};
-    unsigned       blocks = pairBlocks(first, last);
+    unsigned       blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
         densityStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
@@ -343,7 +343,7 @@ void iadStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, co
                  out->z,
                  out->h,
                  out->rho};
-    unsigned blocks = pairBlocks(first, last);
+    unsigned blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
         iadStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
@@ -386,7 +386,7 @@ void momentumEnergyStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fiel
                  out->p,
                  out->alpha,
                  {out->c11, out->c12, out->c13, out->c22, out->c23, out->c33}};
-    unsigned blocks = pairBlocks(first, last);
+    unsigned blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
     if (p->hydro_type == SPHX_F32)
         momentumStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));

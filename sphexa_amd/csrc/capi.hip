@@ -407,6 +407,34 @@ int sphx_neighbor_stats(const sphx_workspace_t* ws, uint64_t* sumPairs, uint32_t
     return SPHX_OK;
 }
 
+int sphx_compute_groups(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f, uint32_t first,
+                        uint32_t last, float tolFactor, sphx_groups_t* out)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && out, SPHX_ERR_ARG, "null argument");
+        computeGroups(ws, makeDevBox(box), f->x, f->y, f->z, first, last, tolFactor, out);
+        return SPHX_OK;
+    }, false);
+}
+
+int sphx_set_groups(sphx_workspace_t* ws, const sphx_groups_t* groups)
+{
+    return guarded(ws, [&] {
+        if (groups == nullptr)
+        {
+            ws->groupsSet = false;
+            ws->groupView = sphx_groups_t{};
+            return SPHX_OK;
+        }
+        SPHX_REQUIRE(groups->firstBody <= groups->lastBody, SPHX_ERR_ARG, "invalid group range");
+        SPHX_REQUIRE(groups->numGroups == 0 || (groups->groupStart && groups->groupEnd), SPHX_ERR_ARG,
+                     "null group arrays");
+        ws->groupsSet = true;
+        ws->groupView = *groups;
+        return SPHX_OK;
+    }, false);
+}
+
 int sphx_search_tests(const sphx_workspace_t* ws, uint64_t* tests)
 {
     if (ws == nullptr || tests == nullptr) { return SPHX_ERR_ARG; }

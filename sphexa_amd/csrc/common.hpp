@@ -157,6 +157,17 @@ struct Tree
     }
 };
 
+/*! target groups of a call over [first, last) (cstone::GroupView): consecutive groups of 64 from
+ * `first` when start is null, else [start[g], min(end[g], start[g] + 64)) for g < num */
+struct Groups
+{
+    const uint32_t* start = nullptr;
+    const uint32_t* end   = nullptr;
+    float*          dt    = nullptr; // optional per-group Courant step (min-updated)
+    uint32_t        num   = 0;
+    uint32_t        first = 0, last = 0;
+};
+
 } // namespace sphx
 
 struct sphx_workspace
@@ -180,6 +191,13 @@ struct sphx_workspace
     const uint32_t*        nbNc = nullptr; // nc array the lists were built with
     bool                   nbValid = false;
     bool                   nbWide  = false; // some target's 2h reaches half a box length (FP32 loops: exact separations)
+
+    // target groups set by sphx_set_groups, and the spatial groups of sphx_compute_groups
+    bool                   groupsSet = false;
+    sphx_groups_t          groupView{};
+    sphx::DevBuf<uint32_t> groupBuf;   // group starts then last
+    sphx::DevBuf<uint32_t> groupCount; // per fixed group: sub-groups, then their exclusive scan
+    sphx::DevBuf<uint64_t> groupSplit; // per fixed group: split mask
     uint64_t               statSumPairs = 0;
     uint32_t               statMaxCount = 0;
     uint64_t               statTests    = 0; // candidate distance tests of the last search
@@ -215,7 +233,7 @@ struct sphx_workspace
     uint64_t      eosSerial = 0;
     sphx_fields_t eosFields{};
     int           eosKind = 0;
-    sphx::DevBuf<uint32_t> counters;     // small device counters / flags
+    sphx::DevBuf<uint32_t> counters;     // small device counters / flags: kCounterSlots 64-bit slots
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions
 
@@ -271,6 +289,70 @@ inline void* scratchBytes(sphx_workspace* ws, size_t bytes)
 }
 
 inline unsigned ceilDiv(size_t a, unsigned b) { return unsigned((a + b - 1) / b); }
+
+//! the target groups of a call over [first, last): the view of sphx_set_groups when it covers exactly
+//! that range, else consecutive groups of 64 from first
+inline Groups groupsFor(const sphx_workspace* ws, uint32_t first, uint32_t last)
+{
+    Groups g;
+    g.first = first, g.last = last;
+    if (ws->groupsSet && ws->groupView.firstBody == first && ws->groupView.lastBody == last)
+    {
+        g.start = ws->groupView.groupStart, g.end = ws->groupView.groupEnd, g.dt = ws->groupView.groupDt;
+        g.num   = ws->groupView.numGroups;
+    }
+    else { g.num = ceilDiv(last - first, uint32_t(kGroup)); }
+    return g;
+}
+
+/*! the particle range [i0, i1) of group g of a call (device): clipped to the call's range and to 64
+ * particles; sets `oversized` when the caller's group holds more than 64 */
+__device__ __forceinline__ void groupRange(const Groups& G, uint32_t g, uint32_t& i0, uint32_t& i1, bool& oversized);
+
+//! threads an elementwise kernel over the call's targets launches: one per target, or 64 per group
+inline size_t groupedThreads(const Groups& G) { return G.start ? size_t(G.num) * kGroup : size_t(G.last - G.first); }
+
+//! particle of this thread of an elementwise kernel over the call's targets (false: none)
+__device__ __forceinline__ bool groupedIndex(const Groups& G, uint32_t& i)
+{
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (G.start == nullptr)
+    {
+        i = G.first + tid;
+        return i < G.last;
+    }
+    const uint32_t g = tid / uint32_t(kGroup);
+    if (g >= G.num) { return false; }
+    uint32_t i0, i1;
+    bool     oversized;
+    groupRange(G, g, i0, i1, oversized);
+    i = i0 + tid % uint32_t(kGroup);
+    return i < i1;
+}
+
+__device__ __forceinline__ void groupRange(const Groups& G, uint32_t g, uint32_t& i0, uint32_t& i1, bool& oversized)
+{
+    if (G.start == nullptr)
+    {
+        i0        = G.first + g * uint32_t(kGroup);
+        i1        = min(i0 + uint32_t(kGroup), G.last);
+        oversized = false;
+        return;
+    }
+    const uint32_t s = G.start[g], e = G.end[g];
+    oversized        = e > s + uint32_t(kGroup);
+    i0               = max(s, G.first);
+    i1               = min(min(e, s + uint32_t(kGroup)), G.last);
+    if (i1 < i0) { i1 = i0; }
+}
+
+/* 64-bit slots of the workspace counters: the search's statistics (stored pairs, maximum count,
+ * non-converged targets) in 0-2, the Courant dt of the momentum loops in 3, the device maxima of the
+ * time step in 4, the search's wide-support count, candidate tests and oversized groups in 5-7 */
+constexpr int kCounterSlots  = 8;
+constexpr int kStatWide      = 5;
+constexpr int kStatTests     = 6;
+constexpr int kStatBadGroups = 7;
 
 constexpr unsigned kNumXcd = 8; // MI355X: 8 accelerator complex dies, one L2 each
 
@@ -415,6 +497,9 @@ void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n);
 void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
                  size_t n, bool keepRemoved);
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
+//! spatial target groups of [first, last) into the workspace (computeGroupSplits)
+void computeGroups(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z,
+                   uint32_t first, uint32_t last, float tolFactor, sphx_groups_t* out);
 void gather(sphx_workspace* ws, const uint32_t* order, const void* in, void* out, size_t n, int elemBytes);
 //! permute every field in place by `order` (4- or 8-byte elements), batched per element size
 //! min x, max x, min y, max y, min z, max z of n coordinates (+-INFINITY for n = 0), on the host

@@ -300,14 +300,14 @@ __global__ void SPHX_MOM_LOOP(T)
         st<T>(f.az, i, T(a.K * double(momz)));
         dtLane = courantDt(maxvs, hi, ri.c, Kcour);
     }
-    reduceMinDt<T>(minDt, dtLane);
+    reduceMinDt<T>(minDt, dtLane, a.L.groups);
 }
 
 template<class T>
 void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p, bool toRho,
                    void* out)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
     RecMass<T>* rec = recordBuffer<RecMass<T>>(ws, a.numParticles, 0);
     packMass<T><<<ceilDiv(a.numParticles, kPackThreads), kPackThreads, 0, ws->stream>>>(a.box, a.x, a.y, a.z,
@@ -337,7 +337,7 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
 template<class T>
 void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainVol, a, f, p);
     if (blocks == 0) { return; }
     RecVol<T>* rec = recordBuffer<RecVol<T>>(ws, a.numParticles, in < 0 ? 0 : in);
@@ -369,7 +369,7 @@ template<class T>
 void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields& mf, const sphx_fields_t* f,
                        const sphx_params_t* p, float Kcour, unsigned long long* dt, bool perAlpha)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainMomStd, a, f, p);
     if (blocks == 0) { return; }
     RecMomStd<T>* rec = recordBuffer<RecMomStd<T>>(ws, a.numParticles, in < 0 ? 0 : in);
@@ -473,7 +473,7 @@ double momentumEnergyStd(sphx_workspace* ws, const DevBox& box, const sphx_field
     SPHX_REQUIRE(f->vx && f->vy && f->vz && f->h && f->m && f->rho && f->p && f->c && f->c11 && f->c12 && f->c13 &&
                      f->c22 && f->c23 && f->c33 && f->ax && f->ay && f->az && f->du,
                  SPHX_ERR_ARG, "momentum-energy needs v, h, m, rho, p, c, c11..c33, ax, ay, az and du");
-    ws->counters.reserve(8);
+    ws->counters.reserve(2 * kCounterSlots);
     unsigned long long* dt = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
     fill64(ws, reinterpret_cast<uint64_t*>(dt), 0x7ff0000000000000ull, 1); // +inf
     const bool perAlpha = (p->flags & SPHX_FLAG_STD_ALPHA) != 0;

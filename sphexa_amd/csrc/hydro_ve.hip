@@ -555,12 +555,12 @@ __global__ void SPHX_MOM_LOOP(T)
         st<T>(f.az, i, T(-a.K * double(momz)));
         dtLane = courantDt(maxvs, hi, ri.c, Kcour);
     }
-    reduceMinDt<T>(minDt, dtLane);
+    reduceMinDt<T>(minDt, dtLane, a.L.groups);
 }
 
 unsigned long long* resetDt(sphx_workspace* ws)
 {
-    ws->counters.reserve(8);
+    ws->counters.reserve(2 * kCounterSlots);
     unsigned long long* dt = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 3;
     fill64(ws, reinterpret_cast<uint64_t*>(dt), 0x7ff0000000000000ull, 1); // +inf
     return dt;
@@ -579,7 +579,7 @@ double readDt(sphx_workspace* ws, unsigned long long* dt)
 template<class T>
 void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, const sphx_params_t* p)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainGradh, a, f, p);
     if (blocks == 0) { return; }
     RecGradh<T>* rec = recordBuffer<RecGradh<T>>(ws, a.numParticles, in < 0 ? 0 : in);
@@ -610,7 +610,7 @@ template<class T>
 void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf, const sphx_fields_t* f,
                  const sphx_params_t* p)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainIadVe, a, f, p);
     if (blocks == 0) { return; }
     RecIadVe<T>* rec = recordBuffer<RecIadVe<T>>(ws, a.numParticles, in < 0 ? 0 : in);
@@ -636,7 +636,7 @@ template<class T>
 void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const sphx_fields_t* f,
               const sphx_params_t* p)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainAv, a, f, p);
     if (blocks == 0) { return; }
     RecAv<T>* rec = recordBuffer<RecAv<T>>(ws, a.numParticles, in < 0 ? 0 : in);
@@ -663,7 +663,7 @@ template<class T>
 void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, const sphx_fields_t* f,
                  const sphx_params_t* p, bool avClean, unsigned long long* dt)
 {
-    unsigned blocks = pairBlocks(a.L.first, a.L.last);
+    unsigned blocks = pairBlocks(a.L);
     int      in     = takeChain(ws, kChainMomVe, a, f, p);
     if (blocks == 0) { return; }
     RecMomVe<T>* rec = recordBuffer<RecMomVe<T>>(ws, a.numParticles, in < 0 ? 0 : in);

@@ -51,7 +51,7 @@ struct PosArgs
     DevBox      box;
     int         fbc[3];
     double      dt, dt_m1, gamma, constCv;
-    uint32_t    first, last;
+    Groups      groups; // the call's targets
     double *    x, *y, *z, *temp, *u;
     const double* du;
     void *      vx, *vy, *vz, *x_m1, *y_m1, *z_m1, *du_m1;
@@ -64,8 +64,8 @@ template<class T>
 __global__ void __launch_bounds__(kIntThreads) positionsKernel(PosArgs a)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.last) { return; }
+    uint32_t i;
+    if (!groupedIndex(a.groups, i)) { return; }
     T* vx = static_cast<T*>(a.vx);
     T* vy = static_cast<T*>(a.vy);
     T* vz = static_cast<T*>(a.vz);
@@ -120,12 +120,12 @@ __global__ void __launch_bounds__(kIntThreads) positionsKernel(PosArgs a)
 
 //! updateSmoothingLengthGpuKernel (sph/include/sph/update_h_gpu.cu:40-50): h = updateH(ng0, nc, h)
 template<class T>
-__global__ void __launch_bounds__(kIntThreads) updateHKernel(uint32_t first, uint32_t last, unsigned ng0,
-                                                           const uint32_t* __restrict__ nc, T* __restrict__ h)
+__global__ void __launch_bounds__(kIntThreads) updateHKernel(Groups G, unsigned ng0, const uint32_t* __restrict__ nc,
+                                                           T* __restrict__ h)
 {
 #pragma clang fp contract(off)
-    const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= last) { return; }
+    uint32_t i;
+    if (!groupedIndex(G, i)) { return; }
     constexpr T c0 = 1023.0;
     constexpr T ex = 1.0 / 10.0;
     h[i]           = h[i] * T(0.5) * pow(T(1) + c0 * ng0 / T(nc[i]), ex);
@@ -181,7 +181,7 @@ template<class T>
 double deviceMax(sphx_workspace* ws, uint32_t first, uint32_t last, const void* x, const void* y, const void* z,
                  bool norm)
 {
-    ws->counters.reserve(16);
+    ws->counters.reserve(2 * kCounterSlots);
     unsigned long long* slot = reinterpret_cast<unsigned long long*>(ws->counters.get()) + 4;
     fill64(ws, reinterpret_cast<uint64_t*>(slot), 0, 1); // below the key of every double
     const unsigned blocks = std::min(ceilDiv(last - first, kIntThreads), kMaxBlocks);
@@ -208,7 +208,7 @@ void launchPositions(sphx_workspace* ws, PosArgs& a, const sphx_fields_t* f, con
     if (hist->mui) { a.constCv = -1.0; }
     else { a.constCv = constCv >= 0 ? constCv : double(idealGasCv(T(p->muiConst), p->gamma)); }
     KernelScope timer(ws, "positions");
-    positionsKernel<T><<<ceilDiv(a.last - a.first, kIntThreads), kIntThreads, 0, ws->stream>>>(a);
+    positionsKernel<T><<<ceilDiv(groupedThreads(a.groups), kIntThreads), kIntThreads, 0, ws->stream>>>(a);
     SPHX_CHECK_LAUNCH();
 }
 
@@ -261,7 +261,7 @@ void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fiel
     }
     SPHX_REQUIRE(!anyFbc || f->h, SPHX_ERR_ARG, "fixed boundaries need h");
     a.dt = dt, a.dt_m1 = dt_m1, a.gamma = p->gamma, a.constCv = -1;
-    a.first = first, a.last = last;
+    a.groups = groupsFor(ws, first, last);
     a.x = f->x, a.y = f->y, a.z = f->z, a.temp = f->temp, a.u = f->temp ? nullptr : f->u, a.du = f->du;
     a.vx = f->vx, a.vy = f->vy, a.vz = f->vz, a.ax = f->ax, a.ay = f->ay, a.az = f->az, a.h = f->h;
     a.x_m1 = hist->x_m1, a.y_m1 = hist->y_m1, a.z_m1 = hist->z_m1, a.du_m1 = hist->du_m1, a.mui = hist->mui;
@@ -275,12 +275,13 @@ void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sph
     SPHX_REQUIRE(f->h && f->nc, SPHX_ERR_ARG, "updateSmoothingLength needs h and nc");
     SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
     if (first == last) { return; }
-    const unsigned blocks = ceilDiv(last - first, kIntThreads);
+    const Groups   G      = groupsFor(ws, first, last);
+    const unsigned blocks = ceilDiv(groupedThreads(G), kIntThreads);
     KernelScope    timer(ws, "update_h");
     if (p->hydro_type == SPHX_F32)
-        updateHKernel<float><<<blocks, kIntThreads, 0, ws->stream>>>(first, last, p->ng0, f->nc, (float*)f->h);
+        updateHKernel<float><<<blocks, kIntThreads, 0, ws->stream>>>(G, p->ng0, f->nc, (float*)f->h);
     else
-        updateHKernel<double><<<blocks, kIntThreads, 0, ws->stream>>>(first, last, p->ng0, f->nc, (double*)f->h);
+        updateHKernel<double><<<blocks, kIntThreads, 0, ws->stream>>>(G, p->ng0, f->nc, (double*)f->h);
     SPHX_CHECK_LAUNCH();
 }
 

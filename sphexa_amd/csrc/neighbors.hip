@@ -51,11 +51,12 @@ struct SearchArgs
     DevBox               box;
     double               slack; // absolute enlargement of search radii (rounding margin)
     uint32_t             first, last, ng0, ngmax, ngmax4;
+    Groups               groups; // target groups, one wavefront each
     int                  iterateH;
     uint32_t*            nbrs;
     uint32_t*            nc;
-    unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles, [3] wide support,
-                                // [4] candidate distance tests (swept candidates x 64 lanes)
+    unsigned long long*  stats; // [0] stored pairs, [1] max count, [2] non-converged particles, [kStatWide] wide
+                                // support, [kStatTests] candidate distance tests (swept candidates x 64 lanes)
     double               minLen; // shortest box length
     // fused XMass: masses, kernel table (TabPair of h's type), normalisation, output (xm or rho)
     const void*          m;
@@ -747,20 +748,26 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
 
     const unsigned wid  = threadIdx.x / kWave;
     const unsigned lane = laneId();
-    const uint32_t g    = xcdBlock(blockIdx.x, gridDim.x) * kWavesPerBlock + wid;
-    const uint32_t base = a.first + g * kGroup;
-    if (base >= a.last) { return; }
+    const uint32_t g    = uniform(xcdBlock(blockIdx.x, gridDim.x) * kWavesPerBlock + wid);
+    if (g >= a.groups.num) { return; }
+    uint32_t base, end;
+    bool     oversized;
+    groupRange(a.groups, g, base, end, oversized);
+    base = uniform(base), end = uniform(end);
+    if (oversized && lane == 0) { atomicAdd(&a.stats[kStatBadGroups], 1ull); }
     WaveLds& lds = ldsMem[wid];
 
     Target t;
     t.i      = base + lane;
-    t.active = t.i < a.last;
+    t.active = t.i < end;
     Th hi    = t.active ? h[t.i] : Th(0);
     t.x      = t.active ? a.x[t.i] : 0.0;
     t.y      = t.active ? a.y[t.i] : 0.0;
     t.z      = t.active ? a.z[t.i] : 0.0;
     setRadius(t, hi, a.slack);
-    uint32_t* myList = a.nbrs + listOffset(g, a.ngmax4, 0, lane);
+    // lists are indexed by particle: target i at group (i - first) / 64, lane (i - first) % 64
+    const uint32_t r      = t.active ? t.i - a.first : 0u;
+    uint32_t*      myList = a.nbrs + listOffset(r / kGroup, a.ngmax4, 0, r % kGroup);
 
     using Sink = std::conditional_t<XMass, XMassSum<Th>, ListOnly>;
     Sink      sink;
@@ -830,8 +837,8 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
         atomicAdd(&a.stats[0], (unsigned long long)sum);
         atomicMax(&a.stats[1], (unsigned long long)mx);
         if (fails) { atomicAdd(&a.stats[2], (unsigned long long)fails); }
-        if (wide) { atomicAdd(&a.stats[3], (unsigned long long)wide); }
-        atomicAdd(&a.stats[4], (unsigned long long)swept * kWave);
+        if (wide) { atomicAdd(&a.stats[kStatWide], (unsigned long long)wide); }
+        atomicAdd(&a.stats[kStatTests], (unsigned long long)swept * kWave);
     }
 }
 
@@ -859,12 +866,14 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     SPHX_REQUIRE(p->ng0 <= p->ngmax, SPHX_ERR_ARG, "ng0 should be smaller than ngmax");
     SPHX_REQUIRE(p->ngmax > 0, SPHX_ERR_ARG, "ngmax must be positive");
 
-    const Tree& t       = ws->tree;
-    uint32_t    numTgt  = last - first;
-    uint32_t    groups  = ceilDiv(numTgt, kGroup);
-    ws->nbrs.reserve(std::max<size_t>(1, size_t(groups) * paddedNgmax(p->ngmax) * kGroup));
-    ws->counters.reserve(8);
-    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 5);
+    const Tree&  t      = ws->tree;
+    const Groups grp    = groupsFor(ws, first, last);
+    const uint32_t groups = grp.num;
+    // list blocks of 64 consecutive particles from first, whatever the target groups
+    ws->nbrs.reserve(std::max<size_t>(1, size_t(ceilDiv(last - first, kGroup)) * paddedNgmax(p->ngmax) * kGroup));
+    ws->counters.reserve(2 * kCounterSlots);
+    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 3);
+    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()) + kStatWide, 0, 3);
 
     double scale = 0;
     for (int d = 0; d < 3; ++d)
@@ -882,6 +891,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     a.box            = box;
     a.slack          = 1e-12 * scale;
     a.first = first, a.last = last, a.ng0 = p->ng0, a.ngmax = p->ngmax, a.ngmax4 = paddedNgmax(p->ngmax);
+    a.groups = grp;
     a.iterateH = iterateH ? 1 : 0;
     a.nbrs     = ws->nbrs.get();
     a.nc       = f->nc;
@@ -920,7 +930,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
         }
         SPHX_CHECK_LAUNCH();
     }
-    unsigned long long stats[5];
+    unsigned long long stats[kCounterSlots];
     SPHX_HIP(hipMemcpyAsync(stats, ws->counters.get(), sizeof(stats), hipMemcpyDeviceToHost, ws->stream));
     SPHX_HIP(hipStreamSynchronize(ws->stream));
 
@@ -931,8 +941,13 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     ws->nbValid      = true;
     ws->statSumPairs = stats[0];
     ws->statMaxCount = uint32_t(stats[1]);
-    ws->nbWide       = stats[3] > 0;
-    ws->statTests    = stats[4];
+    ws->nbWide       = stats[kStatWide] > 0;
+    ws->statTests    = stats[kStatTests];
+    if (stats[kStatBadGroups] > 0)
+    {
+        ws->nbValid = false;
+        throw Error{SPHX_ERR_ARG, std::to_string(stats[kStatBadGroups]) + " target groups hold more than 64 particles"};
+    }
     return stats[2] > 0 ? SPHX_ERR_NOT_CONVERGED : SPHX_OK;
 }
 

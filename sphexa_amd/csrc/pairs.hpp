@@ -71,7 +71,8 @@ struct Lists
 {
     const uint32_t* nbrs;
     const uint32_t* nc;
-    uint32_t        first, last, ngmax, ngmax4;
+    uint32_t        first, last, ngmax, ngmax4; // [first, last): the searched targets
+    Groups          groups;                     // the call's target groups, one wavefront each
 };
 
 struct Lane
@@ -82,22 +83,32 @@ struct Lane
     const uint32_t* list;  // this lane's slot 0; slot k at list[laneSlot(k)]
 };
 
+//! the wave's group index among the call's target groups
+__device__ __forceinline__ uint32_t waveGroup()
+{
+    return __builtin_amdgcn_readfirstlane(xcdBlock(blockIdx.x, gridDim.x) * kPairWaves + threadIdx.x / kWave);
+}
+
+/*! lane l of the wave of group g takes the group's l-th target; its list sits at block (i - first) / 64,
+ * lane (i - first) % 64 of the searched range, whatever the grouping. Targets outside the searched
+ * range [first, last) are not taken. */
 __device__ __forceinline__ Lane laneTarget(const Lists& L)
 {
     Lane           t;
     const unsigned lane = threadIdx.x % kWave;
-    const uint32_t g    = xcdBlock(blockIdx.x, gridDim.x) * kPairWaves + threadIdx.x / kWave;
-    t.i                 = L.first + g * kGroup + lane;
-    t.valid             = t.i < L.last;
-    t.n                 = t.valid ? min(L.nc[t.i] - 1, L.ngmax) : 0u;
-    t.list              = L.nbrs + listOffset(g, L.ngmax4, 0, lane);
+    const uint32_t g    = waveGroup();
+    uint32_t       i0 = 0, i1 = 0;
+    bool           oversized;
+    if (g < L.groups.num) { groupRange(L.groups, g, i0, i1, oversized); }
+    t.i            = i0 + lane;
+    t.valid        = t.i < i1 && t.i >= L.first && t.i < L.last;
+    t.n            = t.valid ? min(L.nc[t.i] - 1, L.ngmax) : 0u;
+    const uint32_t r = t.valid ? t.i - L.first : 0u;
+    t.list         = L.nbrs + listOffset(r / kGroup, L.ngmax4, 0, r % kGroup);
     return t;
 }
 
-inline unsigned pairBlocks(uint32_t first, uint32_t last)
-{
-    return ceilDiv(ceilDiv(last - first, kGroup), kPairWaves);
-}
+inline unsigned pairBlocks(const Lists& L) { return ceilDiv(L.groups.num, kPairWaves); }
 
 //! lt::lookup (sph/include/sph/table_lookup.hpp:10-26): linear interpolation of a 20000-point
 //! table on [0,2], zero beyond the support. The table is stored per interval (both end values
@@ -665,14 +676,15 @@ inline PairArgs makePairArgs(sphx_workspace* ws, const DevBox& box, const sphx_f
                              uint32_t first, uint32_t last)
 {
     SPHX_REQUIRE(ws->nbValid, SPHX_ERR_STATE, "pair loop called before find_neighbors");
-    SPHX_REQUIRE(first == ws->nbFirst && last <= ws->nbLast && first <= last, SPHX_ERR_ARG,
-                 "target range must start at the searched first index and stay inside the searched range");
+    SPHX_REQUIRE(ws->nbFirst <= first && first <= last && last <= ws->nbLast, SPHX_ERR_ARG,
+                 "target range must stay inside the searched range");
     SPHX_REQUIRE(p->ngmax == ws->nbNgmax, SPHX_ERR_ARG, "ngmax differs from the one used by find_neighbors");
     SPHX_REQUIRE(ws->tablesValid, SPHX_ERR_STATE, "kernel tables not set (sphx_workspace_set_kernel)");
     SPHX_REQUIRE(f->x && f->y && f->z && f->nc, SPHX_ERR_ARG, "x, y, z and nc are required");
     PairArgs c;
     c.x = f->x, c.y = f->y, c.z = f->z, c.box = box, c.K = p->K;
-    c.L            = Lists{ws->nbrs.get(), f->nc, first, last, p->ngmax, paddedNgmax(p->ngmax)};
+    c.L            = Lists{ws->nbrs.get(), f->nc, ws->nbFirst, ws->nbLast, p->ngmax, paddedNgmax(p->ngmax),
+                               groupsFor(ws, first, last)};
     c.numParticles = uint32_t(ws->tree.numParticles);
     return c;
 }
@@ -696,12 +708,18 @@ inline SincKernel<T> sincKernel(sphx_workspace* ws)
     return k;
 }
 
-//! wave minimum then one atomic per wave
+/*! wave minimum then one atomic per wave; with per-group steps (block time steps), also
+ * groupDt[g] = min(groupDt[g], the group's minimum) as a float (hydro_ve/momentum_energy_gpu.cu:96-104) */
 template<class T>
-__device__ __forceinline__ void reduceMinDt(unsigned long long* dst, double dt)
+__device__ __forceinline__ void reduceMinDt(unsigned long long* dst, double dt, const Groups& G)
 {
     double m = waveMin(dt);
-    if (threadIdx.x % kWave == 0) { atomicMinPositive(dst, m); }
+    if (threadIdx.x % kWave == 0)
+    {
+        atomicMinPositive(dst, m);
+        const uint32_t g = waveGroup();
+        if (G.dt != nullptr && g < G.num) { G.dt[g] = fminf(G.dt[g], float(m)); }
+    }
 }
 
 } // namespace sphx

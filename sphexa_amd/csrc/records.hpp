@@ -132,10 +132,11 @@ enum ChainKind
     kChainVol    = 6  // sphx_density_std -> (sphx_eos_std) -> sphx_iad_std
 };
 
-//! the call takes part in record chaining: flag set and the full particle range
+//! the call takes part in record chaining: flag set, the full particle range, consecutive target groups
 inline bool chainable(const PairArgs& a, const sphx_params_t* p)
 {
-    return (p->flags & SPHX_FLAG_CHAIN) && a.L.first == 0 && a.L.last == a.numParticles;
+    return (p->flags & SPHX_FLAG_CHAIN) && a.L.groups.first == 0 && a.L.groups.last == a.numParticles &&
+           a.L.groups.start == nullptr;
 }
 
 //! the record buffer holding the records of `kind` that the previous call wrote for this one, or -1

@@ -40,6 +40,8 @@ _SIGS = {
     "sphx_copy_neighbors": [_P, _P, _U32, _U32],
     "sphx_neighbor_stats": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
     "sphx_search_tests": [_P, C.POINTER(C.c_uint64)],
+    "sphx_compute_groups": [_P, _BOXP, _FP, _U32, _U32, C.c_float, C.POINTER(abi.Groups)],
+    "sphx_set_groups": [_P, C.POINTER(abi.Groups)],
     "sphx_density_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
     "sphx_eos_std": [_P, _FP, _PP, _U32, _U32],
     "sphx_iad_std": [_P, _BOXP, _FP, _PP, _U32, _U32],
@@ -167,6 +169,17 @@ class Workspace:
         s, m = C.c_uint64(), C.c_uint32()
         self.call("sphx_neighbor_stats", C.byref(s), C.byref(m))
         return s.value, m.value
+
+    def compute_groups(self, box, fields, first, last, tol_factor=2.0):
+        """Spatial target groups of [first, last) (sphx_compute_groups); an abi.Groups view into the
+        workspace."""
+        g = abi.Groups()
+        self.call("sphx_compute_groups", C.byref(box), C.byref(fields), first, last, tol_factor, C.byref(g))
+        return g
+
+    def set_groups(self, groups):
+        """Target groups of the calls that follow (abi.Groups or None for consecutive groups of 64)."""
+        self.call("sphx_set_groups", C.byref(groups) if groups is not None else None)
 
     def search_tests(self):
         """candidate distance tests of the last search (sphx_search_tests)"""

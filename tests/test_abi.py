@@ -44,6 +44,15 @@ def test_struct_layouts_match_header():
     assert names == abi.ADJOINT_FIELDS
     assert C.sizeof(abi.Adjoint) == 8 * len(abi.ADJOINT_FIELDS)
     assert "int32_t  flags;" in text and abi.Params.flags.offset == 11 * 8 + 3 * 4
+    # the params' EOS members follow flags; the group view is 4 uint32 then 3 pointers
+    names = re.findall(r"^\s*(?:double|int32_t|uint32_t)\s+([\w, ]+);",
+                       text[text.index("typedef struct sphx_params"):text.index("} sphx_params_t;")], re.M)
+    flat = [n.strip() for group in names for n in group.split(",")]
+    assert flat == [f[0] for f in abi.Params._fields_]
+    body = text[text.index("typedef struct sphx_groups"):text.index("} sphx_groups_t;")]
+    decls = re.findall(r"^\s*(?:const uint32_t\*|float\*|uint32_t)\s+([\w, ]+);", body, re.M)
+    assert [n.strip() for d in decls for n in d.split(",")] == [f[0] for f in abi.Groups._fields_]
+    assert C.sizeof(abi.Groups) == 4 * 4 + 3 * 8
 
 
 def test_null_workspace_is_rejected_without_touching_the_gpu():

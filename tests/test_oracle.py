@@ -490,6 +490,67 @@ def test_eos_variants_equal_reference(orc, ref, form, eos, hydro):
         assert np.array_equal(o[k], r[k]), k
 
 
+def _groups_for(orc, x, y, z, box, first=0, last=None, tol=2.0):
+    from oracle import oracle as O
+
+    x, y, z = (np.ascontiguousarray(c, np.float64) for c in (x, y, z))
+    keys = O.sfc_keys(orc, box, x, y, z)
+    sk, order = O.sort_keys(orc, keys)
+    xs, ys, zs = (np.ascontiguousarray(c[order]) for c in (x, y, z))
+    tree = O.Tree(orc, sk, box)
+    last = xs.size if last is None else last
+    return O.group_splits(orc, tree, box, xs, ys, zs, first, last, tol), (xs, ys, zs), tree
+
+
+def test_group_splits_lattice_has_no_splits(orc):
+    """On a perfect lattice consecutive particles in SFC order are lattice neighbours, far closer than
+    twice the smallest leaf side: the groups are the fixed groups of 64 (computeGroupSplits,
+    domain/include/cstone/traversal/groups_gpu.cuh:149-216)."""
+    side = 16
+    g1 = (np.arange(side) + 0.5) / side - 0.5
+    x, y, z = (c.ravel().copy() for c in np.meshgrid(g1, g1, g1, indexing="ij"))
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC)
+    g, _, _ = _groups_for(orc, x, y, z, box)
+    np.testing.assert_array_equal(g, np.append(np.arange(0, x.size, 64), x.size))
+    g, _, _ = _groups_for(orc, x, y, z, box, first=10, last=1000)
+    np.testing.assert_array_equal(g, np.append(np.arange(10, 1000, 64), 1000))
+
+
+def test_group_splits_at_cluster_gaps(orc):
+    """Three small lattice clusters (5^3 particles, spacing 0.002) far apart: each holds more than a
+    bucket, so its leaves are a few spacings wide, and consecutive particles inside a cluster stay within
+    twice the smallest leaf side while the gaps between clusters are hundreds of times wider. Every
+    fixed group is split exactly where the SFC order passes from one cluster to the next."""
+    g1 = (np.arange(5) - 2.0) * 0.002
+    cube = np.stack([c.ravel() for c in np.meshgrid(g1, g1, g1, indexing="ij")], 1)
+    centres = np.array([[0.1, 0.1, 0.1], [0.85, 0.2, 0.7], [0.3, 0.9, 0.6]])
+    pts = np.concatenate([c + cube for c in centres])
+    box = abi.Box.make(0.0, 1.0, abi.OPEN)
+    g, (xs, ys, zs), _ = _groups_for(orc, pts[:, 0], pts[:, 1], pts[:, 2], box)
+    n = xs.size
+    which = np.argmin(((np.stack([xs, ys, zs], 1)[:, None, :] - centres[None]) ** 2).sum(2), 1)
+    changes = 1 + np.nonzero(which[1:] != which[:-1])[0]
+    expected = np.unique(np.concatenate([np.arange(0, n, 64), changes, [n]]))
+    np.testing.assert_array_equal(g, expected)
+
+
+def test_group_splits_invariants(orc):
+    """On the uneven Evrard-like sphere: starts increase, every fixed-group start is a group start, no
+    group holds more than 64 particles, the last entry is the range end, and a split leaves a gap wider
+    than the split distance."""
+    arr, spec = ic.evrard_sphere(20000, seed=4)
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.OPEN)
+    g, (xs, ys, zs), tree = _groups_for(orc, arr["x"], arr["y"], arr["z"], box)
+    n = xs.size
+    assert g[0] == 0 and g[-1] == n and np.all(np.diff(g) > 0) and np.max(np.diff(g)) <= 64
+    assert set(range(0, n, 64)) <= set(g.tolist())
+    assert len(g) - 1 > (n + 63) // 64  # the sphere's outskirts split
+    L = 2.0
+    for s in set(g[1:-1].tolist()) - set(range(0, n, 64)):
+        d = np.sqrt((xs[s] - xs[s - 1]) ** 2 + (ys[s] - ys[s - 1]) ** 2 + (zs[s] - zs[s - 1]) ** 2) / L
+        assert d > 2.0 * 2.0 ** -21  # at least twice the finest possible leaf
+
+
 def test_oracle_tree_equals_reference(orc, ref):
     from oracle import oracle as O
 
