@@ -155,6 +155,17 @@ typedef struct sphx_params
  * with or without the flag. */
 #define SPHX_FLAG_CHAIN 2
 
+/* Tree update of the syncs after a Domain's first (domain/include/cstone/domain/domain.hpp:199-212).
+ * Set on sphx_domain_sync, the tree of the workspace's previous sync (same bucket size) is updated
+ * ONCE instead of converged from scratch: one rebalance decided on its counts (the previous keys),
+ * merging sibling groups whose parent holds at most bucketSize particles and splitting in 8 the leaves
+ * holding more, then the counts of the new keys, as the reference's focused octree does on one rank
+ * (FocusedOctreeSingleNode::update, focus/octree_focus.hpp:348-385). Leaves may then hold more than
+ * bucketSize particles until later syncs split them; the neighbour search and the pair loops take any
+ * leaf size. Without a previous sync tree (first call, or after sphx_build_tree) the sync converges,
+ * as the reference's first sync does. The C++ Domain sets it on every sync after its first. */
+#define SPHX_FLAG_TREE_UPDATE 4
+
 /* Previous-step fields of the time integrator (particles_data.hpp:221,233,238). x_m1, y_m1, z_m1
  * (X_n - X_n-1) and du_m1 are XM1Type, which is the HydroType here (double in the plain form,
  * float in the mixed form, as SphTypes); mui is HydroType, NULL for the constant muiConst. */
@@ -289,7 +300,8 @@ int sphx_tree_view(const sphx_workspace_t* ws, sphx_tree_view_t* out);
 int sphx_global_box(sphx_workspace_t* ws, sphx_box_t* box, const double* x, const double* y, const double* z,
                     size_t n, int firstCall);
 
-/* keys -> stable sort -> permute every non-NULL field of `f` -> tree. One-rank Domain::sync
+/* keys -> stable sort -> permute every non-NULL field of `f` -> tree (converged, or updated once with
+ * SPHX_FLAG_TREE_UPDATE in prm->flags). One-rank Domain::sync
  * (domain/include/cstone/domain/domain.hpp:182-234). As there, only the conserved fields need to be
  * passed (std: x y z h m u vx vy vz, HydroProp::ConservedFields, main/src/propagator/std_hydro.hpp:73;
  * VE: x y z h m temp vx vy vz alpha, ve_hydro.hpp:72): a pass recomputes every dependent field

@@ -247,9 +247,11 @@ sphx_timestep_t timestep(const Dataset& d)
 //! one-rank Domain::sync (domain/include/cstone/domain/domain.hpp:182-234) as the propagators call
 //! it (main/src/propagator/std_hydro.hpp:108-122, ve_hydro.hpp:114-128): keys, sort, permute the
 //! conserved fields (x y z h m u temp vx vy vz alpha and the integrator's previous-step fields),
-//! then the tree. The dependent fields keep their order; every pass rewrites them before reading.
+//! then the tree: converged, or with treeUpdate the previous sync's tree updated once (the syncs
+//! after a Domain's first, SPHX_FLAG_TREE_UPDATE). The dependent fields keep their order; every pass
+//! rewrites them before reading.
 template<class Dataset>
-void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucketSize = 64)
+void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucketSize = 64, bool treeUpdate = false)
 {
     auto          b = box.c();
     auto&         s = d.devData;
@@ -257,6 +259,7 @@ void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucket
     f.x = s.x, f.y = s.y, f.z = s.z, f.h = s.h, f.m = s.m, f.u = s.u, f.temp = s.temp;
     f.vx = s.vx, f.vy = s.vy, f.vz = s.vz, f.alpha = s.alpha, f.keys = s.keys;
     auto p = detail::params(d);
+    if (treeUpdate) { p.flags |= SPHX_FLAG_TREE_UPDATE; }
     detail::check(d.workspace, sphx_domain_sync(d.workspace, &b, &f, &p, numParticles, bucketSize));
     void* extra[5];
     int   bytes[5];
@@ -272,7 +275,7 @@ void syncDomain(size_t numParticles, Dataset& d, const Box& box, unsigned bucket
 /*! One-rank cstone::Domain (domain/include/cstone/domain/domain.hpp): owns the box and the
  *  first-call state. sync() refits the box as GlobalAssignment::assign does (assignment.hpp:80-82:
  *  the fitting box on the first call, then limitBoxShrinking; periodic dimensions fixed), then runs
- *  syncDomain. All particles are local: startIndex() = 0, endIndex() = nParticles(). */
+ *  syncDomain, converging the tree on the first call and updating it once on every later one. All particles are local: startIndex() = 0, endIndex() = nParticles(). */
 class Domain
 {
 public:
@@ -291,7 +294,8 @@ public:
         auto  b = box_.c();
         detail::check(d.workspace, sphx_global_box(d.workspace, &b, s.x, s.y, s.z, numParticles, firstCall_));
         box_ = Box(b);
-        syncDomain(numParticles, d, box_, bucketSize_);
+        // the first sync converges the tree, later ones update it once (domain.hpp:199-212)
+        syncDomain(numParticles, d, box_, bucketSize_, !firstCall_);
         numParticles_ = numParticles;
         firstCall_    = false;
     }

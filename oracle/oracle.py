@@ -38,6 +38,12 @@ _SIGS = {
     "tree_sizes": (None, [_P, _P, _P, _P]),
     "tree_copy": (None, [_P] * 11),
     "tree_free": (None, [_P]),
+    "tree_update": (C.c_int, [_P, _P, _SZ, _U32, _BOXP]),
+    "focus_new": (_P, [_U32, C.c_float]),
+    "focus_update": (C.c_int, [_P, _BOXP, _P, _SZ]),
+    "focus_num_leaves": (_SZ, [_P]),
+    "focus_copy": (None, [_P, _P, _P]),
+    "focus_free": (None, [_P]),
     "group_splits": (_SZ, [_P, _BOXP, _P, _P, _P, _U32, _U32, C.c_float, _P, _SZ]),
     "find_neighbors": (_SZ, [_P, _BOXP, _FP, _PP, _U32, _U32, C.c_int, _P]),
     "kernel_K": (C.c_double, [C.c_double]),
@@ -112,6 +118,18 @@ class Tree:
         keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
         self.lib = lib
         self.handle = lib.tree_build(ptr(keys), keys.size, bucket, C.byref(box))
+        self._copy()
+
+    def update(self, sorted_keys, box, bucket=64):
+        """The tree update of a later Domain::sync (one rebalance on the previous counts, then counts of
+        the new keys), in place; returns True when no node changed."""
+        keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+        conv = self.lib.tree_update(self.handle, ptr(keys), keys.size, bucket, C.byref(box))
+        self._copy()
+        return bool(conv)
+
+    def _copy(self):
+        lib = self.lib
         nl, ni, nn = C.c_int32(), C.c_int32(), C.c_int32()
         lib.tree_sizes(self.handle, C.byref(nl), C.byref(ni), C.byref(nn))
         self.numLeafNodes, self.numInternalNodes, self.numNodes = nl.value, ni.value, nn.value
@@ -133,6 +151,36 @@ class Tree:
     def __del__(self):
         try:
             self.lib.tree_free(self.handle)
+        except Exception:
+            pass
+
+
+class FocusTree:
+    """The reference's single-rank focused octree (oracle/_ref only): the leaves and counts of a
+    sequence of one-rank Domain::sync calls. sync() converges on the first call and updates once on
+    every later one."""
+
+    def __init__(self, lib, bucket=64, theta=0.5):
+        self.lib = lib
+        self.handle = lib.focus_new(bucket, theta)
+        self.first = True
+
+    def sync(self, sorted_keys, box):
+        keys = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+        if self.first:
+            while not self.lib.focus_update(self.handle, C.byref(box), ptr(keys), keys.size):
+                pass
+            self.first = False
+        else:
+            self.lib.focus_update(self.handle, C.byref(box), ptr(keys), keys.size)
+        L = self.lib.focus_num_leaves(self.handle)
+        leaves, counts = np.zeros(L + 1, np.uint64), np.zeros(L, np.uint32)
+        self.lib.focus_copy(self.handle, ptr(leaves), ptr(counts))
+        return leaves, counts
+
+    def __del__(self):
+        try:
+            self.lib.focus_free(self.handle)
         except Exception:
             pass
 

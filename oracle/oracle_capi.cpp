@@ -460,6 +460,13 @@ void orc_tree_copy(const void* tp, uint64_t* leaves, uint32_t* counts, uint32_t*
 
 void orc_tree_free(void* tp) { delete static_cast<NsTree*>(tp); }
 
+/*! the tree update of a later sync (updateNsTree) on the tree `tp`, in place, with the new sorted keys;
+ * returns 1 when no node changed */
+int orc_tree_update(void* tp, const uint64_t* sortedKeys, size_t n, uint32_t bucket, const sphx_box_t* b)
+{
+    return updateNsTree(sortedKeys, n, bucket, toBox(b), *static_cast<NsTree*>(tp)) ? 1 : 0;
+}
+
 /* ---- neighbours ------------------------------------------------------------------------------ */
 //! findNeighborsSfc -> findNeighborsSph (sph/include/sph/find_neighbors.hpp:11-56); nc[i], h[i]
 //! for first <= i < last. Returns the number of particles whose h/nc iteration failed.

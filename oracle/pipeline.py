@@ -19,8 +19,11 @@ VE_OUT = ("xm", "kx", "gradh", "prho", "c", "c11", "c12", "c13", "c22", "c23", "
 GRADV = ("dV11", "dV12", "dV13", "dV22", "dV23", "dV33")
 
 
-def run(lib, arrays, box, params, form="std", av_clean=False, bucket=64, iterate_h=True, hydro=np.float64):
-    """arrays: host IC (unsorted). form: 'std', 've' or 'search' (stop after the neighbour search)."""
+def run(lib, arrays, box, params, form="std", av_clean=False, bucket=64, iterate_h=True, hydro=np.float64,
+        prev_tree=None):
+    """arrays: host IC (unsorted). form: 'std', 've' or 'search' (stop after the neighbour search).
+    prev_tree: the tree of the previous sync of the same particles (a later Domain::sync): it is
+    updated once (O.Tree.update) instead of converged anew."""
     times = {}
     t0 = time.perf_counter()
     x, y, z = (np.ascontiguousarray(arrays[k], np.float64) for k in "xyz")
@@ -33,7 +36,11 @@ def run(lib, arrays, box, params, form="std", av_clean=False, bucket=64, iterate
             continue
         dt = np.float64 if k in abi.REAL_FIELDS else hydro
         a[k] = np.ascontiguousarray(np.asarray(v)[order], dtype=dt)
-    tree = O.Tree(lib, skeys, box, bucket)
+    if prev_tree is None:
+        tree = O.Tree(lib, skeys, box, bucket)
+    else:
+        tree = prev_tree
+        tree.update(skeys, box, bucket)
     times["domain::sync"] = time.perf_counter() - t0
 
     t0 = time.perf_counter()

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "cstone/findneighbors.hpp"
+#include "cstone/focus/octree_focus.hpp"
 #include "cstone/focus/source_center.hpp"
 #include "cstone/primitives/gather.hpp"
 #include "cstone/sfc/sfc.hpp"
@@ -245,6 +246,35 @@ void ref_tree_copy(const void* tp, uint64_t* leaves, uint32_t* counts, uint32_t*
 }
 
 void ref_tree_free(void* tp) { delete static_cast<Tree*>(tp); }
+
+/* The tree of a one-rank Domain::sync sequence through the reference's own single-rank focused octree
+ * (cstone::FocusedOctreeSingleNode, focus/octree_focus.hpp:330-400), the focus spanning the whole key
+ * range: ref_focus_update runs one update (rebalance on the previous counts, then counts of `keys`)
+ * and returns whether the tree was unchanged. The first sync of a Domain repeats it until then. */
+void* ref_focus_new(uint32_t bucket, float theta) { return new cstone::FocusedOctreeSingleNode<KeyT>(bucket, theta); }
+
+int ref_focus_update(void* fp, const sphx_box_t* b, const uint64_t* sortedKeys, size_t n)
+{
+    auto* f = static_cast<cstone::FocusedOctreeSingleNode<KeyT>*>(fp);
+    return f->update(toBox(b), std::span<const KeyT>(sortedKeys, n), KeyT(0), cstone::nodeRange<KeyT>(0),
+                     std::span<const KeyT>{})
+               ? 1
+               : 0;
+}
+
+size_t ref_focus_num_leaves(const void* fp)
+{
+    return static_cast<const cstone::FocusedOctreeSingleNode<KeyT>*>(fp)->leafCounts().size();
+}
+
+void ref_focus_copy(const void* fp, uint64_t* leaves, uint32_t* counts)
+{
+    auto* f = static_cast<const cstone::FocusedOctreeSingleNode<KeyT>*>(fp);
+    std::copy(f->treeLeaves().begin(), f->treeLeaves().end(), leaves);
+    std::copy(f->leafCounts().begin(), f->leafCounts().end(), counts);
+}
+
+void ref_focus_free(void* fp) { delete static_cast<cstone::FocusedOctreeSingleNode<KeyT>*>(fp); }
 
 size_t ref_find_neighbors(const void* tp, const sphx_box_t* b, const sphx_fields_t* f, const sphx_params_t* p,
                           uint32_t first, uint32_t last, int iterateH, uint32_t* nbrs)

@@ -394,13 +394,106 @@ struct NsTree
     std::vector<double>     centers, sizes;
 };
 
-inline void buildNsTree(const Key* sortedKeys, size_t n, unsigned bucket, const Box& box, NsTree& t)
+//! link, layout and centres of the leaves and counts already in t
+inline void finishNsTree(const Box& box, NsTree& t)
 {
-    computeOctree(sortedKeys, n, bucket, t.leaves, t.counts);
     buildOctree(t.leaves, t.oct);
     t.layout.assign(t.counts.size() + 1, 0);
     std::inclusive_scan(t.counts.begin(), t.counts.end(), t.layout.begin() + 1);
     nodeCenters(t.oct.prefixes, box, t.centers, t.sizes);
+}
+
+inline void buildNsTree(const Key* sortedKeys, size_t n, unsigned bucket, const Box& box, NsTree& t)
+{
+    computeOctree(sortedKeys, n, bucket, t.leaves, t.counts);
+    finishNsTree(box, t);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Tree update of the syncs after the first (Domain::sync, domain/include/cstone/domain/domain.hpp:
+ * 199-212): the first sync converges the tree; every later one applies ONE rebalance of the previous
+ * leaves, decided on the previous counts, then counts the new keys in the new leaves
+ * (focusTree_.updateTree + updateCounts). On one rank the focus covers the whole key range, which is
+ * FocusedOctreeSingleNode::update (focus/octree_focus.hpp:348-385) around CombinedUpdate::updateFocus
+ * (:65-130):
+ *  - node counts of every node: leaf counts scattered to the leaves, summed upwards (upsweep);
+ *  - mergeCountAndMacOp (focus/rebalance.hpp:31-68): node i > 0 merges (0) when its parent holds at
+ *    most `bucket` particles; a leaf above the deepest level holding more splits in 8; else 1. Every
+ *    node lies in the focus and every sibling group in its fringe, so the MAC never decides;
+ *  - protectAncestors / nzAncestorOp (rebalance.hpp:88-107,219-233): a node marked 0 takes the op of its
+ *    closest ancestor not marked 0 when it is that ancestor's first descendant, else stays 0, so a
+ *    merged subtree becomes one leaf (in index order, every ancestor is decided before its descendants,
+ *    which gives the same result as the reference's parallel in-place loop);
+ *  - the mandatory keys 0 and 2^63 (enforceKeys) are always leaf bounds: nothing to enforce;
+ *  - the leaves' ops to rebalanceTree / processNode (tree/csarray.hpp:340-389); counts of the new
+ *    leaves from the keys (computeNodeCounts).
+ * Leaves may hold more than `bucket` particles afterwards, until later syncs split them. Returns
+ * whether no node changed (the updateFocus result).
+ * ----------------------------------------------------------------------------------------------*/
+inline bool updateNsTree(const Key* sortedKeys, size_t n, unsigned bucket, const Box& box, NsTree& t)
+{
+    const Octree&         o = t.oct;
+    const TreeNodeIndex   N = o.numNodes;
+    std::vector<unsigned> cnt(N, 0);
+    for (TreeNodeIndex i = 0; i < N; ++i)
+        if (o.internalToLeaf[i] >= 0) { cnt[i] = t.counts[o.internalToLeaf[i]]; }
+    // level-major order: every child follows its parent, so a reverse sweep sums children first
+    for (TreeNodeIndex i = N - 1; i >= 0; --i)
+    {
+        if (o.childOffsets[i] == 0) { continue; }
+        unsigned s = 0;
+        for (int c = 0; c < 8; ++c)
+            s += cnt[o.childOffsets[i] + c];
+        cnt[i] = s;
+    }
+    std::vector<int> op(N, 1);
+    for (TreeNodeIndex i = 0; i < N; ++i)
+    {
+        unsigned level = decodePrefixLength(o.prefixes[i]) / 3;
+        if (i > 0 && cnt[o.parents[(i - 1) / 8]] <= bucket)
+        {
+            op[i] = 0;
+            continue;
+        }
+        bool leaf = o.childOffsets[i] == 0;
+        op[i]     = (leaf && level < kMaxLevel && cnt[i] > bucket) ? 8 : 1;
+    }
+    bool converged = true;
+    for (TreeNodeIndex i = 0; i < N; ++i)
+    {
+        if (op[i] == 0)
+        {
+            TreeNodeIndex a = i;
+            while (op[a] == 0) { a = o.parents[(a - 1) / 8]; }
+            op[i] = decodePlaceholder(o.prefixes[i]) == decodePlaceholder(o.prefixes[a]) ? op[a] : 0;
+        }
+        if (op[i] != 1) { converged = false; }
+    }
+    const TreeNodeIndex L = o.numLeafNodes;
+    std::vector<int>    ops(L + 1, 0);
+    for (TreeNodeIndex i = 0; i < N; ++i)
+        if (o.internalToLeaf[i] >= 0) { ops[o.internalToLeaf[i]] = op[i]; }
+    std::exclusive_scan(ops.begin(), ops.end(), ops.begin(), 0);
+    std::vector<Key> next(ops[L] + 1);
+    for (TreeNodeIndex i = 0; i < L; ++i)
+    {
+        Key      node  = t.leaves[i];
+        unsigned level = treeLevel(t.leaves[i + 1] - node);
+        int      k     = ops[i + 1] - ops[i];
+        if (k == 1) { next[ops[i]] = node; }
+        else if (k == 8)
+        {
+            for (int s = 0; s < 8; ++s)
+                next[ops[i] + s] = node + Key(s) * nodeRange(level + 1);
+        }
+    }
+    next.back() = t.leaves.back();
+    t.leaves.swap(next);
+    t.counts.resize(t.leaves.size() - 1);
+    for (size_t i = 0; i + 1 < t.leaves.size(); ++i)
+        t.counts[i] = nodeCount(t.leaves[i], t.leaves[i + 1], sortedKeys, n);
+    finishNsTree(box, t);
+    return converged;
 }
 
 /* ------------------------------------------------------------------------------------------------

@@ -83,6 +83,7 @@ class Params(C.Structure):
 SPHX_EOS_IDEAL_GAS, SPHX_EOS_ISOTHERMAL, SPHX_EOS_POLYTROPIC = 0, 1, 2
 SPHX_FLAG_STD_ALPHA = 1
 SPHX_FLAG_CHAIN = 2
+SPHX_FLAG_TREE_UPDATE = 4
 
 
 def chained(params):

@@ -239,7 +239,8 @@ int sphx_build_tree(sphx_workspace_t* ws, const uint64_t* sortedKeys, size_t n, 
 {
     return guarded(ws, [&] {
         SPHX_REQUIRE(box && (n == 0 || sortedKeys), SPHX_ERR_ARG, "null argument");
-        ws->nbValid = false;
+        ws->nbValid  = false;
+        ws->syncTree = false;
         buildTree(ws, sortedKeys, n, bucketSize, makeDevBox(box));
         return SPHX_OK;
     });
@@ -313,11 +314,17 @@ int sphx_domain_sync(sphx_workspace_t* ws, const sphx_box_t* box, sphx_fields_t*
         }
         {
             KernelScope timer(ws, "tree");
-            buildTree(ws, keys, n, bucketSize, b);
+            const bool update = (prm->flags & SPHX_FLAG_TREE_UPDATE) && ws->syncTree && ws->tree.valid &&
+                                ws->syncBucket == bucketSize;
+            ws->syncTree = false;
+            if (update) { updateTree(ws, keys, n, bucketSize, b); }
+            else { buildTree(ws, keys, n, bucketSize, b); }
         }
         SPHX_HIP(hipStreamSynchronize(ws->stream));
         own.release();
-        ws->syncN = n;
+        ws->syncN      = n;
+        ws->syncTree   = true;
+        ws->syncBucket = bucketSize;
         return SPHX_OK;
     });
 }

@@ -101,12 +101,17 @@ struct TreeBuildScratch
     DevBuf<uint32_t> lo, hi, flag, rank, leafFlag, leafRank, leafNode, leafNodeSorted;
     DevBuf<uint8_t>  level;
     DevBuf<int32_t>  levelStart;
+    // the one-step update of a later sync: node counts, node and leaf ops, the new leaves
+    DevBuf<uint32_t> nodeCnt, changed;
+    DevBuf<int32_t>  nodeOp, leafOp;
+    DevBuf<Key>      nextLeaves;
 
     void releaseAll()
     {
         nodeKey.release(), leafKey.release(), leafKeySorted.release(), lo.release(), hi.release();
         flag.release(), rank.release(), leafFlag.release(), leafRank.release(), leafNode.release();
         leafNodeSorted.release(), level.release(), levelStart.release();
+        nodeCnt.release(), changed.release(), nodeOp.release(), leafOp.release(), nextLeaves.release();
     }
 };
 
@@ -208,6 +213,8 @@ struct sphx_workspace
     sphx::DevBuf<char>     fieldTmp;     // permutation staging
     sphx::DevBuf<uint32_t> order;        // sort permutation of the domain sync
     size_t                 syncN = 0;    // particles of that sync (0: none yet)
+    bool                   syncTree   = false; // the workspace tree is that sync's (its counts are of its keys)
+    uint32_t               syncBucket = 0;     // its bucket size
     sphx::DevBuf<sphx::Chunk16> records; // packed neighbour records of the current pair loop
     sphx::DevBuf<sphx::Chunk16> records2; // records a pair loop's epilogue writes for the next loop
 
@@ -377,6 +384,19 @@ __host__ __device__ inline Key encodePlaceholder(Key key, int prefixLength)
     return (Key(1) << prefixLength) | (key >> (3 * int(kMaxLevel) - prefixLength));
 }
 
+//! length in bits of the prefix of a placeholder-bit key (decodePrefixLength, sfc/common.hpp)
+__host__ __device__ inline unsigned decodePrefixLength(Key code) { return 63u - unsigned(__builtin_clzll(code)); }
+
+//! first SFC key of the node of a placeholder-bit key (decodePlaceholderBit, sfc/common.hpp)
+__host__ __device__ inline Key decodePlaceholder(Key code)
+{
+    const unsigned len = decodePrefixLength(code);
+    return (code ^ (Key(1) << len)) << (3u * kMaxLevel - len);
+}
+
+//! octree level of a node spanning `range` keys (treeLevel, sfc/common.hpp)
+__host__ __device__ inline unsigned treeLevel(Key range) { return (unsigned(__builtin_clzll(range - 1)) - 1u) / 3u; }
+
 //! Hilbert key -> integer coordinates (same curve as domain/include/cstone/sfc/hilbert.hpp:130-172)
 __host__ __device__ inline void hilbertDecode(Key key, unsigned& ox, unsigned& oy, unsigned& oz)
 {
@@ -507,6 +527,9 @@ void coordinateExtent(sphx_workspace* ws, const double* x, const double* y, cons
 void permuteFields(sphx_workspace* ws, const uint32_t* order, const std::vector<std::pair<void*, int>>& fields,
                    size_t n);
 void buildTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);
+//! one rebalance of the workspace tree on its counts, then counts of the new sorted keys (the tree update
+//! of a later Domain::sync); returns whether no node changed
+bool updateTree(sphx_workspace* ws, const Key* sortedKeys, size_t n, unsigned bucket, const DevBox& box);
 
 //! what findNeighbors computes besides the lists: nothing, or the fused XMass sum as xm or as rho
 constexpr int kSearchOnly = 0, kSearchXm = 1, kSearchRho = 2;

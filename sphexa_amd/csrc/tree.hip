@@ -215,10 +215,16 @@ void growCopy(sphx_workspace* ws, DevBuf<T>& buf, size_t used, size_t need)
 
 } // namespace
 
-void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned bucket, const DevBox& box)
+namespace
 {
-    SPHX_REQUIRE(nParticles < (size_t(1) << 32), SPHX_ERR_ARG, "particle count must fit 32-bit indices");
-    SPHX_REQUIRE(bucket >= 1, SPHX_ERR_ARG, "bucket size must be positive");
+
+/* The linked tree whose nodes are exactly the octree nodes holding more than `bucket` of the sorted
+ * keys `keys` (and their children): the converged cornerstone tree of particle keys with the bucket
+ * size, or, with bucket 1 and the leaf starts of a cornerstone array as keys, the octree of that leaf
+ * array (a node is internal iff more than one leaf starts inside it). Leaf counts and layout come out
+ * in units of keys. */
+void buildLinked(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned bucket, const DevBox& box)
+{
     hipStream_t  st = ws->stream;
     uint32_t     n  = uint32_t(nParticles);
     auto&        s   = ws->tree.build;
@@ -352,6 +358,167 @@ void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned 
     t.numNodes         = int32_t(numNodes);
     t.numParticles     = nParticles;
     t.valid            = true;
+}
+
+//! particle count of every node of the tree: a leaf's from its count, an internal node's from the layout
+//! of the leaves it spans
+__global__ void nodeCountKernel(const Key* __restrict__ prefixes, const TreeNodeIndex* __restrict__ childOffsets,
+                                const TreeNodeIndex* __restrict__ internalToLeaf, const Key* __restrict__ leaves,
+                                const uint32_t* __restrict__ layout, const uint32_t* __restrict__ counts, uint32_t L,
+                                uint32_t N, uint32_t* __restrict__ nodeCnt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) { return; }
+    if (childOffsets[i] == 0)
+    {
+        nodeCnt[i] = counts[internalToLeaf[i]];
+        return;
+    }
+    const Key      p     = prefixes[i];
+    const Key      start = decodePlaceholder(p);
+    const Key      end   = start + nodeRange(decodePrefixLength(p) / 3u);
+    const uint32_t a     = lowerBound(leaves, 0, L, start);
+    const uint32_t b     = lowerBound(leaves, a, L + 1, end);
+    nodeCnt[i]           = layout[b] - layout[a];
+}
+
+/* mergeCountAndMacOp (focus/rebalance.hpp:31-68) on one rank, the focus spanning every key: node i > 0
+ * merges (0) when its parent holds at most `bucket` particles; a leaf above the deepest level holding
+ * more splits in 8; else 1 */
+__global__ void nodeOpKernel(const Key* __restrict__ prefixes, const TreeNodeIndex* __restrict__ childOffsets,
+                             const TreeNodeIndex* __restrict__ parents, const uint32_t* __restrict__ nodeCnt,
+                             uint32_t N, unsigned bucket, int32_t* __restrict__ op)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) { return; }
+    if (i > 0 && nodeCnt[parents[(i - 1) / 8]] <= bucket)
+    {
+        op[i] = 0;
+        return;
+    }
+    const unsigned level = decodePrefixLength(prefixes[i]) / 3u;
+    op[i]                = (childOffsets[i] == 0 && level < kMaxLevel && nodeCnt[i] > bucket) ? 8 : 1;
+}
+
+/* protectAncestors / nzAncestorOp (focus/rebalance.hpp:88-107,219-233) for every node, reading the ops
+ * before protection: a node marked 0 takes the op of its closest ancestor not marked 0 when it is that
+ * ancestor's first descendant, else stays 0. An ancestor marked 0 that protection would turn to 1 is the
+ * first descendant of its own closest unmarked ancestor, so walking past it gives the same answer as the
+ * reference's in-place loop. Leaves store their final op by leaf index; any node left != 1 marks a change. */
+__global__ void protectKernel(const Key* __restrict__ prefixes, const TreeNodeIndex* __restrict__ parents,
+                              const TreeNodeIndex* __restrict__ internalToLeaf, const int32_t* __restrict__ op,
+                              uint32_t N, int32_t* __restrict__ leafOp, uint32_t* __restrict__ changed)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) { return; }
+    int32_t o = op[i];
+    if (o == 0) // never the root
+    {
+        TreeNodeIndex a = TreeNodeIndex(i);
+        while (op[a] == 0) { a = parents[(a - 1) / 8]; }
+        o = decodePlaceholder(prefixes[i]) == decodePlaceholder(prefixes[a]) ? op[a] : 0;
+    }
+    if (o != 1) { *changed = 1u; }
+    const TreeNodeIndex leaf = internalToLeaf[i];
+    if (leaf >= 0) { leafOp[leaf] = o; }
+}
+
+//! rebalanceTree / processNode (tree/csarray.hpp:340-389) with ops 0, 1 and 8 at offsets `offs`
+__global__ void rebalanceKernel(const Key* __restrict__ leaves, const int32_t* __restrict__ offs, uint32_t L,
+                                Key* __restrict__ next)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > L) { return; }
+    if (s == L)
+    {
+        next[offs[L]] = leaves[L];
+        return;
+    }
+    const Key      node = leaves[s];
+    const int32_t  k    = offs[s + 1] - offs[s];
+    if (k == 1) { next[offs[s]] = node; }
+    else if (k == 8)
+    {
+        const Key r = nodeRange(treeLevel(leaves[s + 1] - node) + 1u);
+        for (int c = 0; c < 8; ++c)
+            next[offs[s] + c] = node + Key(c) * r;
+    }
+}
+
+//! first particle and count of every leaf of the linked leaf array (computeNodeCounts: binary searches)
+__global__ void leafLayoutKernel(const Key* __restrict__ leaves, uint32_t L, const Key* __restrict__ keys, uint32_t n,
+                                 uint32_t* __restrict__ layout, uint32_t* __restrict__ counts)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > L) { return; }
+    const uint32_t lo = s == L ? n : lowerBound(keys, 0, n, leaves[s]);
+    layout[s]         = lo;
+    if (s < L) { counts[s] = (s + 1 == L ? n : lowerBound(keys, lo, n, leaves[s + 1])) - lo; }
+}
+
+__global__ void setLeafOpEnd(int32_t* leafOp, uint32_t L, uint32_t* changed)
+{
+    leafOp[L] = 0;
+    *changed  = 0u;
+}
+
+} // namespace
+
+void buildTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned bucket, const DevBox& box)
+{
+    SPHX_REQUIRE(nParticles < (size_t(1) << 32), SPHX_ERR_ARG, "particle count must fit 32-bit indices");
+    SPHX_REQUIRE(bucket >= 1, SPHX_ERR_ARG, "bucket size must be positive");
+    buildLinked(ws, keys, nParticles, bucket, box);
+}
+
+bool updateTree(sphx_workspace* ws, const Key* keys, size_t nParticles, unsigned bucket, const DevBox& box)
+{
+    SPHX_REQUIRE(nParticles < (size_t(1) << 32), SPHX_ERR_ARG, "particle count must fit 32-bit indices");
+    SPHX_REQUIRE(bucket >= 1, SPHX_ERR_ARG, "bucket size must be positive");
+    Tree& t = ws->tree;
+    SPHX_REQUIRE(t.valid, SPHX_ERR_STATE, "no tree to update");
+    hipStream_t    st = ws->stream;
+    auto&          s  = t.build;
+    const uint32_t N = uint32_t(t.numNodes), L = uint32_t(t.numLeafNodes);
+    s.nodeCnt.reserve(N), s.nodeOp.reserve(N), s.leafOp.reserve(size_t(L) + 1), s.changed.reserve(1);
+    setLeafOpEnd<<<1, 1, 0, st>>>(s.leafOp.get(), L, s.changed.get());
+    SPHX_CHECK_LAUNCH();
+    nodeCountKernel<<<ceilDiv(N, 256), 256, 0, st>>>(t.prefixes.get(), t.childOffsets.get(), t.internalToLeaf.get(),
+                                                      t.leaves.get(), t.layout.get(), t.counts.get(), L, N,
+                                                      s.nodeCnt.get());
+    SPHX_CHECK_LAUNCH();
+    nodeOpKernel<<<ceilDiv(N, 256), 256, 0, st>>>(t.prefixes.get(), t.childOffsets.get(), t.parents.get(),
+                                                   s.nodeCnt.get(), N, bucket, s.nodeOp.get());
+    SPHX_CHECK_LAUNCH();
+    protectKernel<<<ceilDiv(N, 256), 256, 0, st>>>(t.prefixes.get(), t.parents.get(), t.internalToLeaf.get(),
+                                                    s.nodeOp.get(), N, s.leafOp.get(), s.changed.get());
+    SPHX_CHECK_LAUNCH();
+    size_t bytes = 0;
+    SPHX_HIP(rocprim::exclusive_scan(nullptr, bytes, s.leafOp.get(), s.leafOp.get(), 0, size_t(L) + 1,
+                                     rocprim::plus<int32_t>(), st));
+    SPHX_HIP(rocprim::exclusive_scan(scratchBytes(ws, bytes), bytes, s.leafOp.get(), s.leafOp.get(), 0,
+                                     size_t(L) + 1, rocprim::plus<int32_t>(), st));
+    int32_t  newL    = 0;
+    uint32_t changed = 0;
+    SPHX_HIP(hipMemcpyAsync(&newL, s.leafOp.get() + L, 4, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipMemcpyAsync(&changed, s.changed.get(), 4, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    SPHX_REQUIRE(newL >= 1, SPHX_ERR_STATE, "tree update left no leaf");
+    s.nextLeaves.reserve(size_t(newL) + 1);
+    rebalanceKernel<<<ceilDiv(size_t(L) + 1, 256), 256, 0, st>>>(t.leaves.get(), s.leafOp.get(), L,
+                                                                 s.nextLeaves.get());
+    SPHX_CHECK_LAUNCH();
+
+    // link the new leaf array (its leaf starts as the keys, one per leaf), then count the particles
+    buildLinked(ws, s.nextLeaves.get(), size_t(newL), 1, box);
+    SPHX_REQUIRE(t.numLeafNodes == newL, SPHX_ERR_STATE, "linking the updated leaves changed their number");
+    const uint32_t n = uint32_t(nParticles);
+    leafLayoutKernel<<<ceilDiv(size_t(newL) + 1, 256), 256, 0, st>>>(t.leaves.get(), uint32_t(newL), keys, n,
+                                                                     t.layout.get(), t.counts.get());
+    SPHX_CHECK_LAUNCH();
+    SPHX_HIP(hipStreamSynchronize(st));
+    t.numParticles = nParticles;
+    return changed == 0;
 }
 
 } // namespace sphx

@@ -110,13 +110,24 @@ def search_first_sum(ws, box, f, params, n, rho):
     return converged
 
 
-def std_step(ws, ps, box, params, bucket=64, chain=True):
+def sync_params(params, tree_update):
+    """params for sphx_domain_sync: with tree_update, SPHX_FLAG_TREE_UPDATE (update the previous
+    sync's tree once, as every Domain::sync after the first does)"""
+    if not tree_update:
+        return params
+    p = abi.Params.from_buffer_copy(params)
+    p.flags |= abi.SPHX_FLAG_TREE_UPDATE
+    return p
+
+
+def std_step(ws, ps, box, params, bucket=64, chain=True, tree_update=False):
     """One std-SPH pass; returns (converged, minDtCourant). With chain, the pair passes run with
-    SPHX_FLAG_CHAIN (bitwise the same)."""
+    SPHX_FLAG_CHAIN (bitwise the same); with tree_update, the sync updates the previous sync's tree
+    once instead of converging a new one."""
     n = ps.n
     f = ps.fields()
     cp = abi.chained(params) if chain else params
-    ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
+    ws.domain_sync(box, ps.sync_fields(), sync_params(params, tree_update), n, bucket)
     ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, cp, n, rho=True)
     ws.eos("sphx_eos_std", f, params, 0, n)
@@ -125,14 +136,15 @@ def std_step(ws, ps, box, params, bucket=64, chain=True):
     return converged, dt
 
 
-def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None, chain=True):
+def ve_step(ws, ps, box, params, bucket=64, av_clean=False, ts=None, chain=True, tree_update=False):
     """One VE pass; returns (converged, minDtCourant). With a time-step state ts (abi.Timestep) it
     also sets ts.minDtRho from div v, as HydroVeProp::computeForces does (ve_hydro.hpp:171).
-    With chain, the pair passes run with SPHX_FLAG_CHAIN (bitwise the same)."""
+    With chain, the pair passes run with SPHX_FLAG_CHAIN (bitwise the same); with tree_update, the
+    sync updates the previous sync's tree once."""
     n = ps.n
     f = ps.fields()
     cp = abi.chained(params) if chain else params
-    ws.domain_sync(box, ps.sync_fields(), params, n, bucket)
+    ws.domain_sync(box, ps.sync_fields(), sync_params(params, tree_update), n, bucket)
     ps.reorder_history(ws)
     converged = search_first_sum(ws, box, f, cp, n, rho=False)
     ws.pair("sphx_ve_def_gradh", box, f, cp, 0, n)
@@ -157,20 +169,27 @@ def integrate(ws, ps, box, params, ts):
     ws.update_smoothing_length(f, params, 0, ps.n)
 
 
-def run_steps(ws, ps, box, params, ts, steps, form="std", av_clean=False, refit_box=False, first_sync=True):
+def run_steps(ws, ps, box, params, ts, steps, form="std", av_clean=False, refit_box=False, first_sync=True,
+              tree_update=False, on_step=None):
     """`steps` full time steps (forces, then integrate) of the std or VE propagator on one rank.
     params.minDt follows ts.minDt (the AV switches read it). With refit_box, each step's sync first
     updates `box` in place as Domain::sync does (sphx_global_box: the fitting box on the first sync,
-    then limited shrinking). Returns the Courant step of each."""
+    then limited shrinking). With tree_update, every sync after the first updates the previous tree
+    once (Domain::sync); the first converges it. on_step(step), if given, runs after each step's
+    forces, before its integration. Returns the Courant step of each."""
     out = []
     for step in range(steps):
+        first = first_sync and step == 0
         if refit_box:
-            ws.global_box(box, ps.t["x"], ps.t["y"], ps.t["z"], ps.n, first_sync and step == 0)
+            ws.global_box(box, ps.t["x"], ps.t["y"], ps.t["z"], ps.n, first)
         params.minDt = ts.minDt
+        upd = tree_update and not first
         if form == "std":
-            conv, dtc = std_step(ws, ps, box, params)
+            conv, dtc = std_step(ws, ps, box, params, tree_update=upd)
         else:
-            conv, dtc = ve_step(ws, ps, box, params, av_clean=av_clean, ts=ts)
+            conv, dtc = ve_step(ws, ps, box, params, av_clean=av_clean, ts=ts, tree_update=upd)
+        if on_step is not None:
+            on_step(step)
         if not conv:
             raise RuntimeError("coupled nc/h-updated failed to converge")
         ts.minDtCourant = dtc

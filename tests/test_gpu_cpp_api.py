@@ -72,7 +72,8 @@ def test_cpp_interface_pass(tmp_path, orc, form, calls):
 def test_cpp_interface_steps(tmp_path, orc, periodic):
     """Three full std steps through the C++ interface (Domain::sync ... computeMomentumEnergySTD,
     then computeTimestep, computePositions, updateSmoothingLength) against the oracle's three
-    steps: positions, velocities, h, u and the time-step state. In the open box the domain refits
+    steps, both updating the tree once per sync after the first: positions, velocities, h, u and
+    the time-step state. In the open box the domain refits
     the box at every sync, and so does the oracle."""
     from test_gpu_integrate import _oracle_steps
 
@@ -98,8 +99,9 @@ def test_cpp_interface_steps(tmp_path, orc, periodic):
     for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
         arr[k] = np.zeros(n)
     ts = abi.Timestep.defaults(minDt=1e-4, minDt_m1=1e-4)
+    # the C++ Domain converges the tree on its first sync and updates it once on each later one
     a, _ = _oracle_steps(orc, arr, box, abi.Params.defaults(orc.kernel_K(6.0)), ts, 3, np.float64,
-                         refit_box=not periodic)
+                         refit_box=not periodic, tree_update=True)
     assert abs(minDt - ts.minDt) <= 1e-12 * ts.minDt and abs(minDt_m1 - ts.minDt_m1) <= 1e-12 * ts.minDt_m1
     assert abs(ttot - ts.ttot) <= 1e-12 * ts.ttot
     # the conserved quantities of the final state against the oracle's on its own final state

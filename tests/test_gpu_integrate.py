@@ -173,7 +173,9 @@ def test_domain_reorder_applies_the_sync_permutation(ws):
         ws.domain_reorder([big[:n], big[n // 2: n // 2 + n]], n)
 
 
-def _oracle_steps(orc, arr, box, params, ts, steps, hydro, refit_box=False):
+def _oracle_steps(orc, arr, box, params, ts, steps, hydro, refit_box=False, tree_update=False):
+    """the oracle's std steps; with tree_update every sync after the first updates the previous tree
+    once (Domain::sync) instead of converging a new one"""
     from oracle import pipeline
 
     from test_integrate_oracle import run_positions as orc_positions
@@ -181,12 +183,14 @@ def _oracle_steps(orc, arr, box, params, ts, steps, hydro, refit_box=False):
     a = dict(arr)
     n = len(a["x"])
     dts = []
+    tree = None
     for step in range(steps):
         if refit_box:
             x, y, z = (np.ascontiguousarray(a[k], np.float64) for k in "xyz")
             orc.global_box(C.byref(box), x.ctypes.data, y.ctypes.data, z.ctypes.data, n, int(step == 0))
         params.minDt = ts.minDt
-        o = pipeline.run(orc, a, box, params, form="std", hydro=hydro)
+        o = pipeline.run(orc, a, box, params, form="std", hydro=hydro, prev_tree=tree if tree_update else None)
+        tree = o["tree"]
         a = {k: v for k, v in o["a"].items() if v is not None}
         ts.minDtCourant = o["minDt"]
         orc.compute_timestep(C.byref(abi.fields_from(a)), C.byref(params), 0, n, C.byref(ts))
