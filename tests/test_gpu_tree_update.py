@@ -4,7 +4,7 @@ oracle's updateNsTree (pinned against the reference's focused octree in test_tre
 * A sequence of syncs of a collapsing, rotating particle set: after every sync the sorted keys and the
   whole tree (leaves, counts, layout, prefixes, child offsets, parents, internalToLeaf, level ranges,
   centres, sizes) are bit-identical to the oracle's, and the tree changes at most syncs.
-* Four full FP64 std steps (forces, time step, positions, h) of a converging flow with the tree updated
+* Five full FP64 std steps (forces, time step, positions, h) of a converging flow with the tree updated
   at every sync after the first, as the reference's Domain does: per step the leaves, the neighbour
   counts and every neighbour list entry in order equal the oracle's chain, and the Courant steps agree.
 """
@@ -131,26 +131,35 @@ def _oracle_steps_updated(orc, arr, box, params, ts, steps, record):
 def test_std_steps_with_tree_update_match_oracle(ws, orc):
     import torch
 
-    arr, spec = ic.sedov_lattice(18, seed=23)
+    # 31^3 particles placed at random: about 58 per level-3 leaf, so that many leaves and sibling
+    # groups sit close to the split and merge thresholds of bucket size 64
+    arr, spec = ic.sedov_lattice(31, seed=23)
     n = arr["x"].size
+    rng = np.random.default_rng(5)
+    for k in ("x", "y", "z"):
+        arr[k] = rng.uniform(-0.5, 0.5, n)
     # a converging, rotating flow in cold gas: large displacements per Courant step
     r = np.sqrt(arr["x"] ** 2 + arr["y"] ** 2 + arr["z"] ** 2)
     arr["vx"] = -2.0 * arr["x"] * np.exp(-r * r / 0.08) - 0.5 * arr["y"]
     arr["vy"] = -2.0 * arr["y"] * np.exp(-r * r / 0.08) + 0.5 * arr["x"]
     arr["vz"] = -2.0 * arr["z"] * np.exp(-r * r / 0.08)
     arr["u"] = np.full(n, 1e-4)
-    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
-        arr[k] = np.zeros(n)
+    dt0 = 1e-2
+    # the position update moves particles by x_m1 / dt_m1 (the previous step's displacement), so the
+    # initial displacements carry the velocity field, as the reference's initial conditions set them
+    for k in ("x", "y", "z"):
+        arr[k + "_m1"] = arr["v" + k] * dt0
+    arr["du_m1"] = np.zeros(n)
     box_o = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
     steps = 5
     params64 = abi.Params.defaults(ws.K)
-    ts_o = abi.Timestep.defaults(minDt=1e-3, minDt_m1=1e-3)
+    ts_o = abi.Timestep.defaults(minDt=dt0, minDt_m1=dt0)
     rec_o = []
     _, dts_o = _oracle_steps_updated(orc, arr, box_o, params64, ts_o, steps, rec_o)
 
     box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
     params = abi.Params.defaults(ws.K)
-    ts = abi.Timestep.defaults(minDt=1e-3, minDt_m1=1e-3)
+    ts = abi.Timestep.defaults(minDt=dt0, minDt_m1=dt0)
     ps = ParticleSet.from_host(arr, STD_FIELDS + ("x_m1", "y_m1", "z_m1", "du_m1"))
     ngmax = params.ngmax
     changes = 0
