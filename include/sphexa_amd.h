@@ -250,7 +250,8 @@ int sphx_workspace_destroy(sphx_workspace_t* ws);
  * (sph/include/sph/particles_data.hpp:380-387) + DeviceParticlesData::uploadTables. */
 int sphx_workspace_set_kernel(sphx_workspace_t* ws, double sincIndex, double* K);
 
-/* Human-readable text for the last error recorded on this workspace. */
+/* Human-readable text for the last error recorded on this workspace; with NULL, that of the last
+ * call without a workspace (sphx_sort_keys_noalloc) on the calling thread. */
 const char* sphx_last_error(const sphx_workspace_t* ws);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks and profiling). When on, the
@@ -275,6 +276,18 @@ int sphx_compute_sfc_keys(sphx_workspace_t* ws, const sphx_box_t* box, const dou
  * Replaces sortByKeyGpu / GpuSfcSorter::setMapFromCodes (domain/include/cstone/primitives/
  * primitives_gpu.cu:323-346, primitives/gather.cuh:50-128). */
 int sphx_sort_keys(sphx_workspace_t* ws, uint64_t* keys, uint32_t* order, size_t n);
+
+/* sphx_sort_keys without a workspace and without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same keys and order, bit for bit, as
+ * sphx_sort_keys. Runs on `stream` (a hipStream_t, NULL = default stream) of GPU `device`, which it
+ * makes current. Its temporaries live in a fixed device array of the library, 32 MiB on every GPU
+ * it runs on, held until the process ends: n is limited to SPHX_SORT_NOALLOC_MAX_KEYS, and larger n
+ * returns SPHX_ERR_ARG with the message in sphx_last_error(NULL) (per thread). Calls share that
+ * array per GPU, so calls that overlap on one GPU (other streams, other threads) race on it and
+ * must be serialised by the caller. */
+#define SPHX_SORT_NOALLOC_MAX_KEYS 1048576
+int sphx_sort_keys_noalloc(int device, void* stream, uint64_t* keys, uint32_t* order, size_t n);
 
 /* out[i] = in[order[i]] for elements of `elemBytes` (4 or 8). Replaces gatherGpu
  * (domain/include/cstone/primitives/primitives_gpu.h:34-40). */

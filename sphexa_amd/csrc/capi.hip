@@ -13,11 +13,19 @@ namespace sphx
 namespace
 {
 
+//! the last error of a call without a workspace on this thread (sphx_last_error(NULL)): a workspace
+//! entry point called with NULL, or an entry point that takes a device instead
+thread_local std::string g_lastErrorNoWs;
+
 //! runs an entry point's body with error translation; `writes` = it may change particle fields
 template<class F>
 int guarded(sphx_workspace* ws, F&& body, bool writes = true)
 {
-    if (ws == nullptr) { return SPHX_ERR_ARG; }
+    if (ws == nullptr)
+    {
+        g_lastErrorNoWs = "null workspace";
+        return SPHX_ERR_ARG;
+    }
     if (writes) { ++ws->callSerial; }
     try
     {
@@ -34,6 +42,30 @@ int guarded(sphx_workspace* ws, F&& body, bool writes = true)
     catch (const std::exception& e)
     {
         ws->lastError = e.what();
+        return SPHX_ERR_ARG;
+    }
+}
+
+//! guarded() for the entry points that take a device instead of a workspace
+template<class F>
+int guardedNoWorkspace(int device, F&& body)
+{
+    try
+    {
+        SPHX_REQUIRE(device >= 0, SPHX_ERR_ARG, "negative device index");
+        SPHX_HIP(hipSetDevice(device));
+        int r = body();
+        if (r == SPHX_OK) { g_lastErrorNoWs.clear(); }
+        return r;
+    }
+    catch (const Error& e)
+    {
+        g_lastErrorNoWs = e.msg;
+        return e.code;
+    }
+    catch (const std::exception& e)
+    {
+        g_lastErrorNoWs = e.what();
         return SPHX_ERR_ARG;
     }
 }
@@ -160,7 +192,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     return SPHX_OK;
 }
 
-const char* sphx_last_error(const sphx_workspace_t* ws) { return ws ? ws->lastError.c_str() : "null workspace"; }
+const char* sphx_last_error(const sphx_workspace_t* ws) { return ws ? ws->lastError.c_str() : g_lastErrorNoWs.c_str(); }
 
 int sphx_workspace_set_profiling(sphx_workspace_t* ws, int on)
 {
@@ -221,6 +253,14 @@ int sphx_sort_keys(sphx_workspace_t* ws, uint64_t* keys, uint32_t* order, size_t
         SPHX_REQUIRE(n == 0 || (keys && order), SPHX_ERR_ARG, "null argument");
         SPHX_REQUIRE(n < (size_t(1) << 32), SPHX_ERR_ARG, "more than 2^32-1 particles");
         sortKeys(ws, keys, order, n);
+        return SPHX_OK;
+    });
+}
+
+int sphx_sort_keys_noalloc(int device, void* stream, uint64_t* keys, uint32_t* order, size_t n)
+{
+    return guardedNoWorkspace(device, [&] {
+        sortKeysNoAlloc(device, static_cast<hipStream_t>(stream), keys, order, n);
         return SPHX_OK;
     });
 }

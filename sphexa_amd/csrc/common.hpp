@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -517,6 +518,12 @@ void fill64(sphx_workspace* ws, uint64_t* p, uint64_t v, size_t n);
 void computeKeys(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z, Key* keys,
                  size_t n, bool keepRemoved);
 void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n);
+//! the allocation-free sort: fixed device scratch of kNoAllocScratchBytes per GPU, at most kNoAllocMaxKeys keys
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocMaxKeys      = size_t(1) << 20;
+static_assert(kNoAllocMaxKeys == SPHX_SORT_NOALLOC_MAX_KEYS, "the C header states the same limit");
+constexpr int    kMaxDevices          = 64;
+void sortKeysNoAlloc(int device, hipStream_t stream, Key* keys, uint32_t* order, size_t n);
 //! spatial target groups of [first, last) into the workspace (computeGroupSplits)
 void computeGroups(sphx_workspace* ws, const DevBox& box, const double* x, const double* y, const double* z,
                    uint32_t first, uint32_t last, float tolFactor, sphx_groups_t* out);

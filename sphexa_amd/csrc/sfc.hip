@@ -98,6 +98,62 @@ void sortKeys(sphx_workspace* ws, Key* keys, uint32_t* order, size_t n)
     SPHX_HIP(hipMemcpyAsync(keys, ws->keyTmp.get(), n * sizeof(Key), hipMemcpyDeviceToDevice, ws->stream));
 }
 
+} // namespace sphx
+
+/* Fixed scratch of the allocation-free sort (sortKeysNoAlloc): the sorted-key buffer, then the
+ * radix sort's temporaries. Every GPU the library runs on holds its own copy for the life of the
+ * process. In host code `scratch` names a host-side stand-in, not device memory: its device
+ * address comes from hipGetSymbolAddress, once per device (noAllocScratch). */
+static __device__ __align__(256) char scratch[sphx::kNoAllocScratchBytes];
+
+namespace sphx
+{
+
+namespace
+{
+std::mutex g_scratchMutex;
+void*      g_scratchPtr[kMaxDevices] = {}; // device address of `scratch` per device, once looked up
+} // namespace
+
+//! the device address of `scratch` on `device` (the current device), looked up on first use
+void* noAllocScratch(int device)
+{
+    SPHX_REQUIRE(device >= 0 && device < kMaxDevices, SPHX_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[device] == nullptr)
+    {
+        void* ptr = nullptr;
+        SPHX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        SPHX_REQUIRE(ptr != nullptr, SPHX_ERR_HIP, "no device address for the sort scratch");
+        g_scratchPtr[device] = ptr;
+    }
+    return g_scratchPtr[device];
+}
+
+/* sortKeys with the fixed scratch instead of workspace buffers: no device allocation. The sorted keys
+ * go to the scratch's first n*8 bytes (rounded up to 256), the radix sort's temporaries after them,
+ * and the keys are copied back, as sortKeys does. n is limited to kNoAllocMaxKeys. */
+void sortKeysNoAlloc(int device, hipStream_t stream, Key* keys, uint32_t* order, size_t n)
+{
+    SPHX_REQUIRE(n <= kNoAllocMaxKeys, SPHX_ERR_ARG,
+                 "sort without allocation: " + std::to_string(n) + " keys exceed the maximum of " +
+                     std::to_string(kNoAllocMaxKeys));
+    SPHX_REQUIRE(n == 0 || (keys && order), SPHX_ERR_ARG, "null keys or order");
+    char* base = static_cast<char*>(noAllocScratch(device));
+    if (n == 0) { return; }
+    const rocprim::counting_iterator<uint32_t> iota(0);
+    Key*         keyTmp   = reinterpret_cast<Key*>(base);
+    const size_t keyBytes = (n * sizeof(Key) + 255) & ~size_t(255);
+    size_t       tmpBytes = 0;
+    SPHX_HIP(rocprim::radix_sort_pairs(nullptr, tmpBytes, keys, keyTmp, iota, order, n, 0, 64, stream));
+    SPHX_REQUIRE(keyBytes + tmpBytes <= kNoAllocScratchBytes, SPHX_ERR_ARG,
+                 "sort without allocation: " + std::to_string(n) + " keys need " +
+                     std::to_string(keyBytes + tmpBytes) + " bytes of scratch, more than " +
+                     std::to_string(kNoAllocScratchBytes));
+    SPHX_HIP(rocprim::radix_sort_pairs(base + keyBytes, tmpBytes, keys, keyTmp, iota, order, n, 0, 64, stream));
+    SPHX_HIP(hipMemcpyAsync(keys, keyTmp, n * sizeof(Key), hipMemcpyDeviceToDevice, stream));
+}
+
 template<class E>
 __global__ void __launch_bounds__(256) gatherKernel(const uint32_t* __restrict__ order, const E* __restrict__ in,
                                                     E* __restrict__ out, size_t n)

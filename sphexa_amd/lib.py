@@ -29,6 +29,7 @@ _SIGS = {
     "sphx_workspace_set_kernel": [_P, C.c_double, C.POINTER(C.c_double)],
     "sphx_compute_sfc_keys": [_P, _BOXP, _P, _P, _P, _P, _SZ],
     "sphx_sort_keys": [_P, _P, _P, _SZ],
+    "sphx_sort_keys_noalloc": [C.c_int, _P, _P, _P, _SZ],
     "sphx_gather": [_P, _P, _P, _P, _SZ, C.c_int],
     "sphx_build_tree": [_P, _P, _SZ, _U32, _BOXP],
     "sphx_tree_view": [_P, C.POINTER(abi.TreeView)],
