@@ -322,10 +322,10 @@ void launchDensity(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f
         launchWide<T>(ws->nbWide, [&](auto wide) {
             constexpr bool W = decltype(wide)::value;
             if (toRho)
-                xmassKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                xmassKernel<T, true, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_XMASS) : 0, ws->stream>>>(
                     a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)out, nullptr, nextVol);
             else
-                xmassKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+                xmassKernel<T, false, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_XMASS) : 0, ws->stream>>>(
                     a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)out, next, nullptr);
         });
         SPHX_CHECK_LAUNCH();
@@ -356,7 +356,7 @@ void launchIadStd(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f,
     {
         KernelScope timer(ws, "iad_std");
         launchWide<T>(ws->nbWide, [&](auto wide) {
-            iadStdKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            iadStdKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_IAD) : 0, ws->stream>>>(
                 a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->c11, (T*)f->c12, (T*)f->c13, (T*)f->c22,
                 (T*)f->c23, (T*)f->c33, nf, next);
         });
@@ -382,10 +382,10 @@ void launchMomentumStd(sphx_workspace* ws, const PairArgs& a, const MomStdFields
     launchWide<T>(ws->nbWide, [&](auto wide) {
         constexpr bool W = decltype(wide)::value;
         if (perAlpha)
-            momentumStdKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws),
+            momentumStdKernel<T, true, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_MOM) : 0, ws->stream>>>(a, mf, rec, sincKernel<T>(ws),
                                                                                         Kcour, dt);
         else
-            momentumStdKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, mf, rec,
+            momentumStdKernel<T, false, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_MOM) : 0, ws->stream>>>(a, mf, rec,
                                                                                          sincKernel<T>(ws), Kcour, dt);
     });
     SPHX_CHECK_LAUNCH();

@@ -597,7 +597,7 @@ void launchGradh(sphx_workspace* ws, const PairArgs& a, const sphx_fields_t* f, 
     {
         KernelScope timer(ws, "ve_def_gradh");
         launchWide<T>(ws->nbWide, [&](auto wide) {
-            veDefGradhKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            veDefGradhKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_GRADH) : 0, ws->stream>>>(
                 a, rec, (const T*)f->h, sincKernel<T>(ws), (T*)f->kx, (T*)f->gradh, next, (const T*)f->vx,
                 (const T*)f->vy, (const T*)f->vz);
         });
@@ -624,7 +624,7 @@ void launchIadVe(sphx_workspace* ws, const PairArgs& a, const IadVeFields& vf, c
     {
         KernelScope timer(ws, "iad_divv_curlv");
         launchWide<T>(ws->nbWide, [&](auto wide) {
-            iadDivvCurlvKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            iadDivvCurlvKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_IAD) : 0, ws->stream>>>(
                 a, vf, rec, sincKernel<T>(ws), next, (const T*)f->c);
         });
         SPHX_CHECK_LAUNCH();
@@ -651,7 +651,7 @@ void launchAv(sphx_workspace* ws, const PairArgs& a, const AvFields& af, const s
     {
         KernelScope timer(ws, "av_switches");
         launchWide<T>(ws->nbWide, [&](auto wide) {
-            avSwitchesKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            avSwitchesKernel<T, decltype(wide)::value><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_AV) : 0, ws->stream>>>(
                 a, af, rec, sincKernel<T>(ws), p->minDt, T(p->alphamin), T(p->alphamax), T(p->decay_constant), next);
         });
         SPHX_CHECK_LAUNCH();
@@ -676,10 +676,10 @@ void launchMomVe(sphx_workspace* ws, const PairArgs& a, const MomVeFields& mf, c
     launchWide<T>(ws->nbWide, [&](auto wide) {
         constexpr bool W = decltype(wide)::value;
         if (avClean)
-            momentumVeKernel<T, true, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            momentumVeKernel<T, true, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_MOM) : 0, ws->stream>>>(
                 a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
         else
-            momentumVeKernel<T, false, W><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(
+            momentumVeKernel<T, false, W><<<blocks, kPairWaves * kWave, isMixed<T> ? occupancyCap(SPHX_CAP_MOM) : 0, ws->stream>>>(
                 a, mf, rec, sincKernel<T>(ws), T(p->Atmin), T(p->Atmax), T(p->ramp), float(p->Kcour), dt);
     });
     SPHX_CHECK_LAUNCH();

@@ -328,7 +328,7 @@ __device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t
  *    window's loads are in flight. The candidate order is therefore the traversal order, as in
  *    the reference. */
 template<class Sink>
-__device__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, const int32_t* leaves, int nLeaf,
+__device__ __forceinline__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, const int32_t* leaves, int nLeaf,
                             const double gc[3],
                             const double ge[3], double reach, unsigned& count, ListWriter& out, Sink& sink)
 {
@@ -572,7 +572,7 @@ __device__ __forceinline__ void sortFoundLeaves(WaveLds& lds, int n)
  * (<= 7*20 = 140 < 160) on top of the stack it started from. A single pop adds at most one leaf,
  * the first in order of all, and the buffer is swept whenever it reaches its spare slots. */
 template<class Sink>
-__device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out,
+__device__ __forceinline__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out,
                                     Sink& sink)
 {
 #pragma clang fp contract(off)
@@ -697,7 +697,7 @@ __device__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double
 
 //! search of the active lanes' targets into their lists (slot 0 at myList); returns the counts
 template<class Sink>
-__device__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList,
+__device__ __forceinline__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList,
                                 Sink& sink, uint32_t& swept)
 {
     ListWriter out{myList};

@@ -110,6 +110,29 @@ __device__ __forceinline__ Lane laneTarget(const Lists& L)
 
 inline unsigned pairBlocks(const Lists& L) { return ceilDiv(L.groups.num, kPairWaves); }
 
+/*! Dynamic LDS per 4-wave block that caps a pair loop at `waves` waves per SIMD (0: no cap): the CU's
+ * 160 KB then holds at most `waves` blocks. A gather-bound loop can run faster with fewer waves in
+ * flight than its register use allows, since more waves evict each other's records from L1. Caps per
+ * loop family of the FP32 form, from a sweep of 2-6 at 8M particles: AV switches 3 (2.98 -> 2.75 ms,
+ * 4 waves by registers otherwise), XMass 6 (1.27 -> 1.22 ms, 8 otherwise); the others ran as fast or
+ * faster uncapped. */
+inline size_t occupancyCap(int waves) { return waves > 0 ? size_t(160 * 1024) / size_t(waves) : 0; }
+#ifndef SPHX_CAP_XMASS
+#define SPHX_CAP_XMASS 6
+#endif
+#ifndef SPHX_CAP_GRADH
+#define SPHX_CAP_GRADH 0
+#endif
+#ifndef SPHX_CAP_IAD
+#define SPHX_CAP_IAD 0
+#endif
+#ifndef SPHX_CAP_AV
+#define SPHX_CAP_AV 3
+#endif
+#ifndef SPHX_CAP_MOM
+#define SPHX_CAP_MOM 0
+#endif
+
 //! lt::lookup (sph/include/sph/table_lookup.hpp:10-26): linear interpolation of a 20000-point
 //! table on [0,2], zero beyond the support. The table is stored per interval (both end values
 //! adjacent), so one lookup is one vector gather instead of two scalar ones. Interval 19999, one
