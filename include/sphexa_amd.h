@@ -506,8 +506,11 @@ int sphx_conserved_quantities(sphx_workspace_t* ws, const sphx_fields_t* f, cons
  * and inverse problems. Each recomputes its pair terms from the neighbour lists of the last
  * sphx_find_neighbors (the lists are a checkpoint: they are held fixed, as is the h of the
  * forward). Kernel derivatives use the tabulated dW/dv interpolated like W in the FP64 form and
- * are evaluated like W in the FP32 form, as the forward loops do. Gradients to neighbours are
- * scattered with atomic adds, so FP results can vary in the last bits. */
+ * are evaluated like W in the FP32 form, as the forward loops do. A pair's derivatives for its
+ * neighbour go to a per-pair buffer of the workspace and are summed per particle in a fixed order
+ * (no atomics), so repeated calls give the same bits. The first adjoint call after a search builds
+ * that pair index (one sort of the search's pairs); the adjoints run over consecutive groups (no
+ * group view set). */
 
 /* rho = K h^-3 (m_i + sum_j m_j W(r_ij/h_i)) (sphx_density_std).
  * in: rho. out: x, y, z, h. fields: x, y, z, h, m, nc. */

@@ -2,11 +2,15 @@
 //
 // No counterpart in the reference. Each kernel re-walks the neighbour lists of the last search
 // (held fixed: they are the checkpoint of the forward), recomputes the pair terms of its forward
-// loop and accumulates the reverse-mode derivatives: the target's own in registers, the
-// neighbour's with an atomic add (a pair gradient scatters to j, and neighbour relations are not
-// symmetric). Kernel derivatives use the tabulated dW/dv interpolated like W. Launch shape and
-// list layout as the forward loops (pairs.hpp). Derivations, with r = x_i - x_j after the fold,
-// d = |r|, v = d / h_i:
+// loop and accumulates the reverse-mode derivatives. A pair term of target i and neighbour j has
+// derivatives for both: the target's are summed in registers and added to its gradient once; the
+// neighbour's are written to the pair's record in a per-pair buffer (no atomics), and a second
+// kernel sums, for every particle j, the records of the pairs that have j as neighbour, in the
+// order of their pair index (AdjointPairs: target i's pairs are p = pairStart[i] + k, and a sort of
+// (j, p) keys gives each j its pairs in increasing p). Results are therefore the same bit for bit
+// from run to run. Kernel derivatives use the tabulated dW/dv interpolated like W (FP64) or are
+// evaluated like W (FP32). Launch shape and list layout as the forward loops (pairs.hpp).
+// Derivations, with r = x_i - x_j after the fold, d = |r|, v = d / h_i:
 //
 //  density   rho_i = K h_i^-3 (m_i + sum_j m_j W(v))
 //            d/dr = K h_i^-4 m_j W'(v) r/d,   d/dh_i = -K h_i^-4 (3 rho0_i + sum_j m_j v W'(v))
@@ -18,6 +22,9 @@
 //  momentum  reverse mode through the pair term of hydro_std/momentum_energy_kern.hpp:11-134 with
 //            a per-particle AV coefficient (the reference's is the constant 1):
 //            a_i = K sum_j (aa C_i r + bb C_j r), du_i = -K/2 sum_j v_ij.(ae C_i r + be C_j r)
+#include <cstring> // rocprim uses memset without including it
+#include <rocprim/rocprim.hpp>
+
 #include "pairs.hpp"
 
 namespace sphx
@@ -26,11 +33,26 @@ namespace sphx
 namespace
 {
 
+//! the target's own gradient: one lane per target, so a plain read-modify-write
 template<class T>
-__device__ __forceinline__ void scatter(void* p, uint32_t j, T v)
+__device__ __forceinline__ void addOwn(void* p, uint32_t i, T v)
 {
-    atomicAdd(static_cast<T*>(p) + j, v);
+    static_cast<T*>(p)[i] += v;
 }
+
+//! where the pair records of a call go: record p (NQ values of T) at buf + p * NQ
+template<class T>
+struct PairOut
+{
+    T*              buf;
+    const uint32_t* pairStart; // per searched target, from the searched range's first
+    uint32_t        nbFirst;
+
+    __device__ __forceinline__ T* record(uint32_t i, unsigned k, int nq) const
+    {
+        return buf + size_t(pairStart[i - nbFirst] + k) * size_t(nq);
+    }
+};
 
 template<class T>
 __device__ __forceinline__ T pow4Inv(T hInv)
@@ -45,9 +67,11 @@ struct DensityVjpArgs
     void *      gx, *gy, *gz, *gh;
 };
 
+constexpr int kDensityNq = 3; // neighbour-side record: gx, gy, gz
+
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    densityStdVjpKernel(PairArgs a, DensityVjpArgs f, SincKernel<T> kern)
+    densityStdVjpKernel(PairArgs a, DensityVjpArgs f, SincKernel<T> kern, PairOut<T> out)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -71,16 +95,14 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             T mj = ld<T>(f.m, j);
             rho0 += mj * w;
             sumVdW += mj * v * dw;
-            if (d > T(0))
-            {
-                T c = coef * mj * dw / d;
-                gxi += c * rx, gyi += c * ry, gzi += c * rz;
-                scatter<T>(f.gx, j, -c * rx), scatter<T>(f.gy, j, -c * ry), scatter<T>(f.gz, j, -c * rz);
-            }
+            T c = d > T(0) ? coef * mj * dw / d : T(0);
+            gxi += c * rx, gyi += c * ry, gzi += c * rz;
+            T* rec = out.record(i, k, kDensityNq);
+            rec[0] = -c * rx, rec[1] = -c * ry, rec[2] = -c * rz;
         }
     });
-    scatter<T>(f.gx, i, gxi), scatter<T>(f.gy, i, gyi), scatter<T>(f.gz, i, gzi);
-    scatter<T>(f.gh, i, -coef * (T(3) * rho0 + sumVdW));
+    addOwn<T>(f.gx, i, gxi), addOwn<T>(f.gy, i, gyi), addOwn<T>(f.gz, i, gzi);
+    addOwn<T>(f.gh, i, -coef * (T(3) * rho0 + sumVdW));
 }
 
 struct IadVjpArgs
@@ -89,9 +111,11 @@ struct IadVjpArgs
     void *      gx, *gy, *gz, *gh, *grho;
 };
 
+constexpr int kIadNq = 4; // neighbour-side record: gx, gy, gz, grho
+
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    iadStdVjpKernel(PairArgs a, IadVjpArgs f, SincKernel<T> kern)
+    iadStdVjpKernel(PairArgs a, IadVjpArgs f, SincKernel<T> kern, PairOut<T> out)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -140,13 +164,13 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             for (int p = 0; p < 3; ++p)
                 dr[p] = radial * r[p] + T(2) * Vj * w * Br[p];
             gxi += dr[0], gyi += dr[1], gzi += dr[2];
-            scatter<T>(f.gx, j, -dr[0]), scatter<T>(f.gy, j, -dr[1]), scatter<T>(f.gz, j, -dr[2]);
             ghi += -Vj * dw * q * d * hInv * hInv;
-            scatter<T>(f.grho, j, -mj / (rhoj * rhoj) * w * q);
+            T* rec = out.record(i, k, kIadNq);
+            rec[0] = -dr[0], rec[1] = -dr[1], rec[2] = -dr[2], rec[3] = -mj / (rhoj * rhoj) * w * q;
         }
     });
-    scatter<T>(f.gx, i, gxi), scatter<T>(f.gy, i, gyi), scatter<T>(f.gz, i, gzi);
-    scatter<T>(f.gh, i, ghi);
+    addOwn<T>(f.gx, i, gxi), addOwn<T>(f.gy, i, gyi), addOwn<T>(f.gz, i, gzi);
+    addOwn<T>(f.gh, i, ghi);
 }
 
 struct MomVjpArgs
@@ -178,9 +202,12 @@ __device__ __forceinline__ void symOuter(const T u[3], const T r[3], T out[6])
     out[5] = u[2] * r[2];
 }
 
+// neighbour-side record: gx, gy, gz, gh, grho, gp, galpha, gC[6]
+constexpr int kMomNq = 13;
+
 template<class T>
 __global__ void __launch_bounds__(kPairWaves * kWave)
-    momentumStdVjpKernel(PairArgs a, MomVjpArgs f, SincKernel<T> kern)
+    momentumStdVjpKernel(PairArgs a, MomVjpArgs f, SincKernel<T> kern, PairOut<T> out)
 {
     Lane t = laneTarget(a.L);
     if (!t.valid) { return; }
@@ -257,13 +284,14 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
             for (int p = 0; p < 3; ++p)
                 rBar[p] = Ci[0][p] * tAiBar[0] + Ci[1][p] * tAiBar[1] + Ci[2][p] * tAiBar[2] + Cj[0][p] * tAjBar[0] +
                           Cj[1][p] * tAjBar[1] + Cj[2][p] * tAjBar[2];
-            T ci6[6], cj6[6];
+            T* rec = out.record(i, k, kMomNq);
+            T  ci6[6], cj6[6];
             symOuter(tAiBar, r, ci6);
             symOuter(tAjBar, r, cj6);
             for (int q = 0; q < 6; ++q)
             {
                 gC_i[q] += ci6[q];
-                scatter<T>(f.gC[q], j, cj6[q]);
+                rec[7 + q] = cj6[q];
             }
 
             T WiBar    = aaBar * (mj * pro_i + visc * vol_i) + aeBar * (T(2) * mj * pro_i + visc * vol_i);
@@ -276,7 +304,7 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
 
             T wBar = viscBar * dvisc_dw;
             galpha_i += viscBar * dvisc_dalpha;
-            scatter<T>(f.galpha, j, viscBar * dvisc_dalpha);
+            rec[6] = viscBar * dvisc_dalpha;
             if (d > T(0))
             {
                 T dBar = WiBar * hiInv3 * hiInv * dW1 + WjBar * hjInv3 * hjInv * dW2;
@@ -284,21 +312,186 @@ __global__ void __launch_bounds__(kPairWaves * kWave)
                     rBar[p] += dBar * r[p] / d + wBar * (vij[p] / d - w * r[p] / (d * d));
             }
             gh_i += -WiBar * hiInv3 * hiInv * (T(3) * W1 + v1 * dW1);
-            scatter<T>(f.gh, j, -WjBar * hjInv3 * hjInv * (T(3) * W2 + v2 * dW2));
+            rec[3] = -WjBar * hjInv3 * hjInv * (T(3) * W2 + v2 * dW2);
 
             gp_i += proBar_i / (roi * roi);
             grho_i += -T(2) * proBar_i * pri / (roi * roi * roi) - volBar_i * mi / (roi * roi);
-            scatter<T>(f.gp, j, prjBar / roj);
-            scatter<T>(f.grho, j, -volBar_j * mj / (roj * roj) - prjBar * pj / (roj * roj));
+            rec[5] = prjBar / roj;
+            rec[4] = -volBar_j * mj / (roj * roj) - prjBar * pj / (roj * roj);
 
             gx_i += rBar[0], gy_i += rBar[1], gz_i += rBar[2];
-            scatter<T>(f.gx, j, -rBar[0]), scatter<T>(f.gy, j, -rBar[1]), scatter<T>(f.gz, j, -rBar[2]);
+            rec[0] = -rBar[0], rec[1] = -rBar[1], rec[2] = -rBar[2];
         }
     });
-    scatter<T>(f.gx, i, gx_i), scatter<T>(f.gy, i, gy_i), scatter<T>(f.gz, i, gz_i), scatter<T>(f.gh, i, gh_i);
-    scatter<T>(f.grho, i, grho_i), scatter<T>(f.gp, i, gp_i), scatter<T>(f.galpha, i, galpha_i);
+    addOwn<T>(f.gx, i, gx_i), addOwn<T>(f.gy, i, gy_i), addOwn<T>(f.gz, i, gz_i), addOwn<T>(f.gh, i, gh_i);
+    addOwn<T>(f.grho, i, grho_i), addOwn<T>(f.gp, i, gp_i), addOwn<T>(f.galpha, i, galpha_i);
     for (int q = 0; q < 6; ++q)
-        scatter<T>(f.gC[q], i, gC_i[q]);
+        addOwn<T>(f.gC[q], i, gC_i[q]);
+}
+
+/* ---- the pair index of the adjoints --------------------------------------------------------------- */
+
+//! stored pairs of each searched target (the last entry 0, so the exclusive scan ends with the total)
+__global__ void pairCountKernel(const uint32_t* __restrict__ nc, uint32_t nbFirst, uint32_t nt, uint32_t ngmax,
+                                uint32_t* __restrict__ cnt)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) { cnt[t] = min(nc[nbFirst + t] - 1u, ngmax); }
+    else if (t == nt) { cnt[t] = 0; }
+}
+
+//! key (j << pbits) | p of every stored pair p = pairStart[t] + k, j the neighbour in slot k of target t
+__global__ void pairKeysKernel(const uint32_t* __restrict__ nbrs, const uint32_t* __restrict__ pairStart, uint32_t nt,
+                               uint32_t ngmax4, int pbits, uint64_t* __restrict__ keys)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) { return; }
+    const uint32_t p0 = pairStart[t], n = pairStart[t + 1] - p0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const uint32_t j = nbrs[listOffset(t / kGroup, ngmax4, k, t % kGroup)];
+        keys[p0 + k]     = (uint64_t(j) << pbits) | uint64_t(p0 + k);
+    }
+}
+
+//! rowStart[j] = first sorted key of neighbour j (lower bound of j << pbits), j = 0 .. n
+__global__ void rowStartKernel(const uint64_t* __restrict__ sorted, uint64_t pairs, int pbits, uint32_t n,
+                               uint32_t* __restrict__ rowStart)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) { return; }
+    const uint64_t key = uint64_t(j) << pbits;
+    uint64_t       lo = 0, hi = pairs;
+    while (lo < hi)
+    {
+        const uint64_t mid = (lo + hi) / 2;
+        if (sorted[mid] < key) { lo = mid + 1; }
+        else { hi = mid; }
+    }
+    rowStart[j] = uint32_t(lo);
+}
+
+template<class T, int NQ>
+struct PullOut
+{
+    T* g[NQ];
+};
+
+/*! g_q[j] += sum of field q of the records of the pairs that have j as neighbour, in increasing pair index,
+ * counting only the pairs of targets [tFirst, tLast) of the call (pair indices [pairStart[tFirst],
+ * pairStart[tLast])) */
+template<class T, int NQ>
+__global__ void __launch_bounds__(256) pullKernel(const uint64_t* __restrict__ sorted, const uint32_t* __restrict__ rowStart,
+                                                  const uint32_t* __restrict__ pairStart, uint32_t tFirst, uint32_t tLast,
+                                                  uint64_t pmask, const T* __restrict__ buf, PullOut<T, NQ> out, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) { return; }
+    const uint64_t pLo = pairStart[tFirst], pHi = pairStart[tLast];
+    T              acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        acc[q] = T(0);
+    for (uint32_t e = rowStart[j]; e < rowStart[j + 1]; ++e)
+    {
+        const uint64_t p = sorted[e] & pmask;
+        if (p < pLo || p >= pHi) { continue; }
+        const T* rec = buf + p * NQ;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc[q] += rec[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        out.g[q][j] += acc[q];
+}
+
+int bitsFor(uint64_t v)
+{
+    int b = 1;
+    while (b < 64 && (v >> b) != 0) { ++b; }
+    return b;
+}
+
+//! the pair index of the last search (built once per search, on the first adjoint call after it)
+void buildAdjointPairs(sphx_workspace* ws)
+{
+    auto& A = ws->adj;
+    if (A.serial == ws->searchSerial) { return; }
+    const uint32_t nt = ws->nbLast - ws->nbFirst;
+    const uint32_t n  = uint32_t(ws->tree.numParticles);
+    SPHX_REQUIRE(uint64_t(nt) * ws->nbNgmax < (uint64_t(1) << 32), SPHX_ERR_ARG,
+                 "adjoints: more than 2^32 neighbour pairs");
+    A.pairStart.reserve(size_t(nt) + 1);
+    // the counts, then the scan's temporaries, in the workspace scratch
+    size_t tmpBytes = 0;
+    SPHX_HIP(rocprim::exclusive_scan(nullptr, tmpBytes, static_cast<const uint32_t*>(nullptr), A.pairStart.get(), 0u,
+                                     size_t(nt) + 1, rocprim::plus<uint32_t>(), ws->stream));
+    const size_t cntBytes = ((size_t(nt) + 1) * sizeof(uint32_t) + 255) & ~size_t(255);
+    char*        base     = static_cast<char*>(scratchBytes(ws, cntBytes + tmpBytes));
+    uint32_t*    cnt      = reinterpret_cast<uint32_t*>(base);
+    pairCountKernel<<<ceilDiv(size_t(nt) + 1, 256), 256, 0, ws->stream>>>(ws->nbNc, ws->nbFirst, nt, ws->nbNgmax, cnt);
+    SPHX_CHECK_LAUNCH();
+    SPHX_HIP(rocprim::exclusive_scan(base + cntBytes, tmpBytes, cnt, A.pairStart.get(), 0u, size_t(nt) + 1,
+                                     rocprim::plus<uint32_t>(), ws->stream));
+    uint32_t pairs = 0;
+    SPHX_HIP(hipMemcpyAsync(&pairs, A.pairStart.get() + nt, sizeof(pairs), hipMemcpyDeviceToHost, ws->stream));
+    SPHX_HIP(hipStreamSynchronize(ws->stream));
+    const int pbits = bitsFor(pairs), jbits = bitsFor(n);
+    A.keys.reserve(std::max<uint64_t>(pairs, 1));
+    A.sorted.reserve(std::max<uint64_t>(pairs, 1));
+    A.rowStart.reserve(size_t(n) + 1);
+    if (pairs > 0)
+    {
+        pairKeysKernel<<<ceilDiv(nt, 256), 256, 0, ws->stream>>>(ws->nbrs.get(), A.pairStart.get(), nt,
+                                                                  paddedNgmax(ws->nbNgmax), pbits, A.keys.get());
+        SPHX_CHECK_LAUNCH();
+        size_t sortBytes = 0;
+        SPHX_HIP(rocprim::radix_sort_keys(nullptr, sortBytes, A.keys.get(), A.sorted.get(), size_t(pairs), 0,
+                                          pbits + jbits, ws->stream));
+        void* tmp = scratchBytes(ws, sortBytes);
+        SPHX_HIP(rocprim::radix_sort_keys(tmp, sortBytes, A.keys.get(), A.sorted.get(), size_t(pairs), 0,
+                                          pbits + jbits, ws->stream));
+    }
+    rowStartKernel<<<ceilDiv(size_t(n) + 1, 256), 256, 0, ws->stream>>>(A.sorted.get(), pairs, pbits, n,
+                                                                        A.rowStart.get());
+    SPHX_CHECK_LAUNCH();
+    A.pairs  = pairs;
+    A.pbits  = pbits;
+    A.serial = ws->searchSerial;
+}
+
+//! the pair records of a call with nq values of T per pair
+template<class T>
+PairOut<T> pairOut(sphx_workspace* ws, int nq)
+{
+    auto& A = ws->adj;
+    A.pairBuf.reserve(std::max<uint64_t>(A.pairs, 1) * nq * sizeof(T));
+    return PairOut<T>{reinterpret_cast<T*>(A.pairBuf.get()), A.pairStart.get(), ws->nbFirst};
+}
+
+//! add every particle's neighbour-side sums of the call over targets [first, last) to g[0..NQ)
+template<class T, int NQ>
+void pull(sphx_workspace* ws, const PairOut<T>& po, uint32_t first, uint32_t last, void* const (&g)[NQ])
+{
+    const auto&    A = ws->adj;
+    const uint32_t n = uint32_t(ws->tree.numParticles);
+    PullOut<T, NQ> out;
+    for (int q = 0; q < NQ; ++q)
+        out.g[q] = static_cast<T*>(g[q]);
+    const uint64_t pmask = (A.pbits >= 64) ? ~uint64_t(0) : ((uint64_t(1) << A.pbits) - 1);
+    if (n == 0) { return; }
+    pullKernel<T, NQ><<<ceilDiv(n, 256), 256, 0, ws->stream>>>(A.sorted.get(), A.rowStart.get(), A.pairStart.get(),
+                                                                first - ws->nbFirst, last - ws->nbFirst, pmask,
+                                                                po.buf, out, n);
+    SPHX_CHECK_LAUNCH();
+}
+
+//! the adjoints take consecutive groups of 64 (no group view), the layout of their pair index
+void requireConsecutive(const PairArgs& a)
+{
+    SPHX_REQUIRE(a.L.groups.start == nullptr, SPHX_ERR_ARG,
+                 "the adjoints run over consecutive groups; clear the group view (sphx_set_groups with NULL) first");
 }
 
 void requireAll(std::initializer_list<const void*> ptrs, const char* what)
@@ -315,14 +508,21 @@ void densityStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f
     PairArgs a = makePairArgs(ws, box, f, p, first, last);
     requireAll({f->h, f->m, in->rho, out->x, out->y, out->z, out->h},
                "density adjoint needs fields h, m, cotangent rho and gradients x, y, z, h");
+    requireConsecutive(a);
     DensityVjpArgs d{f->h, f->m, in->rho, out->x, out->y, out->z, out->h};
     unsigned       blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
-    if (p->hydro_type == SPHX_F32)
-        densityStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
-    else
-        densityStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
-    SPHX_CHECK_LAUNCH();
+    buildAdjointPairs(ws);
+    void* const g[kDensityNq] = {out->x, out->y, out->z};
+    auto        run           = [&](auto zero) {
+        using T      = decltype(zero);
+        PairOut<T> o = pairOut<T>(ws, kDensityNq);
+        densityStdVjpKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<T>(ws), o);
+        SPHX_CHECK_LAUNCH();
+        pull<T, kDensityNq>(ws, o, first, last, g);
+    };
+    if (p->hydro_type == SPHX_F32) { run(0.0f); }
+    else { run(0.0); }
 }
 
 void iadStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
@@ -343,13 +543,20 @@ void iadStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, co
                  out->z,
                  out->h,
                  out->rho};
+    requireConsecutive(a);
     unsigned blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
-    if (p->hydro_type == SPHX_F32)
-        iadStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
-    else
-        iadStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
-    SPHX_CHECK_LAUNCH();
+    buildAdjointPairs(ws);
+    void* const g[kIadNq] = {out->x, out->y, out->z, out->rho};
+    auto        run       = [&](auto zero) {
+        using T      = decltype(zero);
+        PairOut<T> o = pairOut<T>(ws, kIadNq);
+        iadStdVjpKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<T>(ws), o);
+        SPHX_CHECK_LAUNCH();
+        pull<T, kIadNq>(ws, o, first, last, g);
+    };
+    if (p->hydro_type == SPHX_F32) { run(0.0f); }
+    else { run(0.0); }
 }
 
 void momentumEnergyStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f, const sphx_params_t* p,
@@ -386,13 +593,21 @@ void momentumEnergyStdVjp(sphx_workspace* ws, const DevBox& box, const sphx_fiel
                  out->p,
                  out->alpha,
                  {out->c11, out->c12, out->c13, out->c22, out->c23, out->c33}};
+    requireConsecutive(a);
     unsigned blocks = pairBlocks(a.L);
     if (blocks == 0) { return; }
-    if (p->hydro_type == SPHX_F32)
-        momentumStdVjpKernel<float><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<float>(ws));
-    else
-        momentumStdVjpKernel<double><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<double>(ws));
-    SPHX_CHECK_LAUNCH();
+    buildAdjointPairs(ws);
+    void* const g[kMomNq] = {out->x,   out->y,   out->z,   out->h,   out->rho, out->p,  out->alpha,
+                             out->c11, out->c12, out->c13, out->c22, out->c23, out->c33};
+    auto        run       = [&](auto zero) {
+        using T      = decltype(zero);
+        PairOut<T> o = pairOut<T>(ws, kMomNq);
+        momentumStdVjpKernel<T><<<blocks, kPairWaves * kWave, 0, ws->stream>>>(a, d, sincKernel<T>(ws), o);
+        SPHX_CHECK_LAUNCH();
+        pull<T, kMomNq>(ws, o, first, last, g);
+    };
+    if (p->hydro_type == SPHX_F32) { run(0.0f); }
+    else { run(0.0); }
 }
 
 } // namespace sphx

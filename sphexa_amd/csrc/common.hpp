@@ -245,6 +245,21 @@ struct sphx_workspace
     sphx::DevBuf<uint32_t> hostVisible;  // (unused placeholder for future pinned results)
     sphx::DevBuf<double>   reduce64;     // per-block partial reductions
 
+    // the adjoints' pair index of the last search (built by the first adjoint call after it): target i's
+    // stored pairs are p = pairStart[i - nbFirst] + k; sorted holds (j << pbits | p) in increasing order, so
+    // the pairs that have particle j as neighbour are sorted[rowStart[j] .. rowStart[j + 1]); pairBuf holds
+    // the neighbour-side contributions of each pair, summed per neighbour in that order
+    uint64_t searchSerial = 0; // incremented by every search
+    struct AdjointPairs
+    {
+        uint64_t               serial = 0; // searchSerial the index was built for (0: none)
+        uint64_t               pairs  = 0;
+        int                    pbits  = 0;
+        sphx::DevBuf<uint32_t> pairStart, rowStart;
+        sphx::DevBuf<uint64_t> keys, sorted;
+        sphx::DevBuf<char>     pairBuf;
+    } adj;
+
     // optional per-kernel timing: HIP events around the main kernel of each pass
     struct KernelEvents
     {

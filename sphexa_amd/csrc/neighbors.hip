@@ -29,7 +29,13 @@ namespace sphx
 namespace
 {
 
-constexpr int kWavesPerBlock = 4;
+// one wave per block: a group's search time varies with its candidates, and a block's LDS is held
+// until its slowest wave ends, so single-wave blocks free their share as soon as their group is done
+// (8M turbulence 5.29 -> 5.10 ms, 4M Evrard 3.59 -> 3.44 ms against 4-wave blocks)
+#ifndef SPHX_SEARCH_WAVES_PER_BLOCK
+#define SPHX_SEARCH_WAVES_PER_BLOCK 1
+#endif
+constexpr int kWavesPerBlock = SPHX_SEARCH_WAVES_PER_BLOCK;
 constexpr int kStackCap      = 952;  // LDS node stack per wave (bound argued at searchGroupInto)
 constexpr int kLeafCap       = 64;   // leaves per sweep of candidate leaves (one per lane)
 constexpr int kBufCap        = 128;  // LDS buffer of found leaves awaiting their turn in visiting order
@@ -83,8 +89,8 @@ struct WaveLds
     double   tm[2][kWave]; // its mass (fused XMass only)
 };
 
-// four 4-wave blocks share a CU's 160 KB of LDS
-static_assert(kWavesPerBlock * sizeof(WaveLds) <= 160 * 1024 / 4, "search LDS must leave room for four blocks per CU");
+// sixteen waves share a CU's 160 KB of LDS
+static_assert(sizeof(WaveLds) <= 160 * 1024 / 16, "search LDS must leave room for 16 waves per CU");
 
 //! per-target state of a lane
 struct Target
@@ -939,6 +945,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     ws->nbNgmax      = p->ngmax;
     ws->nbNc         = f->nc;
     ws->nbValid      = true;
+    ++ws->searchSerial;
     ws->statSumPairs = stats[0];
     ws->statMaxCount = uint32_t(stats[1]);
     ws->nbWide       = stats[kStatWide] > 0;

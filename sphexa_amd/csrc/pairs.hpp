@@ -20,7 +20,10 @@
 namespace sphx
 {
 
-constexpr int kPairWaves = 4;
+#ifndef SPHX_PAIR_WAVES_PER_BLOCK
+#define SPHX_PAIR_WAVES_PER_BLOCK 4
+#endif
+constexpr int kPairWaves = SPHX_PAIR_WAVES_PER_BLOCK;
 
 template<class T>
 constexpr bool isMixed = std::is_same_v<T, float>;
@@ -110,13 +113,17 @@ __device__ __forceinline__ Lane laneTarget(const Lists& L)
 
 inline unsigned pairBlocks(const Lists& L) { return ceilDiv(L.groups.num, kPairWaves); }
 
-/*! Dynamic LDS per 4-wave block that caps a pair loop at `waves` waves per SIMD (0: no cap): the CU's
- * 160 KB then holds at most `waves` blocks. A gather-bound loop can run faster with fewer waves in
+/*! Dynamic LDS per block that caps a pair loop at `waves` waves per SIMD (0: no cap): the CU's 160 KB
+ * then holds at most 4 `waves` / kPairWaves blocks. A gather-bound loop can run faster with fewer waves in
  * flight than its register use allows, since more waves evict each other's records from L1. Caps per
  * loop family of the FP32 form, from a sweep of 2-6 at 8M particles: AV switches 3 (2.98 -> 2.75 ms,
  * 4 waves by registers otherwise), XMass 6 (1.27 -> 1.22 ms, 8 otherwise); the others ran as fast or
  * faster uncapped. */
-inline size_t occupancyCap(int waves) { return waves > 0 ? size_t(160 * 1024) / size_t(waves) : 0; }
+inline size_t occupancyCap(int waves)
+{
+    // blocks per CU = 4 SIMDs x waves / (waves per block)
+    return waves > 0 ? size_t(160 * 1024) * kPairWaves / (4 * size_t(waves)) : 0;
+}
 #ifndef SPHX_CAP_XMASS
 #define SPHX_CAP_XMASS 6
 #endif

@@ -2,7 +2,10 @@
 
 * FP64 adjoint == the oracle's adjoint (pinned on CPU by finite differences in
   test_adjoint_oracle.py) on the same inputs and lists, to 1e-9 of each gradient's largest
-  magnitude (atomic accumulation order is the only difference).
+  magnitude (the summation order of the neighbour-side terms is the only difference).
+* Repeated calls give the same bits (the neighbour-side terms are summed per particle in pair
+  order, without atomics), in both forms; calls over two halves of the targets add up to the call
+  over all of them.
 * The gradient of L = sum (rho - rho_t)^2 + w sum |a|^2 through the torch.autograd harness
   (tests/sph_autograd.py) matches central finite differences of L on the GPU (FP64, step 1e-6,
   lists held fixed) on sampled particles, to 1e-4 of the largest magnitude among the sampled
@@ -116,6 +119,42 @@ def test_fp64_adjoint_matches_oracle(orc, sys64, case):
         err = np.max(np.abs(got[k] - ref[k])) / scale
         print(f"{case} d/d{k}: max rel err {err:.2e}", flush=True)
         assert err <= 1e-9, (case, k, err)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("case", sorted(GRAD_OUT))
+def test_adjoint_is_deterministic(case, mixed):
+    ws, ps, box, params = build(12, mixed=mixed, seed=6)
+    cot = cotangents(case, ps.n, 5)
+    first = gpu_vjp(ws, ps, box, params, case, cot)
+    for rep in range(3):
+        again = gpu_vjp(ws, ps, box, params, case, cot)
+        for k in first:
+            assert np.array_equal(first[k], again[k]), (case, mixed, k, rep)
+
+
+@pytest.mark.parametrize("case", sorted(GRAD_OUT))
+def test_adjoint_over_target_halves(sys64, case):
+    """the neighbour-side sums of a call cover exactly its targets: two calls over [0, n/2) and
+    [n/2, n), accumulating into the same gradients, equal one call over [0, n) to rounding"""
+    import torch
+
+    from sph_autograd import adjoint
+
+    ws, ps, box, params = sys64
+    cot = cotangents(case, ps.n, 7)
+    full = gpu_vjp(ws, ps, box, params, case, cot)
+    cname, _, _, outs = GRAD_OUT[case]
+    cot_t = {k: torch.from_numpy(v).cuda() for k, v in cot.items()}
+    out = {k: torch.zeros(ps.n, dtype=torch.float64, device="cuda") for k in outs}
+    half = (ps.n // 2) // 64 * 64
+    for lo, hi in ((0, half), (half, ps.n)):
+        ws.vjp(cname, box, ps.fields(), params, lo, hi, adjoint(cot_t), adjoint(out))
+    torch.cuda.synchronize()
+    for k in outs:
+        got = out[k].cpu().numpy()
+        scale = max(np.max(np.abs(full[k])), 1e-300)
+        assert np.max(np.abs(got - full[k])) <= 1e-12 * scale, (case, k)
 
 
 @pytest.mark.parametrize("case", sorted(GRAD_OUT))
