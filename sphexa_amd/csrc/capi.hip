@@ -188,7 +188,7 @@ int sphx_workspace_destroy(sphx_workspace_t* ws)
     ws->tree.releaseAll();
     ws->nbrs.release(), ws->scratch.release(), ws->keyTmp.release();
     ws->fieldTmp.release(), ws->order.release(), ws->records.release(), ws->counters.release(), ws->hostVisible.release(), ws->reduce64.release();
-    ws->adj.pairStart.release(), ws->adj.rowStart.release(), ws->adj.keys.release(), ws->adj.sorted.release();
+    ws->adj.pairStart.release(), ws->adj.rowStart.release(), ws->adj.recordOf.release(), ws->adj.keys.release(), ws->adj.sorted.release();
     ws->adj.pairBuf.release(), ws->records2.release(), ws->groupBuf.release(), ws->groupCount.release();
     ws->groupSplit.release();
     delete ws;

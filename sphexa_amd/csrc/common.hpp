@@ -248,14 +248,15 @@ struct sphx_workspace
     // the adjoints' pair index of the last search (built by the first adjoint call after it): target i's
     // stored pairs are p = pairStart[i - nbFirst] + k; sorted holds (j << pbits | p) in increasing order, so
     // the pairs that have particle j as neighbour are sorted[rowStart[j] .. rowStart[j + 1]); pairBuf holds
-    // the neighbour-side contributions of each pair, summed per neighbour in that order
+    // the neighbour-side contributions of each pair at its place in sorted (recordOf[p]), so each
+    // neighbour sums a contiguous row
     uint64_t searchSerial = 0; // incremented by every search
     struct AdjointPairs
     {
         uint64_t               serial = 0; // searchSerial the index was built for (0: none)
         uint64_t               pairs  = 0;
         int                    pbits  = 0;
-        sphx::DevBuf<uint32_t> pairStart, rowStart;
+        sphx::DevBuf<uint32_t> pairStart, rowStart, recordOf; // recordOf[p]: the pair's place in sorted
         sphx::DevBuf<uint64_t> keys, sorted;
         sphx::DevBuf<char>     pairBuf;
     } adj;
