@@ -3,8 +3,11 @@
 // main/src/propagator/ve_hydro.hpp:130-204), reading the initial conditions from a binary file
 // and writing the SFC-ordered results for comparison with the CPU oracle.
 //
-//   dataset_api <in.bin> <out.bin> std|ve [cpu|gpu] [steps]
+//   dataset_api <in.bin> <out.bin> std|ve|std32|ve32 [cpu|gpu] [steps]
 //
+// "std32"/"ve32" run the mixed form: a DeviceDataset<float> whose HydroType fields are float and
+// whose x y z u temp du stay double, the reference's default types (sph/include/sph/types.hpp:40-45:
+// CoordinateType double, HydroType float).
 // "cpu" (the default) calls findNeighborsSfc and then computeDensity / computeXMass, as the
 // reference's CPU build does; "gpu" calls the fused findNeighborsDensity / findNeighborsXMass,
 // which is how the reference's GPU build runs the search (inside computeDensity / computeXMass).
@@ -17,7 +20,8 @@
 // in.bin : uint64 n, uint32 periodic, double lo, double hi, then n doubles each of
 //          x y z h m u vx vy vz alpha (u is the temperature for the VE pass)
 // out.bin: n doubles each of x, then the pass outputs (std: rho p c c11..c33 ax ay az du;
-//          VE: xm kx gradh prho c c11..c33 divv curlv alpha ax ay az du), n uint32 nc, double minDt
+//          VE: xm kx gradh prho c c11..c33 divv curlv alpha ax ay az du), n uint32 nc, double minDt;
+//          float fields are widened to double on the host
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,14 +45,175 @@ using namespace sphexa_amd;
         }                                                                                                              \
     } while (0)
 
+namespace
+{
+
+//! device buffers of both element types, freed at exit
+struct Owned
+{
+    std::vector<void*> ptrs;
+    ~Owned()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template<class T>
+    void operator()(T*& p, size_t n)
+    {
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw std::runtime_error("hipMalloc");
+        if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) throw std::runtime_error("hipMemset");
+        ptrs.push_back(p);
+    }
+};
+
+template<class T>
+bool upload(T* dst, const std::vector<double>& src)
+{
+    std::vector<T> buf(src.begin(), src.end());
+    return hipMemcpy(dst, buf.data(), buf.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+template<class T>
+bool download(FILE* out, const T* src, size_t n)
+{
+    std::vector<T> buf(n);
+    if (hipMemcpy(buf.data(), src, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    std::vector<double> wide(buf.begin(), buf.end());
+    return std::fwrite(wide.data(), 8, n, out) == n;
+}
+
+template<class H>
+int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, double lo, double hi, bool ve,
+        bool fused, int steps, const char* outPath)
+{
+    DeviceDataset<H> d;
+    if (sphx_workspace_create(&d.workspace, 0, nullptr) != SPHX_OK) { return 2; }
+    if (sphx_workspace_set_kernel(d.workspace, 6.0, &d.K) != SPHX_OK) { return 2; }
+    // the propagators change no field between the hydro calls of a pass, so the pair loops may
+    // write the next loop's neighbour records (bitwise the same results)
+    d.sphxFlags = SPHX_FLAG_CHAIN;
+    d.minDt     = steps > 0 ? 1e-4 : 1e-3;
+    d.minDt_m1  = d.minDt;
+
+    Owned owned;
+    auto& s = d.devData;
+    try
+    {
+        for (double** p : {&s.x, &s.y, &s.z, &s.du}) owned(*p, n);
+        for (H** p : {&s.h, &s.m, &s.vx, &s.vy, &s.vz, &s.c, &s.c11, &s.c12, &s.c13, &s.c22, &s.c23, &s.c33, &s.ax,
+                      &s.ay, &s.az})
+            owned(*p, n);
+        if (ve)
+        {
+            owned(s.temp, n);
+            for (H** p : {&s.prho, &s.xm, &s.kx, &s.gradh, &s.divv, &s.curlv, &s.alpha}) owned(*p, n);
+        }
+        else
+        {
+            owned(s.u, n);
+            for (H** p : {&s.rho, &s.p}) owned(*p, n);
+        }
+        if (steps > 0)
+            for (H** p : {&s.x_m1, &s.y_m1, &s.z_m1, &s.du_m1}) owned(*p, n);
+        owned(s.keys, n);
+        owned(s.nc, n);
+    }
+    catch (const std::exception& e)
+    {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 2;
+    }
+    bool ok = upload(s.x, ic[0]) && upload(s.y, ic[1]) && upload(s.z, ic[2]) && upload(s.h, ic[3]) &&
+              upload(s.m, ic[4]) && upload(ve ? s.temp : s.u, ic[5]) && upload(s.vx, ic[6]) &&
+              upload(s.vy, ic[7]) && upload(s.vz, ic[8]) && (!ve || upload(s.alpha, ic[9]));
+    if (!ok) { return 2; }
+
+    Box       box0(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
+    Domain    domain(0, 1, 64, 64, 0.5f, box0);
+    GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
+    try
+    {
+        for (int step = 0; step < std::max(steps, 1); ++step)
+        {
+            // the pass alone keeps the given box (the CPU oracle's); steps refit it as Domain::sync does
+            if (steps > 0) { domain.sync(n, d); }
+            else { syncDomain(n, d, box0); }
+            const Box& box = steps > 0 ? domain.box() : box0;
+            if (!fused) { findNeighborsSfc(0, n, d, box); }
+            if (ve)
+            {
+                if (fused) { findNeighborsXMass(grp, d, box); }
+                else { computeXMass(grp, d, box); }
+                computeVeDefGradh(grp, d, box);
+                computeEOS(0, n, d);
+                computeIadDivvCurlv(grp, d, box);
+                computeAVswitches(grp, d, box);
+                computeMomentumEnergy<false>(grp, nullptr, d, box);
+            }
+            else
+            {
+                if (fused) { findNeighborsDensity(grp, d, box); }
+                else { computeDensity(grp, d, box); }
+                computeEOS_HydroStd(0, n, d);
+                computeIAD(grp, d, box);
+                computeMomentumEnergySTD(grp, d, box);
+            }
+            if (steps > 0)
+            {
+                computeTimestep(0, n, d);
+                computePositions(grp, d, box, float(d.minDt), {float(d.minDt_m1)});
+                updateSmoothingLength(grp, d);
+            }
+        }
+    }
+    catch (const std::exception& e)
+    {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 3;
+    }
+
+    FILE* out = std::fopen(outPath, "wb");
+    if (!out) { return 1; }
+    if (steps > 0)
+    {
+        for (double* p : {s.x, s.y, s.z}) ok = ok && download(out, p, n);
+        for (H* p : {s.vx, s.vy, s.vz, s.h}) ok = ok && download(out, p, n);
+        ok = ok && download(out, s.u, n);
+        for (double v : {d.minDt, d.minDt_m1, d.ttot}) std::fwrite(&v, 8, 1, out);
+        auto q = computeConservedQuantities(0, n, d);
+        std::fwrite(q.data(), 8, q.size(), out);
+    }
+    else
+    {
+        ok = download(out, s.x, n);
+        std::vector<H*> outs = ve ? std::vector<H*>{s.xm, s.kx, s.gradh, s.prho, s.c, s.c11, s.c12, s.c13, s.c22,
+                                                    s.c23, s.c33, s.divv, s.curlv, s.alpha, s.ax, s.ay, s.az}
+                                  : std::vector<H*>{s.rho, s.p, s.c, s.c11, s.c12, s.c13, s.c22, s.c23, s.c33, s.ax,
+                                                    s.ay, s.az};
+        for (H* p : outs) ok = ok && download(out, p, n);
+        ok = ok && download(out, s.du, n);
+        std::vector<uint32_t> nc(n);
+        ok = ok && hipMemcpy(nc.data(), s.nc, n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+        std::fwrite(nc.data(), 4, n, out);
+        std::fwrite(&d.minDtCourant, 8, 1, out);
+    }
+    std::fclose(out);
+    sphx_workspace_destroy(d.workspace);
+    return ok ? 0 : 2;
+}
+
+} // namespace
+
 int main(int argc, char** argv)
 {
     if (argc < 4 || argc > 6) { return 1; }
     std::string form  = argv[3];
-    bool        fused = argc >= 5 && std::string(argv[4]) == "gpu";
-    int         steps = argc == 6 ? std::atoi(argv[5]) : 0;
+    bool        mixed = form == "std32" || form == "ve32";
+    if (mixed) { form.resize(form.size() - 2); }
+    if (form != "std" && form != "ve") { return 1; }
+    bool fused = argc >= 5 && std::string(argv[4]) == "gpu";
+    int  steps = argc == 6 ? std::atoi(argv[5]) : 0;
     if (steps > 0 && form != "std") { return 1; }
-    FILE*       in   = std::fopen(argv[1], "rb");
+    FILE* in = std::fopen(argv[1], "rb");
     if (!in) { return 1; }
     uint64_t n;
     uint32_t periodic;
@@ -61,119 +226,7 @@ int main(int argc, char** argv)
         if (std::fread(v.data(), 8, n, in) != n) return 1;
     std::fclose(in);
 
-    DeviceDataset<double> d;
-    if (sphx_workspace_create(&d.workspace, 0, nullptr) != SPHX_OK) { return 2; }
-    if (sphx_workspace_set_kernel(d.workspace, 6.0, &d.K) != SPHX_OK) { return 2; }
-    // the propagators change no field between the hydro calls of a pass, so the pair loops may
-    // write the next loop's neighbour records (bitwise the same results)
-    d.sphxFlags = SPHX_FLAG_CHAIN;
-    d.minDt = steps > 0 ? 1e-4 : 1e-3;
-    d.minDt_m1 = d.minDt;
-
-    std::vector<double*> owned;
-    auto alloc = [&](double*& p) {
-        if (hipMalloc(&p, n * 8) != hipSuccess) throw std::runtime_error("hipMalloc");
-        if (hipMemset(p, 0, n * 8) != hipSuccess) throw std::runtime_error("hipMemset");
-        owned.push_back(p);
-    };
-    auto& s = d.devData;
-    bool  ve = form == "ve";
-    for (double** p : {&s.x, &s.y, &s.z, &s.du, &s.h, &s.m, &s.vx, &s.vy, &s.vz, &s.c, &s.c11, &s.c12, &s.c13,
-                       &s.c22, &s.c23, &s.c33, &s.ax, &s.ay, &s.az})
-        alloc(*p);
-    if (ve)
-        for (double** p : {&s.temp, &s.prho, &s.xm, &s.kx, &s.gradh, &s.divv, &s.curlv, &s.alpha}) alloc(*p);
-    else
-        for (double** p : {&s.u, &s.rho, &s.p}) alloc(*p);
-    if (steps > 0)
-        for (double** p : {&s.x_m1, &s.y_m1, &s.z_m1, &s.du_m1}) alloc(*p);
-    HIPCHECK(hipMalloc(&s.keys, n * 8));
-    HIPCHECK(hipMalloc(&s.nc, n * 4));
-    double* icDst[10] = {s.x, s.y, s.z, s.h, s.m, ve ? s.temp : s.u, s.vx, s.vy, s.vz, ve ? s.alpha : nullptr};
-    for (int k = 0; k < 10; ++k)
-        if (icDst[k]) HIPCHECK(hipMemcpy(icDst[k], ic[k].data(), n * 8, hipMemcpyHostToDevice));
-
-    Box       box0(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
-    Domain    domain(0, 1, 64, 64, 0.5f, box0);
-    GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
-    try
-    {
-      for (int step = 0; step < std::max(steps, 1); ++step)
-      {
-        // the pass alone keeps the given box (the CPU oracle's); steps refit it as Domain::sync does
-        if (steps > 0) { domain.sync(n, d); }
-        else { syncDomain(n, d, box0); }
-        const Box& box = steps > 0 ? domain.box() : box0;
-        if (!fused) { findNeighborsSfc(0, n, d, box); }
-        if (ve)
-        {
-            if (fused) { findNeighborsXMass(grp, d, box); }
-            else { computeXMass(grp, d, box); }
-            computeVeDefGradh(grp, d, box);
-            computeEOS(0, n, d);
-            computeIadDivvCurlv(grp, d, box);
-            computeAVswitches(grp, d, box);
-            computeMomentumEnergy<false>(grp, nullptr, d, box);
-        }
-        else
-        {
-            if (fused) { findNeighborsDensity(grp, d, box); }
-            else { computeDensity(grp, d, box); }
-            computeEOS_HydroStd(0, n, d);
-            computeIAD(grp, d, box);
-            computeMomentumEnergySTD(grp, d, box);
-        }
-        if (steps > 0)
-        {
-            computeTimestep(0, n, d);
-            computePositions(grp, d, box, float(d.minDt), {float(d.minDt_m1)});
-            updateSmoothingLength(grp, d);
-        }
-      }
-    }
-    catch (const std::exception& e)
-    {
-        std::fprintf(stderr, "%s\n", e.what());
-        return 3;
-    }
-
-    if (steps > 0)
-    {
-        FILE*               out = std::fopen(argv[2], "wb");
-        std::vector<double> buf(n);
-        for (double* p : {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.h, s.u})
-        {
-            HIPCHECK(hipMemcpy(buf.data(), p, n * 8, hipMemcpyDeviceToHost));
-            std::fwrite(buf.data(), 8, n, out);
-        }
-        for (double v : {d.minDt, d.minDt_m1, d.ttot}) std::fwrite(&v, 8, 1, out);
-        auto q = computeConservedQuantities(0, n, d);
-        std::fwrite(q.data(), 8, q.size(), out);
-        std::fclose(out);
-        for (double* p : owned) (void)hipFree(p);
-        (void)hipFree(s.keys), (void)hipFree(s.nc);
-        sphx_workspace_destroy(d.workspace);
-        return 0;
-    }
-    std::vector<double*> outs = ve ? std::vector<double*>{s.x, s.xm, s.kx, s.gradh, s.prho, s.c, s.c11, s.c12,
-                                                          s.c13, s.c22, s.c23, s.c33, s.divv, s.curlv, s.alpha,
-                                                          s.ax, s.ay, s.az, s.du}
-                                   : std::vector<double*>{s.x, s.rho, s.p, s.c, s.c11, s.c12, s.c13, s.c22, s.c23,
-                                                          s.c33, s.ax, s.ay, s.az, s.du};
-    FILE* out = std::fopen(argv[2], "wb");
-    std::vector<double> buf(n);
-    for (double* p : outs)
-    {
-        HIPCHECK(hipMemcpy(buf.data(), p, n * 8, hipMemcpyDeviceToHost));
-        std::fwrite(buf.data(), 8, n, out);
-    }
-    std::vector<uint32_t> nc(n);
-    HIPCHECK(hipMemcpy(nc.data(), s.nc, n * 4, hipMemcpyDeviceToHost));
-    std::fwrite(nc.data(), 4, n, out);
-    std::fwrite(&d.minDtCourant, 8, 1, out);
-    std::fclose(out);
-    for (double* p : owned) (void)hipFree(p);
-    (void)hipFree(s.keys), (void)hipFree(s.nc);
-    sphx_workspace_destroy(d.workspace);
-    return 0;
+    bool ve = form == "ve";
+    return mixed ? run<float>(ic, n, periodic != 0, lo, hi, ve, fused, steps, argv[2])
+                 : run<double>(ic, n, periodic != 0, lo, hi, ve, fused, steps, argv[2]);
 }
