@@ -500,7 +500,7 @@ int sphx_search_tests(const sphx_workspace_t* ws, uint64_t* tests)
         return guarded(ws, [&] {                                                                                       \
             SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");                                              \
             requireTables(ws);                                                                                         \
-            fn(ws, makeDevBox(box), f, prm, first, last);                                                              \
+            fn(ws, pairBox(ws, box), f, prm, first, last);                                                              \
             return SPHX_OK;                                                                                            \
         });                                                                                                            \
     }
@@ -595,7 +595,7 @@ int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const 
     return guarded(ws, [&] {
         SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
         requireTables(ws);
-        double dt = momentumEnergyStd(ws, makeDevBox(box), f, prm, first, last);
+        double dt = momentumEnergyStd(ws, pairBox(ws, box), f, prm, first, last);
         if (minDtCourant) { *minDtCourant = dt; }
         return SPHX_OK;
     });
@@ -608,7 +608,7 @@ int sphx_momentum_energy_std(sphx_workspace_t* ws, const sphx_box_t* box, const 
         return guarded(ws, [&] {                                                                                       \
             SPHX_REQUIRE(box && f && prm && in && out, SPHX_ERR_ARG, "null argument");                                 \
             requireTables(ws);                                                                                         \
-            fn(ws, makeDevBox(box), f, prm, first, last, in, out);                                                     \
+            fn(ws, pairBox(ws, box), f, prm, first, last, in, out);                                                     \
             return SPHX_OK;                                                                                            \
         });                                                                                                            \
     }
@@ -624,7 +624,7 @@ int sphx_momentum_energy_ve(sphx_workspace_t* ws, const sphx_box_t* box, const s
     return guarded(ws, [&] {
         SPHX_REQUIRE(box && f && prm, SPHX_ERR_ARG, "null argument");
         requireTables(ws);
-        double dt = momentumEnergyVe(ws, makeDevBox(box), f, prm, first, last, avClean != 0);
+        double dt = momentumEnergyVe(ws, pairBox(ws, box), f, prm, first, last, avClean != 0);
         if (minDtCourant) { *minDtCourant = dt; }
         return SPHX_OK;
     });

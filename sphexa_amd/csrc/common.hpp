@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -197,6 +198,7 @@ struct sphx_workspace
     const uint32_t*        nbNc = nullptr; // nc array the lists were built with
     bool                   nbValid = false;
     bool                   nbWide  = false; // some target's 2h reaches half a box length (FP32 loops: exact separations)
+    double                 nbHMax  = 0; // largest h of the last search's targets (the FP32 loops' fixed-point period)
 
     // target groups set by sphx_set_groups, and the spatial groups of sphx_compute_groups
     bool                   groupsSet = false;
@@ -372,11 +374,13 @@ __device__ __forceinline__ void groupRange(const Groups& G, uint32_t g, uint32_t
 
 /* 64-bit slots of the workspace counters: the search's statistics (stored pairs, maximum count,
  * non-converged targets) in 0-2, the Courant dt of the momentum loops in 3, the device maxima of the
- * time step in 4, the search's wide-support count, candidate tests and oversized groups in 5-7 */
-constexpr int kCounterSlots  = 8;
+ * time step in 4, the search's wide-support count, candidate tests and oversized groups in 5-7, the
+ * largest h of the searched targets (FP64 bits) in 8 */
+constexpr int kCounterSlots  = 9;
 constexpr int kStatWide      = 5;
 constexpr int kStatTests     = 6;
 constexpr int kStatBadGroups = 7;
+constexpr int kStatHMax      = 8;
 
 constexpr unsigned kNumXcd = 8; // MI355X: 8 accelerator complex dies, one L2 each
 
@@ -501,10 +505,11 @@ struct DevBox
     double lo[3], hi[3], len[3], ilen[3];
     int    pbc[3];
     int    anyPbc;
-    // 32-bit fixed-point coordinates of the FP32 pair loops: u = rint((x - lo) 2^32 / len) mod 2^32
-    double fixScale[3]; // 2^32 / len
-    double fixStepD[3]; // len / 2^32
-    float  fixStep[3];  // len / 2^32, rounded to float
+    // 32-bit fixed-point coordinates of the FP32 pair loops over the period P = len / 2^s (s the
+    // fixed-point shift of the last search, 0 by default): u = rint((x - lo) 2^32 / P) mod 2^32
+    double fixScale[3]; // 2^32 / P
+    double fixStepD[3]; // P / 2^32
+    float  fixStep[3];  // P / 2^32, rounded to float
 };
 
 inline DevBox makeDevBox(const sphx_box_t* b)
@@ -524,6 +529,41 @@ inline DevBox makeDevBox(const sphx_box_t* b)
         d.fixStep[k]  = float(d.fixStepD[k]);
     }
     return d;
+}
+
+//! the box of the pair loops; in the FP32 form its fixed point runs over the period len / 2^shift[k]
+//! per dimension (fixedPointShift)
+inline DevBox makePairBox(const sphx_box_t* b, const int shift[3])
+{
+    DevBox d = makeDevBox(b);
+    for (int k = 0; k < 3; ++k)
+    {
+        const double p = std::ldexp(d.len[k], -shift[k]);
+        d.fixScale[k]  = 4294967296.0 / p;
+        d.fixStepD[k]  = p / 4294967296.0;
+        d.fixStep[k]   = float(d.fixStepD[k]);
+    }
+    return d;
+}
+
+//! host: the fixed-point shift per dimension for a search whose largest h is hMax (see Coord)
+inline void fixedPointShift(const sphx_box_t* b, double hMax, bool wide, int shift[3])
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        const double len = b->hi[d] - b->lo[d];
+        int          s   = 0;
+        while (!wide && s < 16 && std::ldexp(len, -(s + 1)) >= 8.0 * hMax) { ++s; }
+        shift[d] = s;
+    }
+}
+
+//! host: the box of the FP32 pair loops over the last search's lists (FP64 form: the plain box)
+inline DevBox pairBox(const sphx_workspace* ws, const sphx_box_t* b)
+{
+    int shift[3];
+    fixedPointShift(b, ws->nbHMax, ws->nbWide, shift);
+    return makePairBox(b, shift);
 }
 
 /* ---------------------------------------------------------------------------------------------

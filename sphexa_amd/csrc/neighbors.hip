@@ -21,6 +21,8 @@
 // common.hpp (listOffset): a lane
 // shifts its entries through a register quad and stores 16 bytes at a time, and the pair loops read
 // four slots of all 64 lanes with one coalesced load.
+#include <cstring>
+
 #include "pairs.hpp"
 
 namespace sphx
@@ -838,8 +840,12 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
     unsigned fails  = __popcll(ballot(valid && updates >= 10));
     // support 2h reaching half a box length: the FP32 pair loops then take exact separations
     unsigned wide = __popcll(ballot(valid && 4.0 * double(hi) * (1.0 + 1e-6) >= a.minLen));
+    // largest h: sets the fixed-point period of the FP32 pair loops (positive doubles order as
+    // their bit patterns)
+    double hMax = waveMax(valid ? double(hi) : 0.0);
     if (lane == 0)
     {
+        atomicMax(&a.stats[kStatHMax], (unsigned long long)__double_as_longlong(hMax));
         atomicAdd(&a.stats[0], (unsigned long long)sum);
         atomicMax(&a.stats[1], (unsigned long long)mx);
         if (fails) { atomicAdd(&a.stats[2], (unsigned long long)fails); }
@@ -879,7 +885,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     ws->nbrs.reserve(std::max<size_t>(1, size_t(ceilDiv(last - first, kGroup)) * paddedNgmax(p->ngmax) * kGroup));
     ws->counters.reserve(2 * kCounterSlots);
     fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()), 0, 3);
-    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()) + kStatWide, 0, 3);
+    fill64(ws, reinterpret_cast<uint64_t*>(ws->counters.get()) + kStatWide, 0, kCounterSlots - kStatWide);
 
     double scale = 0;
     for (int d = 0; d < 3; ++d)
@@ -950,6 +956,7 @@ int findNeighbors(sphx_workspace* ws, const DevBox& box, const sphx_fields_t* f,
     ws->statMaxCount = uint32_t(stats[1]);
     ws->nbWide       = stats[kStatWide] > 0;
     ws->statTests    = stats[kStatTests];
+    std::memcpy(&ws->nbHMax, &stats[kStatHMax], sizeof(double));
     if (stats[kStatBadGroups] > 0)
     {
         ws->nbValid = false;

@@ -501,19 +501,23 @@ struct Sep
 };
 
 /*! Position of a particle in the neighbour records: FP64 in the plain form; in the FP32 form
- * 32-bit fixed point over the box, u = rint((x - lo) 2^32 / L) mod 2^32 per dimension (L the box
- * length). A record position is 12 bytes instead of 24; coordinates are quantized to L / 2^32
- * (2.3e-10 in a unit box). The separation is converted to FP32 and scaled by L / 2^32 (about one
- * ulp in all). Which integer difference stands for x_i - x_j:
- *  - while every target of the wave has 2h < L/2 in every dimension, the wrapping difference
- *    u_i - u_j read as a signed 32-bit integer. A listed pair is closer than 2h in each
- *    dimension, so this is the raw difference in an open dimension, and in a periodic one the
- *    minimum image, which equals the reference's fold with cut-off 2h (sfc/box.hpp:218-238) for
- *    any pair closer than 2h < L/2;
+ * 32-bit fixed point over a period P = L / 2^s of the box length L, u = rint((x - lo) 2^32 / P)
+ * mod 2^32 per dimension. The shift s is set after each search from its largest h: the largest
+ * s <= 16 with P >= 8 h_max (fixedPointShift), so a listed pair, closer than 2h_max in each
+ * dimension, is closer than P/4. A record position is 12 bytes instead of 24; coordinates are
+ * quantized to P / 2^32, about 2e-9 h_max whatever the box size (an open box much larger than the
+ * particles' extent, or a wide range of h, keeps the resolution of the smoothing lengths, as FP32
+ * offsets from a nearby cell would). The separation is converted to FP32 and scaled by P / 2^32
+ * (about one ulp in all). Which integer difference stands for x_i - x_j:
+ *  - while every target has 2h < L/2 in every dimension, the wrapping difference u_i - u_j read as
+ *    a signed 32-bit integer. A listed pair is closer than 2h < P/2 in each dimension, so this is
+ *    the raw difference in an open dimension, and in a periodic one (P divides L) the minimum
+ *    image, which equals the reference's fold with cut-off 2h (sfc/box.hpp:218-238) for any pair
+ *    closer than 2h < L/2;
  *  - otherwise (2h >= L/2: a box a few smoothing lengths wide, as for a few hundred particles or
- *    fewer), the exact difference u_i - u_j in 64 bits, the raw difference of coordinates that lie
- *    in [lo, hi] (the box update of the domain sync keeps open dimensions there), folded in a
- *    periodic dimension with cut-off 2h as the reference does. */
+ *    fewer; then s = 0, P = L), the exact difference u_i - u_j in 64 bits, the raw difference of
+ *    coordinates that lie in [lo, hi] (the box update of the domain sync keeps open dimensions
+ *    there), folded in a periodic dimension with cut-off 2h as the reference does. */
 template<class T>
 using Coord = std::conditional_t<isMixed<T>, uint32_t, double>;
 

@@ -301,6 +301,43 @@ def test_mixed_small_boxes(ws, orc, side, h, periodic, form):
         assert rel_to_max(ps.host(k), o["a"][k]) <= 1e-4, k
 
 
+@pytest.mark.parametrize("form", ["std", "ve"])
+def test_mixed_open_box_far_wider_than_support(ws, orc, form):
+    """Particles filling [-0.5, 0.5]^3 (2h about 0.2) inside an open box 2e5 wide, a million times
+    2h: the mixed form against the FP64 oracle with its bars (density <= 1e-5 pointwise,
+    accelerations and du <= 1e-4 of their largest magnitude, nc identical). Fixed point over the
+    whole box would quantize positions to 4.7e-5 (2e-4 of 2h); the FP32 loops' fixed point runs
+    over a period of a few h_max instead."""
+    from oracle import pipeline
+
+    if form == "std":
+        arr, _ = ic.sedov_lattice(16, seed=31)
+        arr["vx"] = 0.1 * np.sin(6.0 * arr["x"])
+        arr["vy"] = 0.05 * np.cos(5.0 * arr["y"])
+        names = STD_FIELDS
+    else:
+        arr, _ = _ve_ic(16, 32)
+        names = VE_FIELDS
+    box = abi.Box.make(-1e5, 1e5, abi.OPEN)
+    o = pipeline.run(orc, arr, box, abi.Params.defaults(ws.K, minDt=1e-3), form=form)
+    ps = ParticleSet.from_host(arr, names, mixed=True)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32, minDt=1e-3)
+    conv, dt = (std_step if form == "std" else ve_step)(ws, ps, box, params)
+    assert conv
+    np.testing.assert_array_equal(ps.host("nc"), o["a"]["nc"])
+    if form == "std":
+        rho_g, rho_o = ps.host("rho"), o["a"]["rho"]
+    else:
+        rho_g = ps.host("kx").astype(np.float64) * ps.host("m") / ps.host("xm")
+        rho_o = o["a"]["kx"] * o["a"]["m"] / o["a"]["xm"]
+    errs = {k: rel_to_max(ps.host(k), o["a"][k]) for k in ("ax", "ay", "az", "du")}
+    print(f"{form}: density {max_rel(rho_g, rho_o):.2e}, " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()),
+          flush=True)
+    assert max_rel(rho_g, rho_o) <= 1e-5
+    for k, e in errs.items():
+        assert e <= 1e-4, (k, e)
+
+
 def _ve_ic(side, seed):
     arr, spec = ic.turbulence_box(side, seed=seed)
     arr["temp"] = arr.pop("u") / ideal_gas_cv(10.0, 5.0 / 3.0)
