@@ -477,6 +477,47 @@ int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sp
                            const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first, uint32_t last,
                            double dt, double dt_m1, double constCv);
 
+/* ---- block time steps (the ve-bdt propagators, main/src/propagator/ve_hydro_bdt.hpp) ---------
+ * The entry points below run over the targets of [first, last): the groups of the view set with
+ * sphx_set_groups for that range (the propagator's active rungs), else consecutive groups of 64.
+ * Rungs index the previous steps dtM1[0 .. numRungs) (Timestep::dt_m1, sph/include/sph/timestep.h:26);
+ * rung values must lie below numRungs. */
+#define SPHX_MAX_RUNGS 4
+
+/* sphx_compute_positions with a rung per particle: particle i takes the previous step
+ * dtM1[rung[i]] (dtM1[0] when rung is NULL). Replaces computePositionsGpu with its rung array
+ * (positions_gpu.cu:116-168, :140). */
+int sphx_compute_positions_rungs(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                                 const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first,
+                                 uint32_t last, double dt, const float* dtM1, int numRungs, const uint8_t* rung,
+                                 double constCv);
+
+/* Drift to a time inside a block time-step hierarchy: each particle is taken back by dtBack to the
+ * state of its last force evaluation (positionUpdate with -dtBack) and advanced by dt from there,
+ * without the periodic wrap; vx, vy, vz follow and temp (cv as in sphx_compute_positions) or u is
+ * recovered and advanced with energyUpdate. x_m1, y_m1, z_m1 and du_m1 are read, not written.
+ * Replaces driftPositionsGpu (positions_gpu.cu:17-85) and driftPositions (positions.hpp:151-174).
+ * fields: x, y, z, vx, vy, vz, ax, ay, az, du, temp or u. */
+int sphx_drift_positions(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_history_t* hist,
+                         const sphx_params_t* prm, uint32_t first, uint32_t last, double dt, double dtBack,
+                         const float* dtM1, int numRungs, const uint8_t* rung, double constCv);
+
+/* groupDt[g] = min(groupDt[g], Krho / |max divv over group g|) (the signed maximum, in float).
+ * Replaces groupDivvTimestepGpu (ts_groups.cu:16-46; groupDivvTimestep passes d.Krho,
+ * ts_rungs.hpp:49-56). fields: divv. */
+int sphx_group_divv_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                             uint32_t first, uint32_t last, float Krho, float* groupDt);
+
+/* groupDt[g] = min(groupDt[g], etaAcc / sqrt(sqrt(max |a|^2 over group g))) in float, |a|^2 =
+ * ax^2 + (ay^2 + az^2) in the HydroType. Replaces groupAccTimestepGpu (ts_groups.cu:51-81;
+ * groupAccTimestep passes etaAcc * sqrt(eps), ts_rungs.hpp:58-67). fields: ax, ay, az. */
+int sphx_group_acc_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                            uint32_t first, uint32_t last, float etaAcc, float* groupDt);
+
+/* particleRungs[i] = rung for every target (groups of at most 64 particles, as for the other
+ * elementwise entry points). Replaces storeRungGpu (ts_groups.cu:84-104). */
+int sphx_store_rung(sphx_workspace_t* ws, uint32_t first, uint32_t last, uint8_t rung, uint8_t* particleRungs);
+
 /* h = updateH(ng0, nc, h) over [first, last). Replaces updateSmoothingLength
  * (sph/include/sph/update_h.hpp:26-35). fields: h, nc. */
 int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,

@@ -486,20 +486,65 @@ void computeTimestep(size_t first, size_t last, Dataset& d)
 }
 
 //! sph::computePositions (sph/include/sph/positions.hpp:177-200), its GPU branch: positions,
-//! velocities, x_m1..z_m1, and temp or u with du_m1. Block time steps (rung != nullptr) are not
-//! supported: every particle advances by dt_forward from the previous step dt_m1[0].
+//! velocities, x_m1..z_m1, and temp or u with du_m1. With a rung array (block time steps, a device
+//! array of one uint8 per particle) particle i takes the previous step dt_m1[rung[i]], else dt_m1[0].
 template<class Dataset>
 void computePositions(const GroupView& grp, Dataset& d, const Box& box, float dt_forward,
                       std::array<float, maxNumRungs> dt_m1, const uint8_t* rung = nullptr)
 {
-    if (rung != nullptr) { throw std::runtime_error("block time-step rungs are not supported"); }
     detail::GroupScope scope(d.workspace, grp);
     auto b = box.c();
     auto f = detail::fields(d);
     auto h = detail::history(d);
     auto p = detail::params(d);
-    detail::check(d.workspace, sphx_compute_positions(d.workspace, &b, &f, &h, &p, grp.firstBody, grp.lastBody,
-                                                      double(dt_forward), double(dt_m1[0]), -1.0));
+    detail::check(d.workspace, sphx_compute_positions_rungs(d.workspace, &b, &f, &h, &p, grp.firstBody,
+                                                            grp.lastBody, double(dt_forward), dt_m1.data(),
+                                                            int(maxNumRungs), rung, -1.0));
+}
+
+//! sph::driftPositions (sph/include/sph/positions.hpp:151-174): the particles of grp, last advanced by
+//! dt_backward within the current block time-step hierarchy, drift to dt_forward from its start
+template<class Dataset>
+void driftPositions(const GroupView& grp, Dataset& d, float dt_forward, float dt_backward,
+                    std::array<float, maxNumRungs> dt_prevRung, const uint8_t* rung)
+{
+    detail::GroupScope scope(d.workspace, grp);
+    auto f = detail::fields(d);
+    auto h = detail::history(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_drift_positions(d.workspace, &f, &h, &p, grp.firstBody, grp.lastBody,
+                                                    double(dt_forward), double(dt_backward), dt_prevRung.data(),
+                                                    int(maxNumRungs), rung, -1.0));
+}
+
+//! sph::groupDivvTimestep (sph/include/sph/ts_rungs.hpp:49-56): groupDt[g] = min(groupDt[g], Krho / |max divv|)
+template<class Dataset>
+void groupDivvTimestep(const GroupView& grp, float* groupDt, Dataset& d)
+{
+    detail::GroupScope scope(d.workspace, grp);
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_group_divv_timestep(d.workspace, &f, &p, grp.firstBody, grp.lastBody,
+                                                        float(d.Krho), groupDt));
+}
+
+//! sph::groupAccTimestep (ts_rungs.hpp:58-67): groupDt[g] = min(groupDt[g], etaAcc sqrt(eps) / |a|max^(1/2))
+template<class Dataset>
+void groupAccTimestep(const GroupView& grp, float* groupDt, Dataset& d)
+{
+    detail::GroupScope scope(d.workspace, grp);
+    auto f = detail::fields(d);
+    auto p = detail::params(d);
+    detail::check(d.workspace, sphx_group_acc_timestep(d.workspace, &f, &p, grp.firstBody, grp.lastBody,
+                                                       float(d.etaAcc * std::sqrt(d.eps)), groupDt));
+}
+
+//! sph::storeRungGpu (sph/include/sph/ts_groups.cu:84-104): particleRungs[i] = rung over grp
+template<class Dataset>
+void storeRung(const GroupView& grp, uint8_t rung, uint8_t* particleRungs, Dataset& d)
+{
+    detail::GroupScope scope(d.workspace, grp);
+    detail::check(d.workspace, sphx_store_rung(d.workspace, grp.firstBody, grp.lastBody, rung, particleRungs));
 }
 
 //! sph::updateSmoothingLength (sph/include/sph/update_h.hpp:26-35)

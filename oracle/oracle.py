@@ -68,6 +68,11 @@ _SIGS = {
     "position_update": (None, [C.c_double, C.c_double, _P, _P, _P, _BOXP, _P, _P, _P]),
     "fbc_check_f32": (C.c_int, [C.c_double, C.c_float, C.c_double, C.c_double, C.c_int]),
     "compute_positions": (None, [_BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double]),
+    "compute_positions_rungs": (None, [_BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, _P, _P]),
+    "drift_positions": (None, [_FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double, _P, _P]),
+    "group_divv_timestep": (None, [_FP, _PP, _P, _P, _U32, C.c_float, _P]),
+    "group_acc_timestep": (None, [_FP, _PP, _P, _P, _U32, C.c_float, _P]),
+    "store_rung": (None, [_P, _P, _U32, C.c_uint8, _P]),
     "update_smoothing_length": (None, [_FP, _PP, _U32, _U32]),
     "compute_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),
     "rho_timestep": (None, [_FP, _PP, _U32, _U32, _TSP]),

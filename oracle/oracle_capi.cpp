@@ -672,6 +672,71 @@ void orc_compute_positions(const sphx_box_t* b, const sphx_fields_t* f, const sp
     else { run(double{}); }
 }
 
+//! computePositions with a rung array (positions_gpu.cu:116-168): previous step dtM1Rung[rung[i]]
+void orc_compute_positions_rungs(const sphx_box_t* b, const sphx_fields_t* f, const sphx_history_t* hs,
+                                 const sphx_params_t* p, uint32_t first, uint32_t last, double dt,
+                                 const float* dtM1Rung, const uint8_t* rung)
+{
+    Box box = toBox(b);
+    auto run = [&](auto tag) {
+        using T        = decltype(tag);
+        double constCv = hs->mui ? -1.0 : double(idealGasCv(T(p->muiConst), p->gamma));
+        double* u      = f->temp ? nullptr : f->u;
+        computePositions<T>(box, dt, double(dtM1Rung[0]), f->x, f->y, f->z, (T*)f->vx, (T*)f->vy, (T*)f->vz,
+                            (T*)hs->x_m1, (T*)hs->y_m1, (T*)hs->z_m1, (const T*)f->ax, (const T*)f->ay,
+                            (const T*)f->az, f->temp, u, f->du, (T*)hs->du_m1, (const T*)f->h, (const T*)hs->mui,
+                            p->gamma, constCv, first, last, dtM1Rung, rung);
+    };
+    if (p->hydro_type == SPHX_F32) { run(float{}); }
+    else { run(double{}); }
+}
+
+//! driftPositionsGpu / driftKernel (positions_gpu.cu:17-85) on host arrays
+void orc_drift_positions(const sphx_fields_t* f, const sphx_history_t* hs, const sphx_params_t* p, uint32_t first,
+                         uint32_t last, double dt, double dtBack, const float* dtM1Rung, const uint8_t* rung)
+{
+    auto run = [&](auto tag) {
+        using T        = decltype(tag);
+        double constCv = hs->mui ? -1.0 : double(idealGasCv(T(p->muiConst), p->gamma));
+        double* u      = f->temp ? nullptr : f->u;
+        driftPositions<T>(dt, dtBack, dtM1Rung, rung, f->x, f->y, f->z, (T*)f->vx, (T*)f->vy, (T*)f->vz,
+                          (const T*)hs->x_m1, (const T*)hs->y_m1, (const T*)hs->z_m1, (const T*)f->ax,
+                          (const T*)f->ay, (const T*)f->az, f->temp, u, f->du, (const T*)hs->du_m1, (const T*)hs->mui,
+                          p->gamma, constCv, first, last);
+    };
+    if (p->hydro_type == SPHX_F32) { run(float{}); }
+    else { run(double{}); }
+}
+
+//! groupDivvTimestepGpu (ts_groups.cu:16-46) over explicit groups
+void orc_group_divv_timestep(const sphx_fields_t* f, const sphx_params_t* p, const uint32_t* groupStart,
+                             const uint32_t* groupEnd, uint32_t numGroups, float Krho, float* groupDt)
+{
+    if (p->hydro_type == SPHX_F32)
+        groupDivvTimestep(Krho, groupStart, groupEnd, numGroups, (const float*)f->divv, groupDt);
+    else
+        groupDivvTimestep(Krho, groupStart, groupEnd, numGroups, (const double*)f->divv, groupDt);
+}
+
+//! groupAccTimestepGpu (ts_groups.cu:51-81) over explicit groups
+void orc_group_acc_timestep(const sphx_fields_t* f, const sphx_params_t* p, const uint32_t* groupStart,
+                            const uint32_t* groupEnd, uint32_t numGroups, float etaAcc, float* groupDt)
+{
+    if (p->hydro_type == SPHX_F32)
+        groupAccTimestep(etaAcc, groupStart, groupEnd, numGroups, (const float*)f->ax, (const float*)f->ay,
+                         (const float*)f->az, groupDt);
+    else
+        groupAccTimestep(etaAcc, groupStart, groupEnd, numGroups, (const double*)f->ax, (const double*)f->ay,
+                         (const double*)f->az, groupDt);
+}
+
+//! storeRungGpu (ts_groups.cu:84-104)
+void orc_store_rung(const uint32_t* groupStart, const uint32_t* groupEnd, uint32_t numGroups, uint8_t rung,
+                    uint8_t* particleRungs)
+{
+    storeRung(groupStart, groupEnd, numGroups, rung, particleRungs);
+}
+
 void orc_update_smoothing_length(const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last)
 {
     if (p->hydro_type == SPHX_F32) { updateSmoothingLength(first, last, p->ng0, f->nc, (float*)f->h); }

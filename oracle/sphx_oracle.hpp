@@ -1655,12 +1655,14 @@ inline bool fbcCheck(double coord, Th h, double top, double bottom, bool fbc)
 }
 
 /*! computePositionsKernel, positions_gpu.cu:116-168, over [first, last). constCv < 0 selects the
- * per-particle idealGasCv(mui[i], gamma). temp (when non-null) takes precedence over u. */
+ * per-particle idealGasCv(mui[i], gamma). temp (when non-null) takes precedence over u. With a rung
+ * array (block time steps), particle i takes the previous step dtM1Rung[rung[i]] (:140). */
 template<class T>
-inline void computePositions(const Box& box, double dt, double dt_m1, double* x, double* y, double* z, T* vx, T* vy,
-                             T* vz, T* x_m1, T* y_m1, T* z_m1, const T* ax, const T* ay, const T* az, double* temp,
-                             double* u, const double* du, T* du_m1, const T* h, const T* mui, double gamma,
-                             double constCv, size_t first, size_t last)
+inline void computePositions(const Box& box, double dt, double dt_m1_all, double* x, double* y, double* z, T* vx,
+                             T* vy, T* vz, T* x_m1, T* y_m1, T* z_m1, const T* ax, const T* ay, const T* az,
+                             double* temp, double* u, const double* du, T* du_m1, const T* h, const T* mui,
+                             double gamma, double constCv, size_t first, size_t last, const float* dtM1Rung = nullptr,
+                             const uint8_t* rung = nullptr)
 {
     bool anyFBC = box.fbc[0] || box.fbc[1] || box.fbc[2];
     for (size_t i = first; i < last; ++i)
@@ -1674,6 +1676,7 @@ inline void computePositions(const Box& box, double dt, double dt_m1, double* x,
                 continue;
             }
         }
+        const double dt_m1 = rung ? double(dtM1Rung[rung[i]]) : dt_m1_all;
         double A[3]  = {double(ax[i]), double(ay[i]), double(az[i])};
         double X[3]  = {x[i], y[i], z[i]};
         double dX[3] = {double(x_m1[i]), double(y_m1[i]), double(z_m1[i])};
@@ -1696,6 +1699,90 @@ inline void computePositions(const Box& box, double dt, double dt_m1, double* x,
             du_m1[i] = du[i];
         }
     }
+}
+
+/*! driftKernel, positions_gpu.cu:17-60, over [first, last): a particle of a block time-step
+ * hierarchy, last advanced by dtBack from (Xn, An, dXn), is taken back to Xn (positionUpdate with
+ * -dtBack) and advanced by dt from there, both without the periodic wrap (the reference drifts in
+ * an open unit box, so that the octree stays valid); velocities follow, and temp or u is recovered
+ * and advanced the same way with energyUpdate. x_m1, y_m1, z_m1 and du_m1 are read only. The
+ * previous step is dtM1Rung[rung[i]], or dtM1Rung[0] without a rung array. */
+template<class T>
+inline void driftPositions(double dt, double dtBack, const float* dtM1Rung, const uint8_t* rung, double* x, double* y,
+                           double* z, T* vx, T* vy, T* vz, const T* x_m1, const T* y_m1, const T* z_m1, const T* ax,
+                           const T* ay, const T* az, double* temp, double* u, const double* du, const T* du_m1,
+                           const T* mui, double gamma, double constCv, size_t first, size_t last)
+{
+    Box noPbc{};
+    for (int d = 0; d < 3; ++d)
+    {
+        noPbc.lo[d] = 0, noPbc.hi[d] = 1, noPbc.len[d] = 1, noPbc.ilen[d] = 1, noPbc.pbc[d] = 0, noPbc.fbc[d] = 0;
+    }
+    for (size_t i = first; i < last; ++i)
+    {
+        const double dt_m1 = double(rung ? dtM1Rung[rung[i]] : dtM1Rung[0]);
+        double A[3]  = {double(ax[i]), double(ay[i]), double(az[i])};
+        double Xb[3] = {x[i], y[i], z[i]};
+        double dX[3] = {double(x_m1[i]), double(y_m1[i]), double(z_m1[i])};
+        double Xn[3], V[3], dXn1[3];
+        positionUpdate(-dtBack, dt_m1, Xb, A, dX, noPbc, Xn, V, dXn1);
+        double Xn1[3];
+        positionUpdate(dt, dt_m1, Xn, A, dX, noPbc, Xn1, V, dXn1);
+        x[i] = Xn1[0], y[i] = Xn1[1], z[i] = Xn1[2];
+        vx[i] = T(V[0]), vy[i] = T(V[1]), vz[i] = T(V[2]);
+
+        if (temp != nullptr)
+        {
+            T      cv      = (constCv < 0) ? idealGasCv(mui[i], gamma) : constCv;
+            double u_recov = energyUpdate(double(temp[i] * cv), -dtBack, dt_m1, du[i], double(du_m1[i]));
+            temp[i]        = energyUpdate(u_recov, dt, dt_m1, du[i], double(du_m1[i])) / cv;
+        }
+        else if (u != nullptr)
+        {
+            double u_recov = energyUpdate(u[i], -dtBack, dt_m1, du[i], double(du_m1[i]));
+            u[i]           = energyUpdate(u_recov, dt, dt_m1, du[i], double(du_m1[i]));
+        }
+    }
+}
+
+/*! groupDivvKernel, ts_groups.cu:16-36: per group the largest divv (signed, in float), then
+ * groupDt[g] = min(groupDt[g], Krho / |max divv|) in float */
+template<class T>
+inline void groupDivvTimestep(float Krho, const uint32_t* groupStart, const uint32_t* groupEnd, size_t numGroups,
+                              const T* divv, float* groupDt)
+{
+    for (size_t g = 0; g < numGroups; ++g)
+    {
+        float localMax = -INFINITY;
+        for (uint32_t i = groupStart[g]; i < groupEnd[g]; ++i)
+            localMax = float(std::max<T>(T(localMax), divv[i]));
+        groupDt[g] = std::min(groupDt[g], Krho / std::abs(localMax));
+    }
+}
+
+/*! groupAccKernel, ts_groups.cu:51-71: per group the largest |a|^2 (norm2, util/array.hpp:236-240
+ * and 316-320: the right fold ax^2 + (ay^2 + az^2) in T, as float), then
+ * groupDt[g] = min(groupDt[g], etaAcc / sqrt(sqrt(max |a|^2))) in float */
+template<class T>
+inline void groupAccTimestep(float etaAcc, const uint32_t* groupStart, const uint32_t* groupEnd, size_t numGroups,
+                             const T* ax, const T* ay, const T* az, float* groupDt)
+{
+    for (size_t g = 0; g < numGroups; ++g)
+    {
+        float maxAcc = 0;
+        for (uint32_t i = groupStart[g]; i < groupEnd[g]; ++i)
+            maxAcc = float(std::max<T>(T(maxAcc), T(ax[i] * ax[i] + (ay[i] * ay[i] + az[i] * az[i]))));
+        groupDt[g] = std::min(groupDt[g], etaAcc / std::sqrt(std::sqrt(maxAcc)));
+    }
+}
+
+//! storeRungKernel, ts_groups.cu:84-95: particleRungs[i] = rung over every group
+inline void storeRung(const uint32_t* groupStart, const uint32_t* groupEnd, size_t numGroups, uint8_t rung,
+                      uint8_t* particleRungs)
+{
+    for (size_t g = 0; g < numGroups; ++g)
+        for (uint32_t i = groupStart[g]; i < groupEnd[g]; ++i)
+            particleRungs[i] = rung;
 }
 
 //! updateSmoothingLengthCpu, update_h.hpp:12-24

@@ -149,11 +149,6 @@ void searchThenSum(const GroupView& g, Data& d, const Box& b, Sum sum)
     check(ws, sum(ws, &bx, &f, &p, g.firstBody, g.lastBody));
 }
 
-[[noreturn]] void unsupported(const char* what)
-{
-    throw std::runtime_error(std::string("sphexa_amd: ") + what + " is not supported");
-}
-
 template<class T>
 constexpr int32_t hydroType()
 {
@@ -340,7 +335,6 @@ void computePositionsGpu(const GroupView& grp, float dt, util::array<float, Time
                          const uint8_t* rung, Tu* temp, Tu* u, Tdu* du, Tm1* du_m1, Thydro* h, Thydro* mui, Tc gamma,
                          Tc constCv, const cstone::Box<Tc>& b)
 {
-    if (rung) { unsupported("block time-step rungs"); }
     Groups        scope(theWorkspace(), grp);
     sphx_fields_t f{};
     f.x = x, f.y = y, f.z = z, f.vx = vx, f.vy = vy, f.vz = vz, f.ax = ax, f.ay = ay, f.az = az;
@@ -350,8 +344,8 @@ void computePositionsGpu(const GroupView& grp, float dt, util::array<float, Time
     p.gamma      = gamma;
     p.hydro_type = hydroType<Thydro>();
     auto bx      = box(b);
-    check(theWorkspace(), sphx_compute_positions(theWorkspace(), &bx, &f, &hs, &p, grp.firstBody, grp.lastBody, dt,
-                                                 dt_m1[0], constCv));
+    check(theWorkspace(), sphx_compute_positions_rungs(theWorkspace(), &bx, &f, &hs, &p, grp.firstBody, grp.lastBody,
+                                                       dt, dt_m1.data(), Timestep::maxNumRungs, rung, constCv));
 }
 
 template void computePositionsGpu(const GroupView&, float, util::array<float, Timestep::maxNumRungs>, double*,
@@ -364,11 +358,22 @@ template void computePositionsGpu(const GroupView&, float, util::array<float, Ti
                                   double, const Box&);
 
 template<class Tc, class Thydro, class Tm1, class Tdu>
-void driftPositionsGpu(const GroupView&, float, float, util::array<float, Timestep::maxNumRungs>, Tc*, Tc*, Tc*,
-                       Thydro*, Thydro*, Thydro*, const Tm1*, const Tm1*, const Tm1*, const Thydro*, const Thydro*,
-                       const Thydro*, const uint8_t*, Tc*, Tc*, Tdu*, Tm1*, Thydro*, Tc, Tc)
+void driftPositionsGpu(const GroupView& grp, float dt, float dt_back, util::array<float, Timestep::maxNumRungs> dt_m1,
+                       Tc* x, Tc* y, Tc* z, Thydro* vx, Thydro* vy, Thydro* vz, const Tm1* x_m1, const Tm1* y_m1,
+                       const Tm1* z_m1, const Thydro* ax, const Thydro* ay, const Thydro* az, const uint8_t* rung,
+                       Tc* temp, Tc* u, Tdu* du, Tm1* du_m1, Thydro* mui, Tc gamma, Tc constCv)
 {
-    unsupported("the drift of block time-step rungs");
+    Groups        scope(theWorkspace(), grp);
+    sphx_fields_t f{};
+    f.x = x, f.y = y, f.z = z, f.vx = vx, f.vy = vy, f.vz = vz;
+    f.ax = const_cast<Thydro*>(ax), f.ay = const_cast<Thydro*>(ay), f.az = const_cast<Thydro*>(az);
+    f.temp = temp, f.u = u, f.du = du;
+    sphx_history_t hs{const_cast<Tm1*>(x_m1), const_cast<Tm1*>(y_m1), const_cast<Tm1*>(z_m1), du_m1, mui};
+    sphx_params_t  p{};
+    p.gamma      = gamma;
+    p.hydro_type = hydroType<Thydro>();
+    check(theWorkspace(), sphx_drift_positions(theWorkspace(), &f, &hs, &p, grp.firstBody, grp.lastBody, dt, dt_back,
+                                               dt_m1.data(), Timestep::maxNumRungs, rung, constCv));
 }
 
 template void driftPositionsGpu(const GroupView&, float, float, util::array<float, Timestep::maxNumRungs>, double*,
@@ -391,15 +396,27 @@ template void updateSmoothingLengthGpu(const GroupView&, unsigned, const unsigne
 template void updateSmoothingLengthGpu(const GroupView&, unsigned, const unsigned*, double*);
 
 template<class T>
-void groupDivvTimestepGpu(float, const GroupView&, const T*, float*)
+void groupDivvTimestepGpu(float Krho, const GroupView& grp, const T* divv, float* groupDt)
 {
-    unsupported("per-group time steps (groupDivvTimestepGpu)");
+    Groups        scope(theWorkspace(), grp);
+    sphx_fields_t f{};
+    f.divv = const_cast<T*>(divv);
+    sphx_params_t p{};
+    p.hydro_type = hydroType<T>();
+    check(theWorkspace(),
+          sphx_group_divv_timestep(theWorkspace(), &f, &p, grp.firstBody, grp.lastBody, Krho, groupDt));
 }
 
 template<class T>
-void groupAccTimestepGpu(float, const GroupView&, const T*, const T*, const T*, float*)
+void groupAccTimestepGpu(float etaAcc, const GroupView& grp, const T* ax, const T* ay, const T* az, float* groupDt)
 {
-    unsupported("per-group time steps (groupAccTimestepGpu)");
+    Groups        scope(theWorkspace(), grp);
+    sphx_fields_t f{};
+    f.ax = const_cast<T*>(ax), f.ay = const_cast<T*>(ay), f.az = const_cast<T*>(az);
+    sphx_params_t p{};
+    p.hydro_type = hydroType<T>();
+    check(theWorkspace(),
+          sphx_group_acc_timestep(theWorkspace(), &f, &p, grp.firstBody, grp.lastBody, etaAcc, groupDt));
 }
 
 template void groupDivvTimestepGpu(float, const GroupView&, const float*, float*);
@@ -407,7 +424,11 @@ template void groupDivvTimestepGpu(float, const GroupView&, const double*, float
 template void groupAccTimestepGpu(float, const GroupView&, const float*, const float*, const float*, float*);
 template void groupAccTimestepGpu(float, const GroupView&, const double*, const double*, const double*, float*);
 
-void storeRungGpu(const GroupView&, uint8_t, uint8_t*) { unsupported("block time-step rungs (storeRungGpu)"); }
+void storeRungGpu(const GroupView& grp, uint8_t rung, uint8_t* particleRungs)
+{
+    Groups scope(theWorkspace(), grp);
+    check(theWorkspace(), sphx_store_rung(theWorkspace(), grp.firstBody, grp.lastBody, rung, particleRungs));
+}
 
 unsigned nsGroupSize() { return 64; } // target groups of at most 64 particles, one wavefront each
 

@@ -569,6 +569,57 @@ int sphx_compute_positions(sphx_workspace_t* ws, const sphx_box_t* box, const sp
     });
 }
 
+int sphx_compute_positions_rungs(sphx_workspace_t* ws, const sphx_box_t* box, const sphx_fields_t* f,
+                                 const sphx_history_t* hist, const sphx_params_t* prm, uint32_t first,
+                                 uint32_t last, double dt, const float* dtM1, int numRungs, const uint8_t* rung,
+                                 double constCv)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(box && f && hist && prm && dtM1, SPHX_ERR_ARG, "null argument");
+        computePositions(ws, box, f, hist, prm, first, last, dt, dtM1[0], constCv, dtM1, numRungs, rung);
+        return SPHX_OK;
+    });
+}
+
+int sphx_drift_positions(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_history_t* hist,
+                         const sphx_params_t* prm, uint32_t first, uint32_t last, double dt, double dtBack,
+                         const float* dtM1, int numRungs, const uint8_t* rung, double constCv)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && hist && prm && dtM1, SPHX_ERR_ARG, "null argument");
+        driftPositions(ws, f, hist, prm, first, last, dt, dtBack, dtM1, numRungs, rung, constCv);
+        return SPHX_OK;
+    });
+}
+
+int sphx_group_divv_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                             uint32_t first, uint32_t last, float Krho, float* groupDt)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
+        groupTimestep(ws, f, prm, first, last, false, Krho, groupDt);
+        return SPHX_OK;
+    });
+}
+
+int sphx_group_acc_timestep(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
+                            uint32_t first, uint32_t last, float etaAcc, float* groupDt)
+{
+    return guarded(ws, [&] {
+        SPHX_REQUIRE(f && prm, SPHX_ERR_ARG, "null argument");
+        groupTimestep(ws, f, prm, first, last, true, etaAcc, groupDt);
+        return SPHX_OK;
+    });
+}
+
+int sphx_store_rung(sphx_workspace_t* ws, uint32_t first, uint32_t last, uint8_t rung, uint8_t* particleRungs)
+{
+    return guarded(ws, [&] {
+        storeRung(ws, first, last, rung, particleRungs);
+        return SPHX_OK;
+    });
+}
+
 int sphx_update_smoothing_length(sphx_workspace_t* ws, const sphx_fields_t* f, const sphx_params_t* prm,
                                  uint32_t first, uint32_t last)
 {

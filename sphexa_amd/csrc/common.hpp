@@ -626,7 +626,15 @@ void rhoTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t
 void computeTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                      uint32_t last, sphx_timestep_t* ts);
 void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
-                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv);
+                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv,
+                      const float* dtM1Rung = nullptr, int numRungs = 0, const uint8_t* rung = nullptr);
+//! block time steps: the drift of a hierarchy's particles, the per-group steps, the rung store
+void driftPositions(sphx_workspace* ws, const sphx_fields_t* f, const sphx_history_t* hist, const sphx_params_t* p,
+                    uint32_t first, uint32_t last, double dt, double dtBack, const float* dtM1Rung, int numRungs,
+                    const uint8_t* rung, double constCv);
+void groupTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                   bool acc, float coeff, float* groupDt);
+void storeRung(sphx_workspace* ws, uint32_t first, uint32_t last, uint8_t rung, uint8_t* particleRungs);
 void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,
                            uint32_t last);
 void conservedQuantities(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,

@@ -46,11 +46,22 @@ __device__ __forceinline__ bool fbcCheck(double coord, Th h, double top, double 
     return fbc && (fabs(top - coord) < double(Th(2) * h) || fabs(bottom - coord) < double(Th(2) * h));
 }
 
+constexpr int kMaxRungs = SPHX_MAX_RUNGS;
+
+//! a particle's rung, clamped to the rungs the arguments hold (a larger one is the caller's error)
+__device__ __forceinline__ unsigned rungOf(const uint8_t* rung, uint32_t i)
+{
+    return min(unsigned(rung[i]), unsigned(kMaxRungs - 1));
+}
+
 struct PosArgs
 {
     DevBox      box;
     int         fbc[3];
     double      dt, dt_m1, gamma, constCv;
+    double      dtBack;                // drift: the step taken since the hierarchy's start
+    float       dtM1Rung[kMaxRungs];   // block time steps: previous step per rung ...
+    const uint8_t* rung;               // ... and each particle's rung (NULL: dt_m1)
     Groups      groups; // the call's targets
     double *    x, *y, *z, *temp, *u;
     const double* du;
@@ -82,7 +93,7 @@ __global__ void __launch_bounds__(kIntThreads) positionsKernel(PosArgs a)
         }
     }
 
-    const double dt = a.dt, dt_m1 = a.dt_m1;
+    const double dt = a.dt, dt_m1 = a.rung ? double(a.dtM1Rung[rungOf(a.rung, i)]) : a.dt_m1;
     const double X[3]  = {a.x[i], a.y[i], a.z[i]};
     const double An[3] = {double(ld<T>(a.ax, i)), double(ld<T>(a.ay, i)), double(ld<T>(a.az, i))};
     const double dX[3] = {double(ld<T>(a.x_m1, i)), double(ld<T>(a.y_m1, i)), double(ld<T>(a.z_m1, i))};
@@ -116,6 +127,97 @@ __global__ void __launch_bounds__(kIntThreads) positionsKernel(PosArgs a)
         a.u[i]   = energyUpdate(a.u[i], dt, dt_m1, a.du[i], double(du_m1[i]));
         du_m1[i] = T(a.du[i]);
     }
+}
+
+/* driftKernel (positions_gpu.cu:17-60): a particle of a block time-step hierarchy, advanced by dtBack
+ * from (Xn, An, dXn) at its last force evaluation, is taken back to Xn with positionUpdate(-dtBack)
+ * and advanced by dt from there, both without the periodic wrap (the reference drifts in an open
+ * box so that the octree stays valid); velocities follow, temp or u is recovered and advanced with
+ * energyUpdate. x_m1, y_m1, z_m1 and du_m1 are read only. */
+template<class T>
+__global__ void __launch_bounds__(kIntThreads) driftKernel(PosArgs a)
+{
+#pragma clang fp contract(off)
+    uint32_t i;
+    if (!groupedIndex(a.groups, i)) { return; }
+    const double dt = a.dt, dtBack = a.dtBack;
+    const double dt_m1 = double(a.dtM1Rung[a.rung ? rungOf(a.rung, i) : 0u]);
+    const double X[3]  = {a.x[i], a.y[i], a.z[i]};
+    const double An[3] = {double(ld<T>(a.ax, i)), double(ld<T>(a.ay, i)), double(ld<T>(a.az, i))};
+    const double dX[3] = {double(ld<T>(a.x_m1, i)), double(ld<T>(a.y_m1, i)), double(ld<T>(a.z_m1, i))};
+    double*      xo[3] = {a.x, a.y, a.z};
+    T*           vo[3] = {static_cast<T*>(a.vx), static_cast<T*>(a.vy), static_cast<T*>(a.vz)};
+    for (int d = 0; d < 3; ++d)
+    {
+        // back: positionUpdate(-dtBack, ...) keeps Xn
+        double Vnmhalf = dX[d] * (1.0 / dt_m1);
+        double Vn      = Vnmhalf + 0.5 * dt_m1 * An[d];
+        double dXback  = (Vn + 0.5 * An[d] * fabs(-dtBack)) * -dtBack;
+        double Xn      = X[d] + dXback;
+        // forward from Xn with the same (An, dXn)
+        double Vnp1  = Vn + An[d] * dt;
+        double dXnp1 = (Vn + 0.5 * An[d] * fabs(dt)) * dt;
+        xo[d][i]     = Xn + dXnp1;
+        vo[d][i]     = T(Vnp1);
+    }
+    const T* du_m1 = static_cast<const T*>(a.du_m1);
+    if (a.temp != nullptr)
+    {
+        const T cv      = (a.constCv < 0) ? idealGasCv(ld<T>(a.mui, i), a.gamma) : T(a.constCv);
+        double  u_recov = energyUpdate(a.temp[i] * double(cv), -dtBack, dt_m1, a.du[i], double(du_m1[i]));
+        a.temp[i]       = energyUpdate(u_recov, dt, dt_m1, a.du[i], double(du_m1[i])) / double(cv);
+    }
+    else if (a.u != nullptr)
+    {
+        double u_recov = energyUpdate(a.u[i], -dtBack, dt_m1, a.du[i], double(du_m1[i]));
+        a.u[i]         = energyUpdate(u_recov, dt, dt_m1, a.du[i], double(du_m1[i]));
+    }
+}
+
+/* groupDivvKernel / groupAccKernel (ts_groups.cu:16-36, 51-71), one wavefront per group instead of
+ * one thread: the group's largest divv (signed), or largest |a|^2 = ax^2 + (ay^2 + az^2) in T
+ * (norm2, the right fold of util/array.hpp:236-240), as a wave maximum; rounding that maximum to
+ * float once equals the reference's running float maximum (rounding is monotone). Then
+ * groupDt[g] = min(groupDt[g], Krho / |max|), or min(groupDt[g], etaAcc / sqrt(sqrt(max))), in
+ * float with correctly rounded divide and root. Groups of any size: lanes stride over [start, end). */
+template<class T, bool Acc>
+__global__ void __launch_bounds__(kIntThreads) groupDtKernel(Groups G, float coeff, const T* __restrict__ a,
+                                                           const T* __restrict__ b, const T* __restrict__ c,
+                                                           float* __restrict__ groupDt)
+{
+#pragma clang fp contract(off)
+    const uint32_t g = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (g >= G.num) { return; }
+    uint32_t s, e;
+    if (G.start == nullptr)
+    {
+        s = G.first + g * uint32_t(kGroup);
+        e = min(s + uint32_t(kGroup), G.last);
+    }
+    else { s = G.start[g], e = G.end[g]; }
+    double m = Acc ? 0.0 : -INFINITY;
+    for (uint32_t i = s + laneId(); i < e; i += kWave)
+    {
+        double v;
+        if constexpr (Acc) { v = double(T(a[i] * a[i] + (b[i] * b[i] + c[i] * c[i]))); }
+        else { v = double(a[i]); }
+        m = v > m ? v : m;
+    }
+    m = waveMax(m);
+    if (laneId() == 0)
+    {
+        const float mf = float(m);
+        const float dt = Acc ? coeff / sqrtf(sqrtf(mf)) : coeff / fabsf(mf);
+        groupDt[g]     = fminf(groupDt[g], dt);
+    }
+}
+
+//! storeRungKernel (ts_groups.cu:84-93): particleRungs[i] = rung over the call's targets
+__global__ void __launch_bounds__(kIntThreads) storeRungKernel(Groups G, uint8_t rung, uint8_t* particleRungs)
+{
+    uint32_t i;
+    if (!groupedIndex(G, i)) { return; }
+    particleRungs[i] = rung;
 }
 
 //! updateSmoothingLengthGpuKernel (sph/include/sph/update_h_gpu.cu:40-50): h = updateH(ng0, nc, h)
@@ -201,15 +303,36 @@ double deviceMax(sphx_workspace* ws, uint32_t first, uint32_t last, const void* 
 
 template<class T>
 void launchPositions(sphx_workspace* ws, PosArgs& a, const sphx_fields_t* f, const sphx_history_t* hist,
-                     const sphx_params_t* p, double constCv)
+                     const sphx_params_t* p, double constCv, bool drift)
 {
     // constCv: -1 selects the per-particle idealGasCv(mui, gamma); without mui the caller's value,
     // or the reference host side's idealGasCv(muiConst, gamma) in the mass type (positions.hpp:183)
     if (hist->mui) { a.constCv = -1.0; }
     else { a.constCv = constCv >= 0 ? constCv : double(idealGasCv(T(p->muiConst), p->gamma)); }
-    KernelScope timer(ws, "positions");
-    positionsKernel<T><<<ceilDiv(groupedThreads(a.groups), kIntThreads), kIntThreads, 0, ws->stream>>>(a);
+    const unsigned blocks = ceilDiv(groupedThreads(a.groups), kIntThreads);
+    if (drift)
+    {
+        KernelScope timer(ws, "drift");
+        driftKernel<T><<<blocks, kIntThreads, 0, ws->stream>>>(a);
+    }
+    else
+    {
+        KernelScope timer(ws, "positions");
+        positionsKernel<T><<<blocks, kIntThreads, 0, ws->stream>>>(a);
+    }
     SPHX_CHECK_LAUNCH();
+}
+
+//! the rung arguments of the block time-step entry points into `a`
+void setRungs(PosArgs& a, const float* dtM1Rung, int numRungs, const uint8_t* rung)
+{
+    SPHX_REQUIRE(dtM1Rung != nullptr, SPHX_ERR_ARG, "the previous steps per rung are missing");
+    SPHX_REQUIRE(numRungs >= 1 && numRungs <= kMaxRungs, SPHX_ERR_ARG,
+                 "numRungs must lie in [1, " + std::to_string(kMaxRungs) + "]");
+    for (int r = 0; r < kMaxRungs; ++r)
+        a.dtM1Rung[r] = r < numRungs ? dtM1Rung[r] : 0.0f;
+    a.rung  = rung;
+    a.dt_m1 = dtM1Rung[0];
 }
 
 } // namespace
@@ -242,16 +365,35 @@ void computeTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_para
     ts->minDt    = dt;
 }
 
-void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
-                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv)
+namespace
+{
+
+PosArgs positionArgs(sphx_workspace* ws, const sphx_fields_t* f, const sphx_history_t* hist, uint32_t first,
+                     uint32_t last)
 {
     SPHX_REQUIRE(f->x && f->y && f->z && f->vx && f->vy && f->vz && f->ax && f->ay && f->az, SPHX_ERR_ARG,
                  "positions need x, y, z, vx, vy, vz, ax, ay, az");
     SPHX_REQUIRE(hist->x_m1 && hist->y_m1 && hist->z_m1, SPHX_ERR_ARG, "positions need x_m1, y_m1, z_m1");
     SPHX_REQUIRE(!(f->temp || f->u) || (f->du && hist->du_m1), SPHX_ERR_ARG, "the energy update needs du and du_m1");
     SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
+    PosArgs a{};
+    a.groups = groupsFor(ws, first, last);
+    a.x = f->x, a.y = f->y, a.z = f->z, a.temp = f->temp, a.u = f->temp ? nullptr : f->u, a.du = f->du;
+    a.vx = f->vx, a.vy = f->vy, a.vz = f->vz, a.ax = f->ax, a.ay = f->ay, a.az = f->az, a.h = f->h;
+    a.x_m1 = hist->x_m1, a.y_m1 = hist->y_m1, a.z_m1 = hist->z_m1, a.du_m1 = hist->du_m1, a.mui = hist->mui;
+    a.rung = nullptr;
+    return a;
+}
+
+} // namespace
+
+void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fields_t* f, const sphx_history_t* hist,
+                      const sphx_params_t* p, uint32_t first, uint32_t last, double dt, double dt_m1, double constCv,
+                      const float* dtM1Rung, int numRungs, const uint8_t* rung)
+{
+    PosArgs a = positionArgs(ws, f, hist, first, last);
+    if (dtM1Rung != nullptr) { setRungs(a, dtM1Rung, numRungs, rung); }
     if (first == last) { return; }
-    PosArgs a;
     a.box = makeDevBox(box);
     bool anyFbc = false;
     for (int d = 0; d < 3; ++d)
@@ -260,13 +402,64 @@ void computePositions(sphx_workspace* ws, const sphx_box_t* box, const sphx_fiel
         anyFbc   = anyFbc || a.fbc[d];
     }
     SPHX_REQUIRE(!anyFbc || f->h, SPHX_ERR_ARG, "fixed boundaries need h");
-    a.dt = dt, a.dt_m1 = dt_m1, a.gamma = p->gamma, a.constCv = -1;
-    a.groups = groupsFor(ws, first, last);
-    a.x = f->x, a.y = f->y, a.z = f->z, a.temp = f->temp, a.u = f->temp ? nullptr : f->u, a.du = f->du;
-    a.vx = f->vx, a.vy = f->vy, a.vz = f->vz, a.ax = f->ax, a.ay = f->ay, a.az = f->az, a.h = f->h;
-    a.x_m1 = hist->x_m1, a.y_m1 = hist->y_m1, a.z_m1 = hist->z_m1, a.du_m1 = hist->du_m1, a.mui = hist->mui;
-    if (p->hydro_type == SPHX_F32) { launchPositions<float>(ws, a, f, hist, p, constCv); }
-    else { launchPositions<double>(ws, a, f, hist, p, constCv); }
+    a.dt = dt, a.gamma = p->gamma, a.constCv = -1;
+    if (dtM1Rung == nullptr) { a.dt_m1 = dt_m1; }
+    if (p->hydro_type == SPHX_F32) { launchPositions<float>(ws, a, f, hist, p, constCv, false); }
+    else { launchPositions<double>(ws, a, f, hist, p, constCv, false); }
+}
+
+void driftPositions(sphx_workspace* ws, const sphx_fields_t* f, const sphx_history_t* hist, const sphx_params_t* p,
+                    uint32_t first, uint32_t last, double dt, double dtBack, const float* dtM1Rung, int numRungs,
+                    const uint8_t* rung, double constCv)
+{
+    PosArgs a = positionArgs(ws, f, hist, first, last);
+    setRungs(a, dtM1Rung, numRungs, rung);
+    if (first == last) { return; }
+    a.dt = dt, a.dtBack = dtBack, a.gamma = p->gamma, a.constCv = -1;
+    if (p->hydro_type == SPHX_F32) { launchPositions<float>(ws, a, f, hist, p, constCv, true); }
+    else { launchPositions<double>(ws, a, f, hist, p, constCv, true); }
+}
+
+void groupTimestep(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first, uint32_t last,
+                   bool acc, float coeff, float* groupDt)
+{
+    SPHX_REQUIRE(groupDt != nullptr, SPHX_ERR_ARG, "groupDt is missing");
+    SPHX_REQUIRE(acc ? (f->ax && f->ay && f->az) : f->divv != nullptr, SPHX_ERR_ARG,
+                 acc ? "the acceleration step needs ax, ay, az" : "the divv step needs divv");
+    SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
+    const Groups G = groupsFor(ws, first, last);
+    if (G.num == 0) { return; }
+    const unsigned blocks = ceilDiv(size_t(G.num) * kWave, kIntThreads);
+    KernelScope    timer(ws, "group_dt");
+    if (p->hydro_type == SPHX_F32)
+    {
+        if (acc)
+            groupDtKernel<float, true><<<blocks, kIntThreads, 0, ws->stream>>>(
+                G, coeff, (const float*)f->ax, (const float*)f->ay, (const float*)f->az, groupDt);
+        else
+            groupDtKernel<float, false><<<blocks, kIntThreads, 0, ws->stream>>>(G, coeff, (const float*)f->divv,
+                                                                               nullptr, nullptr, groupDt);
+    }
+    else
+    {
+        if (acc)
+            groupDtKernel<double, true><<<blocks, kIntThreads, 0, ws->stream>>>(
+                G, coeff, (const double*)f->ax, (const double*)f->ay, (const double*)f->az, groupDt);
+        else
+            groupDtKernel<double, false><<<blocks, kIntThreads, 0, ws->stream>>>(G, coeff, (const double*)f->divv,
+                                                                                nullptr, nullptr, groupDt);
+    }
+    SPHX_CHECK_LAUNCH();
+}
+
+void storeRung(sphx_workspace* ws, uint32_t first, uint32_t last, uint8_t rung, uint8_t* particleRungs)
+{
+    SPHX_REQUIRE(particleRungs != nullptr, SPHX_ERR_ARG, "particleRungs is missing");
+    SPHX_REQUIRE(first <= last, SPHX_ERR_ARG, "invalid range");
+    if (first == last) { return; }
+    const Groups G = groupsFor(ws, first, last);
+    storeRungKernel<<<ceilDiv(groupedThreads(G), kIntThreads), kIntThreads, 0, ws->stream>>>(G, rung, particleRungs);
+    SPHX_CHECK_LAUNCH();
 }
 
 void updateSmoothingLength(sphx_workspace* ws, const sphx_fields_t* f, const sphx_params_t* p, uint32_t first,

@@ -58,6 +58,12 @@ _SIGS = {
     "sphx_rho_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
     "sphx_compute_timestep": [_P, _FP, _PP, _U32, _U32, _TSP],
     "sphx_compute_positions": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double, C.c_double],
+    "sphx_compute_positions_rungs": [_P, _BOXP, _FP, _HP, _PP, _U32, _U32, C.c_double, _P, C.c_int, _P,
+                                     C.c_double],
+    "sphx_drift_positions": [_P, _FP, _HP, _PP, _U32, _U32, C.c_double, C.c_double, _P, C.c_int, _P, C.c_double],
+    "sphx_group_divv_timestep": [_P, _FP, _PP, _U32, _U32, C.c_float, _P],
+    "sphx_group_acc_timestep": [_P, _FP, _PP, _U32, _U32, C.c_float, _P],
+    "sphx_store_rung": [_P, _U32, _U32, C.c_uint8, _P],
     "sphx_update_smoothing_length": [_P, _FP, _PP, _U32, _U32],
     "sphx_conserved_quantities": [_P, _FP, _PP, _U32, _U32, C.c_double, C.POINTER(abi.Conserved)],
     "sphx_density_std_vjp": [_P, _BOXP, _FP, _PP, _U32, _U32, _AP, _AP],
@@ -241,6 +247,30 @@ class Workspace:
         """Press positions and Adams-Bashforth energy; const_cv < 0: cv from params.muiConst."""
         self.call("sphx_compute_positions", C.byref(box), C.byref(fields), C.byref(history), C.byref(params), first,
                   last, dt, dt_m1, const_cv)
+
+    def compute_positions_rungs(self, box, fields, history, params, first, last, dt, dt_m1_rungs, rung_ptr,
+                                const_cv=-1.0):
+        """compute_positions with the previous step per rung (a sequence of floats) and a device
+        uint8 rung per particle (pointer, or None)."""
+        arr = (C.c_float * len(dt_m1_rungs))(*dt_m1_rungs)
+        self.call("sphx_compute_positions_rungs", C.byref(box), C.byref(fields), C.byref(history), C.byref(params),
+                  first, last, dt, arr, len(dt_m1_rungs), rung_ptr, const_cv)
+
+    def drift_positions(self, fields, history, params, first, last, dt, dt_back, dt_m1_rungs, rung_ptr,
+                        const_cv=-1.0):
+        """drift the particles of a block time-step hierarchy back by dt_back and forward by dt"""
+        arr = (C.c_float * len(dt_m1_rungs))(*dt_m1_rungs)
+        self.call("sphx_drift_positions", C.byref(fields), C.byref(history), C.byref(params), first, last, dt, dt_back,
+                  arr, len(dt_m1_rungs), rung_ptr, const_cv)
+
+    def group_divv_timestep(self, fields, params, first, last, krho, group_dt_ptr):
+        self.call("sphx_group_divv_timestep", C.byref(fields), C.byref(params), first, last, krho, group_dt_ptr)
+
+    def group_acc_timestep(self, fields, params, first, last, eta_acc, group_dt_ptr):
+        self.call("sphx_group_acc_timestep", C.byref(fields), C.byref(params), first, last, eta_acc, group_dt_ptr)
+
+    def store_rung(self, first, last, rung, rungs_ptr):
+        self.call("sphx_store_rung", first, last, rung, rungs_ptr)
 
     def conserved_quantities(self, fields, params, first, last, cv=-1.0):
         """Energies and momenta over [first, last) (sphx_conserved_quantities) as a dict; cv < 0: from

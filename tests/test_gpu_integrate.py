@@ -276,3 +276,127 @@ def test_positions_explicit_const_cv(ws, mixed):
         torch.cuda.synchronize()
         out.append(t["temp"].cpu().numpy())
     assert np.array_equal(out[0], out[1])
+
+
+# ---- block time steps ---------------------------------------------------------------------------
+
+def _device_history(t):
+    hs = abi.History()
+    for k in abi.HISTORY_FIELDS:
+        if k in t:
+            setattr(hs, k, t[k].data_ptr())
+    return hs
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("energy", ["u", "temp_mui"])
+def test_positions_rungs_and_drift_match_oracle(ws, orc, mixed, energy):
+    """sphx_compute_positions_rungs (dt_m1 per particle from its rung) and sphx_drift_positions (back
+    by dt_back, forward by dt) against the oracle: positions, velocities and the previous-step fields
+    bit for bit, energies within 2 ulp (the u < 0 guard's exp)."""
+    import torch
+
+    from test_integrate_oracle import RUNG_DT_M1, run_drift, run_positions_rungs
+
+    rng = np.random.default_rng(41)
+    n = 20000
+    hydro = np.float32 if mixed else np.float64
+    params = abi.Params.defaults(1.0, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC)
+    ek = "u" if energy == "u" else "temp"
+    for op in ("positions", "drift"):
+        d = _positions_case(rng, n, hydro, abi.PERIODIC, energy)
+        rung = rng.integers(0, 4, n).astype(np.uint8)
+        t = to_device(d, mixed)
+        rung_t = torch.from_numpy(rung).cuda()
+        print(f"case {op} mixed={mixed} energy={energy}", flush=True)
+        if op == "positions":
+            ws.compute_positions_rungs(box, abi.fields_from(t), _device_history(t), params, 0, n, 0.01, RUNG_DT_M1,
+                                       rung_t.data_ptr())
+            run_positions_rungs(orc, d, box, params, n, 0.01, RUNG_DT_M1, rung)
+        else:
+            ws.drift_positions(abi.fields_from(t), _device_history(t), params, 0, n, 0.03, 0.0125, RUNG_DT_M1,
+                               rung_t.data_ptr())
+            run_drift(orc, d, params, n, 0.03, 0.0125, RUNG_DT_M1, rung)
+        torch.cuda.synchronize()
+        for k in ("x", "y", "z", "vx", "vy", "vz", "x_m1", "y_m1", "z_m1", "du_m1"):
+            g = t[k].cpu().numpy()
+            assert np.array_equal(g, d[k]), (op, k, np.max(np.abs(g - d[k])))
+        assert ulp_diff(t[ek].cpu().numpy(), d[ek]) <= 2, (op, ek)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_group_timesteps_and_rungs_match_oracle(ws, orc, mixed):
+    """sphx_group_divv_timestep and sphx_group_acc_timestep over a group view with groups of 0 to 120
+    particles (one wavefront strides over a group larger than 64), sphx_store_rung over those groups
+    cut to 64, and all three over the consecutive groups of 64 without a view: groupDt and the rungs
+    bit for bit."""
+    import torch
+
+    from test_integrate_oracle import _groups_case
+
+    rng = np.random.default_rng(42)
+    n = 20000
+    hydro = np.float32 if mixed else np.float64
+    params = abi.Params.defaults(1.0, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64)
+    start, end = _groups_case(rng, n)
+    divv = rng.normal(0, 3, n).astype(hydro)
+    acc = [rng.normal(0, 5, n).astype(hydro) for _ in range(3)]
+    host = {"divv": divv, "ax": acc[0], "ay": acc[1], "az": acc[2]}
+    t = {k: torch.from_numpy(v).cuda() for k, v in host.items()}
+    f_dev, f_host = abi.fields_from(t), abi.fields_from(host)
+    eta = float(np.float32(0.2 * np.sqrt(0.005)))
+    for view in (True, False):
+        if view:
+            s, e = start, end
+            s_t = torch.from_numpy(s.astype(np.int32)).cuda()
+            e_t = torch.from_numpy(e.astype(np.int32)).cuda()
+            g = abi.Groups(firstBody=0, lastBody=n, numGroups=s.size)
+            g.groupStart, g.groupEnd = s_t.data_ptr(), e_t.data_ptr()
+            ws.set_groups(g)
+        else:
+            s = np.arange(0, n, 64, dtype=np.uint32)
+            e = np.minimum(s + 64, n).astype(np.uint32)
+        print(f"case group steps mixed={mixed} view={view} groups={s.size}", flush=True)
+        dt0 = rng.uniform(0.01, 0.5, s.size).astype(np.float32)
+        for kind in ("divv", "acc"):
+            gdt = torch.from_numpy(dt0.copy()).cuda()
+            ref = dt0.copy()
+            if kind == "divv":
+                ws.group_divv_timestep(f_dev, params, 0, n, 0.06, gdt.data_ptr())
+                orc.group_divv_timestep(C.byref(f_host), C.byref(params), s.ctypes.data, e.ctypes.data, s.size, 0.06,
+                                        ref.ctypes.data)
+            else:
+                ws.group_acc_timestep(f_dev, params, 0, n, eta, gdt.data_ptr())
+                orc.group_acc_timestep(C.byref(f_host), C.byref(params), s.ctypes.data, e.ctypes.data, s.size, eta,
+                                       ref.ctypes.data)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(gdt.cpu().numpy(), ref)
+        # the rung store, like the elementwise kernels, takes groups of at most 64 particles (one
+        # wavefront each, storeRungKernel: one lane per particle)
+        e64 = np.minimum(e, s + 64).astype(np.uint32)
+        if view:
+            e64_t = torch.from_numpy(e64.astype(np.int32)).cuda()
+            g.groupEnd = e64_t.data_ptr()
+            ws.set_groups(g)
+        rungs = torch.full((n,), 9, dtype=torch.uint8, device="cuda")
+        ws.store_rung(0, n, 3, rungs.data_ptr())
+        ref = np.full(n, 9, np.uint8)
+        orc.store_rung(s.ctypes.data, e64.ctypes.data, s.size, 3, ref.ctypes.data)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(rungs.cpu().numpy(), ref)
+        ws.set_groups(None)
+
+
+def test_block_timestep_arguments_are_checked(ws):
+    """numRungs outside [1, 4] and a missing groupDt are refused before any launch"""
+    import torch
+
+    n = 128
+    t = {k: torch.zeros(n, dtype=torch.float64, device="cuda") for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay",
+                                                                         "az", "x_m1", "y_m1", "z_m1", "divv")}
+    params = abi.Params.defaults(1.0)
+    with pytest.raises(Exception, match="numRungs"):
+        ws.drift_positions(abi.fields_from(t), _device_history(t), params, 0, n, 0.1, 0.0, [0.1] * 5, None)
+    with pytest.raises(Exception, match="groupDt"):
+        ws.group_divv_timestep(abi.fields_from(t), params, 0, n, 0.06, None)

@@ -205,3 +205,163 @@ def test_compute_timestep_closed_form(orc):
     ts2 = abi.Timestep.defaults(minDt=1e-3, minDtCourant=1.0)
     orc.compute_timestep(C.byref(f), C.byref(params), 0, n, C.byref(ts2))
     assert ts2.minDt == 1.1e-3
+
+
+# ---- block time steps ---------------------------------------------------------------------------
+
+def history_of(d):
+    hs = abi.History()
+    for k in abi.HISTORY_FIELDS:
+        if k in d:
+            setattr(hs, k, d[k].ctypes.data)
+    return hs
+
+
+def run_positions_rungs(lib, d, box, params, n, dt, dt_m1_rungs, rung):
+    f = abi.fields_from(d)
+    arr = np.asarray(dt_m1_rungs, np.float32)
+    lib.compute_positions_rungs(C.byref(box), C.byref(f), C.byref(history_of(d)), C.byref(params), 0, n, dt,
+                                arr.ctypes.data, rung.ctypes.data if rung is not None else None)
+
+
+def run_drift(lib, d, params, n, dt, dt_back, dt_m1_rungs, rung):
+    f = abi.fields_from(d)
+    arr = np.asarray(dt_m1_rungs, np.float32)
+    lib.drift_positions(C.byref(f), C.byref(history_of(d)), C.byref(params), 0, n, dt, dt_back, arr.ctypes.data,
+                        rung.ctypes.data if rung is not None else None)
+
+
+RUNG_DT_M1 = (0.0125, 0.025, 0.05, 0.1)
+
+
+@pytest.mark.parametrize("energy", ["u", "temp", "temp_mui"])
+@pytest.mark.parametrize("hydro", [np.float64, np.float32])
+def test_positions_rungs_select_the_rung_step(orc, energy, hydro):
+    """computePositions with a rung array (positions_gpu.cu:140): each particle advances exactly as the
+    plain update (pinned against the reference above) with dt_m1 = dt_m1[rung[i]]; without a rung
+    array every particle takes dt_m1[0]."""
+    rng = np.random.default_rng(31)
+    n = 600
+    d0 = _positions_case(rng, n, hydro, abi.PERIODIC, energy)
+    rung = rng.integers(0, 4, n).astype(np.uint8)
+    box = abi.Box.make(-0.5, 0.5, abi.PERIODIC)
+    params = abi.Params.defaults(1.0, hydro_type=abi.SPHX_F32 if hydro == np.float32 else abi.SPHX_F64)
+    d = {k: v.copy() for k, v in d0.items()}
+    run_positions_rungs(orc, d, box, params, n, 0.01, RUNG_DT_M1, rung)
+    for r in range(4):
+        e = {k: v.copy() for k, v in d0.items()}
+        run_positions(orc, e, box, params, n, 0.01, float(np.float32(RUNG_DT_M1[r])))
+        sel = rung == r
+        for k in d:
+            assert np.array_equal(d[k][sel], e[k][sel]), (k, r)
+    d = {k: v.copy() for k, v in d0.items()}
+    run_positions_rungs(orc, d, box, params, n, 0.01, RUNG_DT_M1, None)
+    e = {k: v.copy() for k, v in d0.items()}
+    run_positions(orc, e, box, params, n, 0.01, float(np.float32(RUNG_DT_M1[0])))
+    for k in d:
+        assert np.array_equal(d[k], e[k]), k
+
+
+@pytest.mark.parametrize("energy", ["u", "temp"])
+def test_drift_composes_the_position_update(orc, energy):
+    """driftKernel (positions_gpu.cu:17-60): back by dt_back, then forward by dt from the recovered
+    state, through positionUpdate and energyUpdate (pinned against the reference above), in an open
+    box; x_m1 and du_m1 unchanged. Invariants: dt = dt_back = 0 is the identity, and a drift with
+    dt_back = 0 lands where the plain update (without its wrap) puts the particle."""
+    rng = np.random.default_rng(32)
+    n, hydro = 300, np.float64
+    d0 = _positions_case(rng, n, hydro, abi.OPEN, energy)
+    rung = rng.integers(0, 4, n).astype(np.uint8)
+    params = abi.Params.defaults(1.0)
+    dt, dt_back = 0.03, 0.0125
+    d = {k: v.copy() for k, v in d0.items()}
+    run_drift(orc, d, params, n, dt, dt_back, RUNG_DT_M1, rung)
+    open_box = abi.Box.make(0.0, 1.0, abi.OPEN)
+    cvc = 8.317e7 / 10.0 / (5.0 / 3.0 - 1.0)
+    for i in range(0, n, 7):
+        dm1 = float(np.float32(RUNG_DT_M1[rung[i]]))
+        X = np.array([d0[k][i] for k in "xyz"])
+        A = np.array([d0[k][i] for k in ("ax", "ay", "az")])
+        dX = np.array([d0[k][i] for k in ("x_m1", "y_m1", "z_m1")], np.float64)
+        Xn, V, dXo = (np.zeros(3) for _ in range(3))
+        orc.position_update(-dt_back, dm1, X.ctypes.data, A.ctypes.data, dX.ctypes.data, C.byref(open_box),
+                            Xn.ctypes.data, V.ctypes.data, dXo.ctypes.data)
+        X1 = np.zeros(3)
+        orc.position_update(dt, dm1, Xn.ctypes.data, A.ctypes.data, dX.ctypes.data, C.byref(open_box), X1.ctypes.data,
+                            V.ctypes.data, dXo.ctypes.data)
+        assert [d[k][i] for k in "xyz"] == list(X1)
+        assert [d[k][i] for k in ("vx", "vy", "vz")] == list(V)
+        if energy == "u":
+            u0 = orc.energy_update(d0["u"][i], -dt_back, dm1, d0["du"][i], d0["du_m1"][i])
+            assert d["u"][i] == orc.energy_update(u0, dt, dm1, d0["du"][i], d0["du_m1"][i])
+        else:
+            u0 = orc.energy_update(d0["temp"][i] * cvc, -dt_back, dm1, d0["du"][i], d0["du_m1"][i])
+            assert d["temp"][i] == orc.energy_update(u0, dt, dm1, d0["du"][i], d0["du_m1"][i]) / cvc
+    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
+        assert np.array_equal(d[k], d0[k]), k
+    d = {k: v.copy() for k, v in d0.items()}
+    run_drift(orc, d, params, n, 0.0, 0.0, RUNG_DT_M1, rung)
+    for k in ("x", "y", "z"):
+        assert np.array_equal(d[k], d0[k]), k
+    d = {k: v.copy() for k, v in d0.items()}
+    run_drift(orc, d, params, n, dt, 0.0, RUNG_DT_M1, None)
+    e = {k: v.copy() for k, v in d0.items()}
+    run_positions(orc, e, abi.Box.make(-5.0, 5.0, abi.OPEN), params, n, dt, float(np.float32(RUNG_DT_M1[0])))
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(d[k], e[k]), k
+
+
+def _groups_case(rng, n):
+    """uneven groups: sizes 0 to 120 (over a wavefront), contiguous, covering [0, n)"""
+    sizes = rng.integers(0, 121, 200)
+    sizes = sizes[np.cumsum(sizes) <= n]
+    start = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint32)
+    end = (start + sizes).astype(np.uint32)
+    return start, end
+
+
+def group_dt_numpy(kind, start, end, a, coeff, dt0):
+    """the reference's per-group steps in float32 (ts_groups.cu:16-36, 51-71), written with numpy"""
+    out = dt0.astype(np.float32).copy()
+    for g, (s, e) in enumerate(zip(start, end)):
+        if kind == "divv":
+            m = np.float32(np.max(a[s:e])) if e > s else np.float32(-np.inf)
+            step = np.float32(coeff) / np.float32(abs(m))
+        else:
+            ax, ay, az = (v[s:e] for v in a)
+            n2 = ax * ax + (ay * ay + az * az)
+            m = np.float32(max(0.0, np.max(n2))) if e > s else np.float32(0.0)
+            step = np.float32(coeff) / np.sqrt(np.sqrt(m))
+        out[g] = min(out[g], step)
+    return out
+
+
+@pytest.mark.parametrize("hydro", [np.float64, np.float32])
+def test_group_timesteps_against_numpy(orc, hydro):
+    """groupDivvTimestep: the signed maximum of divv (a group of compressing particles only, divv < 0,
+    takes its least negative value, not its largest |divv|), Krho / |max| in float; groupAccTimestep:
+    etaAcc / sqrt(sqrt(max |a|^2)); both only lower groupDt; empty groups give 0 (divv) or inf (acc)."""
+    rng = np.random.default_rng(33)
+    n = 6000
+    start, end = _groups_case(rng, n)
+    divv = rng.normal(0, 3, n).astype(hydro)
+    divv[start[3]:end[3]] = -np.abs(divv[start[3]:end[3]]) - 0.5  # all compressing
+    acc = [rng.normal(0, 5, n).astype(hydro) for _ in range(3)]
+    dt0 = rng.uniform(0.01, 0.5, start.size).astype(np.float32)
+    params = abi.Params.defaults(1.0, hydro_type=abi.SPHX_F32 if hydro == np.float32 else abi.SPHX_F64)
+    f = abi.fields_from({"divv": divv, "ax": acc[0], "ay": acc[1], "az": acc[2]})
+    got = dt0.copy()
+    orc.group_divv_timestep(C.byref(f), C.byref(params), start.ctypes.data, end.ctypes.data, start.size, 0.06,
+                            got.ctypes.data)
+    np.testing.assert_array_equal(got, group_dt_numpy("divv", start, end, divv, 0.06, dt0))
+    got = dt0.copy()
+    eta = np.float32(0.2 * np.sqrt(0.005))
+    orc.group_acc_timestep(C.byref(f), C.byref(params), start.ctypes.data, end.ctypes.data, start.size, eta,
+                           got.ctypes.data)
+    np.testing.assert_array_equal(got, group_dt_numpy("acc", start, end, acc, eta, dt0))
+    rungs = np.full(n, 9, np.uint8)
+    orc.store_rung(start.ctypes.data, end.ctypes.data, start.size, 2, rungs.ctypes.data)
+    covered = np.zeros(n, bool)
+    for s, e in zip(start, end):
+        covered[s:e] = True
+    assert np.all(rungs[covered] == 2) and np.all(rungs[~covered] == 9)
