@@ -195,7 +195,7 @@ enum class Fold
 };
 
 //! Test the `n` staged candidates against the lane's target. Hits are recorded as bits of a
-//! per-lane mask over chunks of 32 candidates and appended afterwards in candidate order, so the
+//! per-lane mask over the tile's candidates and appended afterwards in candidate order, so the
 //! list keeps the traversal order while the test loop stays free of the divergent append.
 template<Fold mode, class Sink>
 __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, const double* tx, const double* ty,
@@ -206,10 +206,8 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
 #pragma clang fp contract(off)
     const DevBox&  b     = a.box;
     const unsigned ngmax = a.ngmax;
-    for (uint32_t c0 = 0; c0 < n; c0 += 32)
-    {
-        const uint32_t cn = min(n - c0, 32u);
-        uint32_t       m  = 0;
+    auto test32 = [&](uint32_t c0, uint32_t cn) {
+        uint32_t m = 0;
 #pragma unroll 4
         for (uint32_t c = 0; c < cn; ++c)
         {
@@ -224,6 +222,16 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
             double d2 = dx * dx + dy * dy + dz * dz;
             m |= d2 < t.r2 ? (1u << c) : 0u;
         }
+        return m;
+    };
+    // a tile holds at most 64 candidates: one 64-bit hit mask per lane, drained in one pass (the
+    // drain runs as many rounds as the busiest lane has hits, and one mask over the whole tile
+    // needs fewer rounds than two over its halves)
+    for (uint32_t c0 = 0; c0 < n; c0 += 64)
+    {
+        const uint32_t cn = uniform(min(n - c0, 64u));
+        uint64_t       m  = test32(c0, min(cn, 32u));
+        if (cn > 32u) { m |= uint64_t(test32(c0 + 32u, cn - 32u)) << 32; }
         if (!t.active) { m = 0; }
         while (m)
         {
@@ -231,7 +239,7 @@ __device__ __forceinline__ void sweepTile(const SearchArgs& a, const Target& t, 
             for (int u = 0; u < Sink::kBatch; ++u)
             {
                 const bool     have  = m != 0;
-                const uint32_t c     = c0 + (have ? uint32_t(__builtin_ctz(m)) : 0u);
+                const uint32_t c     = c0 + (have ? uint32_t(__builtin_ctzll(m)) : 0u);
                 const uint32_t j     = tj[c];
                 const bool     keep  = have && j != t.i;
                 const bool     store = keep && count < ngmax;
