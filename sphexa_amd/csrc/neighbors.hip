@@ -77,7 +77,8 @@ struct SearchArgs
 };
 
 //! per-wave LDS: traversal stack, candidate leaves, the kept leaves' particle ranges, and the
-//! double-buffered candidate tile
+//! double-buffered candidate tile (with the candidates' masses for the fused XMass only)
+template<bool Mass>
 struct WaveLds
 {
     int32_t  stack[kStackCap];      // pending nodes, the first in visiting order on top
@@ -88,11 +89,13 @@ struct WaveLds
     int32_t  leafImg[kLeafCap];     // kept leaves: periodic image code (imageShift)
     double   tx[2][kWave], ty[2][kWave], tz[2][kWave];
     uint32_t tj[2][kWave]; // particle index of each staged candidate
-    double   tm[2][kWave]; // its mass (fused XMass only)
+    double   tm[2][Mass ? kWave : 1]; // its mass (fused XMass only)
 };
 
-// sixteen waves share a CU's 160 KB of LDS
-static_assert(sizeof(WaveLds) <= 160 * 1024 / 16, "search LDS must leave room for 16 waves per CU");
+// sixteen waves of the search share a CU's 160 KB of LDS (the fused search + XMass, with its masses,
+// holds three waves per SIMD by its registers)
+static_assert(sizeof(WaveLds<false>) <= 160 * 1024 / 16, "search LDS must leave room for 16 waves per CU");
+static_assert(sizeof(WaveLds<true>) <= 160 * 1024 / 12, "fused search LDS must leave room for 12 waves per CU");
 
 //! per-target state of a lane
 struct Target
@@ -312,8 +315,8 @@ __device__ __forceinline__ double imageShift(const DevBox& b, int code, int d)
 }
 
 //! sweep the first n candidates of tile buffer `buf`, all of image `code`
-template<class Sink>
-__device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t, WaveLds& lds, int buf, uint32_t n,
+template<class Sink, class Lds>
+__device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t, Lds& lds, int buf, uint32_t n,
                                             int code, unsigned& count, ListWriter& out, Sink& sink)
 {
     __builtin_amdgcn_wave_barrier(); // a wave's LDS accesses complete in order
@@ -343,8 +346,8 @@ __device__ __forceinline__ void sweepBuffer(const SearchArgs& a, const Target& t
  *    A full tile of 64 (or the last one, or one whose image ends) is swept, while the next
  *    window's loads are in flight. The candidate order is therefore the traversal order, as in
  *    the reference. */
-template<class Sink>
-__device__ __forceinline__ void flushLeaves(const SearchArgs& a, const Target& t, WaveLds& lds, const int32_t* leaves, int nLeaf,
+template<class Sink, class Lds>
+__device__ __forceinline__ void flushLeaves(const SearchArgs& a, const Target& t, Lds& lds, const int32_t* leaves, int nLeaf,
                             const double gc[3],
                             const double ge[3], double reach, unsigned& count, ListWriter& out, Sink& sink)
 {
@@ -529,7 +532,8 @@ __device__ __forceinline__ void flushLeaves(const SearchArgs& a, const Target& t
 
 //! Sort the first n entries of the found-leaf buffer by rank (bitonic network over the next
 //! power of two, the padding ranked last; ranks of distinct leaves are distinct)
-__device__ __forceinline__ void sortFoundLeaves(WaveLds& lds, int n)
+template<class Lds>
+__device__ __forceinline__ void sortFoundLeaves(Lds& lds, int n)
 {
     const unsigned lane = laneId();
     // leaves found in one pop arrive in order: often there is nothing to sort
@@ -587,8 +591,8 @@ __device__ __forceinline__ void sortFoundLeaves(WaveLds& lds, int n)
  * i.e. it proceeds depth first and adds at most 7 entries per tree level below the popped node
  * (<= 7*20 = 140 < 160) on top of the stack it started from. A single pop adds at most one leaf,
  * the first in order of all, and the buffer is swept whenever it reaches its spare slots. */
-template<class Sink>
-__device__ __forceinline__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, ListWriter& out,
+template<class Sink, class Lds>
+__device__ __forceinline__ unsigned searchGroupInto(const SearchArgs& a, const Target& t, double radius, Lds& lds, ListWriter& out,
                                     Sink& sink)
 {
 #pragma clang fp contract(off)
@@ -712,8 +716,8 @@ __device__ __forceinline__ unsigned searchGroupInto(const SearchArgs& a, const T
 }
 
 //! search of the active lanes' targets into their lists (slot 0 at myList); returns the counts
-template<class Sink>
-__device__ __forceinline__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, WaveLds& lds, uint32_t* myList,
+template<class Sink, class Lds>
+__device__ __forceinline__ unsigned searchGroup(const SearchArgs& a, const Target& t, double radius, Lds& lds, uint32_t* myList,
                                 Sink& sink, uint32_t& swept)
 {
     ListWriter out{myList};
@@ -760,7 +764,7 @@ __global__ void __launch_bounds__(kWavesPerBlock * kWave)
 __attribute__((amdgpu_waves_per_eu(XMass ? 1 : isMixed<Th> ? SPHX_WAVES_SEARCH_F32 : SPHX_WAVES_SEARCH_F64)))
 searchKernel(SearchArgs a, Th* __restrict__ h)
 {
-    __shared__ WaveLds ldsMem[kWavesPerBlock];
+    __shared__ WaveLds<XMass> ldsMem[kWavesPerBlock];
 
     const unsigned wid  = threadIdx.x / kWave;
     const unsigned lane = laneId();
@@ -771,7 +775,7 @@ searchKernel(SearchArgs a, Th* __restrict__ h)
     groupRange(a.groups, g, base, end, oversized);
     base = uniform(base), end = uniform(end);
     if (oversized && lane == 0) { atomicAdd(&a.stats[kStatBadGroups], 1ull); }
-    WaveLds& lds = ldsMem[wid];
+    WaveLds<XMass>& lds = ldsMem[wid];
 
     Target t;
     t.i      = base + lane;
