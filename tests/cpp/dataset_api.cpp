@@ -3,7 +3,7 @@
 // main/src/propagator/ve_hydro.hpp:130-204), reading the initial conditions from a binary file
 // and writing the SFC-ordered results for comparison with the CPU oracle.
 //
-//   dataset_api <in.bin> <out.bin> std|ve|std32|ve32 [cpu|gpu] [steps]
+//   dataset_api <in.bin> <out.bin> std|ve|std32|ve32 [cpu|gpu|bdt] [steps]
 //
 // "std32"/"ve32" run the mixed form: a DeviceDataset<float> whose HydroType fields are float and
 // whose x y z u temp du stay double, the reference's default types (sph/include/sph/types.hpp:40-45:
@@ -11,6 +11,12 @@
 // "cpu" (the default) calls findNeighborsSfc and then computeDensity / computeXMass, as the
 // reference's CPU build does; "gpu" calls the fused findNeighborsDensity / findNeighborsXMass,
 // which is how the reference's GPU build runs the search (inside computeDensity / computeXMass).
+// "bdt" (VE only) runs the "cpu" pass and then the block time-step calls of HydroVeBdtProp
+// (main/src/propagator/ve_hydro_bdt.hpp:258-291, 337-360) over consecutive groups of 64:
+// groupDivvTimestep and groupAccTimestep from groupDt = 1e10, storeRung (rung 1), computePositions
+// with the rungs (dt 2e-4, dt_m1 per rung 1e-4, 2e-4, 4e-4, 8e-4) and driftPositions (forward 3e-4,
+// back 1e-4), and writes out.bin: the groups' dt (floats widened to double), n uint8 rungs, then n
+// doubles each of x y z vx vy vz temp.
 // With `steps` the driver runs that many full std steps, forces then the integrate phase of
 // HydroProp (main/src/propagator/std_hydro.hpp:170-181: computeTimestep, computePositions,
 // updateSmoothingLength) from minDt = minDt_m1 = 1e-4, and writes instead
@@ -25,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -83,7 +90,7 @@ bool download(FILE* out, const T* src, size_t n)
 
 template<class H>
 int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, double lo, double hi, bool ve,
-        bool fused, int steps, const char* outPath)
+        bool fused, bool bdt, int steps, const char* outPath)
 {
     DeviceDataset<H> d;
     if (sphx_workspace_create(&d.workspace, 0, nullptr) != SPHX_OK) { return 2; }
@@ -112,7 +119,7 @@ int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, dou
             owned(s.u, n);
             for (H** p : {&s.rho, &s.p}) owned(*p, n);
         }
-        if (steps > 0)
+        if (steps > 0 || bdt)
             for (H** p : {&s.x_m1, &s.y_m1, &s.z_m1, &s.du_m1}) owned(*p, n);
         owned(s.keys, n);
         owned(s.nc, n);
@@ -130,8 +137,16 @@ int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, dou
     Box       box0(lo, hi, periodic ? BoundaryType::periodic : BoundaryType::open);
     Domain    domain(0, 1, 64, 64, 0.5f, box0);
     GroupView grp{0, uint32_t(n), 0, nullptr, nullptr};
+    const uint32_t numGroups = uint32_t((n + 63) / 64);
+    float*         groupDt   = nullptr;
+    uint8_t*       rungs     = nullptr;
     try
     {
+        if (bdt)
+        {
+            owned(groupDt, numGroups);
+            owned(rungs, n);
+        }
         for (int step = 0; step < std::max(steps, 1); ++step)
         {
             // the pass alone keeps the given box (the CPU oracle's); steps refit it as Domain::sync does
@@ -157,6 +172,19 @@ int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, dou
                 computeIAD(grp, d, box);
                 computeMomentumEnergySTD(grp, d, box);
             }
+            if (bdt)
+            {
+                GroupView all{0, uint32_t(n), numGroups, nullptr, nullptr};
+                std::vector<float> big(numGroups, 1e10f);
+                if (hipMemcpy(groupDt, big.data(), numGroups * 4, hipMemcpyHostToDevice) != hipSuccess)
+                    throw std::runtime_error("hipMemcpy");
+                groupDivvTimestep(all, groupDt, d);
+                groupAccTimestep(all, groupDt, d);
+                storeRung(all, 1, rungs, d);
+                std::array<float, maxNumRungs> dtM1{1e-4f, 2e-4f, 4e-4f, 8e-4f};
+                computePositions(all, d, box, 2e-4f, dtM1, rungs);
+                driftPositions(all, d, 3e-4f, 1e-4f, dtM1, rungs);
+            }
             if (steps > 0)
             {
                 computeTimestep(0, n, d);
@@ -173,7 +201,17 @@ int run(const std::vector<std::vector<double>>& ic, size_t n, bool periodic, dou
 
     FILE* out = std::fopen(outPath, "wb");
     if (!out) { return 1; }
-    if (steps > 0)
+    if (bdt)
+    {
+        ok = download(out, groupDt, numGroups);
+        std::vector<uint8_t> r(n);
+        ok = ok && hipMemcpy(r.data(), rungs, n, hipMemcpyDeviceToHost) == hipSuccess;
+        std::fwrite(r.data(), 1, n, out);
+        for (double* p : {s.x, s.y, s.z}) ok = ok && download(out, p, n);
+        for (H* p : {s.vx, s.vy, s.vz}) ok = ok && download(out, p, n);
+        ok = ok && download(out, s.temp, n);
+    }
+    else if (steps > 0)
     {
         for (double* p : {s.x, s.y, s.z}) ok = ok && download(out, p, n);
         for (H* p : {s.vx, s.vy, s.vz, s.h}) ok = ok && download(out, p, n);
@@ -211,8 +249,10 @@ int main(int argc, char** argv)
     if (mixed) { form.resize(form.size() - 2); }
     if (form != "std" && form != "ve") { return 1; }
     bool fused = argc >= 5 && std::string(argv[4]) == "gpu";
+    bool bdt   = argc >= 5 && std::string(argv[4]) == "bdt";
     int  steps = argc == 6 ? std::atoi(argv[5]) : 0;
     if (steps > 0 && form != "std") { return 1; }
+    if (bdt && (form != "ve" || steps > 0)) { return 1; }
     FILE* in = std::fopen(argv[1], "rb");
     if (!in) { return 1; }
     uint64_t n;
@@ -227,6 +267,6 @@ int main(int argc, char** argv)
     std::fclose(in);
 
     bool ve = form == "ve";
-    return mixed ? run<float>(ic, n, periodic != 0, lo, hi, ve, fused, steps, argv[2])
-                 : run<double>(ic, n, periodic != 0, lo, hi, ve, fused, steps, argv[2]);
+    return mixed ? run<float>(ic, n, periodic != 0, lo, hi, ve, fused, bdt, steps, argv[2])
+                 : run<double>(ic, n, periodic != 0, lo, hi, ve, fused, bdt, steps, argv[2]);
 }

@@ -190,3 +190,57 @@ def test_cpp_interface_steps(tmp_path, orc, periodic, mixed):
         err = np.max(np.abs(got[k][og] - ref)) / scale
         tol = ({"x": 1e-9, "y": 1e-9, "z": 1e-9, "h": 3e-3}.get(k, 1e-4)) if mixed else 1e-10
         assert err < tol, (k, err)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_cpp_interface_block_timesteps(tmp_path, ws, mixed):
+    """The block time-step calls of the C++ interface (groupDivvTimestep, groupAccTimestep, storeRung,
+    computePositions with rungs, driftPositions; tests/cpp/dataset_api.cpp "bdt") after a VE pass,
+    bitwise the same C ABI calls driven from Python with the arguments HydroVeBdtProp passes
+    (Krho, etaAcc sqrt(eps) as float; the previous steps per rung), which the GPU integration tests
+    check against the oracle."""
+    import torch
+
+    from sphexa_amd.particles import VE_FIELDS
+
+    arr, spec, third = _pass_ic("ve")
+    n = arr["x"].size
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(np.array([n], np.uint64).tobytes())
+        fh.write(np.array([1], np.uint32).tobytes())
+        fh.write(np.array([spec["lo"], spec["hi"]], np.float64).tobytes())
+        for col in (arr["x"], arr["y"], arr["z"], arr["h"], arr["m"], third, arr["vx"], arr["vy"], arr["vz"],
+                    arr["alpha"]):
+            fh.write(np.ascontiguousarray(col, np.float64).tobytes())
+    r = subprocess.run([DRIVER, str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), "ve32" if mixed else "ve",
+                        "bdt"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(tmp_path / "out.bin", "rb").read()
+    ng = (n + 63) // 64
+    gdt_cpp = np.frombuffer(raw[:8 * ng], np.float64)
+    rungs_cpp = np.frombuffer(raw[8 * ng:8 * ng + n], np.uint8)
+    rest = np.frombuffer(raw[8 * ng + n:], np.float64).reshape(7, n)
+
+    box = abi.Box.make(spec["lo"], spec["hi"], abi.PERIODIC)
+    params = abi.Params.defaults(ws.K, hydro_type=abi.SPHX_F32 if mixed else abi.SPHX_F64, minDt=1e-3)
+    for k in ("x_m1", "y_m1", "z_m1", "du_m1"):
+        arr[k] = np.zeros(n)
+    ps = ParticleSet.from_host(arr, VE_FIELDS + ("x_m1", "y_m1", "z_m1", "du_m1"), mixed=mixed)
+    conv, _ = ve_step(ws, ps, box, params)
+    assert conv
+    f = ps.fields()
+    gdt = torch.full((ng,), 1e10, dtype=torch.float32, device="cuda")
+    ws.group_divv_timestep(f, params, 0, n, float(np.float32(0.06)), gdt.data_ptr())
+    ws.group_acc_timestep(f, params, 0, n, float(np.float32(0.2 * np.sqrt(0.005))), gdt.data_ptr())
+    rungs = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    ws.store_rung(0, n, 1, rungs.data_ptr())
+    dt_m1 = [1e-4, 2e-4, 4e-4, 8e-4]
+    ws.compute_positions_rungs(box, f, ps.history(), params, 0, n, float(np.float32(2e-4)), dt_m1, rungs.data_ptr())
+    ws.drift_positions(f, ps.history(), params, 0, n, float(np.float32(3e-4)), float(np.float32(1e-4)), dt_m1,
+                       rungs.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(gdt_cpp, gdt.cpu().numpy().astype(np.float64))
+    assert np.all(gdt_cpp < 1e10)  # both limits applied
+    assert np.array_equal(rungs_cpp, rungs.cpu().numpy()) and np.all(rungs_cpp == 1)
+    for row, k in zip(rest, ("x", "y", "z", "vx", "vy", "vz", "temp")):
+        assert np.array_equal(row, ps.host(k).astype(np.float64)), k
