@@ -106,8 +106,10 @@ struct DeviceDataset
     DeviceFields<HydroType> devData;
     sphx_workspace_t*       workspace{nullptr};
 
+    uint64_t iteration{1}, numParticlesGlobal{0}; // step attributes of the file I/O (particles_data.hpp:87-88)
     unsigned ng0{100}, ngmax{150};
     double   K{0}, Kcour{0.2}, gamma{5.0 / 3.0}, muiConst{10.0};
+    double   sincIndex{6.0}; // the kernel's sinc exponent (sphx_workspace_set_kernel takes it)
     double   minDt{1e-12}, minDtCourant{INFINITY};
     double   alphamin{0.05}, alphamax{1.0}, decay_constant{0.2};
     double   Atmin{0.1}, Atmax{0.2}, ramp{1.0 / (0.2 - 0.1)};

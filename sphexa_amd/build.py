@@ -51,6 +51,34 @@ def build_cpp_driver():
     return CPP_DRIVER
 
 
+HDF5_ROOT = os.environ.get("SPHX_HDF5_ROOT", "/opt/conda")
+IO_SRC = os.path.join(HERE, "io", "h5part.cpp")
+IO_LIB = os.path.join(HERE, "libsphexa_amd_io.so")
+
+
+def build_io():
+    """sphexa_amd/libsphexa_amd_io.so: fields and attributes in the H5Part layout of HDF5 files
+    (include/sphexa_amd_io.h), linked against the image's HDF5; skipped when HDF5 is absent."""
+    if not os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
+        print(f"HDF5 not found under {HDF5_ROOT}: libsphexa_amd_io.so not built")
+        return None
+    lib = os.path.join(HDF5_ROOT, "lib")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{os.path.join(HDF5_ROOT, 'include')}", IO_SRC, f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}",
+           "-o", IO_LIB]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"I/O library build failed:\n{r.stdout}\n{r.stderr}")
+    # tests/cpp/h5_api: the C++ I/O interface (include/sphexa_amd_io.hpp), run by the CPU tests
+    drv = os.path.join(ROOT, "tests", "cpp", "h5_api")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", drv + ".cpp", f"-L{HERE}",
+           "-lsphexa_amd_io", "-Wl,-rpath,$ORIGIN/../../sphexa_amd", "-o", drv]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"tests/cpp/h5_api build failed:\n{r.stdout}\n{r.stderr}")
+    return IO_LIB
+
+
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
@@ -64,8 +92,9 @@ def build(verbose=True):
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_cpp_driver()
+    io = build_io()
     if verbose:
-        print(f"built {LIB} and {CPP_DRIVER}")
+        print(f"built {LIB}, {CPP_DRIVER}" + (f" and {io}" if io else ""))
     return LIB
 
 
