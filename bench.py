@@ -77,7 +77,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-std", action="store_true", help="skip the std-pass side measurement")
     ap.add_argument("--no-side", action="store_true",
-                    help="skip the side workloads (Evrard sphere, adjoint pass)")
+                    help="skip the side workloads (Evrard sphere, adjoint pass, checkpoint I/O)")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (counter calibration under rocprofv3)")
@@ -284,6 +284,66 @@ def run_std(arr, spec, mixed, steps, warmup):
     del ps, ws
     torch.cuda.empty_cache()
     return out
+
+
+CHECKPOINT_FIELDS = ("x", "y", "z", "h", "m", "u", "vx", "vy", "vz", "x_m1", "y_m1", "z_m1", "du_m1")
+
+
+def run_checkpoint_io(arr, spec, mixed, reps=2):
+    """Side measurement of the H5Part field I/O: the std propagator's checkpoint fields (the conserved
+    fields with the x, y, z and du history) of the 8M particles written from HBM into one step of an
+    H5Part file and read back into HBM, through the host staging buffer. Wall-clock times of the
+    whole call (HDF5 + PCIe), best of `reps`; the read follows the write, so it comes from the page
+    cache. None when the I/O library is not built."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from sphexa_amd import h5io
+    from sphexa_amd.particles import ParticleSet
+
+    if not os.path.exists(h5io.LIB_PATH):
+        return None
+    a = {k: arr[k] for k in CHECKPOINT_FIELDS if k in arr}
+    a["u"] = arr["temp"]
+    for k in CHECKPOINT_FIELDS:
+        a.setdefault(k, np.zeros_like(arr["x"]))
+    ps = ParticleSet.from_host(a, CHECKPOINT_FIELDS, mixed=mixed)
+    n = ps.n
+    nbytes = sum(ps.t[k][:n].numel() * ps.t[k].element_size() for k in CHECKPOINT_FIELDS)
+    back = {k: torch.empty_like(ps.t[k][:n]) for k in CHECKPOINT_FIELDS}
+    tmp = tempfile.mkdtemp(prefix="sphx_ckpt_")
+    path = os.path.join(tmp, "checkpoint.h5")
+    tw, tr = [], []
+    try:
+        for _ in range(reps):
+            if os.path.exists(path):
+                os.remove(path)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            with h5io.H5File(path, h5io.WRITE) as f:
+                f.add_step(n)
+                for k in CHECKPOINT_FIELDS:
+                    f.write_field(k, ps.t[k][:n])
+            tw.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            with h5io.H5File(path) as f:
+                f.set_step(-1)
+                for k in CHECKPOINT_FIELDS:
+                    f.read_field(k, out=back[k])
+            torch.cuda.synchronize()
+            tr.append(time.perf_counter() - t0)
+        same = all(torch.equal(back[k], ps.t[k][:n]) for k in CHECKPOINT_FIELDS)
+        fsize = os.path.getsize(path)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    del ps, back
+    torch.cuda.empty_cache()
+    return dict(fields=len(CHECKPOINT_FIELDS), particles=n, field_bytes=nbytes, file_bytes=fsize,
+                write_s=round(min(tw), 4), read_s=round(min(tr), 4),
+                write_gbs=round(nbytes / min(tw) / 1e9, 3), read_gbs=round(nbytes / min(tr) / 1e9, 3),
+                round_trip_bitwise=bool(same))
 
 
 EVRARD_KERNELS = ("search", "xmass", "ve_def_gradh", "iad_divv_curlv", "av_switches", "momentum_ve", "sfc_keys",
@@ -563,7 +623,14 @@ def main():
     if not args.no_side:
         side_runs["evrard_4M_ve"] = {f: run_evrard(4_000_000, f == "mixed", args.steps, args.warmup) for f in forms}
         side_runs["adjoint_2M_std"] = {f: run_adjoint(126, f == "mixed", args.steps, args.warmup) for f in forms}
+        io = {f: run_checkpoint_io(arr, spec, f == "mixed") for f in forms}
+        if all(v is not None for v in io.values()):
+            side_runs["checkpoint_io_8M"] = io
+            print("# checkpoint_io_8M: " + ", ".join(f"{f} write {v['write_gbs']} GB/s read {v['read_gbs']} GB/s"
+                                                      for f, v in io.items()), file=sys.stderr, flush=True)
         for name, r in side_runs.items():
+            if name == "checkpoint_io_8M":
+                continue
             print(f"# {name}: " + ", ".join(f"{f} {v['ms_per_step']:.3f} ms" for f, v in r.items()), file=sys.stderr,
                   flush=True)
 
@@ -620,8 +687,8 @@ def main():
         # the kernel with the largest share of the headline pass, then every kernel of the pass per form
         "roofline": roofline,
         "roofline_kernels": rl,
-        # the load-balance workload (Evrard-like sphere, 4M, VE) and the differentiation workload
-        # (2M, std forward + adjoint), per form
+        # the load-balance workload (Evrard-like sphere, 4M, VE), the differentiation workload
+        # (2M, std forward + adjoint) and the checkpoint I/O of the 8M particles (H5Part), per form
         "side_workloads": side_runs or None,
         "cpu_baseline": cpu,
     }

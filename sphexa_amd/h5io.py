@@ -161,3 +161,24 @@ class H5File:
 
     def file_attributes(self):
         return self.names(1, step=False)
+
+
+def read_template_block(path):
+    """x, y, z (float64) of the last step of `path`: readTemplateBlock (main/src/init/utils.hpp:74-87),
+    how the reference's initialisers load a glass block."""
+    with H5File(path) as f:
+        f.set_step(-1)
+        return tuple(f.read_field(k, np.float64) for k in ("x", "y", "z"))
+
+
+def read_file_attributes(path, settings=None):
+    """readFileAttributes (main/src/init/utils.hpp:90-121): every file attribute of `path` holding one
+    value, read as a double into `settings` (a dict, overriding what it holds); returns `settings`."""
+    settings = {} if settings is None else settings
+    with H5File(path) as f:
+        if f.num_steps() > 0:
+            f.set_step(-1)
+        for name in f.file_attributes():
+            if f.attrib_info(name, step=False)[1] == 1:
+                settings[name] = float(f.read_attrib(name, np.float64, step=False)[0])
+    return settings

@@ -203,3 +203,30 @@ def test_cpp_interface_checkpoint(tmp_path):
     assert ref.read_attrib("iteration", np.int64, 1)[0] == 43 and ref.read_attrib("time", np.float64, 1)[0] == 0.125
     assert sorted(ref.dataset_names()) == ["h", "nc", "temp", "x"] and ref.num_particles() == 1000
     ref.close()
+
+
+@needs_ref
+def test_template_block_and_settings_from_reference_file(tmp_path):
+    """readTemplateBlock and readFileAttributes (main/src/init/utils.hpp:74-121) on a file the
+    reference's H5Part wrote: the block is x, y, z of the last step, read as double from float
+    datasets too; the settings are the one-value file attributes as doubles, overriding defaults,
+    and an attribute of several values is left out."""
+    rng = np.random.default_rng(4)
+    path = tmp_path / "glass.h5"
+    blocks = [rng.uniform(-0.5, 0.5, (3, 100)), rng.uniform(-0.5, 0.5, (3, 125)).astype(np.float32)]
+    w = R.RefH5Part(path, R.H5PART_WRITE)
+    w.write_attrib("gamma", np.array([1.4]), step=False)
+    w.write_attrib("ng0", np.array([120], np.int32), step=False)
+    w.write_attrib("eosChoice", np.array([1], np.int8), step=False)
+    w.write_attrib("box", np.array([-0.5, 0.5]), step=False)
+    for s, b in enumerate(blocks):
+        w.set_step(s)
+        w.set_num_particles(b.shape[1])
+        for k, v in zip("xyz", b):
+            w.write(k, v)
+    w.close()
+    x, y, z = h5io.read_template_block(path)
+    for got, want in zip((x, y, z), blocks[1]):
+        assert got.dtype == np.float64 and np.array_equal(got, want.astype(np.float64))
+    settings = h5io.read_file_attributes(path, {"gamma": 5 / 3, "Kcour": 0.2})
+    assert settings == {"gamma": 1.4, "ng0": 120.0, "eosChoice": 1.0, "Kcour": 0.2}
