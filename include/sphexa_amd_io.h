@@ -81,8 +81,9 @@ int sphx_h5_write_attrib(sphx_h5file_t* f, int step, const char* name, int type,
 int sphx_h5_read_attrib(sphx_h5file_t* f, int step, const char* name, int type, void* value, uint64_t count);
 
 /* Names: number of fields of the current step (step = 1 with kind 0), of its attributes (step = 1,
- * kind 1) or of the file attributes (step = 0, kind 1), and name i of them into buf; the size and
- * element type of an attribute. */
+ * kind 1) or of the file attributes (step = 0, kind 1), and name i of them into buf (a name that does
+ * not fit in len bytes with its terminator is SPHX_H5_ERR_ARG, buf untouched: call again with a
+ * larger buffer); the size and element type of an attribute. */
 int sphx_h5_count(sphx_h5file_t* f, int step, int kind, int64_t* out);
 int sphx_h5_name(sphx_h5file_t* f, int step, int kind, int64_t i, char* buf, size_t len);
 int sphx_h5_attrib_info(sphx_h5file_t* f, int step, const char* name, int* type, uint64_t* count);

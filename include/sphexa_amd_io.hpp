@@ -49,9 +49,16 @@ inline std::vector<std::string> names(sphx_h5file_t* f, int step, int kind)
     std::vector<std::string> out;
     for (int64_t i = 0; i < n; ++i)
     {
-        char buf[256];
-        check(sphx_h5_name(f, step, kind, i, buf, sizeof(buf)));
-        out.emplace_back(buf);
+        std::string buf(256, '\0');
+        int         code;
+        // a name that does not fit is an argument error: grow the buffer and ask again
+        while ((code = sphx_h5_name(f, step, kind, i, buf.data(), buf.size())) == SPHX_H5_ERR_ARG &&
+               buf.size() < (size_t(1) << 20))
+        {
+            buf.resize(buf.size() * 4);
+        }
+        check(code);
+        out.emplace_back(buf.c_str());
     }
     return out;
 }

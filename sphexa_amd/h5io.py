@@ -148,8 +148,14 @@ class H5File:
         _check(lib.sphx_h5_count(self.h, int(step), kind, C.byref(k)))
         out = []
         for i in range(k.value):
-            buf = C.create_string_buffer(256)
-            _check(lib.sphx_h5_name(self.h, int(step), kind, i, buf, 256))
+            size = 256
+            while True:
+                buf = C.create_string_buffer(size)
+                code = lib.sphx_h5_name(self.h, int(step), kind, i, buf, size)
+                if code != ERR_ARG or size >= 1 << 20:
+                    break
+                size *= 4  # the name did not fit
+            _check(code)
             out.append(buf.value.decode())
         return out
 

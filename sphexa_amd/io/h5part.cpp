@@ -93,6 +93,19 @@ void check(herr_t r, const std::string& what)
     if (r < 0) { fail(SPHX_H5_ERR_IO, "HDF5: " + what); }
 }
 
+//! name of link or attribute i of a group, whatever its length (`get` is H5Lget_name_by_idx or
+//! H5Aget_name_by_idx: called without a buffer it returns the length)
+template<class Get>
+std::string nameAt(Get get, hid_t group, hsize_t i)
+{
+    ssize_t len = get(group, ".", H5_INDEX_NAME, H5_ITER_INC, i, nullptr, 0, H5P_DEFAULT);
+    if (len < 0) { fail(SPHX_H5_ERR_IO, "HDF5: name of entry " + std::to_string(i)); }
+    std::string name(size_t(len) + 1, '\0');
+    check(get(group, ".", H5_INDEX_NAME, H5_ITER_INC, i, name.data(), name.size(), H5P_DEFAULT), "name");
+    name.resize(size_t(len));
+    return name;
+}
+
 //! closes an HDF5 object at scope exit
 struct Handle
 {
@@ -114,6 +127,16 @@ bool onDevice(const void* p)
 
 std::string stepName(int64_t k) { return "Step#" + std::to_string(k); }
 
+//! the dataset name H5Part stores for a field name: names over 64 characters keep their first 63
+//! (_normalize_dataset_name, H5Part.c:906-921, with H5PART_DATANAME_LEN = 64), on writing and on
+//! reading alike
+std::string datasetName(const char* name)
+{
+    std::string s(name);
+    if (s.size() > 64) { s.resize(63); }
+    return s;
+}
+
 //! "Step#<digits>" groups of the root (H5PartGetNumSteps)
 int64_t countSteps(hid_t file)
 {
@@ -122,10 +145,7 @@ int64_t countSteps(hid_t file)
     int64_t steps = 0;
     for (hsize_t i = 0; i < info.nlinks; ++i)
     {
-        char    buf[256];
-        ssize_t len = H5Lget_name_by_idx(file, ".", H5_INDEX_NAME, H5_ITER_INC, i, buf, sizeof(buf), H5P_DEFAULT);
-        if (len < 0) { continue; }
-        std::string name(buf);
+        const std::string name = nameAt(H5Lget_name_by_idx, file, i);
         if (name.rfind("Step#", 0) == 0 && name.size() > 5 &&
             name.find_first_not_of("0123456789", 5) == std::string::npos)
         {
@@ -160,12 +180,10 @@ std::vector<std::string> names(hid_t group, int kind)
         check(H5Gget_info(group, &info), "group info");
         for (hsize_t i = 0; i < info.nlinks; ++i)
         {
-            char    buf[256];
-            ssize_t len = H5Lget_name_by_idx(group, ".", H5_INDEX_NAME, H5_ITER_INC, i, buf, sizeof(buf), H5P_DEFAULT);
-            if (len < 0) { continue; }
-            H5O_info_t oi;
-            if (H5Oget_info_by_name(group, buf, &oi, H5P_DEFAULT) >= 0 && oi.type == H5O_TYPE_DATASET)
-                out.emplace_back(buf);
+            std::string name = nameAt(H5Lget_name_by_idx, group, i);
+            H5O_info_t  oi;
+            if (H5Oget_info_by_name(group, name.c_str(), &oi, H5P_DEFAULT) >= 0 && oi.type == H5O_TYPE_DATASET)
+                out.push_back(std::move(name));
         }
     }
     else
@@ -173,11 +191,7 @@ std::vector<std::string> names(hid_t group, int kind)
         H5O_info_t oi;
         check(H5Oget_info(group, &oi), "object info");
         for (hsize_t i = 0; i < oi.num_attrs; ++i)
-        {
-            char    buf[256];
-            ssize_t len = H5Aget_name_by_idx(group, ".", H5_INDEX_NAME, H5_ITER_INC, i, buf, sizeof(buf), H5P_DEFAULT);
-            if (len >= 0) { out.emplace_back(buf); }
-        }
+            out.push_back(nameAt(H5Aget_name_by_idx, group, i));
     }
     return out;
 }
@@ -319,6 +333,7 @@ int sphx_h5_write_field(sphx_h5file_t* f, const char* name, int type, const void
 {
     return guarded([&] {
         handle(f);
+        if (f->mode == SPHX_H5_READ) { fail(SPHX_H5_ERR_ARG, "file opened for reading"); }
         hid_t g = stepGroup(f);
         if (name == nullptr || (data == nullptr && f->n > 0)) { fail(SPHX_H5_ERR_ARG, "null argument"); }
         const hid_t t = nativeType(type);
@@ -334,11 +349,15 @@ int sphx_h5_write_field(sphx_h5file_t* f, const char* name, int type, const void
         const hsize_t dims = f->n;
         Handle        space{H5Screate_simple(1, &dims, nullptr), H5Sclose};
         // an existing dataset of the step is written over, as H5PartWriteData* does (H5Part.c:950-960)
-        hid_t d = H5Lexists(g, name, H5P_DEFAULT) > 0 ? H5Dopen2(g, name, H5P_DEFAULT)
-                                                     : H5Dcreate2(g, name, t, space.id, H5P_DEFAULT, H5P_DEFAULT,
-                                                                  H5P_DEFAULT);
+        const std::string dn = datasetName(name);
+        hid_t d = H5Lexists(g, dn.c_str(), H5P_DEFAULT) > 0
+                      ? H5Dopen2(g, dn.c_str(), H5P_DEFAULT)
+                      : H5Dcreate2(g, dn.c_str(), t, space.id, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
         if (d < 0) { fail(SPHX_H5_ERR_IO, "cannot create dataset " + std::string(name)); }
         Handle dh{d, H5Dclose};
+        Handle ds{H5Dget_space(d), H5Sclose};
+        if (uint64_t(H5Sget_simple_extent_npoints(ds.id)) != f->n)
+            fail(SPHX_H5_ERR_ARG, "dataset " + std::string(name) + " exists with another particle count");
         if (f->n > 0) { check(H5Dwrite(d, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, src), "write " + std::string(name)); }
     });
 }
@@ -350,8 +369,10 @@ int sphx_h5_read_field(sphx_h5file_t* f, const char* name, int type, void* data)
         hid_t g = stepGroup(f);
         if (name == nullptr || (data == nullptr && f->n > 0)) { fail(SPHX_H5_ERR_ARG, "null argument"); }
         const hid_t t = nativeType(type);
-        if (H5Lexists(g, name, H5P_DEFAULT) <= 0) { fail(SPHX_H5_ERR_MISSING, "Could not read field: " + std::string(name)); }
-        Handle d{H5Dopen2(g, name, H5P_DEFAULT), H5Dclose};
+        const std::string dn = datasetName(name);
+        if (H5Lexists(g, dn.c_str(), H5P_DEFAULT) <= 0)
+            fail(SPHX_H5_ERR_MISSING, "Could not read field: " + std::string(name));
+        Handle d{H5Dopen2(g, dn.c_str(), H5P_DEFAULT), H5Dclose};
         Handle s{H5Dget_space(d.id), H5Sclose};
         if (uint64_t(H5Sget_simple_extent_npoints(s.id)) != f->n)
             fail(SPHX_H5_ERR_ARG, "field " + std::string(name) + " does not hold the step's particle count");
@@ -438,8 +459,10 @@ int sphx_h5_name(sphx_h5file_t* f, int step, int kind, int64_t i, char* buf, siz
         if (buf == nullptr || len == 0 || (kind == 0 && !step)) { fail(SPHX_H5_ERR_ARG, "invalid argument"); }
         auto all = names(kind == 0 ? stepGroup(f) : attribOwner(f, step), kind);
         if (i < 0 || i >= int64_t(all.size())) { fail(SPHX_H5_ERR_MISSING, "no entry " + std::to_string(i)); }
-        std::strncpy(buf, all[size_t(i)].c_str(), len - 1);
-        buf[len - 1] = '\0';
+        const std::string& name = all[size_t(i)];
+        if (name.size() + 1 > len)
+            fail(SPHX_H5_ERR_ARG, "name buffer too small: " + std::to_string(name.size() + 1) + " bytes needed");
+        std::memcpy(buf, name.c_str(), name.size() + 1);
     });
 }
 

@@ -230,3 +230,54 @@ def test_template_block_and_settings_from_reference_file(tmp_path):
         assert got.dtype == np.float64 and np.array_equal(got, want.astype(np.float64))
     settings = h5io.read_file_attributes(path, {"gamma": 5 / 3, "Kcour": 0.2})
     assert settings == {"gamma": 1.4, "ng0": 120.0, "eosChoice": 1.0, "Kcour": 0.2}
+
+
+@needs_ref
+def test_long_names_and_write_errors(tmp_path):
+    """attribute names of any length list and read back; a field name over 64 characters is stored
+    as its first 63, on writing and reading alike (H5Part.c:906-921), so the reference's H5Part reads
+    it under the full name too. Writing in a file opened for reading, or over a dataset of the step
+    with another particle count, is an error."""
+    path = tmp_path / "names.h5"
+    long_field, long_attr = "f" * 300, "a" * 280
+    with h5io.H5File(path, h5io.WRITE) as f:
+        f.add_step(10)
+        f.write_field(long_field, np.arange(10.0))
+        f.write_attrib(long_attr, 2.5)
+        f.write_attrib("file_" + "b" * 260, np.int32(3), step=False)
+    with h5io.H5File(path) as f:
+        f.set_step(0)
+        assert f.fields() == [long_field[:63]] and f.step_attributes() == [long_attr]
+        assert f.file_attributes() == ["file_" + "b" * 260]
+        assert np.array_equal(f.read_field(long_field), np.arange(10.0))
+        with pytest.raises(h5io.H5Error) as e:
+            f.write_field("x", np.zeros(10))
+        assert e.value.code == h5io.ERR_ARG
+    r = R.RefH5Part(path, R.H5PART_READ)
+    r.set_step(0)
+    assert np.array_equal(r.read(long_field, np.float64), np.arange(10.0))
+    r.close()
+    with h5io.H5File(path, h5io.APPEND) as f:
+        f.add_step(4)
+        f.write_field("x", np.zeros(4))
+        f.set_step(0)  # back to the first step, whose datasets hold 10 particles
+        f.write_field(long_field, np.ones(10))
+    # a step whose datasets differ in length (H5PartSetNumParticles between writes): the step's count
+    # is its first dataset's, and writing over a dataset of another length is refused
+    mixed = tmp_path / "ragged.h5"
+    w = R.RefH5Part(mixed, R.H5PART_WRITE)
+    w.set_step(0)
+    w.set_num_particles(4)
+    w.write("x", np.zeros(4))
+    w.set_num_particles(6)
+    w.write("y", np.zeros(6))
+    w.close()
+    with h5io.H5File(mixed, h5io.APPEND) as f:
+        assert f.set_step(0) == 4
+        f.write_field("x", np.ones(4))
+        with pytest.raises(h5io.H5Error) as e:
+            f.write_field("y", np.ones(4))
+        assert e.value.code == h5io.ERR_ARG and "another particle count" in str(e.value)
+        with pytest.raises(h5io.H5Error) as e:
+            f.read_field("y")
+        assert e.value.code == h5io.ERR_ARG
