@@ -756,7 +756,7 @@ __device__ __forceinline__ void setRadius(Target& t, Th hi, double slack)
 #define SPHX_WAVES_SEARCH_F32 4
 #endif
 #ifndef SPHX_WAVES_SEARCH_F64
-#define SPHX_WAVES_SEARCH_F64 3
+#define SPHX_WAVES_SEARCH_F64 4
 #endif
 
 template<class Th, bool XMass>
