@@ -487,7 +487,7 @@ def kernel_rooflines(r, mixed):
     times the units of the launch) over the kernel's mean duration from HIP events on its stream,
     against the FP64 vector peak for FP64 work (the search's distance tests in both forms) and the
     FP32 peaks for the FP32 pair loops: 157.3 TFLOP/s with packed FP32 issue, 78.6 TFLOP/s without
-    (this build compiles with packed FP32 off)."""
+    (the library is compiled with packed FP32 off)."""
     out = {}
     side = round(r["n"] ** (1.0 / 3.0))
     form = "mixed" if mixed else "fp64"
@@ -591,7 +591,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (multi-GPU sharding is a "
-              f"design note, not part of this build)", file=sys.stderr)
+              f"design note, not implemented)", file=sys.stderr)
         sys.exit(2)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if world > 1:
