@@ -19,6 +19,8 @@ struct sphx_h5file
     hid_t    step = -1; // group of the current step
     int      mode = 0;
     uint64_t n    = 0;  // particles of the current step
+    void*    pinned      = nullptr; // page-locked staging of device fields, grown to the largest field
+    size_t   pinnedBytes = 0;
 };
 
 namespace
@@ -123,6 +125,28 @@ bool onDevice(const void* p)
     bool                  dev = hipPointerGetAttributes(&a, p) == hipSuccess && a.type == hipMemoryTypeDevice;
     (void)hipGetLastError(); // host memory unknown to HIP leaves an error to clear
     return dev;
+}
+
+//! page-locked host buffer of at least `bytes` for staging a device field (the DMA engines copy
+//! page-locked memory directly; pageable memory goes through the runtime's own bounce buffers), or
+//! the pageable `fallback` when page-locked memory cannot be had
+void* stagingBuffer(sphx_h5file* f, size_t bytes, std::vector<char>& fallback)
+{
+    if (bytes > f->pinnedBytes)
+    {
+        if (f->pinned) { (void)hipHostFree(f->pinned); }
+        f->pinned      = nullptr;
+        f->pinnedBytes = 0;
+        if (hipHostMalloc(&f->pinned, bytes, hipHostMallocDefault) == hipSuccess) { f->pinnedBytes = bytes; }
+        else
+        {
+            f->pinned = nullptr;
+            (void)hipGetLastError();
+        }
+    }
+    if (f->pinned) { return f->pinned; }
+    fallback.resize(bytes);
+    return fallback.data();
 }
 
 std::string stepName(int64_t k) { return "Step#" + std::to_string(k); }
@@ -271,6 +295,7 @@ int sphx_h5_close(sphx_h5file_t* f)
         if (f == nullptr) { return; }
         if (f->step >= 0) { H5Gclose(f->step); }
         herr_t r = f->file >= 0 ? H5Fclose(f->file) : 0;
+        if (f->pinned) { (void)hipHostFree(f->pinned); }
         delete f;
         check(r, "close");
     });
@@ -337,14 +362,15 @@ int sphx_h5_write_field(sphx_h5file_t* f, const char* name, int type, const void
         hid_t g = stepGroup(f);
         if (name == nullptr || (data == nullptr && f->n > 0)) { fail(SPHX_H5_ERR_ARG, "null argument"); }
         const hid_t t = nativeType(type);
-        std::vector<char> staging;
+        std::vector<char> fallback;
         const void*       src = data;
         if (f->n > 0 && onDevice(data))
         {
-            staging.resize(f->n * typeSize(type));
-            if (hipMemcpy(staging.data(), data, staging.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            const size_t bytes = f->n * typeSize(type);
+            void*        host  = stagingBuffer(f, bytes, fallback);
+            if (hipMemcpy(host, data, bytes, hipMemcpyDeviceToHost) != hipSuccess)
                 fail(SPHX_H5_ERR_IO, "device to host copy of field " + std::string(name));
-            src = staging.data();
+            src = host;
         }
         const hsize_t dims = f->n;
         Handle        space{H5Screate_simple(1, &dims, nullptr), H5Sclose};
@@ -377,11 +403,12 @@ int sphx_h5_read_field(sphx_h5file_t* f, const char* name, int type, void* data)
         if (uint64_t(H5Sget_simple_extent_npoints(s.id)) != f->n)
             fail(SPHX_H5_ERR_ARG, "field " + std::string(name) + " does not hold the step's particle count");
         if (f->n == 0) { return; }
-        const bool        dev = onDevice(data);
-        std::vector<char> staging(dev ? f->n * typeSize(type) : 0);
-        void*             dst = dev ? staging.data() : data;
+        const bool        dev   = onDevice(data);
+        const size_t      bytes = f->n * typeSize(type);
+        std::vector<char> fallback;
+        void*             dst = dev ? stagingBuffer(f, bytes, fallback) : data;
         check(H5Dread(d.id, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, dst), "read " + std::string(name));
-        if (dev && hipMemcpy(data, dst, staging.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (dev && hipMemcpy(data, dst, bytes, hipMemcpyHostToDevice) != hipSuccess)
             fail(SPHX_H5_ERR_IO, "host to device copy of field " + std::string(name));
     });
 }
