@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,8 +20,10 @@ struct sphx_h5file
     hid_t    step = -1; // group of the current step
     int      mode = 0;
     uint64_t n    = 0;  // particles of the current step
-    void*    pinned      = nullptr; // page-locked staging of device fields, grown to the largest field
-    size_t   pinnedBytes = 0;
+    void*       pinned      = nullptr; // page-locked staging of device fields: two chunk buffers
+    size_t      pinnedBytes = 0;
+    hipStream_t copies      = nullptr; // stream of the staging copies (default priority)
+    hipEvent_t  done[2]     = {nullptr, nullptr}; // copy of the chunk in staging buffer 0 / 1 complete
 };
 
 namespace
@@ -127,26 +130,139 @@ bool onDevice(const void* p)
     return dev;
 }
 
-//! page-locked host buffer of at least `bytes` for staging a device field (the DMA engines copy
-//! page-locked memory directly; pageable memory goes through the runtime's own bounce buffers), or
-//! the pageable `fallback` when page-locked memory cannot be had
-void* stagingBuffer(sphx_h5file* f, size_t bytes, std::vector<char>& fallback)
+//! device fields move in chunks of this many bytes, through two page-locked staging buffers, so
+//! the copy of one chunk overlaps HDF5's work on the other
+constexpr size_t kChunkBytes = size_t(16) << 20;
+
+//! two page-locked chunk buffers (each `chunk` bytes), the copy stream and its events; false when
+//! page-locked memory or the stream cannot be had (the caller then stages through pageable memory)
+bool pinnedStaging(sphx_h5file* f, size_t chunk)
 {
-    if (bytes > f->pinnedBytes)
+    if (f->copies == nullptr)
+    {
+        if (hipStreamCreate(&f->copies) != hipSuccess)
+        {
+            f->copies = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+        for (auto& e : f->done)
+        {
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            {
+                e = nullptr;
+                (void)hipGetLastError();
+                return false;
+            }
+        }
+    }
+    if (f->done[0] == nullptr || f->done[1] == nullptr) { return false; }
+    if (2 * chunk > f->pinnedBytes)
     {
         if (f->pinned) { (void)hipHostFree(f->pinned); }
         f->pinned      = nullptr;
         f->pinnedBytes = 0;
-        if (hipHostMalloc(&f->pinned, bytes, hipHostMallocDefault) == hipSuccess) { f->pinnedBytes = bytes; }
+        if (hipHostMalloc(&f->pinned, 2 * chunk, hipHostMallocDefault) == hipSuccess) { f->pinnedBytes = 2 * chunk; }
         else
         {
             f->pinned = nullptr;
             (void)hipGetLastError();
         }
     }
-    if (f->pinned) { return f->pinned; }
-    fallback.resize(bytes);
-    return fallback.data();
+    return f->pinned != nullptr;
+}
+
+void releaseStaging(sphx_h5file* f)
+{
+    if (f->copies) { (void)hipStreamSynchronize(f->copies); }
+    for (auto& e : f->done)
+    {
+        if (e) { (void)hipEventDestroy(e); }
+        e = nullptr;
+    }
+    if (f->copies) { (void)hipStreamDestroy(f->copies); }
+    f->copies = nullptr;
+    if (f->pinned) { (void)hipHostFree(f->pinned); }
+    f->pinned      = nullptr;
+    f->pinnedBytes = 0;
+}
+
+//! elements [off, off + cnt) of dataset d from or into host memory `buf` (H5Dwrite / H5Dread on a
+//! hyperslab of the 1-D file space)
+void slab(hid_t d, hid_t t, hsize_t off, hsize_t cnt, void* buf, bool write, const std::string& name)
+{
+    Handle fs{H5Dget_space(d), H5Sclose};
+    check(H5Sselect_hyperslab(fs.id, H5S_SELECT_SET, &off, nullptr, &cnt, nullptr), "select " + name);
+    Handle ms{H5Screate_simple(1, &cnt, nullptr), H5Sclose};
+    if (write) { check(H5Dwrite(d, t, ms.id, fs.id, H5P_DEFAULT, buf), "write " + name); }
+    else { check(H5Dread(d, t, ms.id, fs.id, H5P_DEFAULT, buf), "read " + name); }
+}
+
+//! waits for the staging copies however the caller leaves (an HDF5 error mid-field included)
+struct CopiesDrained
+{
+    hipStream_t s;
+    ~CopiesDrained()
+    {
+        if (s) { (void)hipStreamSynchronize(s); }
+    }
+};
+
+/*! n elements of size es between device memory `dev` and dataset d, in chunks through the two
+ *  page-locked buffers: writing, the device-to-host copy of chunk c + 1 runs while HDF5 writes
+ *  chunk c; reading, the host-to-device copy of chunk c runs while HDF5 reads chunk c + 1. */
+void pipelined(sphx_h5file* f, hid_t d, hid_t t, size_t es, uint64_t n, void* dev, bool write, const std::string& name)
+{
+    const uint64_t ce      = std::max<uint64_t>(1, kChunkBytes / es); // elements per chunk
+    const uint64_t nChunks = (n + ce - 1) / ce;
+    char*          pin     = static_cast<char*>(f->pinned);
+    char*          buf[2]  = {pin, pin + (nChunks > 1 ? ce * es : 0)}; // one chunk: buffer 0 only
+    char*          base    = static_cast<char*>(dev);
+    CopiesDrained  drain{f->copies};
+    auto           count   = [&](uint64_t c) { return std::min<uint64_t>(ce, n - c * ce); };
+    auto           hipOk   = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            fail(SPHX_H5_ERR_IO, std::string(what) + " of field " + name);
+        }
+    };
+    // the copy stream is ordered after no other stream: finish the device work that produced or
+    // still uses the field first, as a synchronous copy would
+    hipOk(hipDeviceSynchronize(), "device synchronisation");
+    if (write)
+    {
+        hipOk(hipMemcpyAsync(buf[0], base, count(0) * es, hipMemcpyDeviceToHost, f->copies), "device to host copy");
+        hipOk(hipEventRecord(f->done[0], f->copies), "event");
+        for (uint64_t c = 0; c < nChunks; ++c)
+        {
+            const int b = int(c & 1);
+            if (c + 1 < nChunks)
+            {
+                // buffer b ^ 1 held chunk c - 1, whose HDF5 write has returned
+                hipOk(hipMemcpyAsync(buf[b ^ 1], base + (c + 1) * ce * es, count(c + 1) * es, hipMemcpyDeviceToHost,
+                                     f->copies),
+                      "device to host copy");
+                hipOk(hipEventRecord(f->done[b ^ 1], f->copies), "event");
+            }
+            hipOk(hipEventSynchronize(f->done[b]), "device to host copy");
+            slab(d, t, c * ce, count(c), buf[b], true, name);
+        }
+    }
+    else
+    {
+        for (uint64_t c = 0; c < nChunks; ++c)
+        {
+            const int b = int(c & 1);
+            // buffer b held chunk c - 2: its host-to-device copy must be done before HDF5 refills it
+            if (c >= 2) { hipOk(hipEventSynchronize(f->done[b]), "host to device copy"); }
+            slab(d, t, c * ce, count(c), buf[b], false, name);
+            hipOk(hipMemcpyAsync(base + c * ce * es, buf[b], count(c) * es, hipMemcpyHostToDevice, f->copies),
+                  "host to device copy");
+            hipOk(hipEventRecord(f->done[b], f->copies), "event");
+        }
+        hipOk(hipStreamSynchronize(f->copies), "host to device copy");
+    }
 }
 
 std::string stepName(int64_t k) { return "Step#" + std::to_string(k); }
@@ -295,7 +411,7 @@ int sphx_h5_close(sphx_h5file_t* f)
         if (f == nullptr) { return; }
         if (f->step >= 0) { H5Gclose(f->step); }
         herr_t r = f->file >= 0 ? H5Fclose(f->file) : 0;
-        if (f->pinned) { (void)hipHostFree(f->pinned); }
+        releaseStaging(f);
         delete f;
         check(r, "close");
     });
@@ -362,16 +478,6 @@ int sphx_h5_write_field(sphx_h5file_t* f, const char* name, int type, const void
         hid_t g = stepGroup(f);
         if (name == nullptr || (data == nullptr && f->n > 0)) { fail(SPHX_H5_ERR_ARG, "null argument"); }
         const hid_t t = nativeType(type);
-        std::vector<char> fallback;
-        const void*       src = data;
-        if (f->n > 0 && onDevice(data))
-        {
-            const size_t bytes = f->n * typeSize(type);
-            void*        host  = stagingBuffer(f, bytes, fallback);
-            if (hipMemcpy(host, data, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-                fail(SPHX_H5_ERR_IO, "device to host copy of field " + std::string(name));
-            src = host;
-        }
         const hsize_t dims = f->n;
         Handle        space{H5Screate_simple(1, &dims, nullptr), H5Sclose};
         // an existing dataset of the step is written over, as H5PartWriteData* does (H5Part.c:950-960)
@@ -384,7 +490,22 @@ int sphx_h5_write_field(sphx_h5file_t* f, const char* name, int type, const void
         Handle ds{H5Dget_space(d), H5Sclose};
         if (uint64_t(H5Sget_simple_extent_npoints(ds.id)) != f->n)
             fail(SPHX_H5_ERR_ARG, "dataset " + std::string(name) + " exists with another particle count");
-        if (f->n > 0) { check(H5Dwrite(d, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, src), "write " + std::string(name)); }
+        if (f->n == 0) { return; }
+        const size_t es = typeSize(type);
+        if (!onDevice(data))
+        {
+            check(H5Dwrite(d, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "write " + std::string(name));
+            return;
+        }
+        if (pinnedStaging(f, std::min<size_t>(kChunkBytes, f->n * es) + es))
+        {
+            pipelined(f, d, t, es, f->n, const_cast<void*>(data), true, name);
+            return;
+        }
+        std::vector<char> host(f->n * es); // no page-locked memory: one pageable copy
+        if (hipMemcpy(host.data(), data, host.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            fail(SPHX_H5_ERR_IO, "device to host copy of field " + std::string(name));
+        check(H5Dwrite(d, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, host.data()), "write " + std::string(name));
     });
 }
 
@@ -403,12 +524,20 @@ int sphx_h5_read_field(sphx_h5file_t* f, const char* name, int type, void* data)
         if (uint64_t(H5Sget_simple_extent_npoints(s.id)) != f->n)
             fail(SPHX_H5_ERR_ARG, "field " + std::string(name) + " does not hold the step's particle count");
         if (f->n == 0) { return; }
-        const bool        dev   = onDevice(data);
-        const size_t      bytes = f->n * typeSize(type);
-        std::vector<char> fallback;
-        void*             dst = dev ? stagingBuffer(f, bytes, fallback) : data;
-        check(H5Dread(d.id, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, dst), "read " + std::string(name));
-        if (dev && hipMemcpy(data, dst, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        const size_t es = typeSize(type);
+        if (!onDevice(data))
+        {
+            check(H5Dread(d.id, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "read " + std::string(name));
+            return;
+        }
+        if (pinnedStaging(f, std::min<size_t>(kChunkBytes, f->n * es) + es))
+        {
+            pipelined(f, d.id, t, es, f->n, data, false, name);
+            return;
+        }
+        std::vector<char> host(f->n * es); // no page-locked memory: one pageable copy
+        check(H5Dread(d.id, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, host.data()), "read " + std::string(name));
+        if (hipMemcpy(data, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
             fail(SPHX_H5_ERR_IO, "host to device copy of field " + std::string(name));
     });
 }

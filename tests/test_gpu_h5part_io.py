@@ -103,3 +103,32 @@ def test_restart_from_checkpoint_is_bitwise(tmp_path, ws, mixed):
     assert ts2.minDt == ts.minDt and ts2.ttot == ts.ttot
     for k in CONSERVED:
         assert np.array_equal(ps2.host(k), cont[k]), k
+
+
+def test_device_fields_larger_than_a_staging_chunk(tmp_path):
+    """device fields of several 16 MiB staging chunks, with a ragged last chunk (3,000,001 doubles are
+    two chunks, 9,000,003 floats three): written from and read back into device memory bit for bit,
+    and the same values through a host read"""
+    import torch
+
+    n = 3_000_001
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.rand(n, device="cuda", dtype=torch.float64, generator=g)
+    big = torch.rand(9_000_003, device="cuda", dtype=torch.float32, generator=g)
+    path = tmp_path / "chunks.h5"
+    print(f"case chunked device fields n={n} and {big.numel()}", flush=True)
+    with h5io.H5File(path, h5io.WRITE) as f:
+        f.add_step(n)
+        f.write_field("x", x)
+        f.add_step(big.numel())
+        f.write_field("h", big)
+    with h5io.H5File(path) as f:
+        assert f.set_step(0) == n
+        back = torch.empty_like(x)
+        f.read_field("x", out=back)
+        assert torch.equal(back, x)
+        assert np.array_equal(f.read_field("x"), x.cpu().numpy())
+        assert f.set_step(1) == big.numel()
+        back32 = torch.empty_like(big)
+        f.read_field("h", out=back32)
+        assert torch.equal(back32, big)
