@@ -9,8 +9,9 @@
  *
  * One rank's share of a file: the library opens files serially, as the reference's H5Part build
  * without parallel HDF5 does on one rank (h5part_wrapper.hpp:332-354). Field pointers may be device
- * (HIP) or host memory; device data is staged through host memory. Calls on one handle must not
- * overlap. Separate library (libsphexa_amd_io.so, linked against HDF5) so that the compute library
+ * (HIP) or host memory. A device field is read or written after a device synchronisation, in chunks
+ * through page-locked host buffers the handle keeps until it closes; the call returns when the
+ * data is in the file or in device memory. Calls on one handle must not overlap. Separate library (libsphexa_amd_io.so, linked against HDF5) so that the compute library
  * does not depend on HDF5. */
 #ifndef SPHEXA_AMD_IO_H
 #define SPHEXA_AMD_IO_H
